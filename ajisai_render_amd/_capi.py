@@ -1,0 +1,154 @@
+"""ctypes binding of libajisai_gpu.so (the C-ABI declared in include/ajisai_gpu.h).
+
+The library is built in-tree by ``__graft_entry__.build()``. Loading fails
+loudly when it is missing: there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libajisai_gpu.so")
+
+AJG_OK, AJG_ERR_HIP, AJG_ERR_OOM, AJG_ERR_INVALID, AJG_ERR_CANCELLED, AJG_ERR_INTERNAL = range(6)
+GEOM_QUAD, GEOM_CUBE, GEOM_MESH = 0, 1, 2
+MAT_DIFFUSE, MAT_DISNEY = 0, 1
+LIGHT_DIRECTIONAL, LIGHT_ENV = 0, 1
+
+STATUS_NAMES = {0: "AJG_OK", 1: "AJG_ERR_HIP", 2: "AJG_ERR_OOM", 3: "AJG_ERR_INVALID",
+                4: "AJG_ERR_CANCELLED", 5: "AJG_ERR_INTERNAL"}
+
+
+class AjgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class MeshView(C.Structure):
+    _fields_ = [("position_buffer", C.c_void_p), ("position_offset", C.c_size_t),
+                ("position_stride", C.c_size_t), ("position_size", C.c_size_t),
+                ("index_buffer", C.c_void_p), ("index_offset", C.c_size_t),
+                ("index_stride", C.c_size_t), ("index_size", C.c_size_t)]
+
+
+class AccelInfo(C.Structure):
+    _fields_ = [("tri_count", C.c_uint32), ("node_count", C.c_uint32),
+                ("iterations", C.c_uint32), ("depth", C.c_uint32),
+                ("device_bytes", C.c_uint64), ("scene_box", C.c_float * 6)]
+
+
+class Primitive(C.Structure):
+    _fields_ = [("view", MeshView), ("kind", C.c_uint32), ("two_sided", C.c_uint32),
+                ("material", C.c_uint32), ("reserved", C.c_uint32),
+                ("normal_and_vs", C.c_void_p)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("base_color", C.c_float * 3),
+                ("subsurface", C.c_float), ("metallic", C.c_float), ("specular", C.c_float),
+                ("specular_tint", C.c_float), ("roughness", C.c_float),
+                ("anisotropic", C.c_float), ("sheen", C.c_float), ("sheen_tint", C.c_float),
+                ("clearcoat", C.c_float), ("clearcoat_gloss", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("dir", C.c_float * 3), ("angle", C.c_float),
+                ("radiance", C.c_float * 3), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
+                ("env_texels", C.c_void_p), ("rot", C.c_float), ("scaler", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("raster2camera", C.c_float * 16), ("camera2world", C.c_float * 16),
+                ("resolution", C.c_float * 2), ("lens_radius", C.c_float),
+                ("focal_distance", C.c_float)]
+
+
+class PTArgs(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("tile_size", C.c_int32), ("min_bounces", C.c_int32),
+                ("max_bounces", C.c_int32), ("cont_prob", C.c_float), ("use_mis", C.c_int32),
+                ("seed_mode", C.c_int32), ("seed_base", C.c_uint32),
+                ("env_cdf_fp64", C.c_int32), ("max_slots", C.c_int32)]
+
+
+class Film(C.Structure):
+    _fields_ = [("sum", C.c_void_p), ("sum_sq", C.c_void_p), ("count", C.c_void_p),
+                ("per_sample", C.c_void_p)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("closest_queries", C.c_uint64), ("any_queries", C.c_uint64),
+                ("paths", C.c_uint64), ("iterations", C.c_uint32), ("slots", C.c_uint32),
+                ("ms_total", C.c_float), ("ms_trace_closest", C.c_float),
+                ("ms_trace_any", C.c_float), ("ms_shade", C.c_float)]
+
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("tri_id", "<u4"), ("u", "<f4"), ("v", "<f4")])
+NODE_DTYPE = np.dtype([("mn", "<f4", 3), ("mx", "<f4", 3), ("count", "<u4"), ("first", "<u4")])
+
+_P = C.c_void_p
+_SIGS = {
+    "ajg_abi_version": ([], C.c_int),
+    "ajg_ctx_create": ([C.c_int, C.POINTER(_P)], C.c_int),
+    "ajg_ctx_destroy": ([_P], C.c_int),
+    "ajg_last_error": ([_P], C.c_char_p),
+    "ajg_ctx_inject_failure": ([_P, C.c_int, C.c_int], C.c_int),
+    "ajg_accel_build": ([_P, C.POINTER(MeshView), C.c_uint32, C.c_uint32, C.POINTER(_P)], C.c_int),
+    "ajg_accel_build_soup": ([_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.POINTER(_P)], C.c_int),
+    "ajg_accel_destroy": ([_P], C.c_int),
+    "ajg_accel_get_info": ([_P, C.POINTER(AccelInfo)], C.c_int),
+    "ajg_accel_download": ([_P, _P, _P], C.c_int),
+    "ajg_accel_trace_closest": ([_P, _P, C.c_uint32, _P, _P], C.c_int),
+    "ajg_accel_trace_any": ([_P, _P, C.c_uint32, _P, _P], C.c_int),
+    "ajg_accel_trace_counts": ([_P, _P, C.c_uint32, _P, _P, _P], C.c_int),
+    "ajg_scene_create": ([_P, C.POINTER(Primitive), C.c_uint32, C.POINTER(Material), C.c_uint32,
+                          C.POINTER(Light), C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)],
+                         C.c_int),
+    "ajg_scene_destroy": ([_P], C.c_int),
+    "ajg_scene_get_accel": ([_P, C.POINTER(_P)], C.c_int),
+    "ajg_scene_env_tables": ([_P, C.c_uint32, _P, _P, _P, _P], C.c_int),
+    "ajg_render_pt": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), C.POINTER(Film),
+                       C.POINTER(RenderStats), _P], C.c_int),
+    "ajg_film_resolve": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P], C.c_int),
+    "ajg_camera_setup": ([C.POINTER(Camera), _P, _P, _P, C.c_float, C.c_float, C.c_float,
+                          C.c_float, C.c_float], C.c_int),
+}
+
+_lib = None
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points this binding declares."""
+    return sorted(_SIGS)
+
+
+def load(path: str = LIB_PATH):
+    """Load libajisai_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first "
+                           "(the HIP extension is required; there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(status, ctx=None, what=""):
+    if status != AJG_OK:
+        msg = ""
+        if ctx is not None and _lib is not None:
+            raw = _lib.ajg_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise AjgError(status, f"{what}: {msg}")

@@ -1,0 +1,345 @@
+"""Host-side mirror of Ajisai's Aggregate and Renderer interfaces over the C-ABI.
+
+* ``PLOCAggregate`` mirrors ``Aggregate`` / ``CreatePLOCAggregate(search_radius)``
+  (ajisai/core/aggregate/aggregate.h:32-46): ``Build(primitives)``, batched
+  ``Intersect(rays)`` and ``Occlude(rays)``.
+* ``GpuScene`` mirrors ``DefaultScene`` (scene/default_scene.cpp:29-103).
+* ``PathTracer`` mirrors ``CreatePTRenderer(PTRendererArgs)`` /
+  ``Renderer::Render`` (renderer/renderer.h:37-80, pt.cpp:40-73).
+
+Device buffers are torch tensors on ``cuda``; torch is only used for device
+memory and streams. Every compute call goes through libajisai_gpu.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _capi as capi
+from .scenes import Light, Material, Primitive, Scene
+
+
+def _torch():
+    import torch  # noqa: WPS433 (imported lazily: plumbing only)
+    return torch
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return 0
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())
+
+
+class Context:
+    """ajg_ctx: one per device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = capi.load()
+        h = C.c_void_p()
+        st = self.lib.ajg_ctx_create(device, C.byref(h))
+        if st != capi.AJG_OK:
+            raise capi.AjgError(st, f"ajg_ctx_create(device={device})")
+        self.h = h
+        self.device = device
+
+    def check(self, st, what=""):
+        capi.check(st, self.h, what)
+
+    def inject_failure(self, countdown: int, status: int):
+        self.check(self.lib.ajg_ctx_inject_failure(self.h, countdown, status))
+
+    def close(self):
+        if self.h:
+            self.lib.ajg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mesh_views(prims: Sequence[Primitive]):
+    """MeshView structs over the primitives' numpy buffers (kept alive by
+    the returned list)."""
+    arr = (capi.MeshView * max(1, len(prims)))()
+    keep = []
+    for i, p in enumerate(prims):
+        pos = np.ascontiguousarray(p.positions, np.float32)
+        idx = np.ascontiguousarray(p.index_words, np.uint32)
+        keep += [pos, idx]
+        arr[i] = capi.MeshView(pos.ctypes.data, 0, pos.strides[0], pos.shape[0],
+                               idx.ctypes.data, 0, 12, idx.shape[0])
+    return arr, keep
+
+
+def c_primitives(prims: Sequence[Primitive]):
+    views, keep = mesh_views(prims)
+    arr = (capi.Primitive * max(1, len(prims)))()
+    for i, p in enumerate(prims):
+        nrm = None
+        if p.normal_and_vs is not None:
+            nrm = np.ascontiguousarray(p.normal_and_vs, np.float32)
+            keep.append(nrm)
+        arr[i] = capi.Primitive(views[i], p.kind, int(p.two_sided), p.material, 0,
+                                nrm.ctypes.data if nrm is not None else None)
+    return arr, keep
+
+
+def c_materials(mats: Sequence[Material]):
+    arr = (capi.Material * max(1, len(mats)))()
+    for i, m in enumerate(mats):
+        arr[i] = capi.Material(m.type, (C.c_float * 3)(*m.base_color), m.subsurface, m.metallic,
+                               m.specular, m.specular_tint, m.roughness, m.anisotropic,
+                               m.sheen, m.sheen_tint, m.clearcoat, m.clearcoat_gloss)
+    return arr
+
+
+def c_lights(lights: Sequence[Light]):
+    arr = (capi.Light * max(1, len(lights)))()
+    keep = []
+    for i, l in enumerate(lights):
+        if l.env is not None:
+            env = np.ascontiguousarray(l.env, np.float32)
+            keep.append(env)
+            h, w = env.shape[:2]
+            ptr = env.ctypes.data
+        else:
+            h = w = 0
+            ptr = None
+        arr[i] = capi.Light(l.type, (C.c_float * 3)(*l.dir), l.angle,
+                            (C.c_float * 3)(*l.radiance), w, h, ptr, l.rot, l.scaler)
+    return arr, keep
+
+
+class PLOCAggregate:
+    """Aggregate "ploc" on the GPU (CreatePLOCAggregate, aggregate.h:45)."""
+
+    def __init__(self, ctx: Context, search_radius: int = 14, handle=None, owner=True):
+        self.ctx = ctx
+        self.search_radius = search_radius
+        self.h = handle
+        self.owner = owner
+
+    # Aggregate::Build (aggregate.h:36)
+    def Build(self, primitives: Sequence[Primitive]):
+        self._release()
+        views, keep = mesh_views(primitives)
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.ajg_accel_build(self.ctx.h, views, len(primitives),
+                                                    self.search_radius, C.byref(h)),
+                       "ajg_accel_build")
+        self.h = h
+        del keep
+        return self
+
+    def build_soup(self, tri_dev, scene_box: Optional[np.ndarray] = None, stream=None):
+        """Build from a device-resident (n, 9) float32 soup."""
+        self._release()
+        box = None if scene_box is None else np.ascontiguousarray(scene_box, np.float32)
+        h = C.c_void_p()
+        n = int(tri_dev.shape[0])
+        self.ctx.check(self.ctx.lib.ajg_accel_build_soup(
+            self.ctx.h, _ptr(tri_dev), n, None if box is None else box.ctypes.data,
+            self.search_radius, stream, C.byref(h)), "ajg_accel_build_soup")
+        self.h = h
+        return self
+
+    def info(self) -> capi.AccelInfo:
+        out = capi.AccelInfo()
+        self.ctx.check(self.ctx.lib.ajg_accel_get_info(self.h, C.byref(out)))
+        return out
+
+    def download(self):
+        inf = self.info()
+        nodes = np.zeros(inf.node_count, capi.NODE_DTYPE)
+        perm = np.zeros(inf.tri_count, np.uint32)
+        self.ctx.check(self.ctx.lib.ajg_accel_download(self.h, nodes.ctypes.data,
+                                                       perm.ctypes.data), "download")
+        return nodes, perm
+
+    # Aggregate::Intersect (aggregate.h:38-39), batched over device rays
+    def Intersect(self, rays_dev, hits_dev=None, stream=None):
+        torch = _torch()
+        n = rays_dev.numel() // 32
+        if hits_dev is None:
+            hits_dev = torch.empty(n * 16, dtype=torch.uint8, device=rays_dev.device)
+        self.ctx.check(self.ctx.lib.ajg_accel_trace_closest(self.h, _ptr(rays_dev), n,
+                                                            _ptr(hits_dev), stream),
+                       "trace_closest")
+        return hits_dev
+
+    # Aggregate::Occlude (aggregate.h:41), batched
+    def Occlude(self, rays_dev, occ_dev=None, stream=None):
+        torch = _torch()
+        n = rays_dev.numel() // 32
+        if occ_dev is None:
+            occ_dev = torch.empty(n, dtype=torch.uint8, device=rays_dev.device)
+        self.ctx.check(self.ctx.lib.ajg_accel_trace_any(self.h, _ptr(rays_dev), n,
+                                                        _ptr(occ_dev), stream), "trace_any")
+        return occ_dev
+
+    def counts(self, rays_dev, stream=None):
+        torch = _torch()
+        n = rays_dev.numel() // 32
+        nodes = torch.empty(n, dtype=torch.int32, device=rays_dev.device)
+        tris = torch.empty(n, dtype=torch.int32, device=rays_dev.device)
+        self.ctx.check(self.ctx.lib.ajg_accel_trace_counts(self.h, _ptr(rays_dev), n,
+                                                           _ptr(nodes), _ptr(tris), stream),
+                       "trace_counts")
+        return nodes, tris
+
+    def _release(self):
+        if self.h and self.owner:
+            self.ctx.lib.ajg_accel_destroy(self.h)
+        self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self._release()
+        except Exception:
+            pass
+
+
+class GpuScene:
+    """DefaultScene on the device (default_scene.cpp:29-103) with the GPU
+    PLOC aggregate."""
+
+    def __init__(self, ctx: Context, scene: Scene, env_cdf_fp64: int = 1):
+        self.ctx = ctx
+        self.scene = scene
+        prims, keep = c_primitives(scene.primitives)
+        mats = c_materials(scene.materials)
+        lights, keep2 = c_lights(scene.lights)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.ajg_scene_create(ctx.h, prims, len(scene.primitives), mats,
+                                           len(scene.materials), lights, len(scene.lights),
+                                           scene.search_radius, env_cdf_fp64, C.byref(h)),
+                  "ajg_scene_create")
+        self.h = h
+        del keep, keep2
+        acc = C.c_void_p()
+        ctx.check(ctx.lib.ajg_scene_get_accel(self.h, C.byref(acc)))
+        self.accel = PLOCAggregate(ctx, scene.search_radius, handle=acc, owner=False)
+
+    def env_tables(self, li):
+        l = self.scene.lights[li]
+        h, w = l.env.shape[:2]
+        cc = np.zeros((h, w + 1), np.float32)
+        cf = np.zeros(h, np.float32)
+        mc = np.zeros(h + 1, np.float32)
+        mf = np.zeros(1, np.float32)
+        self.ctx.check(self.ctx.lib.ajg_scene_env_tables(self.h, li, cc.ctypes.data,
+                                                         cf.ctypes.data, mc.ctypes.data,
+                                                         mf.ctypes.data), "env tables")
+        return cc, cf, mc, mf[0]
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ajg_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PathTracer:
+    """CreatePTRenderer(PTRendererArgs) (renderer.h:37-45, 80) for TraceMIS on
+    the device. Renders into per-traced-pixel film sums."""
+
+    def __init__(self, gscene: GpuScene, spp=None, tile_size=None, min_bounces=None,
+                 max_bounces=None, cont_prob=None, seed_mode=0, seed_base=0, env_cdf_fp64=1,
+                 max_slots=0):
+        pt = gscene.scene.pt
+        self.g = gscene
+        self.args = capi.PTArgs(pt.spp if spp is None else spp,
+                                pt.tile_size if tile_size is None else tile_size,
+                                pt.min_bounces if min_bounces is None else min_bounces,
+                                pt.max_bounces if max_bounces is None else max_bounces,
+                                pt.cont_prob if cont_prob is None else cont_prob, 1, seed_mode,
+                                seed_base, env_cdf_fp64, max_slots)
+
+    def traced_dims(self):
+        w, h = self.g.scene.camera.res
+        ts = self.args.tile_size
+        return (w + ts - 1) // ts * ts, (h + ts - 1) // ts * ts
+
+    def camera(self, cam_override=None) -> capi.Camera:
+        if cam_override is None:
+            return camera_setup(self.g.scene.camera)
+        cam = capi.Camera()
+        C.memmove(C.byref(cam), C.byref(cam_override), C.sizeof(capi.Camera))
+        return cam
+
+    def render_device(self, cam_override=None, per_sample=False, stream=None, film=None):
+        """Render into device tensors; returns (film dict, RenderStats)."""
+        torch = _torch()
+        wp, hp = self.traced_dims()
+        n = wp * hp
+        if film is None:
+            film = dict(sum=torch.empty(n * 3, dtype=torch.float32, device="cuda"),
+                        sum_sq=torch.empty(n * 3, dtype=torch.float32, device="cuda"),
+                        count=torch.empty(n, dtype=torch.int32, device="cuda"))
+            if per_sample:
+                film["per_sample"] = torch.empty(n * self.args.spp * 3, dtype=torch.float32,
+                                                 device="cuda")
+        cf = capi.Film(_ptr(film["sum"]), _ptr(film["sum_sq"]), _ptr(film["count"]),
+                       _ptr(film.get("per_sample")))
+        stats = capi.RenderStats()
+        cam = self.camera(cam_override)
+        self.g.ctx.check(self.g.ctx.lib.ajg_render_pt(self.g.h, C.byref(cam), C.byref(self.args),
+                                                      C.byref(cf), C.byref(stats), stream),
+                         "ajg_render_pt")
+        return film, stats
+
+    # Renderer::Render (renderer.h:64-65)
+    def Render(self, cam_override=None, per_sample=False):
+        film, stats = self.render_device(cam_override, per_sample)
+        out = {k: v.cpu().numpy() for k, v in film.items()}
+        out["sum"] = out["sum"].reshape(-1, 3)
+        out["sum_sq"] = out["sum_sq"].reshape(-1, 3)
+        out["count"] = out["count"].view(np.uint32)
+        if "per_sample" in out:
+            out["per_sample"] = out["per_sample"].reshape(-1, 3)
+        out["stats"] = stats
+        return out
+
+    def resolve(self, film_dev):
+        """Film resolve on the device -> (H, W, 3) numpy image."""
+        torch = _torch()
+        w, h = self.g.scene.camera.res
+        rgb = torch.empty(w * h * 3, dtype=torch.float32, device="cuda")
+        self.g.ctx.check(self.g.ctx.lib.ajg_film_resolve(self.g.ctx.h, w, h, self.args.tile_size,
+                                                         _ptr(film_dev["sum"]),
+                                                         _ptr(film_dev["count"]), _ptr(rgb), None),
+                         "ajg_film_resolve")
+        return rgb.cpu().numpy().reshape(h, w, 3)
+
+
+def camera_setup(cam_spec) -> capi.Camera:
+    lib = capi.load()
+    cam = capi.Camera()
+    f3 = lambda v: np.ascontiguousarray(v, np.float32)  # noqa: E731
+    pos, la, up = f3(cam_spec.pos), f3(cam_spec.look_at), f3(cam_spec.up)
+    st = lib.ajg_camera_setup(C.byref(cam), pos.ctypes.data, la.ctypes.data, up.ctypes.data,
+                              cam_spec.fov, cam_spec.res[0], cam_spec.res[1],
+                              cam_spec.lens_radius, cam_spec.focal_distance)
+    capi.check(st, None, "ajg_camera_setup")
+    return cam
+
+
+def rays_to_bytes(rays: np.ndarray) -> np.ndarray:
+    assert rays.dtype == capi.RAY_DTYPE
+    return np.ascontiguousarray(rays).view(np.uint8).reshape(-1)
+
+
+def hits_from_bytes(buf: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(buf).view(capi.HIT_DTYPE)

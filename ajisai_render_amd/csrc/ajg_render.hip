@@ -1,0 +1,1143 @@
+// ajg_render.hip — DefaultScene on the device and the wavefront MIS path
+// tracer (PathTracing::TraceMIS, ajisai/core/renderer/pt.cpp:98-227, with
+// the helpers of renderer/helper.h and TiledIntegrator::Render, pt.cpp:40-73).
+//
+// Stages per wavefront iteration:
+//   trace_closest  extension rays + MIS-BSDF rays   (exact-order traversal)
+//   trace_any      shadow rays of the NEE samples
+//   shade          1) finish the previous bounce's direct lighting
+//                     (direct = sum_lights beta*NEE + beta*MIS-BSDF, added to
+//                     the path's radiance in the reference's order),
+//                  2) shade the extension hit: PostIntersect, BSDF, light
+//                     samples -> shadow rays, MIS-BSDF sample -> ray,
+//                     continuation sample, Russian roulette,
+//                  3) when a sample ends, accumulate it (Kahan, FP32) and
+//                     start the slot's next sample (ray generation).
+// Every path slot keeps its own PCG32 stream, so the random numbers each
+// sample consumes are those of the reference's per-sample loop.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ajg_internal.h"
+#include "ajg_shade.h"
+#include "ajg_traverse.h"
+
+struct ajg_scene {
+  ajg_ctx *ctx = nullptr;
+  ajg_accel *accel = nullptr;
+  uint4 *tri_shade = nullptr;  // per leaf slot: flags|material, n_i, n_j, n_k
+  float4 *normals = nullptr;   // mesh normal_and_vs, all meshes concatenated
+  ajg::DevMaterial *mats = nullptr;
+  uint32_t n_mats = 0;
+  ajg::DevLight lights[ajg::kMaxLights];
+  uint32_t n_lights = 0;
+  int env_index = -1;
+  std::vector<void *> env_bufs;
+  float world[6] = {0, 0, 0, 0, 0, 0};
+  float radius = 0.f;
+};
+
+namespace ajg {
+
+int injected(ajg_ctx *ctx);
+int flatten_views(ajg_ctx *ctx, const ajg_mesh_view *views, uint32_t n_views,
+                  std::vector<float> *tri, float *box);
+int upload_and_build(ajg_ctx *ctx, const std::vector<float> &tri, const float *box,
+                     uint32_t r, hipStream_t s, ajg_accel **out);
+
+// tri_shade flags
+static constexpr uint32_t kKindShift = 28;  // 2 bits
+static constexpr uint32_t kTwoSidedBit = 1u << 27;
+static constexpr uint32_t kMatMask = (1u << 27) - 1;
+
+// ------------------------------------------------------------ env tables
+// EnvLight constructor weights (env_light.cpp:33-45): luminance of the
+// bilinear texture sample at texel centres times sin(theta), luminance in
+// double (spectrum.h:80-82).
+__global__ void __launch_bounds__(256)
+k_env_weights(DevEnv e, float *wts) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= e.w * e.h) return;
+  const uint32_t x = i % e.w, y = i / e.w;
+  const float v = ((float)y + 0.5f) / (float)e.h;
+  const float sin_t = sinf(AJG_PI * ((float)y + 0.5f) / (float)e.h);
+  const float u = ((float)x + 0.5f) / (float)e.w;
+  wts[i] = luminance(tex_sample(e, u, v)) * sin_t;
+}
+
+// Distribution::SetFunction (distribution.h:40-58) for one row (or the
+// marginal): prefix sums in FP64 (rounded to float) or FP32 as the
+// reference, then normalisation by funcInt.
+__device__ void distrib_build(const float *f, uint32_t n, float *func, float *cdf,
+                              float *func_int, int fp64) {
+  const float inv = 1.f / (float)n;
+  cdf[0] = 0.f;
+  if (fp64) {
+    double acc = 0.0;
+    for (uint32_t i = 1; i <= n; ++i) {
+      func[i - 1] = f[i - 1];
+      acc += (double)f[i - 1] * (double)inv;
+      cdf[i] = (float)acc;
+    }
+  } else {
+    for (uint32_t i = 1; i <= n; ++i) {
+      func[i - 1] = f[i - 1];
+      cdf[i] = cdf[i - 1] + f[i - 1] * inv;
+    }
+  }
+  const float fi = cdf[n];
+  *func_int = fi;
+  if (fi > 0.f) {
+    for (uint32_t i = 0; i <= n; ++i) cdf[i] /= fi;
+  } else {
+    for (uint32_t i = 0; i <= n; ++i) cdf[i] = (float)i / (float)n;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+k_env_rows(const float *wts, uint32_t w, uint32_t h, float *cond_func, float *cond_cdf,
+           float *cond_int, int fp64) {
+  const uint32_t y = blockIdx.x * 64 + threadIdx.x;
+  if (y >= h) return;
+  distrib_build(wts + (size_t)y * w, w, cond_func + (size_t)y * w, cond_cdf + (size_t)y * (w + 1),
+                cond_int + y, fp64);
+}
+
+__global__ void k_env_marginal(const float *cond_int, uint32_t h, float *marg_func,
+                               float *marg_cdf, float *marg_int, int fp64) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    distrib_build(cond_int, h, marg_func, marg_cdf, marg_int, fp64);
+}
+
+// --------------------------------------------------------------- render
+struct DevScene {
+  DevBVH bvh;
+  const uint4 *tri_shade;
+  const float4 *normals;
+  const DevMaterial *mats;
+  DevLight lights[kMaxLights];
+  uint32_t n_lights;
+  int env_index;
+  float radius;
+};
+
+struct RenderParams {
+  DevCamera cam;
+  uint32_t W, H, Wp, n_pix;
+  uint32_t spp, chunk, n_chunks, total_work;
+  int min_bounces, max_bounces;
+  float cont_prob;
+  int seed_mode;
+  uint32_t seed_base;
+  uint32_t slots;
+};
+
+// pend_flags
+static constexpr uint32_t kPendActive = 1u;
+static constexpr uint32_t kPendEnded = 2u;
+static constexpr uint32_t kPendLast = 4u;
+static constexpr uint32_t kPendMis = 8u;
+static constexpr uint32_t kPendLight0 = 16u;  // + k
+
+struct PathState {
+  uint32_t *work;       // current work item (kNone = none)
+  uint32_t *sample;     // sample index within the work item's range (absolute)
+  uint32_t *sample_end;
+  uint32_t *bounce;
+  uint32_t *ext_idx;    // index of the extension ray in the closest queue, kNone
+  uint32_t *bank;       // accumulator bank of the current work item
+  uint64_t *rng;
+  float *L, *beta;      // 3 per slot
+  // pending direct lighting of the last shaded bounce
+  uint32_t *pend_flags, *pend_mis, *pend_shadow;  // shadow: kMaxLights per slot
+  uint32_t *pend_work, *pend_sample, *pend_bank;
+  float *pend_L, *pend_beta, *pend_c, *pend_f, *pend_pdf, *pend_ns;
+  // accumulators: 2 banks x {sum3, comp3, sq3, compsq3}
+  float *acc;
+  uint32_t *acc_n;      // 2 banks
+};
+
+struct Queues {
+  DevRay *c_rays_in, *c_rays_out;
+  float4 *c_hits_in;   // t, leaf slot (int bits), u, v
+  uint32_t *c_count_in, *c_count_out;
+  DevRay *a_rays_in, *a_rays_out;
+  uint8_t *a_occ_in;
+  uint32_t *a_count_in, *a_count_out;
+  uint32_t *live;
+  uint32_t *work_counter;
+  float *partial;      // total_work x 6 (sum3, sq3) or the film when n_chunks == 1
+  uint32_t *partial_n;
+  float *per_sample;   // optional
+  unsigned long long *counters;  // closest, any, paths
+};
+
+__device__ __forceinline__ void kahan_add(float &sum, float &c, float x) {
+  const float y = x - c;
+  const float t = sum + y;
+  c = (t - sum) - y;
+  sum = t;
+}
+
+// Wave-aggregated append: returns this lane's slot in the queue.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+  const uint64_t mask = __ballot(want);
+  if (mask == 0) return kNone;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  return want ? base + (uint32_t)__popcll(mask & lt) : kNone;
+}
+
+struct Inct {
+  f3 pos, ng, ns, wr;
+  uint32_t material;
+};
+
+// MT normal + PostIntersect (ploc_aggregate.cpp:199-211; quad.cpp:73-90,
+// cube.cpp:96-108 (tuple order reversed in the index buffer), mesh.cpp:183-204,
+// two_sided.cpp:44-53).
+__device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay &r, float t, int ls,
+                                               float u, float v) {
+  Inct it;
+  f3 a, b, c;
+  load_leaf(sc.bvh.leaf_tris, ls, &a, &b, &c);
+  const uint4 info = sc.tri_shade[ls];
+  const uint32_t kind = info.x >> kKindShift;
+  it.material = info.x & kMatMask;
+  it.pos = add(r.o, scale(r.d, t));
+  it.wr = neg(r.d);
+  if (kind == AJG_GEOM_CUBE) {
+    it.ng = normalize(cross(sub(b, c), sub(a, c)));
+    it.ns = it.ng;
+  } else {
+    it.ng = normalize(cross(sub(b, a), sub(c, a)));
+    if (kind == AJG_GEOM_MESH) {
+      const float w = 1 - u - v;
+      const float4 ni = sc.normals[info.y], nj = sc.normals[info.z], nk = sc.normals[info.w];
+      it.ns = add(add(scale(mk3(ni.x, ni.y, ni.z), w), scale(mk3(nj.x, nj.y, nj.z), u)),
+                  scale(mk3(nk.x, nk.y, nk.z), v));
+    } else {
+      it.ns = it.ng;
+    }
+  }
+  if ((info.x & kTwoSidedBit) && dot(it.ng, it.wr) < 0) {
+    it.ng = scale(it.ng, -1.f);
+    it.ns = scale(it.ns, -1.f);
+  }
+  return it;
+}
+
+// Intersection::EpsOffset (intersection.h:42-47)
+__device__ __forceinline__ f3 eps_offset(const Inct &it, f3 dir) {
+  if (dot(dir, it.ng) > 0.f) return add(it.pos, scale(it.ng, AJG_EPS));
+  return sub(it.pos, scale(it.ng, AJG_EPS));
+}
+
+__device__ __forceinline__ void store_ray(DevRay *q, uint32_t i, const DevRay &r) {
+  float4 *p = reinterpret_cast<float4 *>(q + i);
+  p[0] = make_float4(r.o.x, r.o.y, r.o.z, r.d.x);
+  p[1] = make_float4(r.d.y, r.d.z, r.tmin, r.tmax);
+}
+__device__ __forceinline__ DevRay load_ray(const DevRay *q, uint32_t i) {
+  const float4 *p = reinterpret_cast<const float4 *>(q + i);
+  const float4 a = p[0], b = p[1];
+  DevRay r;
+  r.o = mk3(a.x, a.y, a.z);
+  r.d = mk3(a.w, b.x, b.y);
+  r.tmin = b.z;
+  r.tmax = b.w;
+  return r;
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256)
+k_q_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= *count) return;
+  const DevRay ray = load_ray(rays, i);
+  const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+  const THit h = trace_closest<STACK, false>(bvh, r, nullptr, nullptr);
+  hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256)
+k_q_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= *count) return;
+  const DevRay ray = load_ray(rays, i);
+  const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+  occ[i] = trace_any<STACK>(bvh, r) ? 1 : 0;
+}
+
+__global__ void k_reset_counters(uint32_t *a, uint32_t *b, uint32_t *c) {
+  if (threadIdx.x == 0) {
+    *a = 0;
+    *b = 0;
+    *c = 0;
+  }
+}
+
+__device__ __forceinline__ f3 ld_f3(const float *p, uint32_t i) {
+  return mk3(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
+}
+__device__ __forceinline__ void st_f3(float *p, uint32_t i, f3 v) {
+  p[3 * (size_t)i] = v.x;
+  p[3 * (size_t)i + 1] = v.y;
+  p[3 * (size_t)i + 2] = v.z;
+}
+
+// Accumulate a finished sample into its work item's bank; flush the bank
+// when it was the item's last sample.
+__device__ void accumulate(const PathState &ps, const Queues &q, const RenderParams &P,
+                           uint32_t slot, uint32_t bank, uint32_t work, uint32_t sample, f3 L,
+                           bool last) {
+  float *a = ps.acc + ((size_t)slot * 2 + bank) * 12;
+  kahan_add(a[0], a[3], L.x);
+  kahan_add(a[1], a[4], L.y);
+  kahan_add(a[2], a[5], L.z);
+  kahan_add(a[6], a[9], L.x * L.x);
+  kahan_add(a[7], a[10], L.y * L.y);
+  kahan_add(a[8], a[11], L.z * L.z);
+  uint32_t &n = ps.acc_n[(size_t)slot * 2 + bank];
+  n += 1;
+  const uint32_t p = work % P.n_pix;
+  if (q.per_sample) {
+    float *o = q.per_sample + 3 * ((size_t)p * P.spp + sample);
+    o[0] = L.x;
+    o[1] = L.y;
+    o[2] = L.z;
+  }
+  if (last) {
+    float *dst = q.partial + (size_t)work * 6;
+    dst[0] = a[0] - a[3];
+    dst[1] = a[1] - a[4];
+    dst[2] = a[2] - a[5];
+    dst[3] = a[6] - a[9];
+    dst[4] = a[7] - a[10];
+    dst[5] = a[8] - a[11];
+    q.partial_n[work] = n;
+    for (int k = 0; k < 12; ++k) a[k] = 0.f;
+    n = 0;
+  }
+}
+
+// Start the next sample of the slot (fetching a new work item when the
+// current one is exhausted) and emit its camera ray. Wave-uniform call.
+__device__ void next_sample(const PathState &ps, const Queues &q, const RenderParams &P,
+                            const DevScene &sc, uint32_t slot, bool need, Pcg32 &rng,
+                            bool &live) {
+  uint32_t work = need ? ps.work[slot] : kNone;
+  uint32_t sample = 0;
+  bool fresh = false;
+  if (need && work != kNone) {
+    sample = ps.sample[slot] + 1;
+    if (sample >= ps.sample_end[slot]) work = kNone;
+  }
+  const bool fetch = need && work == kNone;
+  const uint32_t w = wave_append(fetch, q.work_counter);
+  if (fetch) {
+    if (w < P.total_work) {
+      work = w;
+      fresh = true;
+      const uint32_t c = w / P.n_pix;
+      sample = c * P.chunk;
+      ps.sample_end[slot] = min(P.spp, sample + P.chunk);
+      ps.bank[slot] ^= 1u;
+    } else {
+      work = kNone;
+    }
+  }
+  if (!need) return;
+  ps.work[slot] = work;
+  if (work == kNone) {
+    ps.ext_idx[slot] = kNone;
+    return;
+  }
+  ps.sample[slot] = sample;
+  const uint32_t p = work % P.n_pix;
+  const uint32_t x = p % P.Wp, y = p / P.Wp;
+  if (P.seed_mode == 0) {
+    if (fresh) pcg_seed(rng, (uint64_t)(int64_t)(int)(x + y * P.W));
+  } else {
+    pcg_seed(rng, sample_seed(P.seed_base, p, sample));
+  }
+  // TiledIntegrator::Render per-sample draws (pt.cpp:58-61)
+  const float u = (float)x + pcg_next_float(rng);
+  const float v = (float)y + pcg_next_float(rng);
+  const float l0 = pcg_next_float(rng);
+  const float l1 = pcg_next_float(rng);
+  const DevRay r = camera_ray(P.cam, u, v, l0, l1);
+  st_f3(ps.L, slot, mk3(0.f, 0.f, 0.f));
+  st_f3(ps.beta, slot, mk3(1.f, 1.f, 1.f));
+  ps.bounce[slot] = 0;
+  live = true;
+  // Russian roulette for bounce 0 only if min_bounces < 0 (pt.cpp:107-110)
+  bool alive = true;
+  if (0 > P.min_bounces) {
+    if (pcg_next_float(rng) > P.cont_prob) {
+      alive = false;
+    } else {
+      st_f3(ps.beta, slot, divs(mk3(1.f, 1.f, 1.f), P.cont_prob));
+    }
+  }
+  if (P.max_bounces <= 0) alive = false;
+  (void)alive;
+  // A path that dies before its first intersection contributes zero.
+  const uint32_t idx = wave_append(alive, q.c_count_out);
+  if (alive) {
+    store_ray(q.c_rays_out, idx, r);
+    ps.ext_idx[slot] = idx;
+  } else {
+    ps.ext_idx[slot] = kNone;
+  }
+  if (!alive) {
+    // zero-radiance sample: accumulate immediately, then the next sample
+    // starts in the following iteration (slot stays live).
+    const bool last = sample + 1 >= ps.sample_end[slot];
+    accumulate(ps, q, P, slot, ps.bank[slot], work, sample, mk3(0.f, 0.f, 0.f), last);
+    ps.ext_idx[slot] = kNone;
+    ps.pend_flags[slot] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
+  const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = slot < P.slots;
+  bool live = false;
+  Pcg32 rng;
+  rng.state = valid ? ps.rng[slot] : 0ull;
+  bool need_new = false;
+
+  if (valid) {
+    // ---- 1. finish the previous bounce's direct lighting
+    const uint32_t pf = ps.pend_flags[slot];
+    if (pf & kPendActive) {
+      f3 direct = mk3(0.f, 0.f, 0.f);
+      for (uint32_t k = 0; k < sc.n_lights; ++k) {
+        f3 c = mk3(0.f, 0.f, 0.f);
+        if (pf & (kPendLight0 << k)) {
+          const uint32_t si = ps.pend_shadow[(size_t)slot * kMaxLights + k];
+          if (!q.a_occ_in[si]) c = ld_f3(ps.pend_c, slot * kMaxLights + k);
+        }
+        direct = add(direct, c);
+      }
+      f3 mis = mk3(0.f, 0.f, 0.f);
+      if (pf & kPendMis) {
+        const uint32_t mi = ps.pend_mis[slot];
+        const float4 h = q.c_hits_in[mi];
+        if (__float_as_int(h.y) < 0 && sc.env_index >= 0) {
+          const DevEnv &e = sc.lights[sc.env_index].env;
+          const DevRay mr = load_ray(q.c_rays_in, mi);
+          const f3 le = env_radiance(e, mr.d);
+          if (!is_black(le)) {
+            const f3 f = scale(mul(le, ld_f3(ps.pend_f, slot)),
+                               fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
+            const float pdf = ps.pend_pdf[slot];
+            const float lpdf = env_pdf(e, mr.d);
+            mis = scale(divs(f, pdf), power_heuristic(pdf, lpdf));
+          }
+        }
+      }
+      direct = add(direct, mul(ld_f3(ps.pend_beta, slot), mis));
+      const f3 Lf = add(ld_f3(ps.pend_L, slot), scale(direct, 1.f));
+      if (pf & kPendEnded) {
+        accumulate(ps, q, P, slot, ps.pend_bank[slot], ps.pend_work[slot], ps.pend_sample[slot],
+                   Lf, (pf & kPendLast) != 0);
+      } else {
+        st_f3(ps.L, slot, Lf);
+      }
+      ps.pend_flags[slot] = 0;
+    }
+
+    // ---- 2. shade the extension hit of the current sample
+    const uint32_t ei = ps.ext_idx[slot];
+    if (ei == kNone) {
+      if (ps.work[slot] != kNone) {
+        // a sample ended before its first intersection: start the next one
+        need_new = true;
+        live = true;
+      } else if (ps.sample_end[slot] == 0) {
+        need_new = true;  // never had work: try to fetch some
+      }
+    }
+    if (ei != kNone) {
+      live = true;
+      const DevRay ray = load_ray(q.c_rays_in, ei);
+      const float4 h = q.c_hits_in[ei];
+      const int ls = __float_as_int(h.y);
+      const uint32_t b = ps.bounce[slot];
+      f3 L = ld_f3(ps.L, slot);
+      f3 beta = ld_f3(ps.beta, slot);
+      const uint32_t work = ps.work[slot], sample = ps.sample[slot];
+      const bool last = sample + 1 >= ps.sample_end[slot];
+      if (ls < 0) {
+        if (b == 0 && sc.env_index >= 0) L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
+        accumulate(ps, q, P, slot, ps.bank[slot], work, sample, L, last);
+        need_new = true;
+      } else {
+        const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w);
+        const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
+        uint32_t pf2 = kPendActive;
+        st_f3(ps.pend_L, slot, L);
+        st_f3(ps.pend_beta, slot, beta);
+        ps.pend_work[slot] = work;
+        ps.pend_sample[slot] = sample;
+        ps.pend_bank[slot] = ps.bank[slot];
+        // NEE for each light (MISSampleLight, helper.h:100-133)
+        for (uint32_t k = 0; k < sc.n_lights; ++k) {
+          const DevLight &l = sc.lights[k];
+          LightSample lsmp;
+          if (l.type == AJG_LIGHT_ENV) {
+            const float first = pcg_next_float(rng);
+            const float second = pcg_next_float(rng);
+            lsmp = env_sample(l.env, it.pos, second, first, sc.radius);
+          } else {
+            const float u0 = pcg_next_float(rng);
+            const float u1 = pcg_next_float(rng);
+            lsmp = dir_sample(l, it.pos, u0, u1, sc.radius);
+          }
+          bool emit = false;
+          DevRay sr;
+          if (!(is_black(lsmp.radiance) || lsmp.pdf <= 0.f)) {
+            const float len = length(sub(lsmp.pos, it.pos)) - AJG_EPS;
+            if (!(len < AJG_EPS)) {
+              const f3 to_light = normalize(sub(lsmp.pos, it.pos));
+              const f3 bf = bsdf_eval_all(bs, to_light, it.wr);
+              if (!is_black(bf)) {
+                const f3 nrm = normalize(it.ns);
+                const f3 f = scale(mul(lsmp.radiance, bf), fabsf(dot(to_light, nrm)));
+                const float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
+                const f3 c = mul(beta, scale(divs(f, lsmp.pdf), power_heuristic(lsmp.pdf, bpdf)));
+                st_f3(ps.pend_c, slot * kMaxLights + k, c);
+                sr = make_ray(it.pos, to_light, AJG_EPS, 0.99f * len);
+                emit = true;
+              }
+            }
+          }
+          const uint32_t si = wave_append(emit, q.a_count_out);
+          if (emit) {
+            store_ray(q.a_rays_out, si, sr);
+            ps.pend_shadow[(size_t)slot * kMaxLights + k] = si;
+            pf2 |= kPendLight0 << k;
+          }
+        }
+        // MIS-BSDF sample (MISSampleBsdf, helper.h:209-246)
+        {
+          f3 sam;
+          sam.x = pcg_next_float(rng);
+          sam.y = pcg_next_float(rng);
+          sam.z = pcg_next_float(rng);
+          BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
+          bool emit = false;
+          DevRay mr;
+          if (s.valid && !(is_black(s.f) || s.pdf == 0.f) && sc.env_index >= 0) {
+            const f3 dn = normalize(s.dir);
+            mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
+            st_f3(ps.pend_f, slot, s.f);
+            ps.pend_pdf[slot] = s.pdf;
+            st_f3(ps.pend_ns, slot, it.ns);
+            emit = true;
+          }
+          const uint32_t mi = wave_append(emit, q.c_count_out);
+          if (emit) {
+            store_ray(q.c_rays_out, mi, mr);
+            ps.pend_mis[slot] = mi;
+            pf2 |= kPendMis;
+          }
+        }
+        // continuation (pt.cpp:215-223) and Russian roulette of the next
+        // bounce (pt.cpp:107-110)
+        bool cont;
+        DevRay nr;
+        {
+          f3 sam;
+          sam.x = pcg_next_float(rng);
+          sam.y = pcg_next_float(rng);
+          sam.z = pcg_next_float(rng);
+          const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
+          cont = s.valid && !(is_black(s.f) || s.pdf < 0.f);
+          if (cont) {
+            beta = mul(beta, divs(scale(s.f, fabsf(dot(it.ng, s.dir))), s.pdf));
+            nr = make_ray(eps_offset(it, s.dir), s.dir, AJG_EPS, __int_as_float(0x7f800000));
+            const int nb = (int)b + 1;
+            if (nb >= P.max_bounces) {
+              cont = false;
+            } else if (nb > P.min_bounces) {
+              if (pcg_next_float(rng) > P.cont_prob)
+                cont = false;
+              else
+                beta = divs(beta, P.cont_prob);
+            }
+          }
+        }
+        const uint32_t ni = wave_append(cont, q.c_count_out);
+        if (cont) {
+          store_ray(q.c_rays_out, ni, nr);
+          ps.ext_idx[slot] = ni;
+          ps.bounce[slot] = b + 1;
+          st_f3(ps.beta, slot, beta);
+        } else {
+          pf2 |= kPendEnded;
+          if (last) pf2 |= kPendLast;
+          need_new = true;
+        }
+        ps.pend_flags[slot] = pf2;
+      }
+    }
+    if (ps.pend_flags[slot] & kPendActive) live = true;
+  }
+
+  // ---- 3. next sample / ray generation (wave-uniform control flow)
+  next_sample(ps, q, P, sc, slot, valid && need_new, rng, live);
+  if (valid) {
+    if (!need_new && ps.ext_idx[slot] != kNone && (ps.pend_flags[slot] & kPendActive) == 0 &&
+        ps.work[slot] != kNone) {
+      // extension ray just emitted above; nothing else to do
+    }
+    ps.rng[slot] = rng.state;
+  }
+  const bool any_live = __ballot(valid && live) != 0ull;
+  if ((threadIdx.x & 63) == 0 && any_live) atomicAdd(q.live, 1u);
+}
+
+// Per-chunk partial sums -> film (Kahan over chunks in chunk order).
+__global__ void __launch_bounds__(256)
+k_reduce_chunks(const float *partial, const uint32_t *partial_n, uint32_t n_pix,
+                uint32_t n_chunks, float *sum, float *sum_sq, uint32_t *count) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n_pix) return;
+  float s[6] = {0, 0, 0, 0, 0, 0}, c[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t n = 0;
+  for (uint32_t k = 0; k < n_chunks; ++k) {
+    const float *src = partial + ((size_t)k * n_pix + p) * 6;
+    for (int j = 0; j < 6; ++j) kahan_add(s[j], c[j], src[j]);
+    n += partial_n[(size_t)k * n_pix + p];
+  }
+  for (int j = 0; j < 3; ++j) {
+    sum[3 * (size_t)p + j] = s[j] - c[j];
+    sum_sq[3 * (size_t)p + j] = s[3 + j] - c[3 + j];
+  }
+  count[p] = n;
+}
+
+__global__ void __launch_bounds__(256)
+k_unpack_single(const float *partial, const uint32_t *partial_n, uint32_t n_pix, float *sum,
+                float *sum_sq, uint32_t *count) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n_pix) return;
+  for (int j = 0; j < 3; ++j) {
+    sum[3 * (size_t)p + j] = partial[(size_t)p * 6 + j];
+    sum_sq[3 * (size_t)p + j] = partial[(size_t)p * 6 + 3 + j];
+  }
+  count[p] = partial_n[p];
+}
+
+// Film resolve (film.cpp:27-63, image.h:38-48): each traced pixel's sums
+// splat with weight 1 into {p-1, p} x {p-1, p}, clamped to the film edge.
+__global__ void __launch_bounds__(256)
+k_resolve(uint32_t w, uint32_t h, uint32_t Wp, uint32_t Hp, const float *sum,
+          const uint32_t *count, float *rgb) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= w * h) return;
+  const int qx = (int)(q % w), qy = (int)(q / w);
+  // A traced pixel p adds into clamp(p - 1) and clamp(p) per axis; the
+  // multiplicity m(p -> q) counts how many of the two land on q. Traced
+  // pixels beyond the film edge (whole tiles) all clamp into the last row or
+  // column.
+  const int x_hi = qx == (int)w - 1 ? (int)Wp - 1 : qx + 1;
+  const int y_hi = qy == (int)h - 1 ? (int)Hp - 1 : qy + 1;
+  double acc[3] = {0, 0, 0}, wt = 0;
+  for (int py = qy; py <= y_hi; ++py) {
+    int my = 0;
+    for (int dy = -1; dy <= 0; ++dy) my += (clampi(py + dy, 0, (int)h - 1) == qy);
+    if (!my) continue;
+    for (int px = qx; px <= x_hi; ++px) {
+      int mx = 0;
+      for (int dx = -1; dx <= 0; ++dx) mx += (clampi(px + dx, 0, (int)w - 1) == qx);
+      if (!mx) continue;
+      const size_t p = (size_t)px + (size_t)py * Wp;
+      const double m = (double)(mx * my);
+      for (int k = 0; k < 3; ++k) acc[k] += m * sum[3 * p + k];
+      wt += m * count[p];
+    }
+  }
+  for (int k = 0; k < 3; ++k) rgb[3 * (size_t)q + k] = wt != 0 ? (float)(acc[k] / wt) : 0.f;
+}
+
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace ajg
+
+using namespace ajg;
+
+static void scene_free(ajg_scene *s) {
+  if (!s) return;
+  destroy_accel(s->accel);
+  (void)hipFree(s->tri_shade);
+  (void)hipFree(s->normals);
+  (void)hipFree(s->mats);
+  for (void *p : s->env_bufs) (void)hipFree(p);
+  delete s;
+}
+
+extern "C" {
+
+int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
+                     const ajg_material *mats, uint32_t n_mats, const ajg_light *lights,
+                     uint32_t n_lights, uint32_t search_radius, int32_t env_cdf_fp64,
+                     ajg_scene **out) {
+  if (!ctx || !out || (n_prims && !prims) || (n_mats && !mats) || (n_lights && !lights))
+    return AJG_ERR_INVALID;
+  *out = nullptr;
+  if (int st = injected(ctx)) return st;
+  if (n_lights > (uint32_t)kMaxLights)
+    return fail(ctx, AJG_ERR_INVALID, "scene: at most 4 lights are supported");
+  if (n_mats == 0) return fail(ctx, AJG_ERR_INVALID, "scene: no materials");
+  for (uint32_t i = 0; i < n_mats; ++i)
+    if (mats[i].type != AJG_MAT_DIFFUSE && mats[i].type != AJG_MAT_DISNEY)
+      return fail(ctx, AJG_ERR_INVALID, "scene: unsupported material type");
+  std::vector<ajg_mesh_view> views(n_prims);
+  size_t n_normals = 0;
+  for (uint32_t i = 0; i < n_prims; ++i) {
+    views[i] = prims[i].view;
+    if (prims[i].kind > AJG_GEOM_MESH)
+      return fail(ctx, AJG_ERR_INVALID, "scene: unsupported geometry kind");
+    if (prims[i].material >= n_mats)
+      return fail(ctx, AJG_ERR_INVALID, "scene: material index out of range");
+    if (prims[i].kind == AJG_GEOM_MESH) {
+      if (!prims[i].normal_and_vs)
+        return fail(ctx, AJG_ERR_INVALID, "scene: mesh without normal_and_vs");
+      n_normals += prims[i].view.position_size;
+    }
+  }
+  int env_count = 0;
+  for (uint32_t i = 0; i < n_lights; ++i) {
+    if (lights[i].type == AJG_LIGHT_ENV) {
+      ++env_count;
+      if (!lights[i].env_texels || lights[i].env_w == 0 || lights[i].env_h == 0)
+        return fail(ctx, AJG_ERR_INVALID, "scene: env light without texels");
+    } else if (lights[i].type != AJG_LIGHT_DIRECTIONAL) {
+      return fail(ctx, AJG_ERR_INVALID, "scene: unsupported light type");
+    }
+  }
+  if (env_count > 1) return fail(ctx, AJG_ERR_INVALID, "scene: more than one env light");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, AJG_ERR_HIP, "hipSetDevice");
+
+  std::vector<float> tri;
+  float box[6];
+  if (int st = flatten_views(ctx, views.data(), n_prims, &tri, box)) return st;
+  ajg_scene *s = new ajg_scene();
+  s->ctx = ctx;
+  int st = upload_and_build(ctx, tri, box, search_radius, nullptr, &s->accel);
+  if (st) {
+    scene_free(s);
+    return st;
+  }
+  const uint32_t n = s->accel->n_tri;
+  for (int i = 0; i < 6; ++i) s->world[i] = box[i];
+  // DirectionalLight/EnvLight radius = |max - center| of the world bound
+  {
+    box3 wb;
+    wb.mn = ld3(box);
+    wb.mx = ld3(box + 3);
+    s->radius = length(sub(wb.mx, box_center(wb)));
+  }
+  // per-triangle shading records, permuted into leaf-slot order
+  std::vector<uint32_t> perm(n);
+  if (hipMemcpy(perm.data(), s->accel->perm, 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
+    scene_free(s);
+    return fail(ctx, AJG_ERR_HIP, "perm download");
+  }
+  std::vector<uint4> by_id(n);
+  std::vector<float4> normals(std::max<size_t>(n_normals, 1));
+  {
+    size_t t = 0, noff = 0;
+    for (uint32_t p = 0; p < n_prims; ++p) {
+      const ajg_primitive &pr = prims[p];
+      const uint32_t flags = (pr.kind << kKindShift) | (pr.two_sided ? kTwoSidedBit : 0u) |
+                             (pr.material & kMatMask);
+      for (size_t i = 0; i < pr.view.index_size; ++i, ++t) {
+        uint4 rec = make_uint4(flags, 0, 0, 0);
+        if (pr.kind == AJG_GEOM_MESH) {
+          const char *base = (const char *)pr.view.index_buffer + pr.view.index_offset +
+                             i * pr.view.index_stride;
+          uint32_t w3[3];
+          memcpy(w3, base, 12);
+          rec.y = (uint32_t)(noff + w3[0]);
+          rec.z = (uint32_t)(noff + w3[1]);
+          rec.w = (uint32_t)(noff + w3[2]);
+        }
+        by_id[t] = rec;
+      }
+      if (pr.kind == AJG_GEOM_MESH) {
+        for (size_t v = 0; v < pr.view.position_size; ++v) {
+          const float *src = pr.normal_and_vs + 4 * v;
+          normals[noff + v] = make_float4(src[0], src[1], src[2], src[3]);
+        }
+        noff += pr.view.position_size;
+      }
+    }
+  }
+  std::vector<uint4> by_slot(n);
+  for (uint32_t i = 0; i < n; ++i) by_slot[i] = by_id[perm[i]];
+  std::vector<DevMaterial> dm(n_mats);
+  for (uint32_t i = 0; i < n_mats; ++i) {
+    const ajg_material &m = mats[i];
+    dm[i].type = m.type;
+    for (int k = 0; k < 3; ++k) dm[i].base_color[k] = m.base_color[k];
+    dm[i].subsurface = m.subsurface;
+    dm[i].metallic = m.metallic;
+    dm[i].specular = m.specular;
+    dm[i].specular_tint = m.specular_tint;
+    dm[i].roughness = m.roughness;
+    dm[i].anisotropic = m.anisotropic;
+    dm[i].sheen = m.sheen;
+    dm[i].sheen_tint = m.sheen_tint;
+    dm[i].clearcoat = m.clearcoat;
+    dm[i].clearcoat_gloss = m.clearcoat_gloss;
+  }
+  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(uint4) * n, "tri shade")) ||
+      (st = dev_alloc(ctx, (void **)&s->normals, sizeof(float4) * normals.size(), "normals")) ||
+      (st = dev_alloc(ctx, (void **)&s->mats, sizeof(DevMaterial) * n_mats, "materials"))) {
+    scene_free(s);
+    return st;
+  }
+  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(uint4) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->mats, dm.data(), sizeof(DevMaterial) * n_mats, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    scene_free(s);
+    return hip_fail(ctx, e, "scene upload");
+  }
+  s->n_mats = n_mats;
+  // lights (Light::Process with the world bound, default_scene.cpp:40-48)
+  s->n_lights = n_lights;
+  for (uint32_t i = 0; i < n_lights; ++i) {
+    DevLight &l = s->lights[i];
+    memset(&l, 0, sizeof(l));
+    l.type = lights[i].type;
+    if (l.type == AJG_LIGHT_DIRECTIONAL) {
+      const f3 d = normalize(ld3(lights[i].dir));
+      l.fr = make_frame(d);
+      l.angle_cos = std::cos(lights[i].angle);
+      l.radiance = ld3(lights[i].radiance);
+      continue;
+    }
+    s->env_index = (int)i;
+    const uint32_t w = lights[i].env_w, h = lights[i].env_h;
+    float *tex, *wts, *cf, *cc, *ci, *mf, *mc, *mi;
+    auto al = [&](float **p, size_t cnt, const char *what) {
+      int r = dev_alloc(ctx, (void **)p, sizeof(float) * cnt, what);
+      if (r == AJG_OK) s->env_bufs.push_back(*p);
+      return r;
+    };
+    if ((st = al(&tex, 3 * (size_t)w * h, "env texels")) || (st = al(&wts, (size_t)w * h, "env weights")) ||
+        (st = al(&cf, (size_t)w * h, "env cond func")) || (st = al(&cc, (size_t)(w + 1) * h, "env cond cdf")) ||
+        (st = al(&ci, h, "env cond int")) || (st = al(&mf, h, "env marg func")) ||
+        (st = al(&mc, h + 1, "env marg cdf")) || (st = al(&mi, 1, "env marg int"))) {
+      scene_free(s);
+      return st;
+    }
+    e = hipMemcpy(tex, lights[i].env_texels, sizeof(float) * 3 * (size_t)w * h, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      scene_free(s);
+      return hip_fail(ctx, e, "env upload");
+    }
+    l.env.w = w;
+    l.env.h = h;
+    l.env.tex = tex;
+    l.env.cond_cdf = cc;
+    l.env.cond_func = cf;
+    l.env.cond_int = ci;
+    l.env.marg_cdf = mc;
+    l.env.marg_func = mf;
+    l.env.rot = lights[i].rot;
+    l.env.scaler = lights[i].scaler;
+    hipLaunchKernelGGL(k_env_weights, dim3(cdiv((uint64_t)w * h, 256)), dim3(256), 0, nullptr,
+                       l.env, wts);
+    hipLaunchKernelGGL(k_env_rows, dim3(cdiv(h, 64)), dim3(64), 0, nullptr, (const float *)wts, w, h,
+                       cf, cc, ci, (int)env_cdf_fp64);
+    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(64), 0, nullptr, (const float *)ci, h, mf, mc, mi,
+                       (int)env_cdf_fp64);
+    e = hipMemcpy(&l.env.marg_int, mi, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      scene_free(s);
+      return hip_fail(ctx, e, "env tables");
+    }
+  }
+  *out = s;
+  return AJG_OK;
+}
+
+int ajg_scene_destroy(ajg_scene *scene) {
+  scene_free(scene);
+  return AJG_OK;
+}
+
+int ajg_scene_get_accel(ajg_scene *scene, ajg_accel **out) {
+  if (!scene || !out) return AJG_ERR_INVALID;
+  *out = scene->accel;
+  return AJG_OK;
+}
+
+int ajg_scene_env_tables(ajg_scene *s, uint32_t li, float *cond_cdf, float *cond_func_int,
+                         float *marg_cdf, float *marg_func_int) {
+  if (!s || li >= s->n_lights || s->lights[li].type != AJG_LIGHT_ENV) return AJG_ERR_INVALID;
+  const DevEnv &e = s->lights[li].env;
+  hipError_t r = hipSuccess;
+  if (cond_cdf) r = hipMemcpy(cond_cdf, e.cond_cdf, sizeof(float) * (e.w + 1) * e.h, hipMemcpyDeviceToHost);
+  if (r == hipSuccess && cond_func_int)
+    r = hipMemcpy(cond_func_int, e.cond_int, sizeof(float) * e.h, hipMemcpyDeviceToHost);
+  if (r == hipSuccess && marg_cdf) r = hipMemcpy(marg_cdf, e.marg_cdf, sizeof(float) * (e.h + 1), hipMemcpyDeviceToHost);
+  if (r != hipSuccess) return hip_fail(s->ctx, r, "env tables download");
+  if (marg_func_int) *marg_func_int = e.marg_int;
+  return AJG_OK;
+}
+
+int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
+                  const ajg_film *film, ajg_render_stats *stats, void *stream) {
+  if (!sc || !camera || !args || !film || !film->sum || !film->sum_sq || !film->count)
+    return AJG_ERR_INVALID;
+  ajg_ctx *ctx = sc->ctx;
+  if (int st = injected(ctx)) return st;
+  if (args->spp <= 0 || args->tile_size <= 0 || args->max_bounces < 0 || !args->use_mis)
+    return fail(ctx, AJG_ERR_INVALID, "render: need spp > 0, tile_size > 0, max_bounces >= 0, use_mis");
+  const float rx = camera->resolution[0], ry = camera->resolution[1];
+  if (!(rx >= 1.f) || !(ry >= 1.f) || rx > 65536.f || ry > 65536.f)
+    return fail(ctx, AJG_ERR_INVALID, "render: bad resolution");
+  hipStream_t s = (hipStream_t)stream;
+  const ajg_accel *a = sc->accel;
+
+  RenderParams P;
+  memcpy(P.cam.r2c, camera->raster2camera, sizeof(P.cam.r2c));
+  memcpy(P.cam.c2w, camera->camera2world, sizeof(P.cam.c2w));
+  P.cam.lens_radius = camera->lens_radius;
+  P.cam.focal_distance = camera->focal_distance;
+  P.W = (uint32_t)(int)rx;
+  P.H = (uint32_t)(int)ry;
+  const uint32_t ts = (uint32_t)args->tile_size;
+  P.Wp = (P.W + ts - 1) / ts * ts;
+  const uint32_t Hp = (P.H + ts - 1) / ts * ts;
+  P.n_pix = P.Wp * Hp;
+  P.spp = (uint32_t)args->spp;
+  P.min_bounces = args->min_bounces;
+  P.max_bounces = args->max_bounces;
+  P.cont_prob = args->cont_prob;
+  P.seed_mode = args->seed_mode;
+  P.seed_base = args->seed_base;
+  const uint64_t target = args->max_slots > 0 ? (uint64_t)args->max_slots : (1ull << 20);
+  if (P.seed_mode == 0) {
+    P.chunk = P.spp;
+    P.n_chunks = 1;
+  } else {
+    uint32_t nc = (uint32_t)std::min<uint64_t>(P.spp, std::max<uint64_t>(1, cdiv(target, P.n_pix)));
+    P.chunk = cdiv(P.spp, nc);
+    P.n_chunks = cdiv(P.spp, P.chunk);
+  }
+  const uint64_t total = (uint64_t)P.n_pix * P.n_chunks;
+  if (total > 0xFFFFFFF0ull) return fail(ctx, AJG_ERR_INVALID, "render: too much work");
+  P.total_work = (uint32_t)total;
+  P.slots = (uint32_t)std::min<uint64_t>(target, total);
+  const uint32_t S = P.slots;
+
+  DevScene ds;
+  ds.bvh.fat = a->fat;
+  ds.bvh.leaf_tris = a->leaf_tris;
+  ds.bvh.root_ref = a->root_ref;
+  for (int i = 0; i < 3; ++i) {
+    ds.bvh.root_mn[i] = a->root_box[i];
+    ds.bvh.root_mx[i] = a->root_box[3 + i];
+  }
+  ds.tri_shade = sc->tri_shade;
+  ds.normals = sc->normals;
+  ds.mats = sc->mats;
+  for (int i = 0; i < kMaxLights; ++i) ds.lights[i] = sc->lights[i];
+  ds.n_lights = sc->n_lights;
+  ds.env_index = sc->env_index;
+  ds.radius = sc->radius;
+
+  // ---- allocations
+  std::vector<void *> bufs;
+  auto al = [&](void **p, size_t bytes, const char *what) {
+    int r = dev_alloc(ctx, p, bytes, what);
+    if (r == AJG_OK) bufs.push_back(*p);
+    return r;
+  };
+  auto free_all = [&]() {
+    for (void *p : bufs) (void)hipFree(p);
+  };
+  PathState ps;
+  Queues q;
+  memset(&q, 0, sizeof(q));
+  const size_t cq_cap = 2 * (size_t)S, aq_cap = (size_t)kMaxLights * S;
+  DevRay *c_rays[2], *a_rays[2];
+  float4 *c_hits;
+  uint8_t *a_occ;
+  uint32_t *counters;  // c0 c1 a0 a1 live work
+  int st = 0;
+  if ((st = al((void **)&ps.work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.sample, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.sample_end, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bounce, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.ext_idx, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bank, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.rng, 8 * (size_t)S, "state")) || (st = al((void **)&ps.L, 12 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.beta, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_flags, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_mis, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_shadow, 4 * (size_t)S * kMaxLights, "state")) ||
+      (st = al((void **)&ps.pend_work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.pend_sample, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_bank, 4 * (size_t)S, "state")) || (st = al((void **)&ps.pend_L, 12 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_beta, 12 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_c, 12 * (size_t)S * kMaxLights, "state")) ||
+      (st = al((void **)&ps.pend_f, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_pdf, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_ns, 12 * (size_t)S, "state")) || (st = al((void **)&ps.acc, 4 * 24 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.acc_n, 8 * (size_t)S, "state")) ||
+      (st = al((void **)&c_rays[0], sizeof(DevRay) * cq_cap, "closest queue")) ||
+      (st = al((void **)&c_rays[1], sizeof(DevRay) * cq_cap, "closest queue")) ||
+      (st = al((void **)&c_hits, sizeof(float4) * cq_cap, "closest hits")) ||
+      (st = al((void **)&a_rays[0], sizeof(DevRay) * aq_cap, "shadow queue")) ||
+      (st = al((void **)&a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
+      (st = al((void **)&a_occ, aq_cap, "shadow results")) ||
+      (st = al((void **)&counters, 64, "counters"))) {
+    free_all();
+    return st;
+  }
+  float *partial = nullptr;
+  uint32_t *partial_n = nullptr;
+  if ((st = al((void **)&partial, sizeof(float) * 6 * total, "partial sums")) ||
+      (st = al((void **)&partial_n, 4 * total, "partial counts"))) {
+    free_all();
+    return st;
+  }
+  hipEvent_t ev0, ev1;
+  (void)hipEventCreate(&ev0);
+  (void)hipEventCreate(&ev1);
+  auto done = [&](int code) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    free_all();
+    return code;
+  };
+#define AJG_RCHK(call)                                          \
+  do {                                                         \
+    hipError_t e_ = (call);                                    \
+    if (e_ != hipSuccess) return done(hip_fail(ctx, e_, #call)); \
+  } while (0)
+  AJG_RCHK(hipMemsetAsync(ps.work, 0xFF, 4 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.sample, 0, 4 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.sample_end, 0, 4 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.ext_idx, 0xFF, 4 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.bank, 0, 4 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.rng, 0, 8 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.pend_flags, 0, 4 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.acc, 0, 4 * 24 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(ps.acc_n, 0, 8 * (size_t)S, s));
+  AJG_RCHK(hipMemsetAsync(partial, 0, sizeof(float) * 6 * total, s));
+  AJG_RCHK(hipMemsetAsync(partial_n, 0, 4 * total, s));
+  AJG_RCHK(hipMemsetAsync(counters, 0, 64, s));
+  AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
+  AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
+  const size_t n_pix_sz = P.n_pix;
+  AJG_RCHK(hipMemsetAsync(film->sum, 0, 12 * n_pix_sz, s));
+  AJG_RCHK(hipMemsetAsync(film->sum_sq, 0, 12 * n_pix_sz, s));
+  AJG_RCHK(hipMemsetAsync(film->count, 0, 4 * n_pix_sz, s));
+  if (film->per_sample) AJG_RCHK(hipMemsetAsync(film->per_sample, 0, 12 * n_pix_sz * P.spp, s));
+
+  q.c_hits_in = c_hits;
+  q.a_occ_in = a_occ;
+  q.live = counters + 4;
+  q.work_counter = counters + 5;
+  q.partial = partial;
+  q.partial_n = partial_n;
+  q.per_sample = film->per_sample;
+  const int stack = a->depth <= 64 ? 64 : (a->depth <= 1024 ? 1024 : 0);
+  if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree depth above 1024"));
+
+  (void)hipEventRecord(ev0, s);
+  int cur = 0;
+  uint32_t iters = 0;
+  uint64_t n_closest = 0, n_any = 0;
+  const uint32_t shade_grid = cdiv(S, 256);
+  for (;;) {
+    // shade reads queue `cur` (traced) and writes queue `nxt`
+    const int nxt = cur ^ 1;
+    q.c_rays_in = c_rays[cur];
+    q.c_rays_out = c_rays[nxt];
+    q.c_count_in = counters + cur;
+    q.c_count_out = counters + nxt;
+    q.a_rays_in = a_rays[cur];
+    q.a_rays_out = a_rays[nxt];
+    q.a_count_in = counters + 2 + cur;
+    q.a_count_out = counters + 2 + nxt;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, q.c_count_out, q.a_count_out, q.live);
+    hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q);
+    uint32_t h[6];
+    AJG_RCHK(hipMemcpyAsync(h, counters, 24, hipMemcpyDeviceToHost, s));
+    AJG_RCHK(hipStreamSynchronize(s));
+    ++iters;
+    if (h[4] == 0) break;
+    if (iters > 100000) return done(fail(ctx, AJG_ERR_INTERNAL, "render: iteration limit"));
+    const uint32_t nc = h[nxt], na = h[2 + nxt];
+    n_closest += nc;
+    n_any += na;
+    if (nc) {
+      if (stack == 64)
+        hipLaunchKernelGGL(k_q_closest<64>, dim3(cdiv(nc, 256)), dim3(256), 0, s, ds.bvh,
+                           (const DevRay *)c_rays[nxt], c_hits, (const uint32_t *)(counters + nxt));
+      else
+        hipLaunchKernelGGL(k_q_closest<1024>, dim3(cdiv(nc, 256)), dim3(256), 0, s, ds.bvh,
+                           (const DevRay *)c_rays[nxt], c_hits, (const uint32_t *)(counters + nxt));
+    }
+    if (na) {
+      if (stack == 64)
+        hipLaunchKernelGGL(k_q_any<64>, dim3(cdiv(na, 256)), dim3(256), 0, s, ds.bvh,
+                           (const DevRay *)a_rays[nxt], a_occ, (const uint32_t *)(counters + 2 + nxt));
+      else
+        hipLaunchKernelGGL(k_q_any<1024>, dim3(cdiv(na, 256)), dim3(256), 0, s, ds.bvh,
+                           (const DevRay *)a_rays[nxt], a_occ, (const uint32_t *)(counters + 2 + nxt));
+    }
+    cur = nxt;
+  }
+  if (P.n_chunks > 1)
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(cdiv(P.n_pix, 256)), dim3(256), 0, s, (const float *)partial,
+                       (const uint32_t *)partial_n, P.n_pix, P.n_chunks, film->sum, film->sum_sq,
+                       film->count);
+  else
+    hipLaunchKernelGGL(k_unpack_single, dim3(cdiv(P.n_pix, 256)), dim3(256), 0, s, (const float *)partial,
+                       (const uint32_t *)partial_n, P.n_pix, film->sum, film->sum_sq, film->count);
+  (void)hipEventRecord(ev1, s);
+  AJG_RCHK(hipStreamSynchronize(s));
+  AJG_RCHK(hipGetLastError());
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->closest_queries = n_closest;
+    stats->any_queries = n_any;
+    stats->paths = (uint64_t)P.n_pix * P.spp;
+    stats->iterations = iters;
+    stats->slots = S;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    stats->ms_total = ms;
+  }
+#undef AJG_RCHK
+  return done(AJG_OK);
+}
+
+int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size, const float *sum,
+                     const uint32_t *count, float *rgb, void *stream) {
+  if (!ctx || !sum || !count || !rgb || w == 0 || h == 0 || tile_size == 0) return AJG_ERR_INVALID;
+  if (int st = injected(ctx)) return st;
+  const uint32_t Wp = (w + tile_size - 1) / tile_size * tile_size;
+  const uint32_t Hp = (h + tile_size - 1) / tile_size * tile_size;
+  hipLaunchKernelGGL(k_resolve, dim3(cdiv((uint64_t)w * h, 256)), dim3(256), 0, (hipStream_t)stream, w, h,
+                     Wp, Hp, sum, count, rgb);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(ctx, e, "k_resolve");
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// ajg_trace.hip — batch closest/any-hit kernels behind
+// ajg_accel_trace_closest / ajg_accel_trace_any / ajg_accel_trace_counts.
+#include "ajg_internal.h"
+#include "ajg_traverse.h"
+
+namespace ajg {
+
+static DevBVH dev_bvh(const ajg_accel *a) {
+  DevBVH b;
+  b.fat = a->fat;
+  b.leaf_tris = a->leaf_tris;
+  b.root_ref = a->root_ref;
+  for (int i = 0; i < 3; ++i) {
+    b.root_mn[i] = a->root_box[i];
+    b.root_mx[i] = a->root_box[3 + i];
+  }
+  return b;
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256)
+k_trace_closest(DevBVH bvh, const ajg_ray *rays, uint32_t n, ajg_hit *hits,
+                const uint32_t *perm) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const ajg_ray ray = rays[i];
+  const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
+  const THit h = trace_closest<STACK, false>(bvh, r, nullptr, nullptr);
+  ajg_hit o;
+  o.t = h.t;
+  o.tri_id = h.slot >= 0 ? perm[h.slot] : kNone;
+  o.u = h.u;
+  o.v = h.v;
+  hits[i] = o;
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256)
+k_trace_counts(DevBVH bvh, const ajg_ray *rays, uint32_t n, uint32_t *nodes,
+               uint32_t *tris) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const ajg_ray ray = rays[i];
+  const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
+  uint32_t nn = 0, nt = 0;
+  (void)trace_closest<STACK, true>(bvh, r, &nn, &nt);
+  nodes[i] = nn;
+  tris[i] = nt;
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256)
+k_trace_any(DevBVH bvh, const ajg_ray *rays, uint32_t n, uint8_t *occ) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const ajg_ray ray = rays[i];
+  const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
+  occ[i] = trace_any<STACK>(bvh, r) ? 1 : 0;
+}
+
+static inline uint32_t grid(uint32_t n) { return (n + 255) / 256; }
+
+int launch_trace_closest(ajg_accel *a, const ajg_ray *rays, uint32_t n, ajg_hit *hits,
+                         hipStream_t s) {
+  if (n == 0) return AJG_OK;
+  const DevBVH b = dev_bvh(a);
+  if (a->depth <= 64)
+    hipLaunchKernelGGL(k_trace_closest<64>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, hits,
+                       (const uint32_t *)a->perm);
+  else if (a->depth <= 1024)
+    hipLaunchKernelGGL(k_trace_closest<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, hits,
+                       (const uint32_t *)a->perm);
+  else
+    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(a->ctx, e, "k_trace_closest");
+}
+
+int launch_trace_counts(ajg_accel *a, const ajg_ray *rays, uint32_t n, uint32_t *nodes,
+                        uint32_t *tris, hipStream_t s) {
+  if (n == 0) return AJG_OK;
+  const DevBVH b = dev_bvh(a);
+  if (a->depth <= 64)
+    hipLaunchKernelGGL(k_trace_counts<64>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
+  else if (a->depth <= 1024)
+    hipLaunchKernelGGL(k_trace_counts<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
+  else
+    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(a->ctx, e, "k_trace_counts");
+}
+
+int launch_trace_any(ajg_accel *a, const ajg_ray *rays, uint32_t n, uint8_t *occ,
+                     hipStream_t s) {
+  if (n == 0) return AJG_OK;
+  const DevBVH b = dev_bvh(a);
+  if (a->depth <= 64)
+    hipLaunchKernelGGL(k_trace_any<64>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
+  else if (a->depth <= 1024)
+    hipLaunchKernelGGL(k_trace_any<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
+  else
+    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(a->ctx, e, "k_trace_any");
+}
+
+}  // namespace ajg

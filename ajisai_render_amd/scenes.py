@@ -1,0 +1,381 @@
+"""Seeded procedural scenes for the Ajisai path-tracing hot path.
+
+The scenes are built from the same geometry kinds the reference's factory
+creates (Quad: geometry/quad.cpp, Cube: geometry/cube.cpp, MeshGeo:
+geometry/mesh.cpp, optionally wrapped in TwoSided), with MeshView buffers laid
+out byte for byte as those classes expose them:
+
+* Quad   — 4 Vector3f positions, indices {0,1,2},{0,2,3} (quad.cpp:33-34,115-127)
+* Cube   — 24 Vector3f positions, std::tuple<u32,u32,u32> triangles
+           (idx, idx+2, idx+1), (idx, idx+3, idx+2); libstdc++ stores a tuple's
+           elements in reverse, so the index buffer holds (idx+1, idx+2, idx)...
+           (cube.cpp:66-75, 131-143)
+* MeshGeo — Vector4f position_and_us (stride 16), Vector4f normal_and_vs, u32
+           index triples (mesh.cpp:237-249)
+
+Configurations follow the benchmark definitions: cornell (256x256 Cornell box
+with two rotated boxes, directional + constant environment) and heavy_mesh
+(displaced icosphere of 1,310,720 triangles on a 2-triangle ground, Disney
+materials, 1024x512 HDR environment).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+GEOM_QUAD, GEOM_CUBE, GEOM_MESH = 0, 1, 2
+MAT_DIFFUSE, MAT_DISNEY = 0, 1
+LIGHT_DIRECTIONAL, LIGHT_ENV = 0, 1
+
+
+@dataclasses.dataclass
+class Primitive:
+    kind: int
+    positions: np.ndarray          # (n, 3) float32, or (n, 4) for MESH
+    index_words: np.ndarray        # (m, 3) uint32 in memory order
+    two_sided: bool
+    material: int
+    normal_and_vs: Optional[np.ndarray] = None  # (n, 4) float32, MESH only
+
+    @property
+    def n_tri(self) -> int:
+        return int(self.index_words.shape[0])
+
+
+@dataclasses.dataclass
+class Material:
+    type: int
+    base_color: tuple
+    subsurface: float = 0.0
+    metallic: float = 0.0
+    specular: float = 0.0
+    specular_tint: float = 0.0
+    roughness: float = 0.0
+    anisotropic: float = 0.0
+    sheen: float = 0.0
+    sheen_tint: float = 0.0
+    clearcoat: float = 0.0
+    clearcoat_gloss: float = 0.0
+
+
+@dataclasses.dataclass
+class Light:
+    type: int
+    dir: tuple = (0.0, 1.0, 0.0)
+    angle: float = 0.0
+    radiance: tuple = (0.0, 0.0, 0.0)
+    env: Optional[np.ndarray] = None   # (h, w, 3) float32
+    rot: float = 0.0
+    scaler: float = 1.0
+
+
+@dataclasses.dataclass
+class CameraSpec:
+    pos: tuple
+    look_at: tuple
+    up: tuple
+    fov: float
+    res: tuple
+    lens_radius: float = 0.0
+    focal_distance: float = 1.0
+
+
+@dataclasses.dataclass
+class PTSpec:
+    spp: int = 16
+    tile_size: int = 16
+    min_bounces: int = 5
+    max_bounces: int = 10
+    cont_prob: float = 0.9
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    primitives: List[Primitive]
+    materials: List[Material]
+    lights: List[Light]
+    camera: CameraSpec
+    pt: PTSpec
+    search_radius: int = 14
+
+    @property
+    def n_tri(self) -> int:
+        return sum(p.n_tri for p in self.primitives)
+
+    def soup(self) -> np.ndarray:
+        """Flattened (n, 9) float32 triangles in PLOC order
+        (ploc_aggregate.cpp:257-284)."""
+        out = []
+        for p in self.primitives:
+            pos = p.positions[:, :3]
+            out.append(pos[p.index_words.astype(np.int64)].reshape(-1, 9))
+        return np.ascontiguousarray(np.concatenate(out, axis=0).astype(np.float32))
+
+    def scene_box(self) -> np.ndarray:
+        mn = np.full(3, np.finfo(np.float32).max, np.float32)
+        mx = np.full(3, -np.finfo(np.float32).max, np.float32)
+        for p in self.primitives:
+            pos = p.positions[:, :3]
+            pmn, pmx = pos.min(axis=0), pos.max(axis=0)
+            if np.all(pmn == pmx):
+                continue
+            mn, mx = np.minimum(mn, pmn), np.maximum(mx, pmx)
+        return np.concatenate([mn, mx]).astype(np.float32)
+
+
+# --------------------------------------------------------------- geometry
+def quad(a, b, c, d, material, two_sided=True) -> Primitive:
+    pos = np.array([a, b, c, d], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    return Primitive(GEOM_QUAD, pos, idx, two_sided, material)
+
+
+_CUBE_FACES = np.array([
+    [[-.5, -.5, -.5], [-.5, -.5, .5], [.5, -.5, .5], [.5, -.5, -.5]],
+    [[-.5, .5, .5], [-.5, .5, -.5], [.5, .5, -.5], [.5, .5, .5]],
+    [[-.5, .5, -.5], [-.5, -.5, -.5], [.5, -.5, -.5], [.5, .5, -.5]],
+    [[.5, .5, .5], [.5, -.5, .5], [-.5, -.5, .5], [-.5, .5, .5]],
+    [[-.5, .5, .5], [-.5, -.5, .5], [-.5, -.5, -.5], [-.5, .5, -.5]],
+    [[.5, .5, -.5], [.5, -.5, -.5], [.5, -.5, .5], [.5, .5, .5]],
+], np.float64)
+
+
+def cube(local2world: np.ndarray, material, two_sided=False) -> Primitive:
+    """Cube of cube.cpp:31-75 transformed by a 4x4 (row-major, column vectors)."""
+    v = _CUBE_FACES.reshape(-1, 3)
+    h = np.concatenate([v, np.ones((v.shape[0], 1))], axis=1) @ local2world.T
+    pos = (h[:, :3] / h[:, 3:4]).astype(np.float32)
+    words = []
+    for i in range(6):
+        idx = 4 * i
+        # tuple (idx, idx+2, idx+1) and (idx, idx+3, idx+2), stored reversed
+        words.append([idx + 1, idx + 2, idx])
+        words.append([idx + 2, idx + 3, idx])
+    return Primitive(GEOM_CUBE, pos, np.array(words, np.uint32), two_sided, material)
+
+
+def rot_y(deg: float) -> np.ndarray:
+    t = math.radians(deg)
+    c, s = math.cos(t), math.sin(t)
+    return np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1]], np.float64)
+
+
+def translate(x, y, z) -> np.ndarray:
+    m = np.eye(4)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def scale3(x, y, z) -> np.ndarray:
+    return np.diag([x, y, z, 1.0])
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere: 20*4^subdiv triangles, 10*4^subdiv + 2 vertices."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+                 np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9],
+                  [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2],
+                  [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+                  [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(subdiv):
+        nv = v.shape[0]
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
+        e.sort(axis=1)
+        key = e[:, 0] * (nv + 1) + e[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        a, b = uniq // (nv + 1), uniq % (nv + 1)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        v = np.concatenate([v, mid], axis=0)
+        m = inv.reshape(3, -1) + nv
+        m01, m12, m20 = m[0], m[1], m[2]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], 1), np.stack([f[:, 1], m12, m01], 1),
+            np.stack([f[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)], axis=0)
+    return v, f
+
+
+def _fbm(p: np.ndarray, seed: int, octaves=6, lacunarity=2.0, gain=0.5) -> np.ndarray:
+    """Seeded 3-D gradient-noise fBm (Perlin-style, quintic fade)."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm])
+    g = rng.normal(size=(256, 3))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+
+    def noise(x):
+        i0 = np.floor(x).astype(np.int64)
+        fr = x - i0
+        w = fr * fr * fr * (fr * (fr * 6 - 15) + 10)
+        out = np.zeros(x.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    ii = (i0 + np.array([dx, dy, dz])) & 255
+                    h = perm[perm[perm[ii[:, 0]] + ii[:, 1]] + ii[:, 2]]
+                    d = fr - np.array([dx, dy, dz])
+                    val = np.einsum("ij,ij->i", g[h], d)
+                    wx = w[:, 0] if dx else 1 - w[:, 0]
+                    wy = w[:, 1] if dy else 1 - w[:, 1]
+                    wz = w[:, 2] if dz else 1 - w[:, 2]
+                    out += wx * wy * wz * val
+        return out
+
+    total = np.zeros(p.shape[0])
+    amp, freq = 1.0, 1.5
+    for _ in range(octaves):
+        total += amp * noise(p * freq)
+        amp *= gain
+        freq *= lacunarity
+    return total
+
+
+def displaced_icosphere(subdiv: int, seed: int, amplitude=0.15, radius=1.0,
+                        center=(0.0, 0.0, 0.0), scale=1.0, rotation=None):
+    """MeshGeo buffers of a seeded fBm-displaced icosphere."""
+    v, f = icosphere(subdiv)
+    disp = _fbm(v * 1.0, seed)
+    disp = disp / (np.abs(disp).max() + 1e-12)
+    p = v * (radius + amplitude * disp)[:, None]
+    if rotation is not None:
+        p = p @ rotation.T
+    p = p * scale + np.asarray(center)
+    # area-weighted vertex normals of the displaced surface
+    e0 = p[f[:, 1]] - p[f[:, 0]]
+    e1 = p[f[:, 2]] - p[f[:, 0]]
+    fn = np.cross(e0, e1)
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    n /= np.linalg.norm(n, axis=1, keepdims=True) + 1e-30
+    # spherical uv of the undisplaced direction
+    u = 0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * np.pi)
+    vv = np.arccos(np.clip(v[:, 1], -1, 1)) / np.pi
+    pos4 = np.concatenate([p, u[:, None]], axis=1).astype(np.float32)
+    nrm4 = np.concatenate([n, vv[:, None]], axis=1).astype(np.float32)
+    return pos4, nrm4, f.astype(np.uint32)
+
+
+def mesh(pos4, nrm4, faces, material, two_sided=True) -> Primitive:
+    return Primitive(GEOM_MESH, np.ascontiguousarray(pos4, np.float32),
+                     np.ascontiguousarray(faces, np.uint32), two_sided, material,
+                     np.ascontiguousarray(nrm4, np.float32))
+
+
+# ------------------------------------------------------------ environment
+def hdr_env(width: int, height: int, seed: int, sigma=2.0, sun=8, sun_gain=1e4):
+    """Procedural HDR map: smooth log-normal luminance field (std `sigma` in
+    log space) with one hot sun x sun patch; FP32 RGB, row 0 at theta = 0."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.meshgrid((np.arange(height) + 0.5) / height, (np.arange(width) + 0.5) / width,
+                         indexing="ij")
+    field = np.zeros((height, width))
+    for _ in range(24):
+        kx, ky = rng.integers(1, 9), rng.integers(1, 7)
+        ph = rng.uniform(0, 2 * np.pi, 2)
+        field += rng.normal() * np.sin(2 * np.pi * kx * xx + ph[0]) * np.sin(np.pi * ky * yy + ph[1])
+    field = (field - field.mean()) / (field.std() + 1e-12) * sigma
+    lum = np.exp(field)
+    tint = np.stack([1.0 + 0.15 * np.sin(2 * np.pi * xx), np.ones_like(xx),
+                     1.0 + 0.25 * np.cos(np.pi * yy)], axis=-1)
+    rgb = lum[..., None] * tint
+    med = float(np.median(lum))
+    sx = int(rng.integers(0, width - sun))
+    sy = int(rng.integers(height // 8, height // 2 - sun))
+    rgb[sy:sy + sun, sx:sx + sun, :] = med * sun_gain
+    return np.ascontiguousarray(rgb.astype(np.float32))
+
+
+def constant_env(value: float):
+    return np.full((1, 1, 3), value, np.float32)
+
+
+# --------------------------------------------------------------- configs
+def cornell(seed: int = 20261203, res=(256, 256), spp=16) -> Scene:
+    """Cornell-box-style scene: open-front unit box (5 two-sided quads) and two
+    boxes rotated about y by seeded angles in [0, 30] degrees; Lambert walls
+    0.73, left (0.65, 0.05, 0.05), right (0.12, 0.45, 0.15); a directional
+    light of radiance 5 from a seeded upper-hemisphere direction facing the
+    open side, plus a constant environment of 0.1. Pinhole 256x256, 16 spp,
+    max 8 bounces, Russian roulette after bounce 3."""
+    rng = np.random.default_rng(seed)
+    grey, red, green = 0, 1, 2
+    mats = [Material(MAT_DIFFUSE, (0.73, 0.73, 0.73)),
+            Material(MAT_DIFFUSE, (0.65, 0.05, 0.05)),
+            Material(MAT_DIFFUSE, (0.12, 0.45, 0.15))]
+    prims = [
+        quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), grey),      # floor
+        quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), grey),      # ceiling
+        quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), grey),      # back
+        quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), red),       # left
+        quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), green),     # right
+    ]
+    a1, a2 = rng.uniform(0, 30, 2)
+    prims.append(cube(translate(0.33, 0.3, 0.38) @ rot_y(a1) @ scale3(0.3, 0.6, 0.3), grey))
+    prims.append(cube(translate(0.68, 0.15, 0.65) @ rot_y(-a2) @ scale3(0.3, 0.3, 0.3), grey))
+    while True:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        if d[1] > 0.2 and d[2] > 0.3:
+            break
+    lights = [Light(LIGHT_DIRECTIONAL, dir=tuple(float(x) for x in d), angle=0.0,
+                    radiance=(5.0, 5.0, 5.0)),
+              Light(LIGHT_ENV, env=constant_env(0.1), rot=0.0, scaler=1.0)]
+    cam = CameraSpec(pos=(0.5, 0.5, 2.2), look_at=(0.5, 0.5, 0.0), up=(0.0, 1.0, 0.0),
+                     fov=40.0, res=res)
+    return Scene("cornell", prims, mats, lights, cam,
+                 PTSpec(spp=spp, tile_size=16, min_bounces=3, max_bounces=8, cont_prob=0.9))
+
+
+def _disney_from(rng) -> Material:
+    return Material(MAT_DISNEY, tuple(float(x) for x in rng.uniform(0, 1, 3)),
+                    subsurface=0.0, metallic=float(rng.uniform()), specular=float(rng.uniform()),
+                    specular_tint=0.0, roughness=float(rng.uniform()), anisotropic=0.0,
+                    sheen=0.0, sheen_tint=0.0, clearcoat=float(rng.uniform()),
+                    clearcoat_gloss=0.5)
+
+
+def heavy_mesh(seed: int = 20261204, subdiv: int = 8, res=(1024, 1024), spp=64,
+               env_size=(1024, 512)) -> Scene:
+    """Displaced icosphere (20*4^subdiv triangles; 1,310,720 at subdiv 8) on a
+    2-triangle ground quad; Disney materials drawn per object; seeded HDR env
+    map; 1024x1024, 64 spp, max 16 bounces (RR defaults: min 5, p 0.9)."""
+    rng = np.random.default_rng(seed)
+    mats = [_disney_from(rng), _disney_from(rng)]
+    pos4, nrm4, f = displaced_icosphere(subdiv, seed + 1, amplitude=0.15)
+    prims = [mesh(pos4, nrm4, f, 0, two_sided=True),
+             quad((-6, -1.2, -6), (-6, -1.2, 6), (6, -1.2, 6), (6, -1.2, -6), 1)]
+    env = hdr_env(env_size[0], env_size[1], seed + 2)
+    lights = [Light(LIGHT_ENV, env=env, rot=0.0, scaler=1.0)]
+    cam = CameraSpec(pos=(0.0, 0.6, 4.2), look_at=(0.0, -0.1, 0.0), up=(0.0, 1.0, 0.0),
+                     fov=38.0, res=res)
+    return Scene("heavy_mesh", prims, mats, lights, cam,
+                 PTSpec(spp=spp, tile_size=16, min_bounces=5, max_bounces=16, cont_prob=0.9))
+
+
+def random_soup(n: int, seed: int, extent=1.0, size=0.05) -> np.ndarray:
+    """(n, 9) float32 random triangle soup in [-extent, extent]^3."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-extent, extent, (n, 1, 3))
+    d = rng.normal(scale=size, size=(n, 3, 3))
+    return np.ascontiguousarray((c + d).reshape(n, 9).astype(np.float32))
+
+
+def soup_scene(tri: np.ndarray) -> Scene:
+    """Wrap a triangle soup as one MeshGeo primitive (for accelerator tests)."""
+    n = tri.shape[0]
+    pos = tri.reshape(-1, 3)
+    pos4 = np.concatenate([pos, np.zeros((pos.shape[0], 1), np.float32)], axis=1)
+    nrm4 = np.zeros_like(pos4)
+    nrm4[:, 1] = 1.0
+    faces = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    return Scene("soup", [mesh(pos4, nrm4, faces, 0)], [Material(MAT_DIFFUSE, (0.5, 0.5, 0.5))],
+                 [], CameraSpec((0, 0, 3), (0, 0, 0), (0, 1, 0), 40.0, (64, 64)), PTSpec())

@@ -1,0 +1,273 @@
+/*
+ * ajisai_gpu.h — C-ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * Plain C types only: opaque handles, pointers and sizes, int status codes.
+ * No exception crosses this boundary. Entry points taking a `stream`
+ * argument enqueue work on that hipStream_t (NULL = the null stream) and
+ * return without synchronising unless stated otherwise.
+ *
+ * Each declaration names the reference interface it replaces
+ * (file:line relative to the reference root, siyuanpan/ajisai_render).
+ */
+#ifndef AJISAI_GPU_H
+#define AJISAI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AJG_ABI_VERSION 1
+
+/* Status codes. The reference has no error returns (noexcept hot calls,
+ * std::exit in loaders: factory/creator/geometry_creators.cpp:118-123);
+ * every entry point here returns one of these instead. */
+enum ajg_status {
+  AJG_OK = 0,
+  AJG_ERR_HIP = 1,       /* a HIP runtime call failed */
+  AJG_ERR_OOM = 2,       /* device allocation failed */
+  AJG_ERR_INVALID = 3,   /* bad argument, unsupported type, empty scene */
+  AJG_ERR_CANCELLED = 4, /* the host cancel flag was raised */
+  AJG_ERR_INTERNAL = 5   /* an internal limit or invariant was hit */
+};
+
+typedef struct ajg_ctx ajg_ctx;
+typedef struct ajg_accel ajg_accel;
+typedef struct ajg_scene ajg_scene;
+
+/* ---------------------------------------------------------------- context */
+int ajg_abi_version(void);
+/* One context per device; owns the default allocations and the error text. */
+int ajg_ctx_create(int device, ajg_ctx **out);
+int ajg_ctx_destroy(ajg_ctx *ctx);
+/* Last error message of this context (valid until the next call). */
+const char *ajg_last_error(const ajg_ctx *ctx);
+/* Fault injection for host-side failure tests: the next `countdown`-th
+ * entry point call on this context returns `status` (0 disables). */
+int ajg_ctx_inject_failure(ajg_ctx *ctx, int countdown, int status);
+
+/* ------------------------------------------------------------ accelerator */
+
+/* MeshView (ajisai/core/geometry/geometry.h:33-42): byte-strided position
+ * (>= 3 floats per vertex) and index (3 x u32 per triangle) buffers. */
+typedef struct ajg_mesh_view {
+  const void *position_buffer;
+  size_t position_offset;
+  size_t position_stride;
+  size_t position_size;
+  const void *index_buffer;
+  size_t index_offset;
+  size_t index_stride;
+  size_t index_size;
+} ajg_mesh_view;
+
+/* Ray as stored after the Ray constructor (ajisai/core/ray.h:28-45):
+ * origin already offset, direction normalised. 32 bytes. */
+typedef struct ajg_ray {
+  float o[3];
+  float d[3];
+  float tmin, tmax;
+} ajg_ray;
+
+/* Closest-hit record (the t / id / barycentrics that Triangle::Intersect
+ * writes into PrimitiveIntersection, ploc_aggregate.cpp:184-216). tri_id is
+ * the global flattened triangle index (ploc_aggregate.cpp:257-284);
+ * 0xFFFFFFFF and t = +inf on a miss. 16 bytes. */
+typedef struct ajg_hit {
+  float t;
+  uint32_t tri_id;
+  float u, v;
+} ajg_hit;
+
+/* PLOC node as built (Node, ploc_aggregate.cpp:146-152; size_t fields
+ * narrowed to u32). count = 1 for a leaf, 0 for an interior node; first is
+ * the first child (interior) or the index into the Morton-sorted primitive
+ * order (leaf). The root is node 0. 32 bytes. */
+typedef struct ajg_bvh_node {
+  float mn[3], mx[3];
+  uint32_t count, first;
+} ajg_bvh_node;
+
+typedef struct ajg_accel_info {
+  uint32_t tri_count;
+  uint32_t node_count;     /* 2 * tri_count - 1 */
+  uint32_t iterations;     /* PLOC Cluster iterations */
+  uint32_t depth;          /* tree depth (root = 1) */
+  uint64_t device_bytes;   /* bytes held on the device by the accelerator */
+  float scene_box[6];      /* Join of primitive AABBs (ploc_aggregate.cpp:256-258) */
+} ajg_accel_info;
+
+/* PLOCAggregate::Build (ploc_aggregate.cpp:252-327) reached through
+ * Aggregate::Build (ajisai/core/aggregate/aggregate.h:36) and the factory
+ * CreatePLOCAggregate(size_t search_radius) (aggregate.h:45): flattens the
+ * views on the host in primitive order, uploads, and builds the PLOC BVH on
+ * the device (Morton codes, LDS radix sort, nearest-neighbour merge).
+ * Synchronous. search_radius defaults to 14 in the reference factory
+ * (factory/creator/aggregate_creators.cpp:43). */
+int ajg_accel_build(ajg_ctx *ctx, const ajg_mesh_view *views, uint32_t n_views,
+                    uint32_t search_radius, ajg_accel **out);
+
+/* Same build from a device-resident flattened triangle soup (n x 9 floats,
+ * a,b,c per triangle, global id = position) and the scene box (host,
+ * 6 floats; NULL = Min/Max over all soup vertices, which equals the Join of
+ * primitive boxes when every position is referenced). Enqueued on `stream`,
+ * then synchronised. This is the path timed for build throughput. */
+int ajg_accel_build_soup(ajg_ctx *ctx, const float *tri_dev, uint32_t n,
+                         const float *scene_box, uint32_t search_radius,
+                         void *stream, ajg_accel **out);
+int ajg_accel_destroy(ajg_accel *accel);
+int ajg_accel_get_info(const ajg_accel *accel, ajg_accel_info *out);
+/* Copy the PLOC node array (2n-1 nodes, root 0) and the Morton-sorted
+ * primitive order (primitive_indices_, n entries) to host memory. */
+int ajg_accel_download(const ajg_accel *accel, ajg_bvh_node *nodes,
+                       uint32_t *perm);
+
+/* Aggregate::Intersect batched (aggregate.h:38-39 -> PLOCAggregate::Intersect,
+ * ploc_aggregate.cpp:329-335, IntersectAux :367-391): left-first DFS order
+ * with pop-time culling, so hits are bit-identical to the reference's
+ * recursive traversal. rays/hits are device pointers. */
+int ajg_accel_trace_closest(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
+                            ajg_hit *hits, void *stream);
+/* Aggregate::Occlude batched (aggregate.h:41 -> ploc_aggregate.cpp:337-365):
+ * occluded[i] = 1 if any triangle is hit in (tmin, tmax). Device pointers. */
+int ajg_accel_trace_any(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
+                        uint8_t *occluded, void *stream);
+/* Counting pass of trace_closest for the byte model: per ray, the number of
+ * fat nodes fetched and triangles tested. Device pointers; not timed. */
+int ajg_accel_trace_counts(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
+                           uint32_t *nodes_fetched, uint32_t *tris_tested,
+                           void *stream);
+
+/* --------------------------------------------------------------- renderer */
+
+enum ajg_geom_kind { AJG_GEOM_QUAD = 0, AJG_GEOM_CUBE = 1, AJG_GEOM_MESH = 2 };
+enum ajg_material_type { AJG_MAT_DIFFUSE = 0, AJG_MAT_DISNEY = 1 };
+enum ajg_light_type { AJG_LIGHT_DIRECTIONAL = 0, AJG_LIGHT_ENV = 1 };
+
+/* Primitive (ajisai/core/primitive/primitive.h) reduced to what the hot path
+ * reads: the geometry's MeshView, its kind (selects the PostIntersect rule:
+ * quad.cpp:73-90, cube.cpp:96-108, mesh.cpp:183-204), the TwoSided wrapper
+ * (two_sided.cpp:44-53), the material index and, for meshes, the per-vertex
+ * normal_and_vs buffer (mesh.cpp:190-193; 4 floats per vertex). */
+typedef struct ajg_primitive {
+  ajg_mesh_view view;
+  uint32_t kind;
+  uint32_t two_sided;
+  uint32_t material;
+  uint32_t reserved;
+  const float *normal_and_vs;
+} ajg_primitive;
+
+/* Material: Diffuse (material/diffuse.cpp:35-48; base_color = albedo) or
+ * Disney (material/disney.cpp:497-525; parameter order of CreateDisney,
+ * material/material.h:40-54) with Constant2D textures. */
+typedef struct ajg_material {
+  uint32_t type;
+  float base_color[3];
+  float subsurface, metallic, specular, specular_tint, roughness, anisotropic,
+      sheen, sheen_tint, clearcoat, clearcoat_gloss;
+} ajg_material;
+
+/* Light: DirectionalLight (light/directional_light.cpp:30-58; dir points
+ * toward the light, angle in radians) or EnvLight with an HDR texture
+ * (light/env_light.cpp:29-170; texels w x h RGB float, row-major, row 0 at
+ * theta = 0; rot in degrees; scaler). */
+typedef struct ajg_light {
+  uint32_t type;
+  float dir[3];
+  float angle;
+  float radiance[3];
+  uint32_t env_w, env_h;
+  const float *env_texels;
+  float rot, scaler;
+} ajg_light;
+
+/* Thin-lens camera (camera/thin_lens.cpp:32-83): column-major raster-to-
+ * camera and camera-to-world matrices, film resolution, lens parameters. */
+typedef struct ajg_camera {
+  float raster2camera[16];
+  float camera2world[16];
+  float resolution[2];
+  float lens_radius, focal_distance;
+} ajg_camera;
+
+/* PTRendererArgs (renderer/renderer.h:37-45) plus the build's options. */
+typedef struct ajg_pt_args {
+  int32_t spp, tile_size, min_bounces, max_bounces;
+  float cont_prob;
+  int32_t use_mis;      /* must be 1: TraceMIS (pt.cpp:98) */
+  int32_t seed_mode;    /* 0: per-pixel stream x + y*W, samples in sequence
+                           (pt.cpp:56-64); 1: one stream per sample */
+  uint32_t seed_base;   /* seed_mode 1 only */
+  int32_t env_cdf_fp64; /* env CDF prefix sums: 0 float (distribution.h:44-48),
+                           1 double rounded to float */
+  int32_t max_slots;    /* path slots in flight; 0 = automatic */
+} ajg_pt_args;
+
+/* Film accumulators per traced pixel (traced area = whole tiles,
+ * pt.cpp:43-49): Kahan-compensated float sums of L and L^2 per channel and
+ * the sample count. Device pointers, sized Wp*Hp*3, Wp*Hp*3 and Wp*Hp. */
+typedef struct ajg_film {
+  float *sum;
+  float *sum_sq;
+  uint32_t *count;
+  float *per_sample; /* optional: L of every sample, (pixel*spp + s)*3 */
+} ajg_film;
+
+typedef struct ajg_render_stats {
+  uint64_t closest_queries; /* camera + continuation + MIS-BSDF rays */
+  uint64_t any_queries;     /* shadow rays */
+  uint64_t paths;           /* samples completed */
+  uint32_t iterations;      /* wavefront iterations */
+  uint32_t slots;
+  float ms_total;           /* device time of the render (events) */
+  float ms_trace_closest, ms_trace_any, ms_shade;
+} ajg_render_stats;
+
+/* DefaultScene (scene/default_scene.cpp:29-103) with a GPU PLOC accelerator:
+ * flattens and builds like ajg_accel_build, uploads PostIntersect data,
+ * materials and lights, builds the env Distribution2D tables on the device
+ * (EnvLight ctor, env_light.cpp:29-48) and processes lights against the
+ * world bound. Synchronous. Host pointers are read during the call only. */
+int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims,
+                     uint32_t n_prims, const ajg_material *mats,
+                     uint32_t n_mats, const ajg_light *lights,
+                     uint32_t n_lights, uint32_t search_radius,
+                     int32_t env_cdf_fp64, ajg_scene **out);
+int ajg_scene_destroy(ajg_scene *scene);
+/* Borrowed accelerator of the scene (owned by the scene). */
+int ajg_scene_get_accel(ajg_scene *scene, ajg_accel **out);
+/* Env Distribution2D tables of light li as built on the device (host
+ * copies): cond_cdf[h][w+1], cond_func_int[h], marg_cdf[h+1], marg_func_int. */
+int ajg_scene_env_tables(ajg_scene *scene, uint32_t li, float *cond_cdf,
+                         float *cond_func_int, float *marg_cdf,
+                         float *marg_func_int);
+
+/* Renderer::Render for PathTracing::TraceMIS (renderer.h:64-65,
+ * pt.cpp:40-73 and 98-227) as a wavefront on the device: ray-gen,
+ * closest/any-hit traversal, shade/sample, shadow and accumulate stages.
+ * Fills `film` (device memory, zeroed by the call). Synchronous; stats may
+ * be NULL. */
+int ajg_render_pt(ajg_scene *scene, const ajg_camera *camera,
+                  const ajg_pt_args *args, const ajg_film *film,
+                  ajg_render_stats *stats, void *stream);
+
+/* Film::MergeTile + WriteImage resolve (film.cpp:27-63, film.h:119-152,
+ * image.h:38-48): W x H RGB image from the per-traced-pixel sums. Device
+ * pointers. */
+int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size,
+                     const float *sum, const uint32_t *count, float *rgb,
+                     void *stream);
+
+/* ThinLensCamera matrices (thin_lens.cpp:32-55) from look-at parameters;
+ * fov in degrees. Host only. */
+int ajg_camera_setup(ajg_camera *cam, const float *pos, const float *look_at,
+                     const float *up, float fov_deg, float res_x, float res_y,
+                     float lens_radius, float focal_distance);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

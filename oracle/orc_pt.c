@@ -1,0 +1,1371 @@
+/*
+ * Path tracer of the oracle — TEST INFRASTRUCTURE (see orc.h).
+ * Restates PathTracing::TraceMIS (ajisai/core/renderer/pt.cpp:98-227) with
+ * its helpers (renderer/helper.h), the Lambert (AggregateBSDF + Diffuse) and
+ * Disney BSDFs, directional and environment lights, the thin-lens camera,
+ * the PCG32 sampler and the box-filtered film.
+ *
+ * Parity status of the Disney BSDF: its exact values (including the
+ * world-space wo used by the coat term of Pdf, disney.cpp:208-220) follow
+ * disney.cpp line by line, but no reference output exists to pin them; the
+ * tests pin it only statistically (sample/pdf consistency, MC integrals,
+ * reciprocity of Eval). Parity unpinned for exact Disney values.
+ *
+ * Environment sampling draws its two numbers as the GCC build of the
+ * reference evaluates SampleContinuous(Next1D(), Next1D(), ...)
+ * (env_light.cpp:71): arguments right to left, so the first draw feeds v.
+ * This order is an assumption about the compiler and is unpinned; it does
+ * not change any expectation.
+ */
+#include <pthread.h>
+#include <stdlib.h>
+
+#include "orc.h"
+#include "orc_math.h"
+
+#define ORC_PI 3.141592654f       /* Constants<float>::pi (constants.h:30) */
+#define ORC_PI2 1.570796327f      /* pi2 */
+#define ORC_PI4 0.785398163f      /* pi4 */
+#define ORC_INV_PI (1.f / ORC_PI)
+#define ORC_EPS 1e-4f             /* Ray::Eps (ray.h:41) */
+
+/* ---- PCG32 (random.h:30-69) -------------------------------------------- */
+static const uint64_t kPcgMult = 6364136223846793005ull;
+static const uint64_t kPcgInc = 1442695040888963407ull;
+
+void orc_pcg32_seed(orc_pcg32 *r, uint64_t seed) {
+  r->state = seed + kPcgInc;
+  (void)orc_pcg32_next_u32(r);
+}
+uint32_t orc_pcg32_next_u32(orc_pcg32 *r) {
+  uint64_t x = r->state;
+  unsigned rot = (unsigned)(x >> 59);
+  r->state = x * kPcgMult + kPcgInc;
+  x ^= x >> 18;
+  uint32_t v = (uint32_t)(x >> 27);
+  return v >> rot | v << ((-rot) & 31);
+}
+float orc_pcg32_next_float(orc_pcg32 *r) {
+  uint32_t u = (orc_pcg32_next_u32(r) >> 9) | 0x3f800000u;
+  float f;
+  memcpy(&f, &u, 4);
+  return f - 1.0f;
+}
+void orc_pcg32_floats(uint64_t seed, uint32_t n, float *out) {
+  orc_pcg32 r;
+  orc_pcg32_seed(&r, seed);
+  for (uint32_t i = 0; i < n; ++i) out[i] = orc_pcg32_next_float(&r);
+}
+
+/* Per-sample stream seed (the build's counter-based mode): splitmix64 of
+ * (seed_base, pixel, sample). */
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static uint64_t sample_seed(uint32_t base, uint32_t pixel, uint32_t s) {
+  return splitmix64(((uint64_t)base << 32) ^ splitmix64(((uint64_t)pixel << 32) | s));
+}
+
+/* ---- warps (warp.h:29-112) and frames (coordinate_system.h:37-52) ----- */
+typedef struct v2 {
+  float x, y;
+} v2;
+
+static v2 disk_concentric(v2 s) {
+  v2 off = {2.f * s.x - 1.f, 2.f * s.y - 1.f};
+  float r, phi;
+  if (off.x == 0 && off.y == 0) {
+    v2 z = {0.f, 0.f};
+    return z;
+  } else if (fabsf(off.x) > fabsf(off.y)) {
+    r = off.x;
+    phi = ORC_PI4 * (off.y / off.x);
+  } else {
+    r = off.y;
+    phi = ORC_PI2 - ORC_PI4 * (off.x / off.y);
+  }
+  v2 out = {r * cosf(phi), r * sinf(phi)};
+  return out;
+}
+
+static v3 cosine_hemisphere(v2 s, float *pdf) {
+  v2 p = disk_concentric(s);
+  float r = 0.f;
+  r += p.x * p.x;
+  r += p.y * p.y;
+  float one_minus = 1 - r;
+  float z = sqrtf(std_max(0.0f, one_minus));
+  if (z == 0) z = 1e-10f;
+  if (pdf) *pdf = z * ORC_INV_PI;
+  return V3(p.x, p.y, z);
+}
+
+static v3 uniform_cone(v2 u, float cos_max) {
+  float cos_t = (1.f - u.x) + u.x * cos_max;
+  float a = 1.f - cos_t * cos_t;
+  float sin_t = sqrtf(std_max(0.f, a));
+  float phi = u.y * 2 * ORC_PI;
+  return V3(cosf(phi) * sin_t, sinf(phi) * sin_t, cos_t);
+}
+static float uniform_cone_pdf(float cos_max) {
+  if (cos_max == 1.0f) return 1.0f;
+  return 1.0f / (2.0f * ORC_PI * (1.0f - cos_max));
+}
+
+typedef struct frame {
+  v3 dx, dy, dz;
+} frame;
+
+static frame make_frame(v3 n) {
+  frame f;
+  f.dz = n;
+  float sign = copysignf(1.f, n.z);
+  float a = -1.f / (sign + n.z);
+  float b = n.x * n.y * a;
+  f.dx = V3(1.f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  f.dy = V3(b, sign + n.y * n.y * a, -n.y);
+  return f;
+}
+static v3 to_local(const frame *f, v3 v) {
+  return V3(vdot(v, f->dx), vdot(v, f->dy), vdot(v, f->dz));
+}
+static v3 to_world(const frame *f, v3 v) {
+  return vadd(vadd(vscale(f->dx, v.x), vscale(f->dy, v.y)), vscale(f->dz, v.z));
+}
+
+/* ---- small helpers ------------------------------------------------------ */
+static inline float luminance(v3 s) { /* spectrum.h:80-82, in double */
+  return (float)(0.2126 * (double)s.x + 0.7152 * (double)s.y +
+                 0.0722 * (double)s.z);
+}
+static inline int is_black(v3 s) { return vmaxc(s) <= 0; }
+static inline float clampf_r(float v, float lo, float hi) { /* std::clamp */
+  return v < lo ? lo : (hi < v ? hi : v);
+}
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+/* std::lerp, C++20 (libstdc++ __lerp) */
+static float std_lerp(float a, float b, float t) {
+  if ((a <= 0 && b >= 0) || (a >= 0 && b <= 0)) return t * b + (1 - t) * a;
+  if (t == 1) return b;
+  const float x = a + t * (b - a);
+  return (t > 1) == (b > a) ? (b < x ? x : b) : (b > x ? x : b);
+}
+/* Lerp template of disney.cpp:42-45 */
+static inline float lerp_d(float a, float b, float t) { return a * (1 - t) + b * t; }
+static inline v3 lerp_v(v3 a, v3 b, float t) {
+  return vadd(vscale(a, 1 - t), vscale(b, t));
+}
+static inline float power_heuristic(float a, float b) { /* helper.h:38-42 */
+  a *= a;
+  b *= b;
+  return a / (a + b);
+}
+
+/* ---- BSDFs -------------------------------------------------------------- */
+typedef struct bsdf {
+  const orc_material *m;
+  v3 ng, ns;
+  frame gframe, sframe;
+} bsdf;
+
+typedef struct bsdf_sample {
+  v3 dir, f;
+  float pdf;
+  int valid;
+} bsdf_sample;
+
+static bsdf make_bsdf(const orc_material *m, v3 ng, v3 ns) {
+  bsdf b;
+  b.m = m;
+  b.ng = ng;
+  b.ns = ns;
+  b.gframe = make_frame(ng);
+  b.sframe = make_frame(ns);
+  return b;
+}
+static v3 albedo(const bsdf *b) { return vload(b->m->base_color); }
+
+/* BSDF black-fringe handling (bsdf.h:99-106, bsdf.cpp:30-63) */
+static int black_fringes(const bsdf *b, v3 w) {
+  return (vdot(b->ng, w) > 0) != (vdot(b->ns, w) > 0);
+}
+static float correct_shading_normal(v3 ng, v3 ns, v3 wi) { /* bsdf.h:42-46 */
+  const float denom = fabsf(vdot(ng, wi));
+  return denom == 0 ? 1.f : fabsf(vdot(ns, wi) / denom);
+}
+
+static bsdf_sample invalid_sample(void) {
+  bsdf_sample s = {{0, 0, 0}, {0, 0, 0}, 0.f, 0};
+  return s;
+}
+
+/* Lambert through AggregateBSDF (aggregate_bsdf.cpp:43-131) with a single
+ * DiffuseComponent of weight 1 (diffuse.cpp:35-48, diffuse_component.cpp). */
+static bsdf_sample lambert_sample(const bsdf *b, v3 wo, v3 sam) {
+  if (black_fringes(b, wo)) {
+    if (vdot(b->ng, wo) <= 0.f) return invalid_sample();
+    float pdf;
+    v2 s2 = {sam.x, sam.y};
+    v3 lwi = cosine_hemisphere(s2, &pdf);
+    if (pdf < FLT_EPSILON) return invalid_sample();
+    v3 wi = vnormalize(to_world(&b->gframe, lwi));
+    float factor = correct_shading_normal(b->ng, b->ns, wi);
+    bsdf_sample r = {wi, vscale(vdivs(albedo(b), ORC_PI), factor), pdf, 1};
+    return r;
+  }
+  v3 lwo = vnormalize(to_local(&b->sframe, wo));
+  if (lwo.z <= 0.f) return invalid_sample();
+  /* Distribution over the single weight always selects component 0. */
+  if (lwo.z < 0) return invalid_sample();
+  float pdf;
+  v2 s2 = {sam.y, sam.z};
+  v3 lwi = cosine_hemisphere(s2, &pdf);
+  if ((double)pdf < 1e-3) return invalid_sample();
+  v3 f = vdivs(albedo(b), ORC_PI); /* coef_ */
+  lwi = vnormalize(lwi);
+  pdf *= 1.f;
+  v3 wi = to_world(&b->sframe, lwi);
+  float factor = correct_shading_normal(b->ng, b->ns, wi);
+  bsdf_sample r = {wi, vscale(f, factor), pdf, 1};
+  return r;
+}
+static v3 lambert_eval(const bsdf *b, v3 wi, v3 wo) {
+  if (black_fringes(b, wo)) {
+    if (vdot(b->ng, wi) <= 0.f || vdot(b->ng, wo) <= 0.f) return V3(0, 0, 0);
+    return vdivs(albedo(b), ORC_PI);
+  }
+  v3 lwi = vnormalize(to_local(&b->sframe, wi));
+  v3 lwo = vnormalize(to_local(&b->sframe, wo));
+  if (lwi.z <= 0.f || lwo.z <= 0.f) return V3(0, 0, 0);
+  v3 ret = V3(0.f, 0.f, 0.f);
+  ret = vadd(ret, vdivs(albedo(b), ORC_PI));
+  return ret;
+}
+static float lambert_pdf(const bsdf *b, v3 wi, v3 wo) {
+  if (black_fringes(b, wo)) {
+    if ((vdot(b->ng, wi) > 0.f) != (vdot(b->ng, wo) > 0.f)) return 0.f;
+    v3 local = vnormalize(to_local(&b->gframe, wi));
+    return local.z * ORC_INV_PI;
+  }
+  v3 lwi = vnormalize(to_local(&b->sframe, wi));
+  v3 lwo = vnormalize(to_local(&b->sframe, wo));
+  if (lwi.z <= 0.f || lwo.z <= 0.f) return 0.f;
+  float weight_sum = 0, pdf = 0;
+  weight_sum += 1.f;
+  pdf += 1.f * (lwi.z * ORC_INV_PI);
+  return weight_sum > 0 ? pdf / weight_sum : 0;
+}
+
+/* ---- Disney (material/disney.cpp) -------------------------------------- */
+static float tan_theta(v3 v) {
+  float temp = 1 - v.z * v.z;
+  if (temp <= 0.0f) return 0.0f;
+  return sqrtf(temp) / v.z;
+}
+static float tan_theta2(v3 v) {
+  float tmp = 1.f - v.z * v.z;
+  if (tmp <= 0.f) return 0.f;
+  return tmp / (v.z * v.z);
+}
+static v3 reflect(v3 vi, v3 n) { return vadd(vi, vscale(n, 2 * vdot(vneg(vi), n))); }
+
+static float fresnel_conductor(float cosi, float eta, float k) {
+  float tmp = (eta * eta + k * k) * cosi * cosi;
+  float rparl2 = (tmp - (2.f * eta * cosi) + 1) / (tmp + (2.f * eta * cosi) + 1);
+  float tmp_f = eta * eta + k * k;
+  float rperp2 = (tmp_f - (2.f * eta * cosi) + cosi * cosi) /
+                 (tmp_f + (2.f * eta * cosi) + cosi * cosi);
+  return 0.5f * (rparl2 + rperp2);
+}
+static float fresnel_schlick(const orc_material *m, float cos_d, float nr) {
+  float omc = 1.f - cos_d;
+  float sq = omc * omc;
+  float fresnel = nr + (1.f - nr) * sq * sq * omc;
+  float cond = fresnel_conductor(cos_d, 0.4f, 1.6f);
+  return std_lerp(fresnel, cond, m->metallic);
+}
+static float fresnel_schlick_coat(float cos_d) {
+  float omc = 1.0f - cos_d;
+  float sq = omc * omc;
+  return 0.04f + (1.0f - 0.04f) * sq * sq * omc;
+}
+static float ggx_d(v3 wh, float alpha) {
+  if (wh.z <= 0.f) return 0.f;
+  float t2 = tan_theta2(wh);
+  float c2 = wh.z * wh.z;
+  float root = alpha / (c2 * (alpha * alpha + t2));
+  return ORC_INV_PI * (root * root);
+}
+static float smith_g(v3 v, v3 wh, float alpha) {
+  const float tt = fabsf(tan_theta(v));
+  if (tt == 0.0f) return 1.0f;
+  if (vdot(v, wh) * v.z <= 0) return 0.0f;
+  const float root = alpha * tt;
+  return 2.0f / (1.0f + sqrtf(1.0f + root * root));
+}
+static float ggx_g(v3 wi, v3 wo, v3 wh, float alpha) {
+  return smith_g(wo, wh, alpha) * smith_g(wi, wh, alpha);
+}
+static float ggx_pdf_vn(v3 wi, v3 h, float alpha) {
+  float d = ggx_d(h, alpha);
+  return smith_g(wi, h, alpha) * fabsf(vdot(wi, h)) * d / (fabsf(wi.z) + 1e-4f);
+}
+static v2 ggx_vn_unit(float theta_i, float u1, float u2) {
+  v2 slope;
+  if (theta_i < 1e-4f) {
+    float a = u1 / ((1 - u1) + 1e-6f);
+    float r = sqrtf(std_max(a, 0.0f));
+    float sp = sinf(2 * ORC_PI * u2), cp = cosf(2 * ORC_PI * u2);
+    slope.x = r * cp;
+    slope.y = r * sp;
+    return slope;
+  }
+  float tan_i = tanf(theta_i);
+  float a = 1 / tan_i;
+  float g1a = 1.0f + 1.0f / (a * a);
+  float g1 = 2.0f / (1.0f + sqrtf(std_max(g1a, 0.0f)));
+  float A = 2.0f * u1 / g1 - 1.0f;
+  if (fabsf(A) == 1) A -= (A >= 0.0f ? 1.0f : -1.0f) * 1e-4f;
+  float temp = 1.0f / (A * A - 1.0f);
+  float B = tan_i;
+  float dd = B * B * temp * temp - (A * A - B * B) * temp;
+  float D = sqrtf(std_max(dd, 0.0f));
+  float sx1 = B * temp - D;
+  float sx2 = B * temp + D;
+  slope.x = (A < 0.0f || sx2 > 1.0f / tan_i) ? sx1 : sx2;
+  float S;
+  if (u2 > 0.5f) {
+    S = 1.0f;
+    u2 = 2.0f * (u2 - 0.5f);
+  } else {
+    S = -1.0f;
+    u2 = 2.0f * (0.5f - u2);
+  }
+  double w = (double)u2;
+  float z = (float)((w * (w * (w * (-0.365728915865723) + 0.790235037209296) -
+                          0.424965825137544) +
+                     0.000152998850436920) /
+                    (w * (w * (w * (w * 0.169507819808272 - 0.397203533833404) -
+                               0.232500544458471) +
+                          1) -
+                     0.539825872510702));
+  slope.y = S * z * sqrtf(1.0f + slope.x * slope.x);
+  return slope;
+}
+static v3 ggx_sample_vn(v3 wi_in, float u1, float u2, float *pdf, float alpha) {
+  v3 wi = vnormalize(V3(alpha * wi_in.x, alpha * wi_in.y, wi_in.z));
+  float theta = 0, phi = 0;
+  if (wi.z < 0.99999f) {
+    theta = acosf(wi.z);
+    phi = atan2f(wi.y, wi.x);
+  }
+  float sp = sinf(phi), cp = cosf(phi);
+  v2 s = ggx_vn_unit(theta, u1, u2);
+  v2 r = {cp * s.x - sp * s.y, sp * s.x + cp * s.y};
+  r.x *= alpha;
+  r.y *= alpha;
+  float norm = 1.f / sqrtf(r.x * r.x + r.y * r.y + 1.0f);
+  v3 h = V3(-r.x * norm, -r.y * norm, norm);
+  *pdf = ggx_pdf_vn(wi_in, h, alpha);
+  return h;
+}
+
+static float disney_diffuse_term(const orc_material *m, v3 wi, v3 wo, float idh,
+                                 float rough) {
+  if (m->metallic == 1.f) return 0.f;
+  float oml = 1.f - fabsf(wi.z);
+  float sql = oml * oml;
+  float omv = 1.f - fabsf(wo.z);
+  float sqv = omv * omv;
+  float fd90 = 0.5f + 2.f * idh * rough;
+  return ORC_INV_PI * (1.f + (fd90 - 1.f) * sql * sql * oml) *
+         (1.f + (fd90 - 1.f) * sqv * sqv * omv);
+}
+static float disney_subsurface_term(const orc_material *m, v3 wi, v3 wo,
+                                    float idh, float rough) {
+  if (m->subsurface == 0.0f) return 0.0f;
+  float oml = 1.0f - fabsf(wi.z);
+  float sql = oml * oml;
+  float omv = 1.0f - fabsf(wo.z);
+  float sqv = omv * omv;
+  float fss90 = idh * idh * rough;
+  float S = (1.0f + (fss90 - 1.0f) * sql * sql * oml) *
+            (1.0f + (fss90 - 1.0f) * sqv * sqv * omv);
+  return ORC_INV_PI * 1.25f *
+         (S * (1.0f / (fabsf(wo.z) + fabsf(wi.z)) - 0.5f) + 0.5f);
+}
+static float disney_specular_term(const orc_material *m, v3 wi, v3 wo, v3 wh,
+                                  float odh, float rough) {
+  if (wi.z * wo.z <= 0.f) return 0.f;
+  float D = ggx_d(wh, rough * rough);
+  if (D == 0.f) return 0.f;
+  float nr = std_lerp(0.f, 0.08f, m->specular);
+  float F = fresnel_schlick(m, odh, nr);
+  float rg = (0.5f + 0.5f * rough);
+  float G = ggx_g(wi, wo, wh, rg * rg);
+  return F * D * G / (4.f * fabsf(wi.z) * fabsf(wo.z));
+}
+static float disney_clearcoat_term(const orc_material *m, v3 wi, v3 wo, v3 wh,
+                                   float idh, float gloss) {
+  if (m->clearcoat == 0.0f) return 0.0f;
+  float rough = lerp_d(0.005f, 0.1f, gloss);
+  float D = ggx_d(wh, rough);
+  if (D == 0.0f) return 0.0f;
+  float F = fresnel_schlick_coat(idh);
+  float G = ggx_g(wo, wi, wh, 0.25f);
+  return m->clearcoat * D * F * G / (4.0f * fabsf(wi.z) * fabsf(wo.z));
+}
+
+static v3 disney_eval(const bsdf *b, v3 wi, v3 wo) {
+  const orc_material *m = b->m;
+  v3 lwi = vnormalize(to_local(&b->sframe, wi));
+  v3 lwo = vnormalize(to_local(&b->sframe, wo));
+  if (fabsf(lwi.z) < FLT_EPSILON || fabsf(lwo.z) < FLT_EPSILON) return V3(0, 0, 0);
+  float rough = clampf_r(m->roughness, 0.02f, 1.f);
+  v3 alb = albedo(b);
+  float lum = luminance(alb);
+  v3 untinted = V3(lum, lum, lum);
+  v3 spec_albedo = lerp_v(lerp_v(alb, untinted, 1.0f - m->specular_tint), alb,
+                          m->metallic);
+  v3 sheen_albedo = lerp_v(untinted, alb, m->sheen_tint);
+  v3 wh = vnormalize(vadd(lwo, lwi));
+  float odh = vdot(lwo, wh);
+  float idh = vdot(lwi, wh);
+  float omo = 1.f - odh;
+  spec_albedo = lerp_v(spec_albedo, untinted, omo * omo * omo);
+  float F = fresnel_schlick(m, odh, 0.0f);
+  v3 sheen_term = vscale(sheen_albedo, F * m->sheen);
+  float dl = lerp_d(disney_diffuse_term(m, lwi, lwo, idh, rough),
+                    disney_subsurface_term(m, lwi, lwo, idh, rough),
+                    m->subsurface);
+  v3 base = vscale(vadd(vscale(alb, dl), sheen_term), 1.0f - m->metallic);
+  v3 spec = vscale(spec_albedo, disney_specular_term(m, lwi, lwo, wh, odh, rough));
+  float cc = disney_clearcoat_term(m, lwi, lwo, wh, idh, m->clearcoat_gloss);
+  return vadd(vadd(base, spec), V3(cc, cc, cc));
+}
+
+static float disney_pdf(const bsdf *b, v3 wi, v3 wo) {
+  const orc_material *m = b->m;
+  v3 lwi = vnormalize(to_local(&b->sframe, wi));
+  v3 lwo = vnormalize(to_local(&b->sframe, wo));
+  v3 wh = vnormalize(vadd(lwo, lwi));
+  if (vmaxc(wh) <= 0.f) return 0.f;
+  float rough = clampf_r(m->roughness, 0.02f, 1.f);
+  float mpdf = ggx_pdf_vn(lwo, wh, rough * rough);
+  float pdf = 0.f;
+  float dwhdwi = 1.f / (4.f * fabsf(vdot(lwi, wh)));
+  float spec_pdf = mpdf * dwhdwi;
+  float nr = std_lerp(0.f, 0.08f, m->specular);
+  float odh = vdot(lwo, wh);
+  float prob_spec = fresnel_schlick(m, odh, nr);
+  pdf += spec_pdf * prob_spec;
+  pdf += fabsf(lwi.z) * ORC_INV_PI * (1.f - prob_spec);
+  if (m->clearcoat > 0.f) {
+    float cw = m->clearcoat / (m->clearcoat + luminance(albedo(b)));
+    float fc = fresnel_schlick_coat(fabsf(wo.z)); /* world-space wo */
+    float pc = (fc * cw) / (fc * cw + (1 - fc) * (1 - cw));
+    float coat_rough = std_lerp(0.005f, 0.10f, m->clearcoat_gloss);
+    float chp = ggx_pdf_vn(wo, wh, coat_rough); /* world-space wo */
+    float coat_pdf = chp * dwhdwi;
+    pdf *= 1.0f - pc;
+    pdf += coat_pdf * pc;
+  }
+  return pdf;
+}
+
+static bsdf_sample disney_sample(const bsdf *b, v3 wo, v3 sam) {
+  const orc_material *m = b->m;
+  v3 lwo = vnormalize(to_local(&b->sframe, wo));
+  v3 lwi = V3(0, 0, 0), lwh;
+  int sample_coat = 0;
+  float rs[3] = {sam.x, sam.y, sam.z};
+  if (m->clearcoat > 0.f) {
+    float cw = m->clearcoat / (m->clearcoat + luminance(albedo(b)));
+    float fc = fresnel_schlick_coat(fabsf(lwo.z));
+    float pc = (fc * cw) / (fc * cw + (1.f - fc) * (1.f - cw));
+    if (sam.y < pc) {
+      sample_coat = 1;
+      rs[1] /= pc;
+      float coat_rough = std_lerp(0.005f, 0.1f, m->clearcoat_gloss);
+      float coat_pdf;
+      lwh = ggx_sample_vn(lwo, rs[0], rs[1], &coat_pdf, coat_rough);
+      lwi = reflect(vneg(lwo), lwh);
+    } else {
+      sample_coat = 0;
+      rs[1] = (sam.y - pc) / (1.f - pc);
+    }
+  }
+  if (!sample_coat) {
+    float mpdf;
+    float rough = clampf_r(m->roughness, 0.02f, 1.0f);
+    lwh = ggx_sample_vn(lwo, rs[0], rs[1], &mpdf, rough * rough);
+    float nr = std_lerp(0.0f, 0.08f, m->specular);
+    float odh = vdot(lwo, lwh);
+    float prob_spec = fresnel_schlick(m, odh, nr);
+    if (rs[2] <= prob_spec) {
+      lwi = reflect(vneg(lwo), lwh);
+    } else {
+      v2 s2 = {rs[0], rs[1]};
+      lwi = cosine_hemisphere(s2, NULL);
+      if (lwo.z < 0.0f) lwi.z *= -1.0f;
+    }
+  }
+  if (lwi.z <= 0.0f) return invalid_sample();
+  v3 wi = to_world(&b->sframe, lwi);
+  bsdf_sample r = {wi, disney_eval(b, wi, wo), disney_pdf(b, wi, wo), 1};
+  return r;
+}
+
+static bsdf_sample bsdf_sample_all(const bsdf *b, v3 wo, v3 sam) {
+  return b->m->type == ORC_MAT_DISNEY ? disney_sample(b, wo, sam)
+                                      : lambert_sample(b, wo, sam);
+}
+static v3 bsdf_eval_all(const bsdf *b, v3 wi, v3 wo) {
+  return b->m->type == ORC_MAT_DISNEY ? disney_eval(b, wi, wo)
+                                      : lambert_eval(b, wi, wo);
+}
+static float bsdf_pdf_all(const bsdf *b, v3 wi, v3 wo) {
+  return b->m->type == ORC_MAT_DISNEY ? disney_pdf(b, wi, wo)
+                                      : lambert_pdf(b, wi, wo);
+}
+
+/* ---- Distribution / Distribution2D (utility/distribution.h:34-149) ----- */
+typedef struct distrib {
+  uint32_t n;
+  float *func, *cdf;
+  float func_int;
+} distrib;
+
+static void distrib_init(distrib *d, const float *f, uint32_t n, int fp64) {
+  d->n = n;
+  d->func = (float *)malloc(sizeof(float) * n);
+  d->cdf = (float *)malloc(sizeof(float) * (n + 1));
+  memcpy(d->func, f, sizeof(float) * n);
+  float inv = 1.f / (float)n;
+  d->cdf[0] = 0.f;
+  if (fp64) {
+    double acc = 0.0;
+    for (uint32_t i = 1; i <= n; ++i) {
+      acc += (double)d->func[i - 1] * (double)inv;
+      d->cdf[i] = (float)acc;
+    }
+  } else {
+    for (uint32_t i = 1; i <= n; ++i) d->cdf[i] = d->cdf[i - 1] + d->func[i - 1] * inv;
+  }
+  d->func_int = d->cdf[n];
+  if (d->func_int > 0.f) {
+    for (uint32_t i = 0; i <= n; ++i) d->cdf[i] /= d->func_int;
+  } else {
+    for (uint32_t i = 0; i <= n; ++i) d->cdf[i] = (float)i / (float)n;
+  }
+}
+static void distrib_free(distrib *d) {
+  free(d->func);
+  free(d->cdf);
+}
+/* first index k in [0, n] with cdf[k] >= u (upper_bound with '<='), or n+1 */
+static uint32_t first_ge(const float *cdf, uint32_t count, float u) {
+  uint32_t lo = 0, len = count;
+  while (len > 0) {
+    uint32_t half = len / 2;
+    if (!(u <= cdf[lo + half])) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo;
+}
+static float distrib_sample_cont(const distrib *d, float u, float *pdf, int *off) {
+  int k = (int)first_ge(d->cdf, d->n + 1, u);
+  int offset = clampi(k - 1, 0, (int)d->n - 1);
+  if (off) *off = offset;
+  float du = u - d->cdf[offset];
+  if ((d->cdf[offset + 1] - d->cdf[offset]) > 0) du /= (d->cdf[offset + 1] - d->cdf[offset]);
+  if (pdf) *pdf = (d->func_int > 0) ? d->func[offset] / d->func_int : 0;
+  return (offset + du) / (float)d->n;
+}
+
+/* ---- scene -------------------------------------------------------------- */
+typedef struct light_rt {
+  uint32_t type;
+  v3 dir, radiance;
+  frame fr;
+  float angle_cos;
+  /* env */
+  uint32_t w, h;
+  const float *tex;
+  float rot, scaler;
+  distrib *cond; /* h rows */
+  distrib marg;
+  box3 aabb;
+} light_rt;
+
+struct orc_scene {
+  uint32_t n_tri;
+  float *tri;
+  orc_node *nodes;
+  uint32_t *perm;
+  uint32_t *tri_prim, *tri_local;
+  orc_primitive *prims;
+  uint32_t n_prims;
+  orc_material *mats;
+  uint32_t n_mats;
+  light_rt *lights;
+  uint32_t n_lights;
+  int env_index;
+  box3 world;
+};
+
+/* HDRTexture::LinearSampleImpl (hdrtex.cpp:60-89) with Image clamping
+ * (image.h:38-48). */
+static v3 tex_texel(const light_rt *l, int x, int y) {
+  x = clampi(x, 0, (int)l->w - 1);
+  y = clampi(y, 0, (int)l->h - 1);
+  return vload(l->tex + 3 * ((size_t)x + (size_t)y * l->w));
+}
+static v3 tex_sample(const light_rt *l, float u, float v) {
+  float dimx = (float)l->w, dimy = (float)l->h;
+  float cx = u * dimx - 0.5f, cy = v * dimy - 0.5f;
+  cx = clampf_r(cx, 0.f, dimx - 1.f);
+  cy = clampf_r(cy, 0.f, dimy - 1.f);
+  float x0 = floorf(cx), y0 = floorf(cy), x1 = ceilf(cx), y1 = ceilf(cy);
+  float tx = cx - x0, ty = cy - y0;
+  v3 u0v0 = tex_texel(l, (int)x0, (int)y0);
+  v3 u1v1 = tex_texel(l, (int)x1, (int)y1);
+  v3 u0v1 = tex_texel(l, (int)x0, (int)y1);
+  v3 u1v0 = tex_texel(l, (int)x1, (int)y0);
+  return vadd(vscale(vadd(vscale(u0v0, 1 - tx), vscale(u1v0, tx)), 1 - ty),
+              vscale(vadd(vscale(u0v1, 1 - tx), vscale(u1v1, tx)), ty));
+}
+
+static const float kInv2Pi = 1.f / (2.f * ORC_PI);
+static const float kTwoPi = 2.f * ORC_PI;
+
+static void env_uv(const light_rt *l, v3 d, float *uo, float *vo, float *theta_o) {
+  v3 dir = vnormalize(d);
+  float theta = acosf(clampf_r(dir.y, -1.f, 1.f));
+  float u = (ORC_PI + atan2f(dir.z, dir.x)) * kInv2Pi;
+  float v = theta * ORC_INV_PI;
+  u += l->rot / 180 * ORC_PI * kInv2Pi;
+  v += 0.f;
+  if (u > 1.f) u -= 1.f;
+  if (u < 0.f) u += 1.f;
+  *uo = u;
+  *vo = v;
+  *theta_o = theta;
+}
+/* EnvLight::Radiance (env_light.cpp:50-66) */
+static v3 env_radiance(const light_rt *l, v3 d) {
+  float u, v, th;
+  env_uv(l, d, &u, &v, &th);
+  return vscale(tex_sample(l, u, v), l->scaler);
+}
+/* Distribution2D::Pdf (distribution.h:136-146) */
+static float distrib2d_pdf(const light_rt *l, float u, float v) {
+  int iu = clampi((int)(u * (float)l->w), 0, (int)l->w - 1);
+  int iv = clampi((int)(v * (float)l->h), 0, (int)l->h - 1);
+  if (l->cond[iv].func_int * l->marg.func_int == 0.0f) return 0.f;
+  return (l->cond[iv].func[iu] * l->marg.func[iv]) /
+         (l->cond[iv].func_int * l->marg.func_int);
+}
+/* EnvLight::Pdf (env_light.cpp:152-170) */
+static float env_pdf(const light_rt *l, v3 d) {
+  float u, v, th;
+  env_uv(l, d, &u, &v, &th);
+  float sin_t = sinf(th);
+  return distrib2d_pdf(l, u, v) / (2.f * ORC_PI * ORC_PI * sin_t);
+}
+
+typedef struct light_sample {
+  v3 pos, radiance;
+  float pdf;
+} light_sample;
+
+/* EnvLight::Sample (env_light.cpp:68-103) from two uniform numbers (u0 is
+ * the number passed as SampleContinuous' first argument). */
+static light_sample env_sample_uv(const light_rt *l, v3 ref, float u0, float u1,
+                                  v3 *dir_out) {
+  light_sample r = {{0, 0, 0}, {0, 0, 0}, 0.f};
+  float pdfs[2];
+  int iv;
+  float sv = distrib_sample_cont(&l->marg, u1, &pdfs[1], &iv);
+  float su = distrib_sample_cont(&l->cond[iv], u0, &pdfs[0], NULL);
+  float pdf = pdfs[0] * pdfs[1];
+  if (dir_out) *dir_out = V3(0, 0, 0);
+  if (pdf == 0.f) return r;
+  float u = su, v = sv;
+  u -= l->rot / 180 * ORC_PI * kInv2Pi;
+  if (u > 1.f) u -= 1.f;
+  if (u < 0.f) u += 1.f;
+  float phi = u * kTwoPi;
+  float theta = v * ORC_PI;
+  float sin_t = sinf(theta);
+  pdf = sin_t != 0.f ? pdf / (2.f * ORC_PI * ORC_PI * sin_t) : 0.f;
+  v3 dir = V3(-sinf(theta) * cosf(phi), cosf(theta), -sinf(theta) * sinf(phi));
+  float radius = vlength(vsub(l->aabb.mx, box_center(l->aabb)));
+  r.pos = vadd(ref, vscale(dir, 2.f * radius));
+  r.radiance = tex_sample(l, u, v);
+  r.pdf = pdf;
+  if (dir_out) *dir_out = dir;
+  return r;
+}
+
+/* DirectionalLight::Sample (directional_light.cpp:43-58) */
+static light_sample dir_sample(const light_rt *l, v3 ref, v2 u) {
+  light_sample r;
+  v3 ref2light = to_world(&l->fr, uniform_cone(u, l->angle_cos));
+  float pdf = uniform_cone_pdf(l->angle_cos);
+  float radius = vlength(vsub(l->aabb.mx, box_center(l->aabb)));
+  r.pos = vadd(ref, vscale(ref2light, 2.f * radius));
+  r.radiance = l->radiance;
+  r.pdf = pdf;
+  return r;
+}
+
+static int v_is_point(const orc_mesh_view *mv) { (void)mv; return 0; }
+
+orc_scene *orc_scene_create(const orc_primitive *prims, uint32_t n_prims,
+                            const orc_material *mats, uint32_t n_mats,
+                            const orc_light *lights, uint32_t n_lights,
+                            uint32_t search_radius, int32_t env_cdf_fp64) {
+  (void)v_is_point;
+  orc_scene *s = (orc_scene *)calloc(1, sizeof(orc_scene));
+  orc_mesh_view *views = (orc_mesh_view *)malloc(sizeof(orc_mesh_view) * (n_prims ? n_prims : 1));
+  for (uint32_t i = 0; i < n_prims; ++i) views[i] = prims[i].view;
+  float box[6];
+  s->n_tri = orc_flatten(views, n_prims, NULL, box);
+  s->tri = (float *)malloc(sizeof(float) * 9 * (s->n_tri ? s->n_tri : 1));
+  orc_flatten(views, n_prims, s->tri, box);
+  free(views);
+  s->world.mn = vload(box);
+  s->world.mx = vload(box + 3);
+  s->tri_prim = (uint32_t *)malloc(sizeof(uint32_t) * (s->n_tri ? s->n_tri : 1));
+  s->tri_local = (uint32_t *)malloc(sizeof(uint32_t) * (s->n_tri ? s->n_tri : 1));
+  uint32_t t = 0;
+  for (uint32_t p = 0; p < n_prims; ++p)
+    for (size_t i = 0; i < prims[p].view.index_size; ++i, ++t) {
+      s->tri_prim[t] = p;
+      s->tri_local[t] = (uint32_t)i;
+    }
+  if (s->n_tri > 0) {
+    s->nodes = (orc_node *)malloc(sizeof(orc_node) * (2 * s->n_tri - 1));
+    s->perm = (uint32_t *)malloc(sizeof(uint32_t) * s->n_tri);
+    if (orc_ploc_build(s->tri, s->n_tri, box, search_radius, s->nodes, s->perm) < 0) {
+      orc_scene_destroy(s);
+      return NULL;
+    }
+  }
+  s->prims = (orc_primitive *)malloc(sizeof(orc_primitive) * (n_prims ? n_prims : 1));
+  memcpy(s->prims, prims, sizeof(orc_primitive) * n_prims);
+  s->n_prims = n_prims;
+  s->mats = (orc_material *)malloc(sizeof(orc_material) * (n_mats ? n_mats : 1));
+  memcpy(s->mats, mats, sizeof(orc_material) * n_mats);
+  s->n_mats = n_mats;
+  s->n_lights = n_lights;
+  s->lights = (light_rt *)calloc(n_lights ? n_lights : 1, sizeof(light_rt));
+  s->env_index = -1;
+  for (uint32_t i = 0; i < n_lights; ++i) {
+    light_rt *l = &s->lights[i];
+    l->type = lights[i].type;
+    l->aabb = s->world; /* Light::Process(world_bound) (default_scene.cpp:47) */
+    if (l->type == ORC_LIGHT_DIRECTIONAL) {
+      l->dir = vnormalize(vload(lights[i].dir));
+      l->fr = make_frame(l->dir);
+      l->angle_cos = cosf(lights[i].angle);
+      l->radiance = vload(lights[i].radiance);
+    } else {
+      s->env_index = (int)i;
+      l->w = lights[i].env_w;
+      l->h = lights[i].env_h;
+      l->tex = lights[i].env_texels;
+      l->rot = lights[i].rot;
+      l->scaler = lights[i].scaler;
+      /* EnvLight ctor (env_light.cpp:29-48) */
+      float *wts = (float *)malloc(sizeof(float) * l->w * l->h);
+      for (uint32_t y = 0; y < l->h; ++y) {
+        float v = ((float)y + 0.5f) / (float)l->h;
+        float sin_t = sinf(ORC_PI * ((float)y + 0.5f) / (float)l->h);
+        for (uint32_t x = 0; x < l->w; ++x) {
+          float u = ((float)x + 0.5f) / (float)l->w;
+          wts[x + l->w * y] = luminance(tex_sample(l, u, v)) * sin_t;
+        }
+      }
+      l->cond = (distrib *)malloc(sizeof(distrib) * l->h);
+      float *mf = (float *)malloc(sizeof(float) * l->h);
+      for (uint32_t y = 0; y < l->h; ++y) {
+        distrib_init(&l->cond[y], wts + (size_t)y * l->w, l->w, env_cdf_fp64);
+        mf[y] = l->cond[y].func_int;
+      }
+      distrib_init(&l->marg, mf, l->h, env_cdf_fp64);
+      free(mf);
+      free(wts);
+    }
+  }
+  return s;
+}
+
+void orc_scene_destroy(orc_scene *s) {
+  if (!s) return;
+  for (uint32_t i = 0; i < s->n_lights; ++i) {
+    light_rt *l = &s->lights[i];
+    if (l->type == ORC_LIGHT_ENV && l->cond) {
+      for (uint32_t y = 0; y < l->h; ++y) distrib_free(&l->cond[y]);
+      free(l->cond);
+      distrib_free(&l->marg);
+    }
+  }
+  free(s->lights);
+  free(s->tri);
+  free(s->nodes);
+  free(s->perm);
+  free(s->tri_prim);
+  free(s->tri_local);
+  free(s->prims);
+  free(s->mats);
+  free(s);
+}
+uint32_t orc_scene_tri_count(const orc_scene *s) { return s->n_tri; }
+const orc_node *orc_scene_nodes(const orc_scene *s) { return s->nodes; }
+const uint32_t *orc_scene_perm(const orc_scene *s) { return s->perm; }
+const float *orc_scene_tris(const orc_scene *s) { return s->tri; }
+void orc_scene_world_bound(const orc_scene *s, float *b) {
+  vstore(b, s->world.mn);
+  vstore(b + 3, s->world.mx);
+}
+
+int orc_scene_env_tables(const orc_scene *s, uint32_t li, float *cond_cdf,
+                         float *cond_func_int, float *marg_cdf,
+                         float *marg_func_int) {
+  if (li >= s->n_lights || s->lights[li].type != ORC_LIGHT_ENV) return -1;
+  const light_rt *l = &s->lights[li];
+  for (uint32_t y = 0; y < l->h; ++y) {
+    memcpy(cond_cdf + (size_t)y * (l->w + 1), l->cond[y].cdf, sizeof(float) * (l->w + 1));
+    cond_func_int[y] = l->cond[y].func_int;
+  }
+  memcpy(marg_cdf, l->marg.cdf, sizeof(float) * (l->h + 1));
+  *marg_func_int = l->marg.func_int;
+  return 0;
+}
+void orc_env_sample(const orc_scene *s, uint32_t li, float u0, float u1,
+                    float *dir3, float *pdf, float *rgb) {
+  v3 dir;
+  light_sample ls = env_sample_uv(&s->lights[li], V3(0, 0, 0), u0, u1, &dir);
+  vstore(dir3, dir);
+  *pdf = ls.pdf;
+  vstore(rgb, ls.radiance);
+}
+float orc_env_pdf(const orc_scene *s, uint32_t li, const float *d) {
+  return env_pdf(&s->lights[li], vload(d));
+}
+void orc_env_radiance(const orc_scene *s, uint32_t li, const float *d, float *rgb) {
+  vstore(rgb, env_radiance(&s->lights[li], vload(d)));
+}
+
+void orc_bsdf_sample(const orc_material *m, const float *ng, const float *ns,
+                     const float *wo, const float *sam3, float *wi, float *f,
+                     float *pdf, int *valid) {
+  bsdf b = make_bsdf(m, vload(ng), vload(ns));
+  bsdf_sample r = bsdf_sample_all(&b, vload(wo), vload(sam3));
+  vstore(wi, r.dir);
+  vstore(f, r.f);
+  *pdf = r.pdf;
+  *valid = r.valid;
+}
+void orc_bsdf_eval(const orc_material *m, const float *ng, const float *ns,
+                   const float *wi, const float *wo, float *f) {
+  bsdf b = make_bsdf(m, vload(ng), vload(ns));
+  vstore(f, bsdf_eval_all(&b, vload(wi), vload(wo)));
+}
+float orc_bsdf_pdf(const orc_material *m, const float *ng, const float *ns,
+                   const float *wi, const float *wo) {
+  bsdf b = make_bsdf(m, vload(ng), vload(ns));
+  return bsdf_pdf_all(&b, vload(wi), vload(wo));
+}
+void orc_square_to_disk(const float *s2, float *out2) {
+  v2 s = {s2[0], s2[1]};
+  v2 r = disk_concentric(s);
+  out2[0] = r.x;
+  out2[1] = r.y;
+}
+void orc_cosine_hemisphere(const float *s2, float *out3, float *pdf) {
+  v2 s = {s2[0], s2[1]};
+  vstore(out3, cosine_hemisphere(s, pdf));
+}
+void orc_frame(const float *n, float *dx, float *dy) {
+  frame f = make_frame(vload(n));
+  vstore(dx, f.dx);
+  vstore(dy, f.dy);
+}
+
+/* ---- camera (thin_lens.cpp:32-83, matrix4.h:119-157) ------------------- */
+static v3 xform_point(const float *m, v3 p) {
+  float in[4] = {p.x, p.y, p.z, 1.f}, out[4];
+  for (int r = 0; r < 4; ++r) {
+    float acc = 0.f;
+    for (int c = 0; c < 4; ++c) acc += m[c * 4 + r] * in[c];
+    out[r] = acc;
+  }
+  return vdivs(V3(out[0], out[1], out[2]), out[3]);
+}
+static v3 xform_vector(const float *m, v3 p) {
+  float in[4] = {p.x, p.y, p.z, 0.f}, out[3];
+  for (int r = 0; r < 3; ++r) {
+    float acc = 0.f;
+    for (int c = 0; c < 4; ++c) acc += m[c * 4 + r] * in[c];
+    out[r] = acc;
+  }
+  return V3(out[0], out[1], out[2]);
+}
+/* Ray constructor (ray.h:33-44) */
+static orc_ray make_ray(v3 o, v3 d, float tmin, float tmax) {
+  orc_ray r;
+  v3 dn = vnormalize(d);
+  vstore(r.o, vadd(o, vscale(dn, ORC_EPS)));
+  vstore(r.d, vnormalize(d));
+  r.tmin = tmin;
+  r.tmax = tmax;
+  return r;
+}
+void orc_camera_ray(const orc_camera *cam, float rx, float ry,
+                    const float *lens2, orc_ray *out) {
+  v3 cc = xform_point(cam->raster2camera, V3(rx, ry, 1.f));
+  v3 o = V3(0.f, 0.f, 0.f);
+  v3 d = vnormalize(cc);
+  if (cam->lens_radius > 0 && cam->focal_distance > 0) {
+    v2 s = {lens2[0], lens2[1]};
+    v2 lens = disk_concentric(s);
+    lens.x *= cam->lens_radius;
+    lens.y *= cam->lens_radius;
+    float ft = cam->focal_distance / d.z;
+    v3 focus = vadd(o, vscale(d, ft));
+    o = V3(lens.x, lens.y, 0.f);
+    d = vnormalize(vsub(focus, o));
+  }
+  *out = make_ray(xform_point(cam->camera2world, o),
+                  xform_vector(cam->camera2world, d), ORC_EPS, INFINITY);
+}
+
+static void inv4d(const double *m, double *inv) {
+  /* cofactor inverse of a column-major 4x4 */
+  double a[16];
+  for (int i = 0; i < 16; ++i) a[i] = m[i];
+  inv[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] +
+           a[9] * a[7] * a[14] + a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
+  inv[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] -
+           a[8] * a[7] * a[14] - a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
+  inv[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] +
+           a[8] * a[7] * a[13] + a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
+  inv[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] -
+            a[8] * a[6] * a[13] - a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
+  inv[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] -
+           a[9] * a[3] * a[14] - a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
+  inv[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] +
+           a[8] * a[3] * a[14] + a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
+  inv[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] -
+           a[8] * a[3] * a[13] - a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
+  inv[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] +
+            a[8] * a[2] * a[13] + a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
+  inv[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] +
+           a[5] * a[3] * a[14] + a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
+  inv[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] -
+           a[4] * a[3] * a[14] - a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
+  inv[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] +
+            a[4] * a[3] * a[13] + a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
+  inv[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] -
+            a[4] * a[2] * a[13] - a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
+  inv[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] -
+           a[5] * a[3] * a[10] - a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
+  inv[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] +
+           a[4] * a[3] * a[10] + a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
+  inv[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] -
+            a[4] * a[3] * a[9] - a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
+  inv[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] +
+            a[4] * a[2] * a[9] + a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
+  double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+  for (int i = 0; i < 16; ++i) inv[i] /= det;
+}
+static void mul4d(const double *a, const double *b, double *o) {
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      double acc = 0;
+      for (int k = 0; k < 4; ++k) acc += a[k * 4 + r] * b[c * 4 + k];
+      o[c * 4 + r] = acc;
+    }
+}
+static void norm3d(double *v) {
+  double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= l;
+  v[1] /= l;
+  v[2] /= l;
+}
+static void cross3d(const double *a, const double *b, double *o) {
+  o[0] = a[1] * b[2] - b[1] * a[2];
+  o[1] = a[2] * b[0] - b[2] * a[0];
+  o[2] = a[0] * b[1] - b[0] * a[1];
+}
+/* Camera matrices in double, rounded to float: camera2world = from({right,
+ * realUp, backward}, eye) (matrix4.h:145-153); camera2screen =
+ * perspectiveProjection(fov, aspect, 1, 1000) (matrix4.h:155-170);
+ * screen2raster as thin_lens.cpp:43-46. */
+void orc_camera_setup(orc_camera *cam, const float *pos, const float *look_at,
+                      const float *up, float fov_deg, float res_x, float res_y,
+                      float lens_radius, float focal_distance) {
+  double eye[3] = {pos[0], pos[1], pos[2]};
+  double back[3] = {pos[0] - (double)look_at[0], pos[1] - (double)look_at[1],
+                    pos[2] - (double)look_at[2]};
+  norm3d(back);
+  double upd[3] = {up[0], up[1], up[2]}, right[3], rup[3];
+  cross3d(upd, back, right);
+  norm3d(right);
+  cross3d(back, right, rup);
+  norm3d(rup);
+  double c2w[16] = {right[0], right[1], right[2], 0, rup[0], rup[1], rup[2], 0,
+                    back[0],  back[1],  back[2],  0, eye[0], eye[1], eye[2], 1};
+  double fov = (double)fov_deg * 3.14159265358979323846 / 180.0;
+  double aspect = (double)res_x / (double)res_y;
+  double t = tan(fov * 0.5);
+  double sx = 2.0 / (2.0 * t * aspect), sy = -(2.0 / (2.0 * t));
+  double zs = 1.0 / (1000.0 - 1.0), m22 = 1.0 * zs, m32 = m22 * 1000.0;
+  double c2s[16] = {sx, 0, 0, 0, 0, sy, 0, 0, 0, 0, m22, -1, 0, 0, m32, 0};
+  /* screen2raster = S(res) * S(0.5,-0.5,1) * T(1,-1,0) */
+  double s2r[16] = {res_x * 0.5, 0, 0, 0, 0, res_y * -0.5, 0, 0, 0, 0, 1, 0,
+                    res_x * 0.5 * 1.0, res_y * -0.5 * -1.0, 0, 1};
+  double c2s_inv[16], s2r_inv[16], r2c[16];
+  inv4d(c2s, c2s_inv);
+  inv4d(s2r, s2r_inv);
+  mul4d(c2s_inv, s2r_inv, r2c);
+  for (int i = 0; i < 16; ++i) {
+    cam->raster2camera[i] = (float)r2c[i];
+    cam->camera2world[i] = (float)c2w[i];
+  }
+  cam->resolution[0] = res_x;
+  cam->resolution[1] = res_y;
+  cam->lens_radius = lens_radius;
+  cam->focal_distance = focal_distance;
+}
+
+/* ---- TraceMIS (pt.cpp:98-227) ------------------------------------------ */
+typedef struct sampler {
+  orc_pcg32 rng;
+} sampler;
+static inline float next1(sampler *s) { return orc_pcg32_next_float(&s->rng); }
+
+typedef struct inct {
+  v3 pos, ng, ns, wr;
+} inct;
+
+static int scene_intersect(const orc_scene *s, const orc_ray *r, orc_hit *h,
+                           uint64_t *q) {
+  (*q)++;
+  if (s->n_tri == 0) {
+    h->t = INFINITY;
+    h->id = 0xFFFFFFFFu;
+    return 0;
+  }
+  orc_trace_closest(s->nodes, s->perm, s->tri, r, 1, h);
+  return h->id != 0xFFFFFFFFu;
+}
+static int scene_occlude(const orc_scene *s, const orc_ray *r, uint64_t *q) {
+  uint8_t o = 0;
+  (*q)++;
+  if (s->n_tri == 0) return 0;
+  orc_trace_any(s->nodes, s->perm, s->tri, r, 1, &o);
+  return o;
+}
+
+static inline uint32_t idx_word(const orc_mesh_view *mv, uint32_t local, int k) {
+  uint32_t v;
+  memcpy(&v, (const char *)mv->index_buffer + mv->index_offset +
+                 (size_t)local * mv->index_stride + 4 * k, 4);
+  return v;
+}
+static inline v3 view_pos(const orc_mesh_view *mv, uint32_t i) {
+  float f[3];
+  memcpy(f, (const char *)mv->position_buffer + mv->position_offset +
+                (size_t)i * mv->position_stride, 12);
+  return V3(f[0], f[1], f[2]);
+}
+
+/* MT normal + PostIntersect (ploc_aggregate.cpp:199-211; quad.cpp:73-90,
+ * cube.cpp:96-108, mesh.cpp:183-204, two_sided.cpp:44-53). */
+static inct post_intersect(const orc_scene *s, const orc_ray *r, const orc_hit *h) {
+  inct it;
+  v3 o = vload(r->o), d = vload(r->d);
+  const float *p = s->tri + 9 * (size_t)h->id;
+  v3 a = vload(p), b = vload(p + 3), c = vload(p + 6);
+  v3 n = vnormalize(vcross(vsub(b, a), vsub(c, a)));
+  const orc_primitive *pr = &s->prims[s->tri_prim[h->id]];
+  uint32_t local = s->tri_local[h->id];
+  it.pos = vadd(o, vscale(d, h->t));
+  it.wr = vneg(d);
+  if (pr->kind == ORC_GEOM_QUAD) {
+    it.ng = n;
+    it.ns = n;
+  } else {
+    uint32_t i, j, k;
+    if (pr->kind == ORC_GEOM_CUBE) { /* std::tuple stored in reverse order */
+      i = idx_word(&pr->view, local, 2);
+      j = idx_word(&pr->view, local, 1);
+      k = idx_word(&pr->view, local, 0);
+    } else {
+      i = idx_word(&pr->view, local, 0);
+      j = idx_word(&pr->view, local, 1);
+      k = idx_word(&pr->view, local, 2);
+    }
+    v3 pi = view_pos(&pr->view, i), pj = view_pos(&pr->view, j),
+       pk = view_pos(&pr->view, k);
+    it.ng = vnormalize(vcross(vsub(pj, pi), vsub(pk, pi)));
+    if (pr->kind == ORC_GEOM_MESH) {
+      const float *N = pr->normal_and_vs;
+      float w = 1 - h->u - h->v;
+      it.ns = vadd(vadd(vscale(vload(N + 4 * (size_t)i), w),
+                        vscale(vload(N + 4 * (size_t)j), h->u)),
+                   vscale(vload(N + 4 * (size_t)k), h->v));
+    } else {
+      it.ns = it.ng;
+    }
+  }
+  if (pr->two_sided) {
+    if (vdot(it.ng, it.wr) < 0) {
+      it.ng = vscale(it.ng, -1.f);
+      it.ns = vscale(it.ns, -1.f);
+    }
+  }
+  return it;
+}
+
+static v3 eps_offset(const inct *it, v3 dir) { /* intersection.h:42-47 */
+  if (vdot(dir, it->ng) > 0.f) return vadd(it->pos, vscale(it->ng, ORC_EPS));
+  return vsub(it->pos, vscale(it->ng, ORC_EPS));
+}
+
+/* MISSampleLight / MISSampleEnvLight (helper.h:73-133) */
+static v3 mis_sample_light(const orc_scene *s, const light_rt *l, const inct *it,
+                           const bsdf *b, sampler *smp, uint64_t *q) {
+  light_sample ls;
+  v3 zero = V3(0, 0, 0);
+  if (l->type == ORC_LIGHT_ENV) {
+    float first = next1(smp);
+    float second = next1(smp);
+    ls = env_sample_uv(l, it->pos, second, first, NULL);
+  } else {
+    v2 u;
+    u.x = next1(smp);
+    u.y = next1(smp);
+    ls = dir_sample(l, it->pos, u);
+  }
+  if (is_black(ls.radiance) || ls.pdf <= 0.f) return zero;
+  const float len = vlength(vsub(ls.pos, it->pos)) - ORC_EPS;
+  if (len < ORC_EPS) return zero;
+  const v3 to_light = vnormalize(vsub(ls.pos, it->pos));
+  orc_ray sr = make_ray(it->pos, to_light, ORC_EPS, 0.99f * len);
+  if (scene_occlude(s, &sr, q)) return zero;
+  v3 bf = bsdf_eval_all(b, to_light, it->wr);
+  if (is_black(bf)) return zero;
+  v3 normal = vnormalize(it->ns);
+  v3 f = vscale(vmul(ls.radiance, bf), fabsf(vdot(to_light, normal)));
+  float bpdf = bsdf_pdf_all(b, to_light, it->wr);
+  return vscale(vdivs(f, ls.pdf), power_heuristic(ls.pdf, bpdf));
+}
+
+/* MISSampleBsdf (helper.h:209-265) */
+static v3 mis_sample_bsdf(const orc_scene *s, const inct *it, const bsdf *b,
+                          sampler *smp, uint64_t *q) {
+  v3 zero = V3(0, 0, 0);
+  v3 sam;
+  sam.x = next1(smp);
+  sam.y = next1(smp);
+  sam.z = next1(smp);
+  bsdf_sample bs = bsdf_sample_all(b, it->wr, sam);
+  if (!bs.valid) bs.f = zero, bs.pdf = 0.f;
+  if (is_black(bs.f) || bs.pdf == 0.f) return zero;
+  bs.dir = vnormalize(bs.dir);
+  orc_ray r = make_ray(eps_offset(it, bs.dir), bs.dir, ORC_EPS, INFINITY);
+  orc_hit h;
+  int hit = scene_intersect(s, &r, &h, q);
+  if (!hit) {
+    v3 env = zero;
+    if (s->env_index >= 0) {
+      const light_rt *l = &s->lights[s->env_index];
+      v3 d = vload(r.d);
+      v3 le = env_radiance(l, d);
+      if (is_black(le)) return zero;
+      v3 f = vscale(vmul(le, bs.f), fabsf(vdot(it->ns, d)));
+      float lpdf = env_pdf(l, d);
+      env = vadd(env, vscale(vdivs(f, bs.pdf), power_heuristic(bs.pdf, lpdf)));
+    }
+    return env;
+  }
+  return zero; /* no emissive primitives */
+}
+
+typedef struct path_stats {
+  uint64_t queries;
+} path_stats;
+
+static v3 trace_mis(const orc_scene *s, const orc_pt_args *a, orc_ray ray,
+                    sampler *smp, uint64_t *q) {
+  v3 L = V3(0, 0, 0), beta = V3(1.f, 1.f, 1.f);
+  for (int bounce = 0; bounce < a->max_bounces; bounce++) {
+    if (bounce > a->min_bounces) {
+      if (next1(smp) > a->cont_prob) return L;
+      beta = vdivs(beta, a->cont_prob);
+    }
+    orc_hit h;
+    if (!scene_intersect(s, &ray, &h, q)) {
+      if (bounce == 0 && s->env_index >= 0)
+        L = vadd(L, env_radiance(&s->lights[s->env_index], vload(ray.d)));
+      return L;
+    }
+    inct it = post_intersect(s, &ray, &h);
+    uint32_t mi = s->prims[s->tri_prim[h.id]].material;
+    bsdf b = make_bsdf(&s->mats[mi], it.ng, it.ns);
+    v3 direct = V3(0, 0, 0);
+    for (uint32_t li = 0; li < s->n_lights; ++li)
+      direct = vadd(direct, vmul(beta, mis_sample_light(s, &s->lights[li], &it, &b, smp, q)));
+    direct = vadd(direct, vmul(beta, mis_sample_bsdf(s, &it, &b, smp, q)));
+    L = vadd(L, vscale(direct, 1.f));
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    bsdf_sample bs = bsdf_sample_all(&b, it.wr, sam);
+    if (!bs.valid) bs.f = V3(0, 0, 0), bs.pdf = 0.f;
+    if (is_black(bs.f) || bs.pdf < 0.f) return L;
+    beta = vmul(beta, vdivs(vscale(bs.f, fabsf(vdot(it.ng, bs.dir))), bs.pdf));
+    ray = make_ray(eps_offset(&it, bs.dir), bs.dir, ORC_EPS, INFINITY);
+  }
+  return L;
+}
+
+static void render_pixel(const orc_scene *s, const orc_camera *cam,
+                         const orc_pt_args *a, uint32_t x, uint32_t y,
+                         uint32_t W, uint32_t Wp, double *sum, double *sum_sq,
+                         uint32_t *count, float *per_sample, uint64_t *q) {
+  sampler smp;
+  uint32_t p = x + y * Wp;
+  if (a->seed_mode == 0) orc_pcg32_seed(&smp.rng, (uint64_t)(int64_t)(int)(x + y * W));
+  double acc[3] = {0, 0, 0}, acc2[3] = {0, 0, 0};
+  for (int si = 0; si < a->spp; ++si) {
+    if (a->seed_mode != 0) orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, p, (uint32_t)si));
+    const float u = (float)x + next1(&smp);
+    const float v = (float)y + next1(&smp);
+    float lens[2];
+    lens[0] = next1(&smp);
+    lens[1] = next1(&smp);
+    orc_ray r;
+    orc_camera_ray(cam, u, v, lens, &r);
+    v3 L = trace_mis(s, a, r, &smp, q);
+    if (per_sample) vstore(per_sample + 3 * ((size_t)p * a->spp + si), L);
+    acc[0] += L.x;
+    acc[1] += L.y;
+    acc[2] += L.z;
+    acc2[0] += (double)L.x * L.x;
+    acc2[1] += (double)L.y * L.y;
+    acc2[2] += (double)L.z * L.z;
+  }
+  for (int k = 0; k < 3; ++k) {
+    sum[3 * (size_t)p + k] = acc[k];
+    sum_sq[3 * (size_t)p + k] = acc2[k];
+  }
+  count[p] = (uint32_t)a->spp;
+}
+
+static void traced_dims(const orc_camera *cam, const orc_pt_args *a, uint32_t *W,
+                        uint32_t *H, uint32_t *Wp, uint32_t *Hp) {
+  *W = (uint32_t)(int)cam->resolution[0];
+  *H = (uint32_t)(int)cam->resolution[1];
+  uint32_t ts = (uint32_t)a->tile_size;
+  *Wp = (*W + ts - 1) / ts * ts;
+  *Hp = (*H + ts - 1) / ts * ts;
+}
+
+uint64_t orc_render(const orc_scene *s, const orc_camera *cam,
+                    const orc_pt_args *a, uint32_t pix_begin, uint32_t pix_end,
+                    double *sum, double *sum_sq, uint32_t *count,
+                    float *per_sample) {
+  uint32_t W, H, Wp, Hp;
+  traced_dims(cam, a, &W, &H, &Wp, &Hp);
+  uint64_t q = 0;
+  uint32_t n = Wp * Hp;
+  if (pix_end > n) pix_end = n;
+  for (uint32_t p = pix_begin; p < pix_end; ++p)
+    render_pixel(s, cam, a, p % Wp, p / Wp, W, Wp, sum, sum_sq, count, per_sample, &q);
+  return q;
+}
+
+typedef struct mt_job {
+  const orc_scene *s;
+  const orc_camera *cam;
+  const orc_pt_args *a;
+  uint32_t begin, end, W, Wp;
+  double *sum, *sum_sq;
+  uint32_t *count;
+  uint32_t next; /* guarded by the atomic builtins */
+  uint64_t queries;
+} mt_job;
+
+static void *mt_worker(void *arg) {
+  mt_job *j = (mt_job *)arg;
+  uint64_t q = 0;
+  for (;;) {
+    uint32_t p0 = __atomic_fetch_add(&j->next, 64u, __ATOMIC_RELAXED);
+    if (p0 >= j->end) break;
+    uint32_t p1 = p0 + 64 < j->end ? p0 + 64 : j->end;
+    for (uint32_t p = p0; p < p1; ++p)
+      render_pixel(j->s, j->cam, j->a, p % j->Wp, p / j->Wp, j->W, j->Wp, j->sum,
+                   j->sum_sq, j->count, NULL, &q);
+  }
+  __atomic_fetch_add(&j->queries, q, __ATOMIC_RELAXED);
+  return NULL;
+}
+
+uint64_t orc_render_mt(const orc_scene *s, const orc_camera *cam,
+                       const orc_pt_args *a, uint32_t pix_begin,
+                       uint32_t pix_end, double *sum, double *sum_sq,
+                       uint32_t *count, int n_threads) {
+  uint32_t W, H, Wp, Hp;
+  traced_dims(cam, a, &W, &H, &Wp, &Hp);
+  uint32_t n = Wp * Hp;
+  if (pix_end > n) pix_end = n;
+  mt_job j = {s, cam, a, pix_begin, pix_end, W, Wp, sum, sum_sq, count, pix_begin, 0};
+  if (n_threads < 1) n_threads = 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+  for (int i = 0; i < n_threads; ++i) pthread_create(&th[i], NULL, mt_worker, &j);
+  for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
+  free(th);
+  return j.queries;
+}
+
+/* Film resolve: a sample at traced pixel p splats weight 1 into the pixels
+ * {p-1, p} x {p-1, p} (box radius 0.5, film.cpp:27-48); film writes clamp to
+ * the edge (image.h:38-48); I = radiance / weight (film.h:131-152). */
+void orc_film_resolve(uint32_t w, uint32_t h, uint32_t ts, const double *sum,
+                      const uint32_t *count, float *rgb) {
+  uint32_t Wp = (w + ts - 1) / ts * ts, Hp = (h + ts - 1) / ts * ts;
+  double *rad = (double *)calloc((size_t)w * h * 3, sizeof(double));
+  double *wt = (double *)calloc((size_t)w * h, sizeof(double));
+  for (uint32_t py = 0; py < Hp; ++py)
+    for (uint32_t px = 0; px < Wp; ++px) {
+      size_t p = (size_t)px + (size_t)py * Wp;
+      for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+          int qx = clampi((int)px + dx, 0, (int)w - 1);
+          int qy = clampi((int)py + dy, 0, (int)h - 1);
+          size_t q = (size_t)qx + (size_t)qy * w;
+          for (int k = 0; k < 3; ++k) rad[3 * q + k] += sum[3 * p + k];
+          wt[q] += count[p];
+        }
+    }
+  for (size_t q = 0; q < (size_t)w * h; ++q)
+    for (int k = 0; k < 3; ++k)
+      rgb[3 * q + k] = wt[q] != 0 ? (float)(rad[3 * q + k] / wt[q]) : 0.f;
+  free(rad);
+  free(wt);
+}

@@ -1,0 +1,125 @@
+"""Wavefront path tracer on the GPU through the C-ABI, against the oracle.
+
+Both run the same per-sample random streams (seed_mode 0: the reference's
+per-pixel stream x + y*W with samples in sequence; seed_mode 1: one stream
+per sample), so each sample's radiance is compared directly. Tolerance:
+shading uses transcendental functions (device libm vs glibc, <= 2 ulp), so a
+sample agrees to |dL| <= 1e-4 * (1 + |L|) unless a last-bit difference flips
+a discrete decision (a texel, a triangle edge); at least 99% of samples must
+agree and the per-pixel sums must agree to 1e-3 relative on average.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import orc
+from ajisai_render_amd import scenes
+from ajisai_render_amd.core import GpuScene, PathTracer
+
+pytestmark = pytest.mark.gpu
+
+
+def compare_samples(g, o, min_agree=0.99):
+    gl = g["per_sample"].astype(np.float64)
+    ol = o["per_sample"].astype(np.float64)
+    assert gl.shape == ol.shape
+    diff = np.abs(gl - ol).max(axis=1)
+    tol = 1e-4 * (1 + np.abs(ol).max(axis=1))
+    agree = (diff <= tol).mean()
+    assert agree >= min_agree, f"only {agree:.4f} of samples agree"
+    assert np.array_equal(g["count"], o["count"])
+    rel = np.abs(g["sum"] - o["sum"]).sum() / max(1e-12, np.abs(o["sum"]).sum())
+    assert rel < 1e-3, rel
+    return agree
+
+
+@pytest.mark.parametrize("seed_mode", [0, 1])
+def test_cornell_per_sample_parity(gpu_ctx, seed_mode):
+    print(f"case: cornell 48x40 spp 4 seed_mode {seed_mode}", flush=True)
+    sc = scenes.cornell(res=(48, 40), spp=4)
+    gs = GpuScene(gpu_ctx, sc)
+    o = orc.OracleScene(sc)
+    cam = o.camera()
+    g = PathTracer(gs, spp=4, seed_mode=seed_mode, seed_base=77).Render(cam_override=cam,
+                                                                       per_sample=True)
+    r = o.render(cam=cam, args=o.args(spp=4, seed_mode=seed_mode, seed_base=77), per_sample=True)
+    agree = compare_samples(g, r)
+    print("agree", agree)
+    st = g["stats"]
+    assert st.paths == 64 * 48 * 4
+    assert st.closest_queries > 0 and st.any_queries > 0
+
+
+def test_cornell_slot_limit_same_result(gpu_ctx):
+    """Few path slots (many work items per slot) give the same film."""
+    sc = scenes.cornell(res=(32, 32), spp=3)
+    gs = GpuScene(gpu_ctx, sc)
+    a = PathTracer(gs, seed_mode=1, seed_base=5).Render(per_sample=True)
+    b = PathTracer(gs, seed_mode=1, seed_base=5, max_slots=333).Render(per_sample=True)
+    assert np.array_equal(a["per_sample"], b["per_sample"])
+    assert np.array_equal(a["count"], b["count"])
+    assert np.allclose(a["sum"], b["sum"], rtol=1e-6, atol=1e-6)
+
+
+def test_heavy_mesh_disney_env_parity(gpu_ctx):
+    print("case: heavy mesh subdiv 5, 40x32 spp 2", flush=True)
+    sc = scenes.heavy_mesh(subdiv=5, res=(40, 32), spp=2, env_size=(256, 128))
+    gs = GpuScene(gpu_ctx, sc)
+    o = orc.OracleScene(sc)
+    cam = o.camera()
+    g = PathTracer(gs, spp=2, seed_mode=0).Render(cam_override=cam, per_sample=True)
+    r = o.render(cam=cam, args=o.args(spp=2), per_sample=True)
+    compare_samples(g, r, min_agree=0.98)
+
+
+@pytest.mark.parametrize("fp64", [1, 0])
+def test_env_tables_match_oracle(gpu_ctx, fp64):
+    sc = scenes.heavy_mesh(subdiv=1, res=(16, 16), spp=1, env_size=(1024, 512))
+    gs = GpuScene(gpu_ctx, sc, env_cdf_fp64=fp64)
+    o = orc.OracleScene(sc, env_cdf_fp64=fp64)
+    gc, gf, gm, gmi = gs.env_tables(0)
+    oc, of, om, omi = o.env_tables(0)
+    # weights use sinf (device vs glibc, a few ulp): compare to 1e-5 relative
+    assert np.allclose(gf, of, rtol=1e-5, atol=0)
+    assert np.allclose(gc, oc, rtol=1e-5, atol=1e-7)
+    assert np.allclose(gm, om, rtol=1e-5, atol=1e-7)
+    assert abs(gmi - omi) <= 1e-5 * omi
+    assert np.all(np.diff(gc, axis=1) >= 0) and gc[:, -1].min() == 1.0
+
+
+def test_directional_plane_closed_form_gpu(gpu_ctx):
+    d = (0.6, 0.8, 0.0)
+    a = (0.5, 0.7, 0.9)
+    prims = [scenes.quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50), 0)]
+    cam = scenes.CameraSpec(pos=(0, 1, 0.2), look_at=(0, 0, 0), up=(0, 0, -1), fov=20.0,
+                            res=(16, 16))
+    sc = scenes.Scene("plane", prims, [scenes.Material(0, a)],
+                      [scenes.Light(0, dir=d, radiance=(5.0, 5.0, 5.0))], cam,
+                      scenes.PTSpec(spp=4, tile_size=16, min_bounces=5, max_bounces=4))
+    g = PathTracer(GpuScene(gpu_ctx, sc)).Render()
+    L = g["sum"] / g["count"][:, None]
+    expect = np.array(a) / math.pi * 5.0 * 0.8 / (1 + (0.8 / math.pi) ** 2)
+    assert np.allclose(L, expect, rtol=5e-5)
+
+
+def test_film_resolve_matches_oracle(gpu_ctx):
+    import torch
+    sc = scenes.cornell(res=(40, 24), spp=2)
+    gs = GpuScene(gpu_ctx, sc)
+    pt = PathTracer(gs, seed_mode=1)
+    film, _ = pt.render_device()
+    img = pt.resolve(film)
+    ref = orc.film_resolve(40, 24, 16, film["sum"].cpu().numpy().reshape(-1, 3).astype(np.float64),
+                           film["count"].cpu().numpy().view(np.uint32))
+    assert np.allclose(img, ref, rtol=1e-5, atol=1e-7)
+    torch.cuda.synchronize()
+
+
+def test_render_rejects_bad_args(gpu_ctx):
+    from ajisai_render_amd import _capi as capi
+    sc = scenes.cornell(res=(16, 16), spp=1)
+    gs = GpuScene(gpu_ctx, sc)
+    with pytest.raises(capi.AjgError) as e:
+        PathTracer(gs, spp=0).Render()
+    assert e.value.status == capi.AJG_ERR_INVALID

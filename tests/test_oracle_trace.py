@@ -1,0 +1,140 @@
+"""Pins the oracle's traversal (ploc_aggregate.cpp:154-242, 329-391) by brute
+force over all triangles and by hand-computed special cases."""
+import numpy as np
+
+import orc
+import raygen
+from ajisai_render_amd import scenes
+
+
+def build(tri):
+    box = np.concatenate([tri.reshape(-1, 3).min(0), tri.reshape(-1, 3).max(0)]).astype(np.float32)
+    nodes, perm, it = orc.ploc_build(tri, box, 14)
+    assert it >= 0
+    return nodes, perm, box
+
+
+def test_mt_known_answers():
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    nodes, perm, _ = build(tri)
+    r = raygen.make_rays(np.array([[0.25, 0.25, 1.0], [0.25, 0.25, -1.0], [0.0, 0.0, 1.0],
+                                   [0.6, 0.6, 1.0], [-0.01, 0.5, 1.0], [0.25, 0.25, 1.0]],
+                                  np.float32),
+                         np.array([[0, 0, -1], [0, 0, 1], [0, 0, -1], [0, 0, -1], [0, 0, -1],
+                                   [1, 0, 0]], np.float32), offset=False)
+    h = orc.trace_closest(nodes, perm, tri, r)
+    # straight down onto (0.25, 0.25): t = 1, u = v = 0.25 exactly
+    assert h[0]["id"] == 0 and h[0]["t"] == 1.0 and h[0]["u"] == 0.25 and h[0]["v"] == 0.25
+    assert h[1]["id"] == 0 and h[1]["t"] == 1.0          # from below: two-sided MT
+    # origin on the leaf box's x = 0 and y = 0 planes with d.x = d.y = 0:
+    # (plane - o) * (1/0) = 0 * inf = NaN, the x-first Max fold keeps NaN and
+    # `NaN < t_max` is false, so IntersectP rejects the box: a miss.
+    assert h[2]["id"] == 0xFFFFFFFF
+    assert h[3]["id"] == 0xFFFFFFFF                     # u + v > 1
+    assert h[4]["id"] == 0xFFFFFFFF                     # u < 0
+    assert h[5]["id"] == 0xFFFFFFFF                     # parallel: |det| < 1e-6
+    # t must be > t_min and < t_max
+    r2 = raygen.make_rays(np.array([[0.25, 0.25, 1e-5]], np.float32),
+                          np.array([[0, 0, -1]], np.float32), offset=False)
+    assert orc.trace_closest(nodes, perm, tri, r2)[0]["id"] == 0xFFFFFFFF
+    r3 = raygen.make_rays(np.array([[0.25, 0.25, 1.0]], np.float32),
+                          np.array([[0, 0, -1]], np.float32), tmax=np.float32(1.0), offset=False)
+    assert orc.trace_closest(nodes, perm, tri, r3)[0]["id"] == 0xFFFFFFFF
+    assert orc.trace_any(nodes, perm, tri, r3)[0] == 0
+    # edge a-c of a=(0,0,0) b=(2,0,0) c=(1,2,0) hit from (0.5,1,1) straight
+    # down: det = 4, f = 1/4, u = 0 exactly (accepted: the test is u < 0),
+    # v = 1/2, t = 1
+    tri2 = np.array([[0, 0, 0, 2, 0, 0, 1, 2, 0]], np.float32)
+    n2, p2, _ = build(tri2)
+    r4 = raygen.make_rays(np.array([[0.5, 1.0, 1.0]], np.float32),
+                          np.array([[0, 0, -1]], np.float32), offset=False)
+    h4 = orc.trace_closest(n2, p2, tri2, r4)[0]
+    assert h4["id"] == 0 and h4["u"] == 0.0 and h4["v"] == 0.5 and h4["t"] == 1.0
+
+
+def slab_np(mn, mx, ray, tmax):
+    """IntersectP (ploc_aggregate.cpp:154-162) in numpy float32."""
+    with np.errstate(all="ignore"):
+        o, d = ray["o"].astype(np.float32), ray["d"].astype(np.float32)
+        inv = np.float32(1) / d
+        neg = inv < 0
+        lo = (np.where(neg, mx, mn).astype(np.float32) - o) * inv
+        hi = (np.where(neg, mn, mx).astype(np.float32) - o) * inv
+    fmax = lambda a, b: b if a < b else a  # noqa: E731
+    fmin = lambda a, b: b if b < a else a  # noqa: E731
+    tmin_max = fmax(fmax(lo[0], lo[1]), lo[2])
+    tmax_min = fmin(fmin(hi[0], hi[1]), hi[2])
+    if tmin_max > tmax_min:
+        return False
+    return bool(tmin_max < tmax and tmax_min > ray["tmin"])
+
+
+def culled_by_boxes(nodes, perm, tri_id, ray, tmax):
+    """True when the leaf of tri_id or one of its ancestors fails IntersectP
+    at t_max = tmax, i.e. the reference's box culling explains a miss."""
+    parent = {}
+    leaf = None
+    for k, nd in enumerate(nodes):
+        if nd["count"] == 0:
+            parent[int(nd["first"])] = k
+            parent[int(nd["first"]) + 1] = k
+        elif perm[nd["first"]] == tri_id:
+            leaf = k
+    k = leaf
+    while k is not None:
+        if not slab_np(nodes[k]["mn"], nodes[k]["mx"], ray, tmax):
+            return True
+        k = parent.get(k)
+    return False
+
+
+def compare_with_brute(tri, rays):
+    """Traversal == brute-force MT over all triangles, except where exact-t
+    ties keep the first visited leaf or the reference's own slab test culls
+    the brute-force triangle (flat boxes, NaN slabs, entry == t)."""
+    nodes, perm, _ = build(tri)
+    h = orc.trace_closest(nodes, perm, tri, rays)
+    b = orc.brute_closest(tri, rays)
+    same_t = h["t"].view(np.uint32) == b["t"].view(np.uint32)
+    explained = 0
+    for i in np.nonzero(~same_t)[0]:
+        # the traversal can only lose a closer hit, never invent one
+        assert not (h["t"][i] < b["t"][i])
+        assert culled_by_boxes(nodes, perm, b["id"][i], rays[i], h["t"][i]), i
+        explained += 1
+    ties = int(((h["id"] != b["id"]) & same_t).sum())
+    occ = orc.trace_any(nodes, perm, tri, rays)
+    assert np.array_equal(occ.astype(bool), h["id"] != 0xFFFFFFFF)
+    return h, ties, explained
+
+
+def test_traversal_equals_brute_force_random():
+    tri = scenes.random_soup(4000, 3, extent=1.0, size=0.06)
+    box = np.concatenate([tri.reshape(-1, 3).min(0), tri.reshape(-1, 3).max(0)])
+    rays = raygen.random_rays(box, 3000, 9)
+    h, ties, culled = compare_with_brute(tri, rays)
+    assert (h["id"] != 0xFFFFFFFF).mean() > 0.3
+    assert ties == 0 and culled == 0
+
+
+def test_traversal_equals_brute_force_cornell_adversarial():
+    sc = scenes.cornell()
+    tri = sc.soup()
+    box = sc.scene_box()
+    rays = np.concatenate([raygen.adversarial_rays(tri, box, 4000, 5),
+                           raygen.random_rays(box, 2000, 6),
+                           raygen.camera_rays(sc.camera, n_max=4000)])
+    h, ties, culled = compare_with_brute(tri, rays)
+    assert (h["id"] != 0xFFFFFFFF).mean() > 0.3
+    assert culled < 0.05 * rays.shape[0]
+
+
+def test_shadow_rays_any_hit_agree_with_closest():
+    tri = scenes.random_soup(2000, 4, extent=1.0, size=0.08)
+    box = np.concatenate([tri.reshape(-1, 3).min(0), tri.reshape(-1, 3).max(0)])
+    rays = raygen.random_rays(box, 2000, 10)
+    rays["tmax"] = np.random.default_rng(0).uniform(0.01, 1.0, rays.shape[0]).astype(np.float32)
+    nodes, perm, _ = build(tri)
+    h = orc.trace_closest(nodes, perm, tri, rays)
+    occ = orc.trace_any(nodes, perm, tri, rays)
+    assert np.array_equal(occ.astype(bool), h["id"] != 0xFFFFFFFF)
