@@ -71,7 +71,11 @@ class PTArgs(C.Structure):
     _fields_ = [("spp", C.c_int32), ("tile_size", C.c_int32), ("min_bounces", C.c_int32),
                 ("max_bounces", C.c_int32), ("cont_prob", C.c_float), ("use_mis", C.c_int32),
                 ("seed_mode", C.c_int32), ("seed_base", C.c_uint32),
-                ("env_cdf_fp64", C.c_int32), ("max_slots", C.c_int32)]
+                ("env_cdf_fp64", C.c_int32), ("max_slots", C.c_int32), ("flags", C.c_uint32)]
+
+
+RENDER_TIMING = 1
+RENDER_COUNT = 2
 
 
 class Film(C.Structure):
@@ -83,7 +87,11 @@ class RenderStats(C.Structure):
     _fields_ = [("closest_queries", C.c_uint64), ("any_queries", C.c_uint64),
                 ("paths", C.c_uint64), ("iterations", C.c_uint32), ("slots", C.c_uint32),
                 ("ms_total", C.c_float), ("ms_trace_closest", C.c_float),
-                ("ms_trace_any", C.c_float), ("ms_shade", C.c_float)]
+                ("ms_trace_any", C.c_float), ("ms_shade", C.c_float),
+                ("launches_closest", C.c_uint32), ("launches_any", C.c_uint32),
+                ("launches_shade", C.c_uint32), ("reserved", C.c_uint32),
+                ("nodes_fetched", C.c_uint64), ("tris_tested", C.c_uint64),
+                ("nodes_fetched_any", C.c_uint64), ("tris_tested_any", C.c_uint64)]
 
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])

@@ -257,7 +257,7 @@ class PathTracer:
 
     def __init__(self, gscene: GpuScene, spp=None, tile_size=None, min_bounces=None,
                  max_bounces=None, cont_prob=None, seed_mode=0, seed_base=0, env_cdf_fp64=1,
-                 max_slots=0):
+                 max_slots=0, flags=0):
         pt = gscene.scene.pt
         self.g = gscene
         self.args = capi.PTArgs(pt.spp if spp is None else spp,
@@ -265,7 +265,7 @@ class PathTracer:
                                 pt.min_bounces if min_bounces is None else min_bounces,
                                 pt.max_bounces if max_bounces is None else max_bounces,
                                 pt.cont_prob if cont_prob is None else cont_prob, 1, seed_mode,
-                                seed_base, env_cdf_fp64, max_slots)
+                                seed_base, env_cdf_fp64, max_slots, flags)
 
     def traced_dims(self):
         w, h = self.g.scene.camera.res

@@ -254,26 +254,102 @@ __device__ __forceinline__ DevRay load_ray(const DevRay *q, uint32_t i) {
   return r;
 }
 
-template <int STACK>
-__global__ void __launch_bounds__(256)
-k_q_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= *count) return;
-  const DevRay ray = load_ray(rays, i);
-  const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-  const THit h = trace_closest<STACK, false>(bvh, r, nullptr, nullptr);
-  hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+// Sum of a per-lane count over the wave, added once per wave.
+__device__ __forceinline__ void wave_count_add(uint32_t v, unsigned long long *dst) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-template <int STACK>
+template <int STACK, bool COUNT>
 __global__ void __launch_bounds__(256)
-k_q_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count) {
+k_q_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count,
+            unsigned long long *counters) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= *count) return;
-  const DevRay ray = load_ray(rays, i);
-  const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-  occ[i] = trace_any<STACK>(bvh, r) ? 1 : 0;
+  const bool act = i < *count;
+  uint32_t nn = 0, nt = 0;
+  if (act) {
+    const DevRay ray = load_ray(rays, i);
+    const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+    const THit h = trace_closest<STACK, COUNT>(bvh, r, &nn, &nt);
+    hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+  }
+  if (COUNT) {
+    wave_count_add(nn, counters);
+    wave_count_add(nt, counters + 1);
+  }
 }
+
+template <int STACK, bool COUNT>
+__global__ void __launch_bounds__(256)
+k_q_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count,
+        unsigned long long *counters) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const bool act = i < *count;
+  uint32_t nn = 0, nt = 0;
+  if (act) {
+    const DevRay ray = load_ray(rays, i);
+    const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+    occ[i] = trace_any<STACK, COUNT>(bvh, r, &nn, &nt) ? 1 : 0;
+  }
+  if (COUNT) {
+    wave_count_add(nn, counters);
+    wave_count_add(nt, counters + 1);
+  }
+}
+
+template <bool COUNT>
+static void launch_q(int stack, const DevBVH &bvh, bool closest, const DevRay *rays, void *out,
+                     const uint32_t *count, uint32_t n, unsigned long long *ctr, hipStream_t s) {
+  const dim3 g((n + 255) / 256), b(256);
+  if (closest) {
+    if (stack == 64)
+      hipLaunchKernelGGL((k_q_closest<64, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count, ctr);
+    else
+      hipLaunchKernelGGL((k_q_closest<1024, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count, ctr);
+  } else {
+    if (stack == 64)
+      hipLaunchKernelGGL((k_q_any<64, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count, ctr);
+    else
+      hipLaunchKernelGGL((k_q_any<1024, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count, ctr);
+  }
+}
+
+// Event pairs around launches (AJG_RENDER_TIMING).
+struct StageTimer {
+  bool on = false;
+  std::vector<hipEvent_t> evs[3];
+  size_t used[3] = {0, 0, 0};
+  uint32_t launches[3] = {0, 0, 0};
+  hipEvent_t get(int k) {
+    if (used[k] == evs[k].size()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      evs[k].push_back(e);
+    }
+    return evs[k][used[k]++];
+  }
+  void begin(int k, hipStream_t s) {
+    ++launches[k];
+    if (on) (void)hipEventRecord(get(k), s);
+  }
+  void end(int k, hipStream_t s) {
+    if (on) (void)hipEventRecord(get(k), s);
+  }
+  float total_ms(int k) {
+    float t = 0.f;
+    for (size_t i = 0; i + 1 < used[k]; i += 2) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, evs[k][i], evs[k][i + 1]);
+      t += ms;
+    }
+    return t;
+  }
+  ~StageTimer() {
+    for (auto &v : evs)
+      for (hipEvent_t e : v) (void)hipEventDestroy(e);
+  }
+};
 
 __global__ void k_reset_counters(uint32_t *a, uint32_t *b, uint32_t *c) {
   if (threadIdx.x == 0) {
@@ -1058,6 +1134,11 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   const int stack = a->depth <= 64 ? 64 : (a->depth <= 1024 ? 1024 : 0);
   if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree depth above 1024"));
 
+  const bool counting = (args->flags & AJG_RENDER_COUNT) != 0;
+  StageTimer timer;
+  timer.on = (args->flags & AJG_RENDER_TIMING) != 0;
+  // traversal counters: closest (nodes, tris) at +8, any (nodes, tris) at +12
+  unsigned long long *trav_ctr = (unsigned long long *)(counters + 8);
   (void)hipEventRecord(ev0, s);
   int cur = 0;
   uint32_t iters = 0;
@@ -1075,7 +1156,9 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
     q.a_count_in = counters + 2 + cur;
     q.a_count_out = counters + 2 + nxt;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, q.c_count_out, q.a_count_out, q.live);
+    timer.begin(2, s);
     hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q);
+    timer.end(2, s);
     uint32_t h[6];
     AJG_RCHK(hipMemcpyAsync(h, counters, 24, hipMemcpyDeviceToHost, s));
     AJG_RCHK(hipStreamSynchronize(s));
@@ -1086,20 +1169,20 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
     n_closest += nc;
     n_any += na;
     if (nc) {
-      if (stack == 64)
-        hipLaunchKernelGGL(k_q_closest<64>, dim3(cdiv(nc, 256)), dim3(256), 0, s, ds.bvh,
-                           (const DevRay *)c_rays[nxt], c_hits, (const uint32_t *)(counters + nxt));
+      timer.begin(0, s);
+      if (counting)
+        launch_q<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, nc, trav_ctr, s);
       else
-        hipLaunchKernelGGL(k_q_closest<1024>, dim3(cdiv(nc, 256)), dim3(256), 0, s, ds.bvh,
-                           (const DevRay *)c_rays[nxt], c_hits, (const uint32_t *)(counters + nxt));
+        launch_q<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, nc, trav_ctr, s);
+      timer.end(0, s);
     }
     if (na) {
-      if (stack == 64)
-        hipLaunchKernelGGL(k_q_any<64>, dim3(cdiv(na, 256)), dim3(256), 0, s, ds.bvh,
-                           (const DevRay *)a_rays[nxt], a_occ, (const uint32_t *)(counters + 2 + nxt));
+      timer.begin(1, s);
+      if (counting)
+        launch_q<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, na, trav_ctr + 2, s);
       else
-        hipLaunchKernelGGL(k_q_any<1024>, dim3(cdiv(na, 256)), dim3(256), 0, s, ds.bvh,
-                           (const DevRay *)a_rays[nxt], a_occ, (const uint32_t *)(counters + 2 + nxt));
+        launch_q<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, na, trav_ctr, s);
+      timer.end(1, s);
     }
     cur = nxt;
   }
@@ -1123,6 +1206,22 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
     stats->ms_total = ms;
+    stats->launches_closest = timer.launches[0];
+    stats->launches_any = timer.launches[1];
+    stats->launches_shade = timer.launches[2];
+    if (timer.on) {
+      stats->ms_trace_closest = timer.total_ms(0);
+      stats->ms_trace_any = timer.total_ms(1);
+      stats->ms_shade = timer.total_ms(2);
+    }
+    if (counting) {
+      unsigned long long tc[4] = {0, 0, 0, 0};
+      AJG_RCHK(hipMemcpy(tc, trav_ctr, 32, hipMemcpyDeviceToHost));
+      stats->nodes_fetched = tc[0];
+      stats->tris_tested = tc[1];
+      stats->nodes_fetched_any = tc[2];
+      stats->tris_tested_any = tc[3];
+    }
   }
 #undef AJG_RCHK
   return done(AJG_OK);

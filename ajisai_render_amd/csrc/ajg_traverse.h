@@ -175,8 +175,10 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r,
   return h;
 }
 
-template <int STACK>
-__device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r) {
+template <int STACK, bool COUNT = false>
+__device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r,
+                                          uint32_t *n_nodes = nullptr,
+                                          uint32_t *n_tris = nullptr) {
   float e;
   if (!slab_static(r, bvh.root_mn[0], bvh.root_mn[1], bvh.root_mn[2], bvh.root_mx[0],
                    bvh.root_mx[1], bvh.root_mx[2], &e) ||
@@ -187,11 +189,13 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r) {
   int sp = 0;
   for (;;) {
     if (node < 0) {
+      if (COUNT) ++*n_tris;
       f3 a, b, c;
       load_leaf(bvh.leaf_tris, ~node, &a, &b, &c);
       float t, u, v;
       if (mt_test(a, b, c, r, r.tmax, &t, &u, &v)) return true;
     } else {
+      if (COUNT) ++*n_nodes;
       const FatNode fn = bvh.fat[node];
       float e0, e1;
       const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0) &&

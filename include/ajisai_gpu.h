@@ -204,7 +204,13 @@ typedef struct ajg_pt_args {
   int32_t env_cdf_fp64; /* env CDF prefix sums: 0 float (distribution.h:44-48),
                            1 double rounded to float */
   int32_t max_slots;    /* path slots in flight; 0 = automatic */
+  uint32_t flags;       /* AJG_RENDER_* bits */
 } ajg_pt_args;
+
+/* ajg_pt_args.flags */
+#define AJG_RENDER_TIMING 1u   /* per-stage device times via HIP events */
+#define AJG_RENDER_COUNT 2u    /* count fat nodes / triangles per query (slower;
+                                  for the byte model, not for timing) */
 
 /* Film accumulators per traced pixel (traced area = whole tiles,
  * pt.cpp:43-49): Kahan-compensated float sums of L and L^2 per channel and
@@ -223,7 +229,12 @@ typedef struct ajg_render_stats {
   uint32_t iterations;      /* wavefront iterations */
   uint32_t slots;
   float ms_total;           /* device time of the render (events) */
-  float ms_trace_closest, ms_trace_any, ms_shade;
+  float ms_trace_closest, ms_trace_any, ms_shade; /* AJG_RENDER_TIMING */
+  uint32_t launches_closest, launches_any, launches_shade;
+  uint32_t reserved;
+  uint64_t nodes_fetched;   /* AJG_RENDER_COUNT: fat nodes fetched, closest */
+  uint64_t tris_tested;     /* AJG_RENDER_COUNT: triangles tested, closest */
+  uint64_t nodes_fetched_any, tris_tested_any; /* same for shadow rays */
 } ajg_render_stats;
 
 /* DefaultScene (scene/default_scene.cpp:29-103) with a GPU PLOC accelerator:
