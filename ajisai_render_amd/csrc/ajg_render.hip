@@ -36,6 +36,8 @@ struct ajg_scene {
   std::vector<void *> env_bufs;
   float world[6] = {0, 0, 0, 0, 0, 0};
   float radius = 0.f;
+  int cu_count = 256;
+  void *render_cache = nullptr;  // RenderBuffers reused across renders
 };
 
 namespace ajg {
@@ -261,18 +263,31 @@ __device__ __forceinline__ void wave_count_add(uint32_t v, unsigned long long *d
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-template <int STACK, bool COUNT>
-__global__ void __launch_bounds__(256)
-k_q_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count,
-            unsigned long long *counters) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  const bool act = i < *count;
+// Persistent traversal: a fixed grid of waves strides over 64-ray chunks of
+// the queue, whose size the shade stage wrote on the device; waves past the
+// end exit at once, so a nearly empty queue costs one short launch.
+static constexpr int kTravBlocksPerCU = 8;
+static constexpr int kItersPerCheck = 4;
+
+template <int SPILL, bool COUNT>
+__global__ void __launch_bounds__(kTravThreads)
+k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count,
+             uint32_t *cursor, unsigned long long *counters) {
+  __shared__ int s_node[kLdsStack * kTravThreads];
+  __shared__ float s_entry[kLdsStack * kTravThreads];
+  const uint32_t n = *count;
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (kTravThreads / 64);
   uint32_t nn = 0, nt = 0;
-  if (act) {
-    const DevRay ray = load_ray(rays, i);
-    const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-    const THit h = trace_closest<STACK, COUNT>(bvh, r, &nn, &nt);
-    hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
+    const uint32_t i = c * 64 + lane;
+    if (i < n) {
+      const DevRay ray = load_ray(rays, i);
+      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      const THit h = trace_closest<SPILL, COUNT>(bvh, r, s_node + threadIdx.x,
+                                                 s_entry + threadIdx.x, &nn, &nt);
+      hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+    }
   }
   if (COUNT) {
     wave_count_add(nn, counters);
@@ -280,17 +295,24 @@ k_q_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count,
   }
 }
 
-template <int STACK, bool COUNT>
-__global__ void __launch_bounds__(256)
-k_q_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count,
-        unsigned long long *counters) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  const bool act = i < *count;
+template <int SPILL, bool COUNT>
+__global__ void __launch_bounds__(kTravThreads)
+k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count, uint32_t *cursor,
+         unsigned long long *counters) {
+  __shared__ int s_node[kLdsStack * kTravThreads];
+  __shared__ float s_entry[kLdsStack * kTravThreads];
+  const uint32_t n = *count;
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (kTravThreads / 64);
   uint32_t nn = 0, nt = 0;
-  if (act) {
-    const DevRay ray = load_ray(rays, i);
-    const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-    occ[i] = trace_any<STACK, COUNT>(bvh, r, &nn, &nt) ? 1 : 0;
+  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
+    const uint32_t i = c * 64 + lane;
+    if (i < n) {
+      const DevRay ray = load_ray(rays, i);
+      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      occ[i] = trace_any<SPILL, COUNT>(bvh, r, s_node + threadIdx.x, s_entry + threadIdx.x, &nn,
+                                       &nt) ? 1 : 0;
+    }
   }
   if (COUNT) {
     wave_count_add(nn, counters);
@@ -299,19 +321,24 @@ k_q_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count,
 }
 
 template <bool COUNT>
-static void launch_q(int stack, const DevBVH &bvh, bool closest, const DevRay *rays, void *out,
-                     const uint32_t *count, uint32_t n, unsigned long long *ctr, hipStream_t s) {
-  const dim3 g((n + 255) / 256), b(256);
+static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *rays, void *out,
+                      const uint32_t *count, uint32_t *cursor, unsigned long long *ctr,
+                      uint32_t grid, hipStream_t s) {
+  const dim3 g(grid), b(kTravThreads);
   if (closest) {
-    if (stack == 64)
-      hipLaunchKernelGGL((k_q_closest<64, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count, ctr);
+    if (stack == 48)
+      hipLaunchKernelGGL((k_pq_closest<48, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count,
+                         cursor, ctr);
     else
-      hipLaunchKernelGGL((k_q_closest<1024, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count, ctr);
+      hipLaunchKernelGGL((k_pq_closest<1024, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count,
+                         cursor, ctr);
   } else {
-    if (stack == 64)
-      hipLaunchKernelGGL((k_q_any<64, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count, ctr);
+    if (stack == 48)
+      hipLaunchKernelGGL((k_pq_any<48, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
+                         cursor, ctr);
     else
-      hipLaunchKernelGGL((k_q_any<1024, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count, ctr);
+      hipLaunchKernelGGL((k_pq_any<1024, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
+                         cursor, ctr);
   }
 }
 
@@ -351,11 +378,18 @@ struct StageTimer {
   }
 };
 
-__global__ void k_reset_counters(uint32_t *a, uint32_t *b, uint32_t *c) {
+// Start of a wavefront iteration: add the sizes of the queues traced in the
+// previous iteration to the totals, clear the queues the shade stage is
+// about to fill, the live counter and the traversal fetch cursors.
+__global__ void k_iter_begin(uint32_t *ctr, int cur, int nxt, unsigned long long *totals) {
   if (threadIdx.x == 0) {
-    *a = 0;
-    *b = 0;
-    *c = 0;
+    totals[0] += ctr[cur];
+    totals[1] += ctr[2 + cur];
+    ctr[nxt] = 0;
+    ctr[2 + nxt] = 0;
+    ctr[4] = 0;
+    ctr[6] = 0;
+    ctr[7] = 0;
   }
 }
 
@@ -506,11 +540,14 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
       }
       f3 mis = mk3(0.f, 0.f, 0.f);
       if (pf & kPendMis) {
+        // The MIS-BSDF ray contributes only when it escapes to the
+        // environment (the scene has no emissive primitives, helper.h:248),
+        // so it is traced as an any-hit query: it escapes iff the
+        // reference's closest-hit query misses.
         const uint32_t mi = ps.pend_mis[slot];
-        const float4 h = q.c_hits_in[mi];
-        if (__float_as_int(h.y) < 0 && sc.env_index >= 0) {
+        if (!q.a_occ_in[mi] && sc.env_index >= 0) {
           const DevEnv &e = sc.lights[sc.env_index].env;
-          const DevRay mr = load_ray(q.c_rays_in, mi);
+          const DevRay mr = load_ray(q.a_rays_in, mi);
           const f3 le = env_radiance(e, mr.d);
           if (!is_black(le)) {
             const f3 f = scale(mul(le, ld_f3(ps.pend_f, slot)),
@@ -621,9 +658,9 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
             st_f3(ps.pend_ns, slot, it.ns);
             emit = true;
           }
-          const uint32_t mi = wave_append(emit, q.c_count_out);
+          const uint32_t mi = wave_append(emit, q.a_count_out);
           if (emit) {
-            store_ray(q.c_rays_out, mi, mr);
+            store_ray(q.a_rays_out, mi, mr);
             ps.pend_mis[slot] = mi;
             pf2 |= kPendMis;
           }
@@ -749,6 +786,74 @@ k_resolve(uint32_t w, uint32_t h, uint32_t Wp, uint32_t Hp, const float *sum,
 
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Device buffers of a render: path state, double-buffered ray queues,
+// counters and per-work-item partial sums. Kept on the scene and reused by
+// later renders that fit.
+struct RenderBuffers {
+  uint32_t slots = 0;
+  uint64_t total = 0;
+  std::vector<void *> bufs;
+  PathState ps;
+  DevRay *c_rays[2] = {nullptr, nullptr};
+  DevRay *a_rays[2] = {nullptr, nullptr};
+  float4 *c_hits = nullptr;
+  uint8_t *a_occ = nullptr;
+  uint32_t *counters = nullptr;
+  float *partial = nullptr;
+  uint32_t *partial_n = nullptr;
+  void release() {
+    for (void *p : bufs) (void)hipFree(p);
+    bufs.clear();
+    slots = 0;
+    total = 0;
+  }
+  ~RenderBuffers() { release(); }
+};
+
+static constexpr size_t kCounterWords = 64;  // 256 bytes
+
+static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t total) {
+  if (rb->slots >= S && rb->total >= total && !rb->bufs.empty()) return AJG_OK;
+  rb->release();
+  auto al = [&](void **p, size_t bytes, const char *what) {
+    int r = dev_alloc(ctx, p, bytes, what);
+    if (r == AJG_OK) rb->bufs.push_back(*p);
+    return r;
+  };
+  PathState &ps = rb->ps;
+  const size_t cq_cap = (size_t)S, aq_cap = (size_t)(kMaxLights + 1) * S;
+  int st = 0;
+  if ((st = al((void **)&ps.work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.sample, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.sample_end, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bounce, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.ext_idx, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bank, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.rng, 8 * (size_t)S, "state")) || (st = al((void **)&ps.L, 12 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.beta, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_flags, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_mis, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_shadow, 4 * (size_t)S * kMaxLights, "state")) ||
+      (st = al((void **)&ps.pend_work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.pend_sample, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_bank, 4 * (size_t)S, "state")) || (st = al((void **)&ps.pend_L, 12 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_beta, 12 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_c, 12 * (size_t)S * kMaxLights, "state")) ||
+      (st = al((void **)&ps.pend_f, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_pdf, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_ns, 12 * (size_t)S, "state")) || (st = al((void **)&ps.acc, 4 * 24 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.acc_n, 8 * (size_t)S, "state")) ||
+      (st = al((void **)&rb->c_rays[0], sizeof(DevRay) * cq_cap, "closest queue")) ||
+      (st = al((void **)&rb->c_rays[1], sizeof(DevRay) * cq_cap, "closest queue")) ||
+      (st = al((void **)&rb->c_hits, sizeof(float4) * cq_cap, "closest hits")) ||
+      (st = al((void **)&rb->a_rays[0], sizeof(DevRay) * aq_cap, "shadow queue")) ||
+      (st = al((void **)&rb->a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
+      (st = al((void **)&rb->a_occ, aq_cap, "shadow results")) ||
+      (st = al((void **)&rb->counters, 4 * kCounterWords, "counters")) ||
+      (st = al((void **)&rb->partial, sizeof(float) * 6 * total, "partial sums")) ||
+      (st = al((void **)&rb->partial_n, 4 * total, "partial counts"))) {
+    rb->release();
+    return st;
+  }
+  rb->slots = S;
+  rb->total = total;
+  return AJG_OK;
+}
+
 }  // namespace ajg
 
 using namespace ajg;
@@ -760,6 +865,7 @@ static void scene_free(ajg_scene *s) {
   (void)hipFree(s->normals);
   (void)hipFree(s->mats);
   for (void *p : s->env_bufs) (void)hipFree(p);
+  delete (RenderBuffers *)s->render_cache;
   delete s;
 }
 
@@ -811,6 +917,11 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
   if (int st = flatten_views(ctx, views.data(), n_prims, &tri, box)) return st;
   ajg_scene *s = new ajg_scene();
   s->ctx = ctx;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+      s->cu_count = prop.multiProcessorCount;
+  }
   int st = upload_and_build(ctx, tri, box, search_radius, nullptr, &s->accel);
   if (st) {
     scene_free(s);
@@ -1040,63 +1151,29 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   ds.env_index = sc->env_index;
   ds.radius = sc->radius;
 
-  // ---- allocations
-  std::vector<void *> bufs;
-  auto al = [&](void **p, size_t bytes, const char *what) {
-    int r = dev_alloc(ctx, p, bytes, what);
-    if (r == AJG_OK) bufs.push_back(*p);
-    return r;
-  };
-  auto free_all = [&]() {
-    for (void *p : bufs) (void)hipFree(p);
-  };
-  PathState ps;
+  // ---- buffers (cached on the scene across renders)
+  RenderBuffers *rb = (RenderBuffers *)sc->render_cache;
+  if (!rb) {
+    rb = new RenderBuffers();
+    sc->render_cache = rb;
+  }
+  if (int st = ensure_buffers(ctx, rb, S, total)) return st;
+  PathState ps = rb->ps;
   Queues q;
   memset(&q, 0, sizeof(q));
-  const size_t cq_cap = 2 * (size_t)S, aq_cap = (size_t)kMaxLights * S;
-  DevRay *c_rays[2], *a_rays[2];
-  float4 *c_hits;
-  uint8_t *a_occ;
-  uint32_t *counters;  // c0 c1 a0 a1 live work
-  int st = 0;
-  if ((st = al((void **)&ps.work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.sample, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.sample_end, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bounce, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.ext_idx, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bank, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.rng, 8 * (size_t)S, "state")) || (st = al((void **)&ps.L, 12 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.beta, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_flags, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.pend_mis, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.pend_shadow, 4 * (size_t)S * kMaxLights, "state")) ||
-      (st = al((void **)&ps.pend_work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.pend_sample, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.pend_bank, 4 * (size_t)S, "state")) || (st = al((void **)&ps.pend_L, 12 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.pend_beta, 12 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.pend_c, 12 * (size_t)S * kMaxLights, "state")) ||
-      (st = al((void **)&ps.pend_f, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_pdf, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.pend_ns, 12 * (size_t)S, "state")) || (st = al((void **)&ps.acc, 4 * 24 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.acc_n, 8 * (size_t)S, "state")) ||
-      (st = al((void **)&c_rays[0], sizeof(DevRay) * cq_cap, "closest queue")) ||
-      (st = al((void **)&c_rays[1], sizeof(DevRay) * cq_cap, "closest queue")) ||
-      (st = al((void **)&c_hits, sizeof(float4) * cq_cap, "closest hits")) ||
-      (st = al((void **)&a_rays[0], sizeof(DevRay) * aq_cap, "shadow queue")) ||
-      (st = al((void **)&a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
-      (st = al((void **)&a_occ, aq_cap, "shadow results")) ||
-      (st = al((void **)&counters, 64, "counters"))) {
-    free_all();
-    return st;
-  }
-  float *partial = nullptr;
-  uint32_t *partial_n = nullptr;
-  if ((st = al((void **)&partial, sizeof(float) * 6 * total, "partial sums")) ||
-      (st = al((void **)&partial_n, 4 * total, "partial counts"))) {
-    free_all();
-    return st;
-  }
+  const size_t cq_cap = (size_t)S, aq_cap = (size_t)(kMaxLights + 1) * S;
+  DevRay **c_rays = rb->c_rays, **a_rays = rb->a_rays;
+  float4 *c_hits = rb->c_hits;
+  uint8_t *a_occ = rb->a_occ;
+  uint32_t *counters = rb->counters;  // c0 c1 a0 a1 live work | traversal u64 x4 | totals
+  float *partial = rb->partial;
+  uint32_t *partial_n = rb->partial_n;
   hipEvent_t ev0, ev1;
   (void)hipEventCreate(&ev0);
   (void)hipEventCreate(&ev1);
   auto done = [&](int code) {
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
-    free_all();
     return code;
   };
 #define AJG_RCHK(call)                                          \
@@ -1115,7 +1192,7 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   AJG_RCHK(hipMemsetAsync(ps.acc_n, 0, 8 * (size_t)S, s));
   AJG_RCHK(hipMemsetAsync(partial, 0, sizeof(float) * 6 * total, s));
   AJG_RCHK(hipMemsetAsync(partial_n, 0, 4 * total, s));
-  AJG_RCHK(hipMemsetAsync(counters, 0, 64, s));
+  AJG_RCHK(hipMemsetAsync(counters, 0, 4 * kCounterWords, s));
   AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
   AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
   const size_t n_pix_sz = P.n_pix;
@@ -1131,7 +1208,7 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   q.partial = partial;
   q.partial_n = partial_n;
   q.per_sample = film->per_sample;
-  const int stack = a->depth <= 64 ? 64 : (a->depth <= 1024 ? 1024 : 0);
+  const int stack = spill_for_depth(a->depth);
   if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree depth above 1024"));
 
   const bool counting = (args->flags & AJG_RENDER_COUNT) != 0;
@@ -1142,50 +1219,54 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   (void)hipEventRecord(ev0, s);
   int cur = 0;
   uint32_t iters = 0;
-  uint64_t n_closest = 0, n_any = 0;
   const uint32_t shade_grid = cdiv(S, 256);
+  const uint32_t trav_grid = (uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU;
+  // counter words: 0-1 closest queue sizes, 2-3 shadow queue sizes, 4 live
+  // waves, 5 work items handed out, 6-7 traversal fetch cursors, 8-15 u64
+  // traversal counts, 16-19 u64 query totals (closest, shadow).
+  unsigned long long *totals = (unsigned long long *)(counters + 16);
   for (;;) {
-    // shade reads queue `cur` (traced) and writes queue `nxt`
-    const int nxt = cur ^ 1;
-    q.c_rays_in = c_rays[cur];
-    q.c_rays_out = c_rays[nxt];
-    q.c_count_in = counters + cur;
-    q.c_count_out = counters + nxt;
-    q.a_rays_in = a_rays[cur];
-    q.a_rays_out = a_rays[nxt];
-    q.a_count_in = counters + 2 + cur;
-    q.a_count_out = counters + 2 + nxt;
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, q.c_count_out, q.a_count_out, q.live);
-    timer.begin(2, s);
-    hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q);
-    timer.end(2, s);
-    uint32_t h[6];
-    AJG_RCHK(hipMemcpyAsync(h, counters, 24, hipMemcpyDeviceToHost, s));
-    AJG_RCHK(hipStreamSynchronize(s));
-    ++iters;
-    if (h[4] == 0) break;
-    if (iters > 100000) return done(fail(ctx, AJG_ERR_INTERNAL, "render: iteration limit"));
-    const uint32_t nc = h[nxt], na = h[2 + nxt];
-    n_closest += nc;
-    n_any += na;
-    if (nc) {
+    for (int k = 0; k < kItersPerCheck; ++k) {
+      // shade reads queue `cur` (traced) and writes queue `nxt`
+      const int nxt = cur ^ 1;
+      q.c_rays_in = c_rays[cur];
+      q.c_rays_out = c_rays[nxt];
+      q.c_count_in = counters + cur;
+      q.c_count_out = counters + nxt;
+      q.a_rays_in = a_rays[cur];
+      q.a_rays_out = a_rays[nxt];
+      q.a_count_in = counters + 2 + cur;
+      q.a_count_out = counters + 2 + nxt;
+      hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
+      timer.begin(2, s);
+      hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q);
+      timer.end(2, s);
       timer.begin(0, s);
       if (counting)
-        launch_q<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, nc, trav_ctr, s);
+        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, counters + 6,
+                        trav_ctr, trav_grid, s);
       else
-        launch_q<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, nc, trav_ctr, s);
+        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, counters + 6,
+                         trav_ctr, trav_grid, s);
       timer.end(0, s);
-    }
-    if (na) {
       timer.begin(1, s);
       if (counting)
-        launch_q<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, na, trav_ctr + 2, s);
+        launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, counters + 7,
+                        trav_ctr + 2, trav_grid, s);
       else
-        launch_q<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, na, trav_ctr, s);
+        launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
+                         counters + 7, trav_ctr + 2, trav_grid, s);
       timer.end(1, s);
+      cur = nxt;
+      ++iters;
     }
-    cur = nxt;
+    uint32_t live = 0;
+    AJG_RCHK(hipMemcpyAsync(&live, counters + 4, 4, hipMemcpyDeviceToHost, s));
+    AJG_RCHK(hipStreamSynchronize(s));
+    if (live == 0) break;
+    if (iters > 200000) return done(fail(ctx, AJG_ERR_INTERNAL, "render: iteration limit"));
   }
+  unsigned long long tot[2] = {0, 0};
   if (P.n_chunks > 1)
     hipLaunchKernelGGL(k_reduce_chunks, dim3(cdiv(P.n_pix, 256)), dim3(256), 0, s, (const float *)partial,
                        (const uint32_t *)partial_n, P.n_pix, P.n_chunks, film->sum, film->sum_sq,
@@ -1194,12 +1275,13 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
     hipLaunchKernelGGL(k_unpack_single, dim3(cdiv(P.n_pix, 256)), dim3(256), 0, s, (const float *)partial,
                        (const uint32_t *)partial_n, P.n_pix, film->sum, film->sum_sq, film->count);
   (void)hipEventRecord(ev1, s);
+  AJG_RCHK(hipMemcpyAsync(tot, totals, 16, hipMemcpyDeviceToHost, s));
   AJG_RCHK(hipStreamSynchronize(s));
   AJG_RCHK(hipGetLastError());
   if (stats) {
     memset(stats, 0, sizeof(*stats));
-    stats->closest_queries = n_closest;
-    stats->any_queries = n_any;
+    stats->closest_queries = tot[0];
+    stats->any_queries = tot[1];
     stats->paths = (uint64_t)P.n_pix * P.spp;
     stats->iterations = iters;
     stats->slots = S;

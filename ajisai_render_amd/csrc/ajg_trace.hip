@@ -17,15 +17,21 @@ static DevBVH dev_bvh(const ajg_accel *a) {
   return b;
 }
 
-template <int STACK>
-__global__ void __launch_bounds__(256)
+#define AJG_LDS_STACK                            \
+  __shared__ int s_node[kLdsStack * kTravThreads]; \
+  __shared__ float s_entry[kLdsStack * kTravThreads]
+
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
 k_trace_closest(DevBVH bvh, const ajg_ray *rays, uint32_t n, ajg_hit *hits,
                 const uint32_t *perm) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  AJG_LDS_STACK;
+  const uint32_t i = blockIdx.x * kTravThreads + threadIdx.x;
   if (i >= n) return;
   const ajg_ray ray = rays[i];
   const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
-  const THit h = trace_closest<STACK, false>(bvh, r, nullptr, nullptr);
+  const THit h = trace_closest<SPILL, false>(bvh, r, s_node + threadIdx.x, s_entry + threadIdx.x,
+                                             nullptr, nullptr);
   ajg_hit o;
   o.t = h.t;
   o.tri_id = h.slot >= 0 ? perm[h.slot] : kNone;
@@ -34,28 +40,30 @@ k_trace_closest(DevBVH bvh, const ajg_ray *rays, uint32_t n, ajg_hit *hits,
   hits[i] = o;
 }
 
-template <int STACK>
-__global__ void __launch_bounds__(256)
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
 k_trace_counts(DevBVH bvh, const ajg_ray *rays, uint32_t n, uint32_t *nodes,
                uint32_t *tris) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  AJG_LDS_STACK;
+  const uint32_t i = blockIdx.x * kTravThreads + threadIdx.x;
   if (i >= n) return;
   const ajg_ray ray = rays[i];
   const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
   uint32_t nn = 0, nt = 0;
-  (void)trace_closest<STACK, true>(bvh, r, &nn, &nt);
+  (void)trace_closest<SPILL, true>(bvh, r, s_node + threadIdx.x, s_entry + threadIdx.x, &nn, &nt);
   nodes[i] = nn;
   tris[i] = nt;
 }
 
-template <int STACK>
-__global__ void __launch_bounds__(256)
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
 k_trace_any(DevBVH bvh, const ajg_ray *rays, uint32_t n, uint8_t *occ) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  AJG_LDS_STACK;
+  const uint32_t i = blockIdx.x * kTravThreads + threadIdx.x;
   if (i >= n) return;
   const ajg_ray ray = rays[i];
   const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
-  occ[i] = trace_any<STACK>(bvh, r) ? 1 : 0;
+  occ[i] = trace_any<SPILL>(bvh, r, s_node + threadIdx.x, s_entry + threadIdx.x) ? 1 : 0;
 }
 
 static inline uint32_t grid(uint32_t n) { return (n + 255) / 256; }
@@ -64,10 +72,10 @@ int launch_trace_closest(ajg_accel *a, const ajg_ray *rays, uint32_t n, ajg_hit 
                          hipStream_t s) {
   if (n == 0) return AJG_OK;
   const DevBVH b = dev_bvh(a);
-  if (a->depth <= 64)
-    hipLaunchKernelGGL(k_trace_closest<64>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, hits,
+  if (spill_for_depth(a->depth) == 48)
+    hipLaunchKernelGGL(k_trace_closest<48>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, hits,
                        (const uint32_t *)a->perm);
-  else if (a->depth <= 1024)
+  else if (spill_for_depth(a->depth) == 1024)
     hipLaunchKernelGGL(k_trace_closest<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, hits,
                        (const uint32_t *)a->perm);
   else
@@ -80,9 +88,9 @@ int launch_trace_counts(ajg_accel *a, const ajg_ray *rays, uint32_t n, uint32_t 
                         uint32_t *tris, hipStream_t s) {
   if (n == 0) return AJG_OK;
   const DevBVH b = dev_bvh(a);
-  if (a->depth <= 64)
-    hipLaunchKernelGGL(k_trace_counts<64>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
-  else if (a->depth <= 1024)
+  if (spill_for_depth(a->depth) == 48)
+    hipLaunchKernelGGL(k_trace_counts<48>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
+  else if (spill_for_depth(a->depth) == 1024)
     hipLaunchKernelGGL(k_trace_counts<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
   else
     return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
@@ -94,9 +102,9 @@ int launch_trace_any(ajg_accel *a, const ajg_ray *rays, uint32_t n, uint8_t *occ
                      hipStream_t s) {
   if (n == 0) return AJG_OK;
   const DevBVH b = dev_bvh(a);
-  if (a->depth <= 64)
-    hipLaunchKernelGGL(k_trace_any<64>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
-  else if (a->depth <= 1024)
+  if (spill_for_depth(a->depth) == 48)
+    hipLaunchKernelGGL(k_trace_any<48>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
+  else if (spill_for_depth(a->depth) == 1024)
     hipLaunchKernelGGL(k_trace_any<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
   else
     return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");

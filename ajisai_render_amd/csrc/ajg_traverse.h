@@ -122,9 +122,55 @@ __device__ __forceinline__ void visit_leaf_closest(const DevBVH &bvh, const TRay
   }
 }
 
-template <int STACK, bool COUNT>
-__device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r,
-                                              uint32_t *n_nodes, uint32_t *n_tris) {
+// Traversal stack: the newest kLdsStack entries live in LDS (one column
+// per lane, lanes on consecutive banks); older entries spill to a private
+// array, which only deep paths ever touch. Entries are (node ref, entry t).
+static constexpr int kLdsStack = 8;
+static constexpr int kTravThreads = 256;
+
+template <int SPILL>
+struct TravStack {
+  int *ln;    // LDS node refs of this lane: element k at ln[k * kTravThreads]
+  float *le;  // LDS entry distances
+  int top = 0, base = 0, spilled = 0;
+  int sn[SPILL];
+  float se[SPILL];
+  __device__ __forceinline__ TravStack(int *lds_node, float *lds_entry)
+      : ln(lds_node), le(lds_entry) {}
+  __device__ __forceinline__ void push(int n, float e) {
+    if (top == kLdsStack) {
+      sn[spilled] = ln[base * kTravThreads];
+      se[spilled] = le[base * kTravThreads];
+      ++spilled;
+      base = (base + 1) & (kLdsStack - 1);
+      --top;
+    }
+    const int i = (base + top) & (kLdsStack - 1);
+    ln[i * kTravThreads] = n;
+    le[i * kTravThreads] = e;
+    ++top;
+  }
+  __device__ __forceinline__ bool pop(int *n, float *e) {
+    if (top == 0) {
+      if (spilled == 0) return false;
+      --spilled;
+      *n = sn[spilled];
+      *e = se[spilled];
+      return true;
+    }
+    --top;
+    const int i = (base + top) & (kLdsStack - 1);
+    *n = ln[i * kTravThreads];
+    *e = le[i * kTravThreads];
+    return true;
+  }
+};
+
+// Stack bound: entries <= tree depth - 1, so SPILL >= depth - kLdsStack.
+template <int SPILL, bool COUNT>
+__device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, int *lds_node,
+                                              float *lds_entry, uint32_t *n_nodes,
+                                              uint32_t *n_tris) {
   THit h;
   h.t = __int_as_float(0x7f800000);
   h.u = 0.f;
@@ -137,9 +183,7 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r,
       !(e < tmax))
     return h;
   int node = bvh.root_ref;
-  int stk_node[STACK];
-  float stk_entry[STACK];
-  int sp = 0;
+  TravStack<SPILL> st(lds_node, lds_entry);
   for (;;) {
     if (node < 0) {
       if (COUNT) ++*n_tris;
@@ -150,22 +194,20 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r,
       float e0, e1;
       const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
       const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
-      if (s1) {
-        stk_node[sp] = fn.d.y;
-        stk_entry[sp] = e1;
-        ++sp;
-      }
+      // the right child is visited after the left subtree: keep its entry
+      // distance and compare it with t_max when it is popped
+      if (s1) st.push(fn.d.y, e1);
       if (s0 && e0 < tmax) {
         node = fn.d.x;
         continue;
       }
     }
-    // pop the next node whose entry still beats t_max
     bool found = false;
-    while (sp > 0) {
-      --sp;
-      if (stk_entry[sp] < tmax) {
-        node = stk_node[sp];
+    int n;
+    float en;
+    while (st.pop(&n, &en)) {
+      if (en < tmax) {
+        node = n;
         found = true;
         break;
       }
@@ -175,9 +217,9 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r,
   return h;
 }
 
-template <int STACK, bool COUNT = false>
-__device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r,
-                                          uint32_t *n_nodes = nullptr,
+template <int SPILL, bool COUNT = false>
+__device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int *lds_node,
+                                          float *lds_entry, uint32_t *n_nodes = nullptr,
                                           uint32_t *n_tris = nullptr) {
   float e;
   if (!slab_static(r, bvh.root_mn[0], bvh.root_mn[1], bvh.root_mn[2], bvh.root_mx[0],
@@ -185,8 +227,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r,
       !(e < r.tmax))
     return false;
   int node = bvh.root_ref;
-  int stk[STACK];
-  int sp = 0;
+  TravStack<SPILL> st(lds_node, lds_entry);
   for (;;) {
     if (node < 0) {
       if (COUNT) ++*n_tris;
@@ -203,7 +244,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r,
       const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1) &&
                       e1 < r.tmax;
       if (s0 && s1) {
-        stk[sp++] = fn.d.y;
+        st.push(fn.d.y, e1);
         node = fn.d.x;
         continue;
       }
@@ -216,10 +257,19 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r,
         continue;
       }
     }
-    if (sp == 0) break;
-    node = stk[--sp];
+    int n;
+    float en;
+    if (!st.pop(&n, &en)) break;
+    node = n;
   }
   return false;
+}
+
+// Spill capacity for a tree of the given depth (0: unsupported depth).
+inline int spill_for_depth(uint32_t depth) {
+  if (depth <= (uint32_t)kLdsStack + 48) return 48;
+  if (depth <= 1024) return 1024;
+  return 0;
 }
 
 }  // namespace ajg
