@@ -124,6 +124,13 @@ _SIGS = {
     "ajg_film_resolve": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P], C.c_int),
     "ajg_camera_setup": ([C.POINTER(Camera), _P, _P, _P, C.c_float, C.c_float, C.c_float,
                           C.c_float, C.c_float], C.c_int),
+    "ajg_guide_param_count": ([], C.c_int),
+    "ajg_guide_create": ([_P, C.c_uint32, _P, C.POINTER(_P)], C.c_int),
+    "ajg_guide_destroy": ([_P], C.c_int),
+    "ajg_guide_get_params": ([_P, _P], C.c_int),
+    "ajg_guide_set_params": ([_P, _P], C.c_int),
+    "ajg_guide_forward": ([_P, _P, _P, C.c_uint32, _P, _P, _P], C.c_int),
+    "ajg_guide_train_step": ([_P, _P, C.c_uint32, C.c_float, _P, _P, _P], C.c_int),
 }
 
 _lib = None

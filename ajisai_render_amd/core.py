@@ -6,6 +6,8 @@
 * ``GpuScene`` mirrors ``DefaultScene`` (scene/default_scene.cpp:29-103).
 * ``PathTracer`` mirrors ``CreatePTRenderer(PTRendererArgs)`` /
   ``Renderer::Render`` (renderer/renderer.h:37-80, pt.cpp:40-73).
+* ``PathGuide`` is the learned directional sampler (no reference
+  counterpart): an MLP giving an 8-lobe vMF mixture per shading point.
 
 Device buffers are torch tensors on ``cuda``; torch is only used for device
 memory and streams. Every compute call goes through libajisai_gpu.so.
@@ -322,6 +324,67 @@ class PathTracer:
                                                          _ptr(film_dev["count"]), _ptr(rgb), None),
                          "ajg_film_resolve")
         return rgb.cpu().numpy().reshape(h, w, 3)
+
+
+class PathGuide:
+    """ajg_guide: MLP x(112) -> 64 -> 64 -> 64 -> 48 (BF16 MFMA, FP32
+    accumulation) whose outputs parameterise an 8-lobe vMF mixture."""
+
+    N_IN, N_HID, N_OUT, N_LOBES = 112, 64, 48, 8
+
+    def __init__(self, ctx: Context, scene_box, seed: int = 1):
+        self.ctx = ctx
+        box = np.ascontiguousarray(np.asarray(scene_box, np.float32).reshape(6))
+        h = C.c_void_p()
+        ctx.check(ctx.lib.ajg_guide_create(ctx.h, seed, box.ctypes.data, C.byref(h)),
+                  "ajg_guide_create")
+        self.h = h
+        self.box = box
+        self.n_params = int(ctx.lib.ajg_guide_param_count())
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        self.ctx.check(self.ctx.lib.ajg_guide_get_params(self.h, out.ctypes.data), "get params")
+        return out
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        self.ctx.check(self.ctx.lib.ajg_guide_set_params(self.h, p.ctypes.data), "set params")
+
+    def forward(self, pos_dev, nrm_dev, want_raw=True, stream=None):
+        """Returns (mixture n x 40, raw n x 48) device tensors."""
+        torch = _torch()
+        n = int(pos_dev.shape[0])
+        dev = pos_dev.device
+        mix = torch.empty((n, 40), dtype=torch.float32, device=dev)
+        raw = torch.empty((n, 48), dtype=torch.float32, device=dev) if want_raw else None
+        self.ctx.check(self.ctx.lib.ajg_guide_forward(self.h, _ptr(pos_dev), _ptr(nrm_dev), n,
+                                                      _ptr(mix), _ptr(raw), stream), "forward")
+        return mix, raw
+
+    def train_step(self, samples_dev, lr=1e-3, want_grad=False, stream=None):
+        """One step on device samples (n x 10: pos, nrm, dir, weight).
+        Returns (loss, grad or None)."""
+        n = int(samples_dev.shape[0])
+        loss = C.c_float(0.0)
+        grad = np.zeros(self.n_params, np.float32) if want_grad else None
+        self.ctx.check(self.ctx.lib.ajg_guide_train_step(
+            self.h, _ptr(samples_dev), n, float(lr), C.addressof(loss),
+            grad.ctypes.data if grad is not None else None, stream), "train step")
+        return float(loss.value), grad
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ajg_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def camera_setup(cam_spec) -> capi.Camera:

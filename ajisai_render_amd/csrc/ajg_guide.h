@@ -1,0 +1,114 @@
+// ajg_guide.h — learned path guiding: the network's layout, the input
+// encoding and the von Mises-Fisher mixture that the network parameterises.
+// (No reference counterpart: the reference samples continuation directions
+// from the BSDF only, pt.cpp:215-222.)
+//
+// Network: x(112) -> 64 -> 64 -> 64 -> 48, ReLU on the hidden layers, BF16
+// weights and activations, FP32 accumulation (MFMA 16x16x16 bf16).
+//   x = [p_hat(3), (sin, cos)(pi*(k+1)*n_c) for c in xyz, k in 0..15 (96), 0 (13)]
+//   p_hat = 2 * (p - box_min) / (box_max - box_min) - 1
+// Output o[0..39] (o[40..47] unused): lobe j uses o[5j..5j+4] =
+//   (mu_raw xyz, kappa_raw, weight logit);
+//   mu = mu_raw / |mu_raw|, kappa = exp(clamp(kappa_raw, -3, 7)), pi = softmax.
+#pragma once
+
+#include <hip/hip_bfloat16.h>
+
+#include "ajg_common.h"
+
+namespace ajg {
+
+static constexpr int kGIn = 112;
+static constexpr int kGUsedIn = 99;
+static constexpr int kGHid = 64;
+static constexpr int kGOut = 48;
+static constexpr int kGUsedOut = 40;
+static constexpr int kGLobes = 8;
+static constexpr int kGFreq = 16;
+// parameter offsets (floats) in the master FP32 vector: W[in][out] then b
+static constexpr int kGW1 = 0;
+static constexpr int kGB1 = kGW1 + kGIn * kGHid;
+static constexpr int kGW2 = kGB1 + kGHid;
+static constexpr int kGB2 = kGW2 + kGHid * kGHid;
+static constexpr int kGW3 = kGB2 + kGHid;
+static constexpr int kGB3 = kGW3 + kGHid * kGHid;
+static constexpr int kGW4 = kGB3 + kGHid;
+static constexpr int kGB4 = kGW4 + kGHid * kGOut;
+static constexpr int kGParams = kGB4 + kGOut;  // 18672
+static constexpr float kKappaLo = -3.f, kKappaHi = 7.f;
+
+// Feature f of the input encoding of a shading point (p, n).
+__device__ __forceinline__ float guide_feature(f3 p, f3 n, const float *box, int f) {
+  if (f < 3) {
+    const float pc = f == 0 ? p.x : (f == 1 ? p.y : p.z);
+    return 2.f * (pc - box[f]) / (box[3 + f] - box[f]) - 1.f;
+  }
+  if (f >= kGUsedIn) return 0.f;
+  const int c = (f - 3) / (2 * kGFreq), k = ((f - 3) % (2 * kGFreq)) >> 1;
+  const float nc = c == 0 ? n.x : (c == 1 ? n.y : n.z);
+  const float a = 3.14159265f * (float)(k + 1) * nc;
+  return ((f - 3) & 1) ? cosf(a) : sinf(a);
+}
+
+struct VmfLobe {
+  f3 mu;
+  float kappa, w;
+};
+
+// Decode network outputs into a normalised mixture.
+__host__ __device__ __forceinline__ void guide_decode(const float *o, VmfLobe *lobes) {
+  float mx = -3.4e38f;
+  for (int j = 0; j < kGLobes; ++j) mx = fmaxf(mx, o[5 * j + 4]);
+  float sum = 0.f;
+  for (int j = 0; j < kGLobes; ++j) {
+    const float mx0 = o[5 * j], my = o[5 * j + 1], mz = o[5 * j + 2];
+    const float len = fmaxf(sqrtf(mx0 * mx0 + my * my + mz * mz), 1e-6f);
+    lobes[j].mu = mk3(mx0 / len, my / len, mz / len);
+    lobes[j].kappa = expf(fminf(fmaxf(o[5 * j + 3], kKappaLo), kKappaHi));
+    lobes[j].w = expf(o[5 * j + 4] - mx);
+    sum += lobes[j].w;
+  }
+  for (int j = 0; j < kGLobes; ++j) lobes[j].w /= sum;
+}
+
+// vMF density kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1))
+__host__ __device__ __forceinline__ float vmf_pdf(const VmfLobe &l, f3 w) {
+  const float k = l.kappa;
+  const float norm = k / (6.28318531f * -expm1f(-2.f * k));
+  return norm * expf(k * (l.mu.x * w.x + l.mu.y * w.y + l.mu.z * w.z - 1.f));
+}
+
+__host__ __device__ __forceinline__ float mixture_pdf(const VmfLobe *lobes, f3 w) {
+  float p = 0.f;
+  for (int j = 0; j < kGLobes; ++j) p += lobes[j].w * vmf_pdf(lobes[j], w);
+  return p;
+}
+
+// Sample the mixture with three uniform numbers (lobe choice reuses u0).
+__device__ __forceinline__ f3 mixture_sample(const VmfLobe *lobes, float u0, float u1, float u2) {
+  int j = 0;
+  float c = 0.f;
+  for (; j < kGLobes - 1; ++j) {
+    c += lobes[j].w;
+    if (u0 < c) break;
+  }
+  const VmfLobe &l = lobes[j];
+  // w = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa (Jakob 2012)
+  const float k = l.kappa;
+  float wz = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k;
+  wz = fminf(fmaxf(wz, -1.f), 1.f);
+  const float s = sqrtf(fmaxf(0.f, 1.f - wz * wz));
+  const float phi = 6.28318531f * u2;
+  float sp, cp;
+  sincosf(phi, &sp, &cp);
+  // orthonormal frame around mu (Duff et al.)
+  const f3 n = l.mu;
+  const float sign = copysignf(1.f, n.z);
+  const float a = -1.f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const f3 t1 = mk3(1.f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+  const f3 t2 = mk3(b, sign + n.y * n.y * a, -n.y);
+  return normalize(add(add(scale(t1, s * cp), scale(t2, s * sp)), scale(n, wz)));
+}
+
+}  // namespace ajg

@@ -38,6 +38,8 @@ namespace ajg {
 // Error helpers: record a message on the context and return a status.
 int fail(ajg_ctx *ctx, int status, const std::string &msg);
 int hip_fail(ajg_ctx *ctx, hipError_t e, const char *what);
+// Host fault injection: the status to report from the next entry point.
+int injected(ajg_ctx *ctx);
 // Device allocation that maps hipErrorOutOfMemory to AJG_ERR_OOM.
 int dev_alloc(ajg_ctx *ctx, void **p, size_t bytes, const char *what);
 

@@ -36,6 +36,7 @@ enum ajg_status {
 typedef struct ajg_ctx ajg_ctx;
 typedef struct ajg_accel ajg_accel;
 typedef struct ajg_scene ajg_scene;
+typedef struct ajg_guide ajg_guide;
 
 /* ---------------------------------------------------------------- context */
 int ajg_abi_version(void);
@@ -277,6 +278,39 @@ int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size,
 int ajg_camera_setup(ajg_camera *cam, const float *pos, const float *look_at,
                      const float *up, float fov_deg, float res_x, float res_y,
                      float lens_radius, float focal_distance);
+
+/* ---------------------------------------------------------- path guiding
+ * No counterpart in the reference: its path tracer samples continuation
+ * directions from the BSDF only (integrator/pt.cpp:215-222). A small MLP
+ * (x(112) -> 64 -> 64 -> 64 -> 48, ReLU, BF16 weights and activations, FP32
+ * accumulation) maps a shading point (position, normal) to an 8-lobe
+ * von Mises-Fisher mixture; it is trained online on the renderer's own
+ * path vertices with Adam. */
+
+/* Number of FP32 parameters (masters): W1[112][64] b1 W2[64][64] b2
+ * W3[64][64] b3 W4[64][48] b4, in that order. */
+int ajg_guide_param_count(void);
+/* Creates a guide with a seeded He-uniform initialisation. scene_box: host
+ * pointer to (min xyz, max xyz), used to normalise positions. */
+int ajg_guide_create(ajg_ctx *ctx, uint32_t seed, const float *scene_box,
+                     ajg_guide **out);
+int ajg_guide_destroy(ajg_guide *guide);
+/* Host copies of the FP32 masters (ajg_guide_param_count() floats);
+ * set_params also refreshes the BF16 copies. Synchronous. */
+int ajg_guide_get_params(ajg_guide *guide, float *params);
+int ajg_guide_set_params(ajg_guide *guide, const float *params);
+/* Inference. pos, nrm: device, n x 3 floats. mixture (device, n x 40 or
+ * NULL): lobe j = (mu xyz, kappa, weight) at 5j. raw (device, n x 48 or
+ * NULL): the network outputs. */
+int ajg_guide_forward(ajg_guide *guide, const float *pos, const float *nrm,
+                      uint32_t n, float *mixture, float *raw, void *stream);
+/* One training step on n samples (device, n x 10 floats: position, normal,
+ * direction, weight). Loss = -sum w log q(dir) / sum w. lr > 0 applies an
+ * Adam step (beta 0.9 / 0.999, eps 1e-8); lr = 0 only computes the
+ * gradient. loss_out / grad_out (host, optional) synchronise the stream. */
+int ajg_guide_train_step(ajg_guide *guide, const float *samples, uint32_t n,
+                         float lr, float *loss_out, float *grad_out,
+                         void *stream);
 
 #ifdef __cplusplus
 }
