@@ -305,7 +305,7 @@ k_guide_loss(const float *raw, const float *samples, uint32_t n, const float *sc
 // W[in][out] reads four consecutive `out` of the in-major copy.
 struct alignas(16) GuideBwdLds {
   uint16_t w2[kGHid * kGHid], w3[kGHid * kGHid], w4[kGHid * kGOut];
-  uint16_t a[kGWaves][16 * kGOut];
+  uint16_t a[kGWaves][16 * kGHid];  // dY4 (16 x 48), then dX2 (16 x 64)
   uint16_t b[kGWaves][16 * kGHid];
 };
 
