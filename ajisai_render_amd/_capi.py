@@ -94,6 +94,18 @@ class RenderStats(C.Structure):
                 ("nodes_fetched_any", C.c_uint64), ("tris_tested_any", C.c_uint64)]
 
 
+class GuideArgs(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("train", C.c_int32), ("batch", C.c_uint32),
+                ("lr", C.c_float), ("train_slots", C.c_uint32), ("max_steps", C.c_uint32),
+                ("steps_per_batch", C.c_uint32)]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [("train_steps", C.c_uint32), ("reserved", C.c_uint32), ("records", C.c_uint64),
+                ("ms_infer", C.c_float), ("ms_train", C.c_float), ("last_loss", C.c_float),
+                ("reserved2", C.c_float)]
+
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("tri_id", "<u4"), ("u", "<f4"), ("v", "<f4")])
 NODE_DTYPE = np.dtype([("mn", "<f4", 3), ("mx", "<f4", 3), ("count", "<u4"), ("first", "<u4")])
@@ -124,6 +136,9 @@ _SIGS = {
     "ajg_film_resolve": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P], C.c_int),
     "ajg_camera_setup": ([C.POINTER(Camera), _P, _P, _P, C.c_float, C.c_float, C.c_float,
                           C.c_float, C.c_float], C.c_int),
+    "ajg_render_pt_guided": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), _P, C.POINTER(GuideArgs),
+                              C.POINTER(Film), C.POINTER(RenderStats), C.POINTER(GuideStats), _P],
+                             C.c_int),
     "ajg_guide_param_count": ([], C.c_int),
     "ajg_guide_create": ([_P, C.c_uint32, _P, C.POINTER(_P)], C.c_int),
     "ajg_guide_destroy": ([_P], C.c_int),

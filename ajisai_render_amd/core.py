@@ -281,8 +281,10 @@ class PathTracer:
         C.memmove(C.byref(cam), C.byref(cam_override), C.sizeof(capi.Camera))
         return cam
 
-    def render_device(self, cam_override=None, per_sample=False, stream=None, film=None):
-        """Render into device tensors; returns (film dict, RenderStats)."""
+    def render_device(self, cam_override=None, per_sample=False, stream=None, film=None,
+                      guide=None, guide_args=None):
+        """Render into device tensors; returns (film dict, RenderStats), or
+        (film dict, RenderStats, GuideStats) when a PathGuide is given."""
         torch = _torch()
         wp, hp = self.traced_dims()
         n = wp * hp
@@ -297,14 +299,22 @@ class PathTracer:
                        _ptr(film.get("per_sample")))
         stats = capi.RenderStats()
         cam = self.camera(cam_override)
+        if guide is not None:
+            ga = guide_args if guide_args is not None else guide_defaults()
+            gs = capi.GuideStats()
+            self.g.ctx.check(self.g.ctx.lib.ajg_render_pt_guided(
+                self.g.h, C.byref(cam), C.byref(self.args), guide.h, C.byref(ga), C.byref(cf),
+                C.byref(stats), C.byref(gs), stream), "ajg_render_pt_guided")
+            return film, stats, gs
         self.g.ctx.check(self.g.ctx.lib.ajg_render_pt(self.g.h, C.byref(cam), C.byref(self.args),
                                                       C.byref(cf), C.byref(stats), stream),
                          "ajg_render_pt")
         return film, stats
 
     # Renderer::Render (renderer.h:64-65)
-    def Render(self, cam_override=None, per_sample=False):
-        film, stats = self.render_device(cam_override, per_sample)
+    def Render(self, cam_override=None, per_sample=False, guide=None, guide_args=None):
+        res = self.render_device(cam_override, per_sample, guide=guide, guide_args=guide_args)
+        film, stats = res[0], res[1]
         out = {k: v.cpu().numpy() for k, v in film.items()}
         out["sum"] = out["sum"].reshape(-1, 3)
         out["sum_sq"] = out["sum_sq"].reshape(-1, 3)
@@ -312,6 +322,8 @@ class PathTracer:
         if "per_sample" in out:
             out["per_sample"] = out["per_sample"].reshape(-1, 3)
         out["stats"] = stats
+        if guide is not None:
+            out["guide_stats"] = res[2]
         return out
 
     def resolve(self, film_dev):
@@ -324,6 +336,14 @@ class PathTracer:
                                                          _ptr(film_dev["count"]), _ptr(rgb), None),
                          "ajg_film_resolve")
         return rgb.cpu().numpy().reshape(h, w, 3)
+
+
+def guide_defaults(alpha=0.5, train=True, batch=1 << 18, lr=1e-3, train_slots=0, max_steps=0,
+                   steps_per_batch=1):
+    """ajg_guide_args: one-sample mixture with probability alpha, online Adam
+    training on batches of 2^18 records at lr 1e-3."""
+    return capi.GuideArgs(alpha, 1 if train else 0, batch, lr, train_slots, max_steps,
+                          steps_per_batch)
 
 
 class PathGuide:

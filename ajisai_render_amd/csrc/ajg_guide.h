@@ -12,8 +12,6 @@
 //   mu = mu_raw / |mu_raw|, kappa = exp(clamp(kappa_raw, -3, 7)), pi = softmax.
 #pragma once
 
-#include <hip/hip_bfloat16.h>
-
 #include "ajg_common.h"
 
 namespace ajg {
@@ -84,24 +82,17 @@ __host__ __device__ __forceinline__ float mixture_pdf(const VmfLobe *lobes, f3 w
   return p;
 }
 
-// Sample the mixture with three uniform numbers (lobe choice reuses u0).
-__device__ __forceinline__ f3 mixture_sample(const VmfLobe *lobes, float u0, float u1, float u2) {
-  int j = 0;
-  float c = 0.f;
-  for (; j < kGLobes - 1; ++j) {
-    c += lobes[j].w;
-    if (u0 < c) break;
-  }
-  const VmfLobe &l = lobes[j];
-  // w = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa (Jakob 2012)
+// Direction from one vMF lobe: w = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa
+// (Jakob 2012, "Numerically stable sampling of the von Mises-Fisher
+// distribution on S^2"), azimuth 2 pi u2 in a frame around mu (Duff et al.
+// 2017, "Building an orthonormal basis, revisited").
+__device__ __forceinline__ f3 vmf_sample_dir(const VmfLobe &l, float u1, float u2) {
   const float k = l.kappa;
   float wz = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k;
   wz = fminf(fmaxf(wz, -1.f), 1.f);
   const float s = sqrtf(fmaxf(0.f, 1.f - wz * wz));
-  const float phi = 6.28318531f * u2;
   float sp, cp;
-  sincosf(phi, &sp, &cp);
-  // orthonormal frame around mu (Duff et al.)
+  sincosf(6.28318531f * u2, &sp, &cp);
   const f3 n = l.mu;
   const float sign = copysignf(1.f, n.z);
   const float a = -1.f / (sign + n.z);
@@ -109,6 +100,47 @@ __device__ __forceinline__ f3 mixture_sample(const VmfLobe *lobes, float u0, flo
   const f3 t1 = mk3(1.f + sign * n.x * n.x * a, sign * b, -sign * n.x);
   const f3 t2 = mk3(b, sign + n.y * n.y * a, -n.y);
   return normalize(add(add(scale(t1, s * cp), scale(t2, s * sp)), scale(n, wz)));
+}
+
+// Sample the mixture: u0 picks the lobe, (u1, u2) the direction.
+__device__ __forceinline__ f3 mixture_sample(const VmfLobe *lobes, float u0, float u1, float u2) {
+  int j = 0;
+  float c = 0.f;
+  for (; j < kGLobes - 1; ++j) {
+    c += lobes[j].w;
+    if (u0 < c) break;
+  }
+  return vmf_sample_dir(lobes[j], u1, u2);
+}
+
+// The same mixture read from a decoded record in memory: lobe j at
+// m[5j..5j+4] = (mu xyz, kappa, weight). The renderer keeps one record per
+// closest-hit queue entry and evaluates it in place (no register copy).
+__device__ __forceinline__ float gmix_pdf(const float *m, f3 w) {
+  float p = 0.f;
+#pragma unroll
+  for (int j = 0; j < kGLobes; ++j) {
+    VmfLobe l;
+    l.mu = mk3(m[5 * j], m[5 * j + 1], m[5 * j + 2]);
+    l.kappa = m[5 * j + 3];
+    l.w = m[5 * j + 4];
+    p += l.w * vmf_pdf(l, w);
+  }
+  return p;
+}
+
+__device__ __forceinline__ f3 gmix_sample(const float *m, float u0, float u1, float u2) {
+  int j = 0;
+  float c = 0.f;
+  for (; j < kGLobes - 1; ++j) {
+    c += m[5 * j + 4];
+    if (u0 < c) break;
+  }
+  VmfLobe l[1];
+  l[0].mu = mk3(m[5 * j], m[5 * j + 1], m[5 * j + 2]);
+  l[0].kappa = m[5 * j + 3];
+  l[0].w = 1.f;
+  return vmf_sample_dir(l[0], u1, u2);
 }
 
 }  // namespace ajg

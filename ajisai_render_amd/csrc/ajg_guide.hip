@@ -146,16 +146,19 @@ __device__ __forceinline__ void encode_tile(uint16_t *xa, const float *pos, cons
   }
 }
 
-// Forward pass over a batch. pos/nrm: float rows with `stride` floats.
+// Forward pass over a batch. pos/nrm: float rows with `stride` floats; the
+// batch size is min(n, *n_dev) when n_dev is given.
 // mode 0: inference, writes the decoded mixture (40 floats / sample) to
 // `mix` and, if non-null, the raw outputs (48 / sample) to `raw`.
 // mode 1: training, writes activations x0..x3 (BF16) and raw outputs.
 __global__ void __launch_bounds__(kGThreads)
 k_guide_fwd(const uint16_t *wt, const float *w, const float *pos, const float *nrm, uint32_t n,
-            uint32_t stride, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
+            const uint32_t *n_dev, uint32_t stride, float bx0, float by0, float bz0, float bx1, float by1, float bz1,
             int mode, float *mix, float *raw, uint16_t *x0, uint16_t *x1, uint16_t *x2,
             uint16_t *x3) {
   __shared__ GuideLds s;
+  if (n_dev) n = min(n, *n_dev);  // queue size written on the device
+  if (n == 0) return;
   load_weights_lds(s, wt, w);
   const float box[6] = {bx0, by0, bz0, bx1, by1, bz1};
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -531,16 +534,62 @@ static void guide_free(ajg_guide *gd) {
 }
 
 int guide_forward_launch(ajg_guide *gd, const float *pos, const float *nrm, uint32_t n,
-                         uint32_t stride, float *mix, float *raw, hipStream_t s) {
+                         const uint32_t *n_dev, uint32_t stride, float *mix, float *raw,
+                         hipStream_t s) {
   if (n == 0) return AJG_OK;
   const uint32_t blocks = std::min<uint32_t>(1024, (n + 16 * kGWaves - 1) / (16 * kGWaves));
   hipLaunchKernelGGL(k_guide_fwd, dim3(blocks), dim3(kGThreads), 0, s, gd->wt, gd->w, pos, nrm, n,
-                     stride, gd->box[0], gd->box[1], gd->box[2], gd->box[3], gd->box[4], gd->box[5],
-                     0, mix, raw, (uint16_t *)nullptr, (uint16_t *)nullptr, (uint16_t *)nullptr,
-                     (uint16_t *)nullptr);
+                     n_dev, stride, gd->box[0], gd->box[1], gd->box[2], gd->box[3], gd->box[4],
+                     gd->box[5], 0, mix, raw, (uint16_t *)nullptr, (uint16_t *)nullptr,
+                     (uint16_t *)nullptr, (uint16_t *)nullptr);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AJG_OK : hip_fail(gd->ctx, e, "k_guide_fwd");
 }
+
+// One training step enqueued on s (no host synchronisation).
+int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr, hipStream_t s) {
+  ajg_ctx *ctx = gd->ctx;
+  if (int st = ensure_train(gd, n)) return st;
+  AJG_HIP(ctx, hipMemsetAsync(gd->scal, 0, 16, s));
+  hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(256), 0, s, samples, n, gd->scal);
+  const uint32_t fblocks = std::min<uint32_t>(1024, (n + 16 * kGWaves - 1) / (16 * kGWaves));
+  // samples rows: pos(3) nrm(3) dir(3) weight(1)
+  hipLaunchKernelGGL(k_guide_fwd, dim3(fblocks), dim3(kGThreads), 0, s, (const uint16_t *)gd->wt,
+                     (const float *)gd->w, samples, samples + 3, n, (const uint32_t *)nullptr, 10u,
+                     gd->box[0], gd->box[1], gd->box[2], gd->box[3], gd->box[4], gd->box[5], 1,
+                     (float *)nullptr, gd->out, gd->x0, gd->x1, gd->x2, gd->x3);
+  hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, s, (const float *)gd->out,
+                     samples, n, (const float *)gd->scal, gd->dy4, gd->scal + 1);
+  hipLaunchKernelGGL(k_guide_bwd_data, dim3(fblocks), dim3(kGThreads), 0, s, (const uint16_t *)gd->wi,
+                     (const uint16_t *)gd->dy4, (const uint16_t *)gd->x1, (const uint16_t *)gd->x2,
+                     (const uint16_t *)gd->x3, n, gd->dx3, gd->dx2, gd->dx1);
+  const uint32_t wb = kGradBlocks;
+  hipLaunchKernelGGL((k_guide_bwd_weight<kGHid, kGOut>), dim3(wb), dim3(kGThreads), 0, s,
+                     (const uint16_t *)gd->x3, (const uint16_t *)gd->dy4, n, gd->partial, kGW4, kGB4);
+  hipLaunchKernelGGL((k_guide_bwd_weight<kGHid, kGHid>), dim3(wb), dim3(kGThreads), 0, s,
+                     (const uint16_t *)gd->x2, (const uint16_t *)gd->dx3, n, gd->partial, kGW3, kGB3);
+  hipLaunchKernelGGL((k_guide_bwd_weight<kGHid, kGHid>), dim3(wb), dim3(kGThreads), 0, s,
+                     (const uint16_t *)gd->x1, (const uint16_t *)gd->dx2, n, gd->partial, kGW2, kGB2);
+  hipLaunchKernelGGL((k_guide_bwd_weight<kGIn, kGHid>), dim3(wb), dim3(kGThreads), 0, s,
+                     (const uint16_t *)gd->x0, (const uint16_t *)gd->dx1, n, gd->partial, kGW1, kGB1);
+  hipLaunchKernelGGL(k_guide_reduce, dim3((kGParams + 255) / 256), dim3(256), 0, s,
+                     (const float *)gd->partial, (int)wb, gd->grad);
+  if (lr > 0.f) {
+    gd->step += 1;
+    const float b1c = 1.f - std::pow(0.9f, (float)gd->step);
+    const float b2c = 1.f - std::pow(0.999f, (float)gd->step);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, gd->w, gd->m,
+                       gd->v, (const float *)gd->grad, lr, b1c, b2c);
+    hipLaunchKernelGGL(k_guide_cast, dim3((kGWeightCount + 255) / 256), dim3(256), 0, s,
+                       (const float *)gd->w, gd->wt, gd->wi);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(ctx, e, "guide train step");
+}
+
+int guide_steps(const ajg_guide *gd) { return gd->step; }
+const float *guide_loss_dev(const ajg_guide *gd) { return gd->scal + 1; }
+const float *guide_box(const ajg_guide *gd) { return gd->box; }
 
 }  // namespace ajg
 
@@ -638,49 +687,16 @@ int ajg_guide_forward(ajg_guide *gd, const float *pos, const float *nrm, uint32_
                       float *mixture, float *raw, void *stream) {
   if (!gd || (n && (!pos || !nrm))) return AJG_ERR_INVALID;
   if (int st = injected(gd->ctx)) return st;
-  return guide_forward_launch(gd, pos, nrm, n, 3, mixture, raw, (hipStream_t)stream);
+  return guide_forward_launch(gd, pos, nrm, n, nullptr, 3, mixture, raw, (hipStream_t)stream);
 }
 
 int ajg_guide_train_step(ajg_guide *gd, const float *samples, uint32_t n, float lr,
                          float *loss_out, float *grad_out, void *stream) {
-  if (!gd || !samples || n == 0) return AJG_ERR_INVALID;
+  if (!gd || !samples || n == 0 || !(lr >= 0.f)) return AJG_ERR_INVALID;
   ajg_ctx *ctx = gd->ctx;
   if (int st = injected(ctx)) return st;
-  if (int st = ensure_train(gd, n)) return st;
   hipStream_t s = (hipStream_t)stream;
-  AJG_HIP(ctx, hipMemsetAsync(gd->scal, 0, 16, s));
-  hipLaunchKernelGGL(k_guide_wsum, dim3(256), dim3(256), 0, s, samples, n, gd->scal);
-  const uint32_t fblocks = std::min<uint32_t>(1024, (n + 16 * kGWaves - 1) / (16 * kGWaves));
-  // samples rows: pos(3) nrm(3) dir(3) weight(1)
-  hipLaunchKernelGGL(k_guide_fwd, dim3(fblocks), dim3(kGThreads), 0, s, (const uint16_t *)gd->wt,
-                     (const float *)gd->w, samples, samples + 3, n, 10u, gd->box[0], gd->box[1],
-                     gd->box[2], gd->box[3], gd->box[4], gd->box[5], 1, (float *)nullptr, gd->out,
-                     gd->x0, gd->x1, gd->x2, gd->x3);
-  hipLaunchKernelGGL(k_guide_loss, dim3((n + 255) / 256), dim3(256), 0, s, (const float *)gd->out,
-                     samples, n, (const float *)gd->scal, gd->dy4, gd->scal + 1);
-  hipLaunchKernelGGL(k_guide_bwd_data, dim3(fblocks), dim3(kGThreads), 0, s, (const uint16_t *)gd->wi,
-                     (const uint16_t *)gd->dy4, (const uint16_t *)gd->x1, (const uint16_t *)gd->x2,
-                     (const uint16_t *)gd->x3, n, gd->dx3, gd->dx2, gd->dx1);
-  const uint32_t wb = kGradBlocks;
-  hipLaunchKernelGGL((k_guide_bwd_weight<kGHid, kGOut>), dim3(wb), dim3(kGThreads), 0, s,
-                     (const uint16_t *)gd->x3, (const uint16_t *)gd->dy4, n, gd->partial, kGW4, kGB4);
-  hipLaunchKernelGGL((k_guide_bwd_weight<kGHid, kGHid>), dim3(wb), dim3(kGThreads), 0, s,
-                     (const uint16_t *)gd->x2, (const uint16_t *)gd->dx3, n, gd->partial, kGW3, kGB3);
-  hipLaunchKernelGGL((k_guide_bwd_weight<kGHid, kGHid>), dim3(wb), dim3(kGThreads), 0, s,
-                     (const uint16_t *)gd->x1, (const uint16_t *)gd->dx2, n, gd->partial, kGW2, kGB2);
-  hipLaunchKernelGGL((k_guide_bwd_weight<kGIn, kGHid>), dim3(wb), dim3(kGThreads), 0, s,
-                     (const uint16_t *)gd->x0, (const uint16_t *)gd->dx1, n, gd->partial, kGW1, kGB1);
-  hipLaunchKernelGGL(k_guide_reduce, dim3((kGParams + 255) / 256), dim3(256), 0, s,
-                     (const float *)gd->partial, (int)wb, gd->grad);
-  if (lr > 0.f) {
-    gd->step += 1;
-    const float b1c = 1.f - std::pow(0.9f, (float)gd->step);
-    const float b2c = 1.f - std::pow(0.999f, (float)gd->step);
-    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, s, gd->w, gd->m,
-                       gd->v, (const float *)gd->grad, lr, b1c, b2c);
-    hipLaunchKernelGGL(k_guide_cast, dim3((kGWeightCount + 255) / 256), dim3(256), 0, s,
-                       (const float *)gd->w, gd->wt, gd->wi);
-  }
+  if (int st = guide_train_launch(gd, samples, n, lr, s)) return st;
   if (loss_out) {
     float sc[2];
     AJG_HIP(ctx, hipMemcpyAsync(sc, gd->scal, 8, hipMemcpyDeviceToHost, s));
@@ -691,8 +707,7 @@ int ajg_guide_train_step(ajg_guide *gd, const float *samples, uint32_t n, float 
     AJG_HIP(ctx, hipMemcpyAsync(grad_out, gd->grad, 4 * kGParams, hipMemcpyDeviceToHost, s));
     AJG_HIP(ctx, hipStreamSynchronize(s));
   }
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? AJG_OK : hip_fail(ctx, e, "guide train step");
+  return AJG_OK;
 }
 
 }  // extern "C"

@@ -58,6 +58,16 @@ int launch_trace_any(ajg_accel *a, const ajg_ray *rays, uint32_t n,
 int launch_trace_counts(ajg_accel *a, const ajg_ray *rays, uint32_t n,
                         uint32_t *nodes, uint32_t *tris, hipStream_t s);
 
+// Path-guide network (ajg_guide.hip). guide_forward_launch writes the
+// decoded mixture (40 floats per point) for min(n, *n_dev) points.
+int guide_forward_launch(ajg_guide *gd, const float *pos, const float *nrm, uint32_t n,
+                         const uint32_t *n_dev, uint32_t stride, float *mix, float *raw,
+                         hipStream_t s);
+int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr, hipStream_t s);
+int guide_steps(const ajg_guide *gd);
+const float *guide_loss_dev(const ajg_guide *gd);
+const float *guide_box(const ajg_guide *gd);
+
 // u32 scan on the device (ajg_scan.hip). tmp must hold scan_tmp_words(n).
 size_t scan_tmp_words(uint32_t n);
 void scan_u32(const uint32_t *in, uint32_t *out, uint32_t n, bool inclusive,

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "ajg_guide.h"
 #include "ajg_internal.h"
 #include "ajg_shade.h"
 #include "ajg_traverse.h"
@@ -38,6 +39,7 @@ struct ajg_scene {
   float radius = 0.f;
   int cu_count = 256;
   void *render_cache = nullptr;  // RenderBuffers reused across renders
+  void *guide_cache = nullptr;   // GuideBuffers reused across guided renders
 };
 
 namespace ajg {
@@ -140,7 +142,8 @@ static constexpr uint32_t kPendActive = 1u;
 static constexpr uint32_t kPendEnded = 2u;
 static constexpr uint32_t kPendLast = 4u;
 static constexpr uint32_t kPendMis = 8u;
-static constexpr uint32_t kPendLight0 = 16u;  // + k
+static constexpr uint32_t kPendHist = 16u;    // guided: the vertex has a history entry
+static constexpr uint32_t kPendLight0 = 32u;  // + k
 
 struct PathState {
   uint32_t *work;       // current work item (kNone = none)
@@ -174,6 +177,33 @@ struct Queues {
   float *per_sample;   // optional
   unsigned long long *counters;  // closest, any, paths
 };
+
+// Guided rendering state. The decoded guide mixture of closest-queue entry i
+// is mix[40 i ..]. Slots below train_slots record training data: the NEE
+// and MIS-BSDF directions of each vertex with their MIS-weighted direct
+// radiance, and (through a per-slot vertex history) each continuation
+// direction with the radiance the rest of the path brought back along it.
+// Training slots are spread over the slot range (every train_stride-th), so
+// the records cover the whole image.
+// Training record: pos(3) normal(3) direction(3) weight(1), weight =
+// luminance(L_in) |cos| / pdf.
+struct GuideParams {
+  const float *mix;
+  float alpha;          // probability of sampling the guide
+  uint32_t train_stride;  // every train_stride-th slot records (0: none)
+  uint32_t hist_depth;
+  float *hist;          // (slots / train_stride) x hist_depth x 16: pos nrm dir cw L_upto beta
+  uint32_t *hist_n;     // indexed by slot / train_stride
+  float *rec;
+  uint32_t *rec_count;
+  uint32_t rec_cap;
+  float *pend_pos;      // 3 per slot
+  float *pend_tw;       // kMaxLights per slot: NEE training weight before visibility
+};
+
+static constexpr int kGRec = 10;
+static constexpr int kGHist = 16;
+static constexpr int kGMix = 40;
 
 __device__ __forceinline__ void kahan_add(float &sum, float &c, float x) {
   const float y = x - c;
@@ -238,6 +268,52 @@ __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay 
 __device__ __forceinline__ f3 eps_offset(const Inct &it, f3 dir) {
   if (dot(dir, it.ng) > 0.f) return add(it.pos, scale(it.ng, AJG_EPS));
   return sub(it.pos, scale(it.ng, AJG_EPS));
+}
+
+__device__ __forceinline__ float lum_f(f3 c) {
+  return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+}
+
+// BSDF value as the sampling routine reports it: LambertianReflection's
+// Sample_f multiplies in the shading-normal correction that its f() leaves
+// out (the plain path's continuation and MIS-BSDF ray use the sampled
+// value), Disney's sample returns f() itself.
+__device__ __forceinline__ f3 bsdf_eval_sampled(const Bsdf &b, f3 wi, f3 wo) {
+  if (b.m.type == AJG_MAT_DISNEY) return disney_eval(b, wi, wo);
+  return scale(lambert_eval(b, wi, wo), correct_shading_normal(b.ng, b.ns, wi));
+}
+
+struct GSample {
+  f3 dir, f;
+  float pdf;
+  bool ok;
+};
+
+// One-sample MIS of the guide mixture and the BSDF: with probability alpha
+// the direction comes from the mixture, else from the BSDF sampler; the
+// estimator divides by the mixture density alpha q + (1 - alpha) p_bsdf.
+__device__ __forceinline__ GSample guided_sample(const Bsdf &bs, f3 wr, const float *gm,
+                                                 float alpha, Pcg32 &rng) {
+  const float usel = pcg_next_float(rng);
+  f3 sam;
+  sam.x = pcg_next_float(rng);
+  sam.y = pcg_next_float(rng);
+  sam.z = pcg_next_float(rng);
+  GSample g;
+  if (usel < alpha) {
+    g.dir = gmix_sample(gm, sam.x, sam.y, sam.z);
+    g.ok = true;
+  } else {
+    const BsdfSample s = bsdf_sample_all(bs, wr, sam);
+    g.ok = s.valid;
+    g.dir = normalize(s.dir);
+  }
+  if (!g.ok) return g;
+  g.f = bsdf_eval_sampled(bs, g.dir, wr);
+  const float pb = bsdf_pdf_all(bs, g.dir, wr);
+  g.pdf = alpha > 0.f ? alpha * gmix_pdf(gm, g.dir) + (1.f - alpha) * pb : pb;
+  g.ok = g.pdf > 0.f && isfinite(g.pdf) && !is_black(g.f);
+  return g;
 }
 
 __device__ __forceinline__ void store_ray(DevRay *q, uint32_t i, const DevRay &r) {
@@ -345,9 +421,9 @@ static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *
 // Event pairs around launches (AJG_RENDER_TIMING).
 struct StageTimer {
   bool on = false;
-  std::vector<hipEvent_t> evs[3];
-  size_t used[3] = {0, 0, 0};
-  uint32_t launches[3] = {0, 0, 0};
+  std::vector<hipEvent_t> evs[5];
+  size_t used[5] = {0, 0, 0, 0, 0};
+  uint32_t launches[5] = {0, 0, 0, 0, 0};
   hipEvent_t get(int k) {
     if (used[k] == evs[k].size()) {
       hipEvent_t e;
@@ -400,6 +476,33 @@ __device__ __forceinline__ void st_f3(float *p, uint32_t i, f3 v) {
   p[3 * (size_t)i] = v.x;
   p[3 * (size_t)i + 1] = v.y;
   p[3 * (size_t)i + 2] = v.z;
+}
+
+__device__ __forceinline__ void emit_record(const GuideParams &G, f3 pos, f3 nrm, f3 dir, float w) {
+  const bool want = w > 0.f && isfinite(w);
+  const uint32_t i = wave_append(want, G.rec_count);
+  if (want && i < G.rec_cap) {
+    float *r = G.rec + (size_t)i * kGRec;
+    st3(r, pos);
+    st3(r + 3, nrm);
+    st3(r + 6, dir);
+    r[9] = w;
+  }
+}
+
+// Path end: one record per recorded continuation k with
+// L_in = (L_final - L_upto_k) / beta_{k+1} (what the rest of the path
+// brought back along the direction, per unit throughput).
+__device__ void emit_history(const GuideParams &G, uint32_t hslot, f3 Lf) {
+  const uint32_t n = G.hist_n[hslot];
+  for (uint32_t k = 0; k < n; ++k) {
+    const float *h = G.hist + ((size_t)hslot * G.hist_depth + k) * kGHist;
+    const float lx = h[13] > 0.f ? fmaxf(Lf.x - h[10], 0.f) / h[13] : 0.f;
+    const float ly = h[14] > 0.f ? fmaxf(Lf.y - h[11], 0.f) / h[14] : 0.f;
+    const float lz = h[15] > 0.f ? fmaxf(Lf.z - h[12], 0.f) / h[15] : 0.f;
+    emit_record(G, ld3(h), ld3(h + 3), ld3(h + 6), lum_f(mk3(lx, ly, lz)) * h[9]);
+  }
+  G.hist_n[hslot] = 0;
 }
 
 // Accumulate a finished sample into its work item's bank; flush the bank
@@ -516,10 +619,13 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   }
 }
 
+template <bool GUIDED>
 __global__ void __launch_bounds__(256)
-k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
+k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
   const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
   const bool valid = slot < P.slots;
+  const bool train = GUIDED && valid && G.train_stride && slot % G.train_stride == 0;
+  const uint32_t hslot = train ? slot / G.train_stride : 0u;
   bool live = false;
   Pcg32 rng;
   rng.state = valid ? ps.rng[slot] : 0ull;
@@ -529,12 +635,22 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
     // ---- 1. finish the previous bounce's direct lighting
     const uint32_t pf = ps.pend_flags[slot];
     if (pf & kPendActive) {
+      f3 tpos, tnrm;
+      if (train) {
+        tpos = ld_f3(G.pend_pos, slot);
+        tnrm = normalize(ld_f3(ps.pend_ns, slot));
+      }
       f3 direct = mk3(0.f, 0.f, 0.f);
       for (uint32_t k = 0; k < sc.n_lights; ++k) {
         f3 c = mk3(0.f, 0.f, 0.f);
         if (pf & (kPendLight0 << k)) {
           const uint32_t si = ps.pend_shadow[(size_t)slot * kMaxLights + k];
-          if (!q.a_occ_in[si]) c = ld_f3(ps.pend_c, slot * kMaxLights + k);
+          if (!q.a_occ_in[si]) {
+            c = ld_f3(ps.pend_c, slot * kMaxLights + k);
+            if (train)
+              emit_record(G, tpos, tnrm, load_ray(q.a_rays_in, si).d,
+                          G.pend_tw[(size_t)slot * kMaxLights + k]);
+          }
         }
         direct = add(direct, c);
       }
@@ -554,12 +670,22 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
                                fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
             const float pdf = ps.pend_pdf[slot];
             const float lpdf = env_pdf(e, mr.d);
-            mis = scale(divs(f, pdf), power_heuristic(pdf, lpdf));
+            const float wb = power_heuristic(pdf, lpdf);
+            mis = scale(divs(f, pdf), wb);
+            if (train) emit_record(G, tpos, tnrm, mr.d, lum_f(le) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
           }
         }
       }
       direct = add(direct, mul(ld_f3(ps.pend_beta, slot), mis));
       const f3 Lf = add(ld_f3(ps.pend_L, slot), scale(direct, 1.f));
+      if (train) {
+        if (pf & kPendEnded) {
+          emit_history(G, hslot, Lf);
+        } else if (pf & kPendHist) {
+          const uint32_t hn = G.hist_n[hslot];
+          st3(G.hist + ((size_t)hslot * G.hist_depth + hn - 1) * kGHist + 10, Lf);
+        }
+      }
       if (pf & kPendEnded) {
         accumulate(ps, q, P, slot, ps.pend_bank[slot], ps.pend_work[slot], ps.pend_sample[slot],
                    Lf, (pf & kPendLast) != 0);
@@ -592,17 +718,33 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
       const bool last = sample + 1 >= ps.sample_end[slot];
       if (ls < 0) {
         if (b == 0 && sc.env_index >= 0) L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
+        if (train) emit_history(G, hslot, L);
         accumulate(ps, q, P, slot, ps.bank[slot], work, sample, L, last);
         need_new = true;
       } else {
         const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w);
         const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
+        const float *gm = GUIDED ? G.mix + (size_t)ei * kGMix : nullptr;
+        // The guide is mixed in only where the BSDF's pdf is the density of
+        // its own sampler (Lambert). The Disney sampler's selection between
+        // its specular and diffuse branches reuses the half-vector's random
+        // numbers (material/disney.cpp:113-130), so its Pdf (:186-220) is not
+        // that density
+        // and the plain estimator converges to E[f cos / Pdf] rather than to
+        // the integral of f cos; mixing another strategy in would change the
+        // value the render converges to. Disney vertices therefore sample
+        // the BSDF alone (alpha 0), exactly as the plain path does.
+        const float alpha = (GUIDED && bs.m.type != AJG_MAT_DISNEY) ? G.alpha : 0.f;
         uint32_t pf2 = kPendActive;
         st_f3(ps.pend_L, slot, L);
         st_f3(ps.pend_beta, slot, beta);
         ps.pend_work[slot] = work;
         ps.pend_sample[slot] = sample;
         ps.pend_bank[slot] = ps.bank[slot];
+        if (train) {
+          st_f3(G.pend_pos, slot, it.pos);
+          st_f3(ps.pend_ns, slot, it.ns);
+        }
         // NEE for each light (MISSampleLight, helper.h:100-133)
         for (uint32_t k = 0; k < sc.n_lights; ++k) {
           const DevLight &l = sc.lights[k];
@@ -626,9 +768,20 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
               if (!is_black(bf)) {
                 const f3 nrm = normalize(it.ns);
                 const f3 f = scale(mul(lsmp.radiance, bf), fabsf(dot(to_light, nrm)));
-                const float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
-                const f3 c = mul(beta, scale(divs(f, lsmp.pdf), power_heuristic(lsmp.pdf, bpdf)));
+                float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
+                // guided: the env light's MIS partner is the mixture density
+                // (a delta light is never reached by directional sampling, so
+                // its weight keeps the BSDF pdf, as in the plain path)
+                if (GUIDED && l.type == AJG_LIGHT_ENV && alpha > 0.f)
+                  bpdf = alpha * gmix_pdf(gm, to_light) + (1.f - alpha) * bpdf;
+                const float wl = power_heuristic(lsmp.pdf, bpdf);
+                const f3 c = mul(beta, scale(divs(f, lsmp.pdf), wl));
                 st_f3(ps.pend_c, slot * kMaxLights + k, c);
+                if (train)
+                  G.pend_tw[(size_t)slot * kMaxLights + k] =
+                      l.type == AJG_LIGHT_ENV
+                          ? lum_f(lsmp.radiance) * fabsf(dot(to_light, nrm)) / lsmp.pdf * wl
+                          : 0.f;
                 sr = make_ray(it.pos, to_light, AJG_EPS, 0.99f * len);
                 emit = true;
               }
@@ -643,20 +796,31 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
         }
         // MIS-BSDF sample (MISSampleBsdf, helper.h:209-246)
         {
-          f3 sam;
-          sam.x = pcg_next_float(rng);
-          sam.y = pcg_next_float(rng);
-          sam.z = pcg_next_float(rng);
-          BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
           bool emit = false;
           DevRay mr;
-          if (s.valid && !(is_black(s.f) || s.pdf == 0.f) && sc.env_index >= 0) {
-            const f3 dn = normalize(s.dir);
-            mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
-            st_f3(ps.pend_f, slot, s.f);
-            ps.pend_pdf[slot] = s.pdf;
-            st_f3(ps.pend_ns, slot, it.ns);
-            emit = true;
+          if (GUIDED) {
+            const GSample g = guided_sample(bs, it.wr, gm, alpha, rng);
+            if (g.ok && sc.env_index >= 0) {
+              mr = make_ray(eps_offset(it, g.dir), g.dir, AJG_EPS, __int_as_float(0x7f800000));
+              st_f3(ps.pend_f, slot, g.f);
+              ps.pend_pdf[slot] = g.pdf;
+              st_f3(ps.pend_ns, slot, it.ns);
+              emit = true;
+            }
+          } else {
+            f3 sam;
+            sam.x = pcg_next_float(rng);
+            sam.y = pcg_next_float(rng);
+            sam.z = pcg_next_float(rng);
+            BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
+            if (s.valid && !(is_black(s.f) || s.pdf == 0.f) && sc.env_index >= 0) {
+              const f3 dn = normalize(s.dir);
+              mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
+              st_f3(ps.pend_f, slot, s.f);
+              ps.pend_pdf[slot] = s.pdf;
+              st_f3(ps.pend_ns, slot, it.ns);
+              emit = true;
+            }
           }
           const uint32_t mi = wave_append(emit, q.a_count_out);
           if (emit) {
@@ -669,16 +833,31 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
         // bounce (pt.cpp:107-110)
         bool cont;
         DevRay nr;
+        float cw = 0.f;
         {
-          f3 sam;
-          sam.x = pcg_next_float(rng);
-          sam.y = pcg_next_float(rng);
-          sam.z = pcg_next_float(rng);
-          const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
-          cont = s.valid && !(is_black(s.f) || s.pdf < 0.f);
+          f3 dir, fv;
+          float pdf;
+          if (GUIDED) {
+            const GSample g = guided_sample(bs, it.wr, gm, alpha, rng);
+            cont = g.ok;
+            dir = g.dir;
+            fv = g.f;
+            pdf = g.pdf;
+          } else {
+            f3 sam;
+            sam.x = pcg_next_float(rng);
+            sam.y = pcg_next_float(rng);
+            sam.z = pcg_next_float(rng);
+            const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
+            cont = s.valid && !(is_black(s.f) || s.pdf < 0.f);
+            dir = s.dir;
+            fv = s.f;
+            pdf = s.pdf;
+          }
           if (cont) {
-            beta = mul(beta, divs(scale(s.f, fabsf(dot(it.ng, s.dir))), s.pdf));
-            nr = make_ray(eps_offset(it, s.dir), s.dir, AJG_EPS, __int_as_float(0x7f800000));
+            beta = mul(beta, divs(scale(fv, fabsf(dot(it.ng, dir))), pdf));
+            nr = make_ray(eps_offset(it, dir), dir, AJG_EPS, __int_as_float(0x7f800000));
+            if (train) cw = fabsf(dot(normalize(it.ns), dir)) / pdf;
             const int nb = (int)b + 1;
             if (nb >= P.max_bounces) {
               cont = false;
@@ -696,6 +875,19 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
           ps.ext_idx[slot] = ni;
           ps.bounce[slot] = b + 1;
           st_f3(ps.beta, slot, beta);
+          if (train) {
+            const uint32_t hn = G.hist_n[hslot];
+            if (hn < G.hist_depth) {
+              float *hr = G.hist + ((size_t)hslot * G.hist_depth + hn) * kGHist;
+              st3(hr, it.pos);
+              st3(hr + 3, normalize(it.ns));
+              st3(hr + 6, nr.d);
+              hr[9] = cw;
+              st3(hr + 13, beta);
+              G.hist_n[hslot] = hn + 1;
+              pf2 |= kPendHist;
+            }
+          }
         } else {
           pf2 |= kPendEnded;
           if (last) pf2 |= kPendLast;
@@ -709,15 +901,40 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q) {
 
   // ---- 3. next sample / ray generation (wave-uniform control flow)
   next_sample(ps, q, P, sc, slot, valid && need_new, rng, live);
-  if (valid) {
-    if (!need_new && ps.ext_idx[slot] != kNone && (ps.pend_flags[slot] & kPendActive) == 0 &&
-        ps.work[slot] != kNone) {
-      // extension ray just emitted above; nothing else to do
-    }
-    ps.rng[slot] = rng.state;
-  }
+  if (valid) ps.rng[slot] = rng.state;
   const bool any_live = __ballot(valid && live) != 0ull;
   if ((threadIdx.x & 63) == 0 && any_live) atomicAdd(q.live, 1u);
+}
+
+// Guided rendering: position and unit shading normal of each closest-queue
+// hit (the guide network's input); misses get a zero record.
+__global__ void __launch_bounds__(256)
+k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t *count,
+             float *pts) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const float4 h = hits[i];
+    const int ls = __float_as_int(h.y);
+    f3 p = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 1.f);
+    if (ls >= 0) {
+      const DevRay r = load_ray(rays, i);
+      const Inct it = post_intersect(sc, r, h.x, ls, h.z, h.w);
+      p = it.pos;
+      nn = normalize(it.ns);
+    }
+    float *o = pts + (size_t)i * 6;
+    st3(o, p);
+    st3(o + 3, nn);
+  }
+}
+
+// Guided rendering: clear the training-record counter after the host has
+// consumed the records, adding the count to the running total.
+__global__ void k_rec_reset(uint32_t *rec_count, unsigned long long *total) {
+  if (threadIdx.x == 0) {
+    *total += *rec_count;
+    *rec_count = 0;
+  }
 }
 
 // Per-chunk partial sums -> film (Kahan over chunks in chunk order).
@@ -810,6 +1027,50 @@ struct RenderBuffers {
   ~RenderBuffers() { release(); }
 };
 
+// Guided-rendering buffers (kept on the scene next to RenderBuffers).
+struct GuideBuffers {
+  uint32_t slots = 0, train_slots = 0, depth = 0, rec_cap = 0;
+  std::vector<void *> bufs;
+  float *pts = nullptr, *mix = nullptr, *pend_pos = nullptr, *pend_tw = nullptr;
+  float *hist = nullptr, *rec = nullptr;
+  uint32_t *hist_n = nullptr;
+  void release() {
+    for (void *p : bufs) (void)hipFree(p);
+    bufs.clear();
+    slots = train_slots = depth = rec_cap = 0;
+  }
+  ~GuideBuffers() { release(); }
+};
+
+static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint32_t T, uint32_t D,
+                                uint32_t cap) {
+  if (gb->slots >= S && gb->train_slots >= T && gb->depth >= D && gb->rec_cap >= cap &&
+      !gb->bufs.empty())
+    return AJG_OK;
+  gb->release();
+  auto al = [&](void **p, size_t bytes, const char *what) {
+    int r = dev_alloc(ctx, p, bytes, what);
+    if (r == AJG_OK) gb->bufs.push_back(*p);
+    return r;
+  };
+  int st = 0;
+  if ((st = al((void **)&gb->pts, 24 * (size_t)S, "guide hit points")) ||
+      (st = al((void **)&gb->mix, 4 * (size_t)kGMix * S, "guide mixtures")) ||
+      (st = al((void **)&gb->pend_pos, 12 * (size_t)S, "guide pending positions")) ||
+      (st = al((void **)&gb->pend_tw, 4 * (size_t)kMaxLights * S, "guide NEE weights")) ||
+      (st = al((void **)&gb->hist, 4 * (size_t)kGHist * T * D, "guide vertex history")) ||
+      (st = al((void **)&gb->hist_n, 4 * (size_t)T, "guide history sizes")) ||
+      (st = al((void **)&gb->rec, 4 * (size_t)kGRec * cap, "guide training records"))) {
+    gb->release();
+    return st;
+  }
+  gb->slots = S;
+  gb->train_slots = T;
+  gb->depth = D;
+  gb->rec_cap = cap;
+  return AJG_OK;
+}
+
 static constexpr size_t kCounterWords = 64;  // 256 bytes
 
 static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t total) {
@@ -866,6 +1127,7 @@ static void scene_free(ajg_scene *s) {
   (void)hipFree(s->mats);
   for (void *p : s->env_bufs) (void)hipFree(p);
   delete (RenderBuffers *)s->render_cache;
+  delete (GuideBuffers *)s->guide_cache;
   delete s;
 }
 
@@ -1089,12 +1351,20 @@ int ajg_scene_env_tables(ajg_scene *s, uint32_t li, float *cond_cdf, float *cond
   return AJG_OK;
 }
 
-int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
-                  const ajg_film *film, ajg_render_stats *stats, void *stream) {
+}  // extern "C"
+
+static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
+                       ajg_guide *gd, const ajg_guide_args *ga, const ajg_film *film,
+                       ajg_render_stats *stats, ajg_guide_stats *gstats, void *stream) {
   if (!sc || !camera || !args || !film || !film->sum || !film->sum_sq || !film->count)
     return AJG_ERR_INVALID;
   ajg_ctx *ctx = sc->ctx;
   if (int st = injected(ctx)) return st;
+  const bool guided = gd != nullptr;
+  if (guided) {
+    if (!ga || !(ga->alpha >= 0.f && ga->alpha <= 1.f) || (ga->train && (ga->batch == 0 || !(ga->lr > 0.f))))
+      return fail(ctx, AJG_ERR_INVALID, "guided render: need 0 <= alpha <= 1, batch > 0 and lr > 0");
+  }
   if (args->spp <= 0 || args->tile_size <= 0 || args->max_bounces < 0 || !args->use_mis)
     return fail(ctx, AJG_ERR_INVALID, "render: need spp > 0, tile_size > 0, max_bounces >= 0, use_mis");
   const float rx = camera->resolution[0], ry = camera->resolution[1];
@@ -1159,6 +1429,39 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   }
   if (int st = ensure_buffers(ctx, rb, S, total)) return st;
   PathState ps = rb->ps;
+  GuideParams G;
+  memset(&G, 0, sizeof(G));
+  GuideBuffers *gb = nullptr;
+  uint32_t batch = 0;
+  if (guided) {
+    gb = (GuideBuffers *)sc->guide_cache;
+    if (!gb) {
+      gb = new GuideBuffers();
+      sc->guide_cache = gb;
+    }
+    // default: every slot records, within a 256 MiB vertex-history budget
+    const uint32_t D0 = (uint32_t)std::max(1, std::min(args->max_bounces, 32));
+    const uint32_t by_mem = (uint32_t)((256ull << 20) / (4ull * kGHist * D0));
+    const uint32_t want = ga->train ? (ga->train_slots ? std::min(ga->train_slots, S)
+                                                       : std::min(S, by_mem))
+                                    : 0u;
+    const uint32_t stride = want ? cdiv(S, want) : 0u;
+    const uint32_t T = stride ? cdiv(S, stride) : 0u;  // slots with slot % stride == 0
+    const uint32_t D = (uint32_t)std::max(1, std::min(args->max_bounces, 32));
+    batch = ga->train ? ga->batch : 0u;
+    const uint32_t cap = ga->train ? std::max<uint32_t>(4u * batch, 1u << 16) : 1u;
+    if (int st = ensure_guide_buffers(ctx, gb, S, std::max(T, 1u), D, cap)) return st;
+    G.mix = gb->mix;
+    G.alpha = ga->alpha;
+    G.train_stride = stride;
+    G.hist_depth = D;
+    G.hist = gb->hist;
+    G.hist_n = gb->hist_n;
+    G.rec = gb->rec;
+    G.rec_cap = cap;
+    G.pend_pos = gb->pend_pos;
+    G.pend_tw = gb->pend_tw;
+  }
   Queues q;
   memset(&q, 0, sizeof(q));
   const size_t cq_cap = (size_t)S, aq_cap = (size_t)(kMaxLights + 1) * S;
@@ -1195,6 +1498,10 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   AJG_RCHK(hipMemsetAsync(counters, 0, 4 * kCounterWords, s));
   AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
   AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
+  if (guided) {
+    AJG_RCHK(hipMemsetAsync(gb->hist_n, 0, 4 * (size_t)gb->train_slots, s));
+    G.rec_count = counters + 24;  // u32; running total (u64) at word 26
+  }
   const size_t n_pix_sz = P.n_pix;
   AJG_RCHK(hipMemsetAsync(film->sum, 0, 12 * n_pix_sz, s));
   AJG_RCHK(hipMemsetAsync(film->sum_sq, 0, 12 * n_pix_sz, s));
@@ -1219,6 +1526,9 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
   (void)hipEventRecord(ev0, s);
   int cur = 0;
   uint32_t iters = 0;
+  uint32_t train_steps = 0;
+  // an untrained guide is not sampled (alpha 0 until the first step)
+  if (guided && batch && guide_steps(gd) == 0) G.alpha = 0.f;
   const uint32_t shade_grid = cdiv(S, 256);
   const uint32_t trav_grid = (uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU;
   // counter words: 0-1 closest queue sizes, 2-3 shadow queue sizes, 4 live
@@ -1239,7 +1549,10 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
       q.a_count_out = counters + 2 + nxt;
       hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
       timer.begin(2, s);
-      hipLaunchKernelGGL(k_shade, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q);
+      if (guided)
+        hipLaunchKernelGGL(k_shade<true>, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G);
+      else
+        hipLaunchKernelGGL(k_shade<false>, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G);
       timer.end(2, s);
       timer.begin(0, s);
       if (counting)
@@ -1257,12 +1570,44 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
         launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
                          counters + 7, trav_ctr + 2, trav_grid, s);
       timer.end(1, s);
+      if (guided) {
+        // guide mixtures of the new closest hits, read by the next shade
+        timer.begin(3, s);
+        hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(S, 256), 4096)), dim3(256), 0, s,
+                           ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
+                           (const uint32_t *)(counters + nxt), gb->pts);
+        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, S, counters + nxt, 6, gb->mix,
+                                          nullptr, s))
+          return done(st);
+        timer.end(3, s);
+      }
       cur = nxt;
       ++iters;
     }
-    uint32_t live = 0;
-    AJG_RCHK(hipMemcpyAsync(&live, counters + 4, 4, hipMemcpyDeviceToHost, s));
+    uint32_t hc[32];
+    AJG_RCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     AJG_RCHK(hipStreamSynchronize(s));
+    const uint32_t live = hc[4];
+    if (guided && batch) {
+      const uint32_t avail = std::min(hc[24], G.rec_cap);
+      if (avail >= batch) {
+        timer.begin(4, s);
+        for (uint32_t j = 0; j + batch <= avail; j += batch) {
+          if (ga->max_steps && train_steps >= ga->max_steps) break;
+          for (uint32_t e = 0; e < std::max(1u, ga->steps_per_batch); ++e) {
+            if (ga->max_steps && train_steps >= ga->max_steps) break;
+            if (int st = guide_train_launch(gd, gb->rec + (size_t)j * kGRec, batch, ga->lr, s))
+              return done(st);
+            ++train_steps;
+          }
+        }
+        hipLaunchKernelGGL(k_rec_reset, dim3(1), dim3(64), 0, s, counters + 24,
+                           (unsigned long long *)(counters + 26));
+        timer.end(4, s);
+        // the guide is sampled once it has been trained at least once
+        G.alpha = ga->alpha;
+      }
+    }
     if (live == 0) break;
     if (iters > 200000) return done(fail(ctx, AJG_ERR_INTERNAL, "render: iteration limit"));
   }
@@ -1305,8 +1650,38 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
       stats->tris_tested_any = tc[3];
     }
   }
+  if (gstats) {
+    memset(gstats, 0, sizeof(*gstats));
+    if (guided) {
+      gstats->train_steps = train_steps;
+      unsigned long long rt = 0;
+      uint32_t rc = 0;
+      AJG_RCHK(hipMemcpy(&rt, counters + 26, 8, hipMemcpyDeviceToHost));
+      AJG_RCHK(hipMemcpy(&rc, counters + 24, 4, hipMemcpyDeviceToHost));
+      gstats->records = rt + rc;
+      if (timer.on) {
+        gstats->ms_infer = timer.total_ms(3);
+        gstats->ms_train = timer.total_ms(4);
+      }
+      if (train_steps) AJG_RCHK(hipMemcpy(&gstats->last_loss, guide_loss_dev(gd), 4, hipMemcpyDeviceToHost));
+    }
+  }
 #undef AJG_RCHK
   return done(AJG_OK);
+}
+
+extern "C" {
+
+int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
+                  const ajg_film *film, ajg_render_stats *stats, void *stream) {
+  return render_impl(sc, camera, args, nullptr, nullptr, film, stats, nullptr, stream);
+}
+
+int ajg_render_pt_guided(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
+                         ajg_guide *guide, const ajg_guide_args *gargs, const ajg_film *film,
+                         ajg_render_stats *stats, ajg_guide_stats *gstats, void *stream) {
+  if (!guide || !gargs) return AJG_ERR_INVALID;
+  return render_impl(sc, camera, args, guide, gargs, film, stats, gstats, stream);
 }
 
 int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size, const float *sum,

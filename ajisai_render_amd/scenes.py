@@ -361,6 +361,68 @@ def heavy_mesh(seed: int = 20261204, subdiv: int = 8, res=(1024, 1024), spp=64,
                  PTSpec(spp=spp, tile_size=16, min_bounces=5, max_bounces=16, cont_prob=0.9))
 
 
+def flat_mesh(pos4, faces, material, two_sided=True) -> Primitive:
+    """MeshGeo with unshared vertices whose normals are the face normals, so
+    the interpolated shading normal equals the geometric one."""
+    p = np.asarray(pos4, np.float64)[:, :3]
+    f = np.asarray(faces, np.int64)
+    tri = p[f]                                   # (m, 3, 3)
+    fn = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True) + 1e-30
+    m = f.shape[0]
+    pos = np.concatenate([tri.reshape(-1, 3), np.zeros((3 * m, 1))], axis=1)
+    nrm = np.concatenate([np.repeat(fn, 3, axis=0), np.zeros((3 * m, 1))], axis=1)
+    idx = np.arange(3 * m, dtype=np.uint32).reshape(m, 3)
+    return mesh(pos.astype(np.float32), nrm.astype(np.float32), idx, material, two_sided)
+
+
+def occluded_room(seed: int = 20261206, res=(512, 512), spp=64, env_size=(1024, 512),
+                  furniture=((6, 0.55), (6, 0.5), (5, 0.35), (5, 0.3))) -> Scene:
+    """Closed 4 x 3 x 4 room (x, y, z in [0,4] x [0,3] x [0,4]) whose x = 4
+    wall is pierced by a 0.2 x 1.0 slit (z in [1.9, 2.1], y in [1.2, 2.2]);
+    lit only by the heavy-mesh scene's HDR env map through the slit. Furniture:
+    flat-shaded displaced icospheres (20*4^s triangles each; 204,800 for the
+    default set) on box pedestals, Lambert or Disney (clearcoat 0) from the
+    seed. 512 x 512, max 16 bounces, Russian roulette after bounce 3 (p 0.9)."""
+    rng = np.random.default_rng(seed)
+    wall = Material(MAT_DIFFUSE, (0.75, 0.75, 0.75))
+    floor = Material(MAT_DIFFUSE, (0.62, 0.56, 0.5))
+    mats = [wall, floor]
+    X, Y, Z = 4.0, 3.0, 4.0
+    z0, z1, y0, y1 = 1.9, 2.1, 1.2, 2.2
+    prims = [
+        quad((0, 0, 0), (X, 0, 0), (X, 0, Z), (0, 0, Z), 1),          # floor
+        quad((0, Y, 0), (0, Y, Z), (X, Y, Z), (X, Y, 0), 0),          # ceiling
+        quad((0, 0, 0), (0, 0, Z), (0, Y, Z), (0, Y, 0), 0),          # x = 0
+        quad((0, 0, 0), (0, Y, 0), (X, Y, 0), (X, 0, 0), 0),          # z = 0
+        quad((0, 0, Z), (X, 0, Z), (X, Y, Z), (0, Y, Z), 0),          # z = Z
+        # x = X wall around the slit
+        quad((X, 0, 0), (X, Y, 0), (X, Y, z0), (X, 0, z0), 0),
+        quad((X, 0, z1), (X, Y, z1), (X, Y, Z), (X, 0, Z), 0),
+        quad((X, 0, z0), (X, y0, z0), (X, y0, z1), (X, 0, z1), 0),
+        quad((X, y1, z0), (X, Y, z0), (X, Y, z1), (X, y1, z1), 0),
+    ]
+    spots = [(1.2, 1.1), (2.7, 1.0), (1.0, 2.9), (2.6, 3.1)]
+    for k, ((sub, rad), (cx, cz)) in enumerate(zip(furniture, spots)):
+        if rng.random() < 0.5:
+            m = Material(MAT_DIFFUSE, tuple(float(c) for c in rng.uniform(0.2, 0.9, 3)))
+        else:
+            m = _disney_from(rng)
+            m.clearcoat = 0.0
+        mats.append(m)
+        ped = 0.25 + 0.35 * rng.random()
+        prims.append(cube(translate(cx, ped / 2, cz) @ scale3(0.7, ped, 0.7), 0))
+        pos4, _, f = displaced_icosphere(sub, seed + 10 + k, amplitude=0.15, radius=rad,
+                                         center=(cx, ped + rad, cz))
+        prims.append(flat_mesh(pos4, f, len(mats) - 1, two_sided=True))
+    env = hdr_env(env_size[0], env_size[1], 20261204 + 2)
+    lights = [Light(LIGHT_ENV, env=env, rot=0.0, scaler=1.0)]
+    cam = CameraSpec(pos=(0.5, 1.5, 0.5), look_at=(3.3, 0.8, 3.0), up=(0.0, 1.0, 0.0),
+                     fov=75.0, res=res)
+    return Scene("occluded_room", prims, mats, lights, cam,
+                 PTSpec(spp=spp, tile_size=16, min_bounces=3, max_bounces=16, cont_prob=0.9))
+
+
 def random_soup(n: int, seed: int, extent=1.0, size=0.05) -> np.ndarray:
     """(n, 9) float32 random triangle soup in [-extent, extent]^3."""
     rng = np.random.default_rng(seed)

@@ -312,6 +312,46 @@ int ajg_guide_train_step(ajg_guide *guide, const float *samples, uint32_t n,
                          float lr, float *loss_out, float *grad_out,
                          void *stream);
 
+/* Guided rendering options. */
+typedef struct ajg_guide_args {
+  float alpha;          /* probability of sampling the guide instead of the
+                           BSDF (one-sample MIS: pdf = alpha q + (1-alpha) p) */
+  int32_t train;        /* 1: train online on the render's own radiance
+                           samples; 0: use the guide as it is */
+  uint32_t batch;       /* training records per Adam step (2^18) */
+  float lr;             /* Adam learning rate (1e-3) */
+  uint32_t train_slots; /* path slots that record training data (0: all,
+                           within a 256 MiB vertex-history budget) */
+  uint32_t max_steps;   /* stop training after this many steps (0: none) */
+  uint32_t steps_per_batch; /* Adam steps on each batch (0 or 1: one) */
+} ajg_guide_args;
+
+typedef struct ajg_guide_stats {
+  uint32_t train_steps;   /* Adam steps taken during this render */
+  uint32_t reserved;
+  uint64_t records;       /* training records produced */
+  float ms_infer;         /* AJG_RENDER_TIMING: hit points + MLP inference */
+  float ms_train;         /* AJG_RENDER_TIMING: training steps */
+  float last_loss;        /* loss of the last training step (0 if none) */
+  float reserved2;
+} ajg_guide_stats;
+
+/* ajg_render_pt with learned path guiding (no reference counterpart; the
+ * BSDF-only sampling of integrator/pt.cpp:215-222 and helper.h:209-246 is
+ * replaced by the one-sample mixture of the guide and the BSDF for the
+ * continuation and the MIS-BSDF ray at Lambertian vertices, and env-light
+ * NEE weights use the mixture pdf there). Disney vertices keep BSDF-only
+ * sampling: the reference's Disney Pdf() is not the density of its sampler,
+ * so mixing in another strategy would move the converged value away from
+ * the plain render's. The expectation equals the plain render's wherever
+ * the BSDF's sampled value equals its evaluation (flat-shaded geometry).
+ * Synchronous. */
+int ajg_render_pt_guided(ajg_scene *scene, const ajg_camera *camera,
+                         const ajg_pt_args *args, ajg_guide *guide,
+                         const ajg_guide_args *gargs, const ajg_film *film,
+                         ajg_render_stats *stats, ajg_guide_stats *gstats,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif
