@@ -97,13 +97,14 @@ class RenderStats(C.Structure):
 class GuideArgs(C.Structure):
     _fields_ = [("alpha", C.c_float), ("train", C.c_int32), ("batch", C.c_uint32),
                 ("lr", C.c_float), ("train_slots", C.c_uint32), ("max_steps", C.c_uint32),
-                ("steps_per_batch", C.c_uint32)]
+                ("steps_per_batch", C.c_uint32), ("warmup_passes", C.c_uint32),
+                ("warmup_spp", C.c_uint32), ("warmup_steps_per_batch", C.c_uint32)]
 
 
 class GuideStats(C.Structure):
-    _fields_ = [("train_steps", C.c_uint32), ("reserved", C.c_uint32), ("records", C.c_uint64),
+    _fields_ = [("train_steps", C.c_uint32), ("warmup_steps", C.c_uint32), ("records", C.c_uint64),
                 ("ms_infer", C.c_float), ("ms_train", C.c_float), ("last_loss", C.c_float),
-                ("reserved2", C.c_float)]
+                ("ms_warmup", C.c_float)]
 
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])

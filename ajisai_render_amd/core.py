@@ -339,11 +339,11 @@ class PathTracer:
 
 
 def guide_defaults(alpha=0.5, train=True, batch=1 << 18, lr=1e-3, train_slots=0, max_steps=0,
-                   steps_per_batch=1):
+                   steps_per_batch=4, warmup_passes=0, warmup_spp=16, warmup_steps_per_batch=64):
     """ajg_guide_args: one-sample mixture with probability alpha, online Adam
-    training on batches of 2^18 records at lr 1e-3."""
+    training on batches of 2^18 records at lr 1e-3, optional warm-up passes."""
     return capi.GuideArgs(alpha, 1 if train else 0, batch, lr, train_slots, max_steps,
-                          steps_per_batch)
+                          steps_per_batch, warmup_passes, warmup_spp, warmup_steps_per_batch)
 
 
 class PathGuide:

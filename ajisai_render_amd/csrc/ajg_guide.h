@@ -5,8 +5,13 @@
 //
 // Network: x(112) -> 64 -> 64 -> 64 -> 48, ReLU on the hidden layers, BF16
 // weights and activations, FP32 accumulation (MFMA 16x16x16 bf16).
-//   x = [p_hat(3), (sin, cos)(pi*(k+1)*n_c) for c in xyz, k in 0..15 (96), 0 (13)]
-//   p_hat = 2 * (p - box_min) / (box_max - box_min) - 1
+// Input features x[16j + 4g + i], j = 0..6, g = 0..3, i = 0..3:
+//   j < 6: component c = j / 2, harmonic h = 8 (j % 2) + 2g + i / 2,
+//          x = sin(pi (h+1) n_c) for even i, cos(pi (h+1) n_c) for odd i;
+//   j = 6: x[96 + t] = p_hat_t for t < 3, zero for features 99..111;
+//   p_hat = 2 * (p - box_min) / (box_max - box_min) - 1.
+// (Every 4 consecutive features are two whole harmonics of one component, so
+// each lane of an MFMA operand fragment computes its own four in place.)
 // Output o[0..39] (o[40..47] unused): lobe j uses o[5j..5j+4] =
 //   (mu_raw xyz, kappa_raw, weight logit);
 //   mu = mu_raw / |mu_raw|, kappa = exp(clamp(kappa_raw, -3, 7)), pi = softmax.
@@ -34,19 +39,6 @@ static constexpr int kGW4 = kGB3 + kGHid;
 static constexpr int kGB4 = kGW4 + kGHid * kGOut;
 static constexpr int kGParams = kGB4 + kGOut;  // 18672
 static constexpr float kKappaLo = -3.f, kKappaHi = 7.f;
-
-// Feature f of the input encoding of a shading point (p, n).
-__device__ __forceinline__ float guide_feature(f3 p, f3 n, const float *box, int f) {
-  if (f < 3) {
-    const float pc = f == 0 ? p.x : (f == 1 ? p.y : p.z);
-    return 2.f * (pc - box[f]) / (box[3 + f] - box[f]) - 1.f;
-  }
-  if (f >= kGUsedIn) return 0.f;
-  const int c = (f - 3) / (2 * kGFreq), k = ((f - 3) % (2 * kGFreq)) >> 1;
-  const float nc = c == 0 ? n.x : (c == 1 ? n.y : n.z);
-  const float a = 3.14159265f * (float)(k + 1) * nc;
-  return ((f - 3) & 1) ? cosf(a) : sinf(a);
-}
 
 struct VmfLobe {
   f3 mu;
@@ -113,34 +105,39 @@ __device__ __forceinline__ f3 mixture_sample(const VmfLobe *lobes, float u0, flo
   return vmf_sample_dir(lobes[j], u1, u2);
 }
 
-// The same mixture read from a decoded record in memory: lobe j at
-// m[5j..5j+4] = (mu xyz, kappa, weight). The renderer keeps one record per
-// closest-hit queue entry and evaluates it in place (no register copy).
-__device__ __forceinline__ float gmix_pdf(const float *m, f3 w) {
-  float p = 0.f;
+// The mixture as the renderer stores it per closest-hit queue entry: 40
+// FP16 values, lobe j at m[5j..5j+4] = (mu xyz, kappa, weight). Sampling and
+// density both decode the stored values the same way (mu renormalised,
+// weights divided by their sum), so the density is exactly that of the
+// sampler whatever the FP16 rounding did.
+struct GMixLobes {
+  VmfLobe l[kGLobes];
+};
+
+__device__ __forceinline__ GMixLobes gmix_load(const _Float16 *m) {
+  GMixLobes g;
+  float sw = 0.f;
 #pragma unroll
   for (int j = 0; j < kGLobes; ++j) {
-    VmfLobe l;
-    l.mu = mk3(m[5 * j], m[5 * j + 1], m[5 * j + 2]);
-    l.kappa = m[5 * j + 3];
-    l.w = m[5 * j + 4];
-    p += l.w * vmf_pdf(l, w);
+    g.l[j].mu = normalize(mk3((float)m[5 * j], (float)m[5 * j + 1], (float)m[5 * j + 2]));
+    g.l[j].kappa = (float)m[5 * j + 3];
+    g.l[j].w = (float)m[5 * j + 4];
+    sw += g.l[j].w;
   }
-  return p;
+  const float inv = 1.f / sw;
+#pragma unroll
+  for (int j = 0; j < kGLobes; ++j) g.l[j].w *= inv;
+  return g;
 }
 
-__device__ __forceinline__ f3 gmix_sample(const float *m, float u0, float u1, float u2) {
-  int j = 0;
-  float c = 0.f;
-  for (; j < kGLobes - 1; ++j) {
-    c += m[5 * j + 4];
-    if (u0 < c) break;
-  }
-  VmfLobe l[1];
-  l[0].mu = mk3(m[5 * j], m[5 * j + 1], m[5 * j + 2]);
-  l[0].kappa = m[5 * j + 3];
-  l[0].w = 1.f;
-  return vmf_sample_dir(l[0], u1, u2);
+__device__ __forceinline__ float gmix_pdf(const _Float16 *m, f3 w) {
+  const GMixLobes g = gmix_load(m);
+  return mixture_pdf(g.l, w);
+}
+
+__device__ __forceinline__ f3 gmix_sample(const _Float16 *m, float u0, float u1, float u2) {
+  const GMixLobes g = gmix_load(m);
+  return mixture_sample(g.l, u0, u1, u2);
 }
 
 }  // namespace ajg

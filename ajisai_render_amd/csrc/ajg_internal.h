@@ -59,11 +59,15 @@ int launch_trace_counts(ajg_accel *a, const ajg_ray *rays, uint32_t n,
                         uint32_t *nodes, uint32_t *tris, hipStream_t s);
 
 // Path-guide network (ajg_guide.hip). guide_forward_launch writes the
-// decoded mixture (40 floats per point) for min(n, *n_dev) points.
+// decoded mixture (40 values per point: FP32 to mix, FP16 to mixh, each
+// optional) for min(n, *n_dev) points.
 int guide_forward_launch(ajg_guide *gd, const float *pos, const float *nrm, uint32_t n,
-                         const uint32_t *n_dev, uint32_t stride, float *mix, float *raw,
-                         hipStream_t s);
+                         const uint32_t *n_dev, uint32_t stride, float *mix, _Float16 *mixh,
+                         float *raw, hipStream_t s);
 int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr, hipStream_t s);
+// The guide's training-record pool (grown to at least cap rows of 10 floats;
+// growing empties it); ctr[0] = rows held, ctr[2..3] = u64 running total.
+int guide_records(ajg_guide *gd, uint32_t cap, float **rec, uint32_t **ctr, uint32_t *cap_out);
 int guide_steps(const ajg_guide *gd);
 const float *guide_loss_dev(const ajg_guide *gd);
 const float *guide_box(const ajg_guide *gd);

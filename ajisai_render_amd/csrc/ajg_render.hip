@@ -179,16 +179,15 @@ struct Queues {
 };
 
 // Guided rendering state. The decoded guide mixture of closest-queue entry i
-// is mix[40 i ..]. Slots below train_slots record training data: the NEE
-// and MIS-BSDF directions of each vertex with their MIS-weighted direct
+// is mix[40 i ..] (FP16). Training slots (every train_stride-th slot, so the
+// records cover the whole image) record training data: the NEE and
+// MIS-BSDF directions of each vertex with their MIS-weighted direct
 // radiance, and (through a per-slot vertex history) each continuation
 // direction with the radiance the rest of the path brought back along it.
-// Training slots are spread over the slot range (every train_stride-th), so
-// the records cover the whole image.
 // Training record: pos(3) normal(3) direction(3) weight(1), weight =
 // luminance(L_in) |cos| / pdf.
 struct GuideParams {
-  const float *mix;
+  const _Float16 *mix;
   float alpha;          // probability of sampling the guide
   uint32_t train_stride;  // every train_stride-th slot records (0: none)
   uint32_t hist_depth;
@@ -292,7 +291,7 @@ struct GSample {
 // One-sample MIS of the guide mixture and the BSDF: with probability alpha
 // the direction comes from the mixture, else from the BSDF sampler; the
 // estimator divides by the mixture density alpha q + (1 - alpha) p_bsdf.
-__device__ __forceinline__ GSample guided_sample(const Bsdf &bs, f3 wr, const float *gm,
+__device__ __forceinline__ GSample guided_sample(const Bsdf &bs, f3 wr, const _Float16 *gm,
                                                  float alpha, Pcg32 &rng) {
   const float usel = pcg_next_float(rng);
   f3 sam;
@@ -724,7 +723,7 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
       } else {
         const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w);
         const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
-        const float *gm = GUIDED ? G.mix + (size_t)ei * kGMix : nullptr;
+        const _Float16 *gm = GUIDED ? G.mix + (size_t)ei * kGMix : nullptr;
         // The guide is mixed in only where the BSDF's pdf is the density of
         // its own sampler (Lambert). The Disney sampler's selection between
         // its specular and diffuse branches reuses the half-vector's random
@@ -1029,23 +1028,23 @@ struct RenderBuffers {
 
 // Guided-rendering buffers (kept on the scene next to RenderBuffers).
 struct GuideBuffers {
-  uint32_t slots = 0, train_slots = 0, depth = 0, rec_cap = 0;
+  uint32_t slots = 0, train_slots = 0, depth = 0;
   std::vector<void *> bufs;
-  float *pts = nullptr, *mix = nullptr, *pend_pos = nullptr, *pend_tw = nullptr;
-  float *hist = nullptr, *rec = nullptr;
+  float *pts = nullptr, *pend_pos = nullptr, *pend_tw = nullptr;
+  _Float16 *mix = nullptr;
+  float *hist = nullptr;
   uint32_t *hist_n = nullptr;
   void release() {
     for (void *p : bufs) (void)hipFree(p);
     bufs.clear();
-    slots = train_slots = depth = rec_cap = 0;
+    slots = train_slots = depth = 0;
   }
   ~GuideBuffers() { release(); }
 };
 
-static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint32_t T, uint32_t D,
-                                uint32_t cap) {
-  if (gb->slots >= S && gb->train_slots >= T && gb->depth >= D && gb->rec_cap >= cap &&
-      !gb->bufs.empty())
+static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint32_t T,
+                                uint32_t D) {
+  if (gb->slots >= S && gb->train_slots >= T && gb->depth >= D && !gb->bufs.empty())
     return AJG_OK;
   gb->release();
   auto al = [&](void **p, size_t bytes, const char *what) {
@@ -1055,19 +1054,17 @@ static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint
   };
   int st = 0;
   if ((st = al((void **)&gb->pts, 24 * (size_t)S, "guide hit points")) ||
-      (st = al((void **)&gb->mix, 4 * (size_t)kGMix * S, "guide mixtures")) ||
+      (st = al((void **)&gb->mix, 2 * (size_t)kGMix * S, "guide mixtures")) ||
       (st = al((void **)&gb->pend_pos, 12 * (size_t)S, "guide pending positions")) ||
       (st = al((void **)&gb->pend_tw, 4 * (size_t)kMaxLights * S, "guide NEE weights")) ||
       (st = al((void **)&gb->hist, 4 * (size_t)kGHist * T * D, "guide vertex history")) ||
-      (st = al((void **)&gb->hist_n, 4 * (size_t)T, "guide history sizes")) ||
-      (st = al((void **)&gb->rec, 4 * (size_t)kGRec * cap, "guide training records"))) {
+      (st = al((void **)&gb->hist_n, 4 * (size_t)T, "guide history sizes"))) {
     gb->release();
     return st;
   }
   gb->slots = S;
   gb->train_slots = T;
   gb->depth = D;
-  gb->rec_cap = cap;
   return AJG_OK;
 }
 
@@ -1449,16 +1446,19 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     const uint32_t T = stride ? cdiv(S, stride) : 0u;  // slots with slot % stride == 0
     const uint32_t D = (uint32_t)std::max(1, std::min(args->max_bounces, 32));
     batch = ga->train ? ga->batch : 0u;
-    const uint32_t cap = ga->train ? std::max<uint32_t>(4u * batch, 1u << 16) : 1u;
-    if (int st = ensure_guide_buffers(ctx, gb, S, std::max(T, 1u), D, cap)) return st;
+    if (int st = ensure_guide_buffers(ctx, gb, S, std::max(T, 1u), D)) return st;
+    // records accumulate in the guide's pool across renders until a batch is full
+    uint32_t cap = 0;
+    if (int st = guide_records(gd, ga->train ? std::max<uint32_t>(4u * batch, 1u << 16) : 1u,
+                               &G.rec, &G.rec_count, &cap))
+      return st;
+    G.rec_cap = cap;
     G.mix = gb->mix;
     G.alpha = ga->alpha;
     G.train_stride = stride;
     G.hist_depth = D;
     G.hist = gb->hist;
     G.hist_n = gb->hist_n;
-    G.rec = gb->rec;
-    G.rec_cap = cap;
     G.pend_pos = gb->pend_pos;
     G.pend_tw = gb->pend_tw;
   }
@@ -1498,9 +1498,15 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   AJG_RCHK(hipMemsetAsync(counters, 0, 4 * kCounterWords, s));
   AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
   AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
+  unsigned long long rec_before = 0;
   if (guided) {
     AJG_RCHK(hipMemsetAsync(gb->hist_n, 0, 4 * (size_t)gb->train_slots, s));
-    G.rec_count = counters + 24;  // u32; running total (u64) at word 26
+    uint32_t rc[4] = {0, 0, 0, 0};
+    AJG_RCHK(hipMemcpyAsync(rc, G.rec_count, 16, hipMemcpyDeviceToHost, s));
+    AJG_RCHK(hipStreamSynchronize(s));
+    unsigned long long rt;
+    memcpy(&rt, rc + 2, 8);
+    rec_before = rt + rc[0];
   }
   const size_t n_pix_sz = P.n_pix;
   AJG_RCHK(hipMemsetAsync(film->sum, 0, 12 * n_pix_sz, s));
@@ -1576,33 +1582,34 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
         hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(S, 256), 4096)), dim3(256), 0, s,
                            ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
                            (const uint32_t *)(counters + nxt), gb->pts);
-        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, S, counters + nxt, 6, gb->mix,
-                                          nullptr, s))
+        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, S, counters + nxt, 6, nullptr,
+                                          gb->mix, nullptr, s))
           return done(st);
         timer.end(3, s);
       }
       cur = nxt;
       ++iters;
     }
-    uint32_t hc[32];
+    uint32_t hc[8], held = 0;
     AJG_RCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    if (guided && batch) AJG_RCHK(hipMemcpyAsync(&held, G.rec_count, 4, hipMemcpyDeviceToHost, s));
     AJG_RCHK(hipStreamSynchronize(s));
     const uint32_t live = hc[4];
     if (guided && batch) {
-      const uint32_t avail = std::min(hc[24], G.rec_cap);
+      const uint32_t avail = std::min(held, G.rec_cap);
       if (avail >= batch) {
         timer.begin(4, s);
         for (uint32_t j = 0; j + batch <= avail; j += batch) {
           if (ga->max_steps && train_steps >= ga->max_steps) break;
           for (uint32_t e = 0; e < std::max(1u, ga->steps_per_batch); ++e) {
             if (ga->max_steps && train_steps >= ga->max_steps) break;
-            if (int st = guide_train_launch(gd, gb->rec + (size_t)j * kGRec, batch, ga->lr, s))
+            if (int st = guide_train_launch(gd, G.rec + (size_t)j * kGRec, batch, ga->lr, s))
               return done(st);
             ++train_steps;
           }
         }
-        hipLaunchKernelGGL(k_rec_reset, dim3(1), dim3(64), 0, s, counters + 24,
-                           (unsigned long long *)(counters + 26));
+        hipLaunchKernelGGL(k_rec_reset, dim3(1), dim3(64), 0, s, G.rec_count,
+                           (unsigned long long *)(G.rec_count + 2));
         timer.end(4, s);
         // the guide is sampled once it has been trained at least once
         G.alpha = ga->alpha;
@@ -1654,11 +1661,11 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     memset(gstats, 0, sizeof(*gstats));
     if (guided) {
       gstats->train_steps = train_steps;
-      unsigned long long rt = 0;
-      uint32_t rc = 0;
-      AJG_RCHK(hipMemcpy(&rt, counters + 26, 8, hipMemcpyDeviceToHost));
-      AJG_RCHK(hipMemcpy(&rc, counters + 24, 4, hipMemcpyDeviceToHost));
-      gstats->records = rt + rc;
+      uint32_t rc[4] = {0, 0, 0, 0};
+      AJG_RCHK(hipMemcpy(rc, G.rec_count, 16, hipMemcpyDeviceToHost));
+      unsigned long long rt;
+      memcpy(&rt, rc + 2, 8);
+      gstats->records = rt + rc[0] - rec_before;
       if (timer.on) {
         gstats->ms_infer = timer.total_ms(3);
         gstats->ms_train = timer.total_ms(4);
@@ -1680,8 +1687,46 @@ int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *ar
 int ajg_render_pt_guided(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
                          ajg_guide *guide, const ajg_guide_args *gargs, const ajg_film *film,
                          ajg_render_stats *stats, ajg_guide_stats *gstats, void *stream) {
-  if (!guide || !gargs) return AJG_ERR_INVALID;
-  return render_impl(sc, camera, args, guide, gargs, film, stats, gstats, stream);
+  if (!sc || !args || !guide || !gargs) return AJG_ERR_INVALID;
+  ajg_guide_stats total;
+  memset(&total, 0, sizeof(total));
+  if (gargs->warmup_passes) {
+    if (gargs->warmup_spp == 0 || gargs->batch == 0 || !(gargs->lr > 0.f))
+      return fail(sc->ctx, AJG_ERR_INVALID, "guided render: warm-up needs spp, batch and lr");
+    // Warm-up passes render into the caller's film (cleared again by the
+    // final render) from seed_mode 1 streams whose base no render uses.
+    ajg_pt_args wa = *args;
+    wa.spp = (int32_t)gargs->warmup_spp;
+    wa.seed_mode = 1;
+    ajg_guide_args wg = *gargs;
+    wg.train = 1;
+    wg.steps_per_batch = gargs->warmup_steps_per_batch;
+    wg.warmup_passes = 0;
+    for (uint32_t k = 0; k < gargs->warmup_passes; ++k) {
+      wa.seed_base = (args->seed_base ^ 0x5bd1e995u) + 0x9e3779b9u * (k + 1);
+      ajg_render_stats ws;
+      ajg_guide_stats wgs;
+      if (int st = render_impl(sc, camera, &wa, guide, &wg, film, &ws, &wgs, stream)) return st;
+      total.train_steps += wgs.train_steps;
+      total.records += wgs.records;
+      total.ms_infer += wgs.ms_infer;
+      total.ms_train += wgs.ms_train;
+      total.ms_warmup += ws.ms_total;
+      if (wgs.train_steps) total.last_loss = wgs.last_loss;
+    }
+    total.warmup_steps = total.train_steps;
+  }
+  ajg_guide_stats fs;
+  if (int st = render_impl(sc, camera, args, guide, gargs, film, stats, &fs, stream)) return st;
+  if (gstats) {
+    *gstats = total;
+    gstats->train_steps += fs.train_steps;
+    gstats->records += fs.records;
+    gstats->ms_infer += fs.ms_infer;
+    gstats->ms_train += fs.ms_train;
+    if (fs.train_steps) gstats->last_loss = fs.last_loss;
+  }
+  return AJG_OK;
 }
 
 int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size, const float *sum,

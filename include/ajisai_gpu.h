@@ -318,22 +318,31 @@ typedef struct ajg_guide_args {
                            BSDF (one-sample MIS: pdf = alpha q + (1-alpha) p) */
   int32_t train;        /* 1: train online on the render's own radiance
                            samples; 0: use the guide as it is */
-  uint32_t batch;       /* training records per Adam step (2^18) */
+  uint32_t batch;       /* training records per Adam step (2^18); records
+                           collect in the guide across renders until a batch
+                           is full */
   float lr;             /* Adam learning rate (1e-3) */
   uint32_t train_slots; /* path slots that record training data (0: all,
                            within a 256 MiB vertex-history budget) */
   uint32_t max_steps;   /* stop training after this many steps (0: none) */
   uint32_t steps_per_batch; /* Adam steps on each batch (0 or 1: one) */
+  /* Warm-up: before the render, `warmup_passes` guided passes of
+   * `warmup_spp` samples per pixel train the guide (steps_per_batch
+   * `warmup_steps_per_batch`); their images are discarded. They draw from
+   * random streams independent of the render's own (seed_mode 1 streams
+   * from a seed base no render uses), so the trained guide is independent
+   * of the final samples. */
+  uint32_t warmup_passes, warmup_spp, warmup_steps_per_batch;
 } ajg_guide_args;
 
 typedef struct ajg_guide_stats {
-  uint32_t train_steps;   /* Adam steps taken during this render */
-  uint32_t reserved;
-  uint64_t records;       /* training records produced */
+  uint32_t train_steps;   /* Adam steps taken (warm-up included) */
+  uint32_t warmup_steps;  /* of which during the warm-up passes */
+  uint64_t records;       /* training records produced (warm-up included) */
   float ms_infer;         /* AJG_RENDER_TIMING: hit points + MLP inference */
   float ms_train;         /* AJG_RENDER_TIMING: training steps */
   float last_loss;        /* loss of the last training step (0 if none) */
-  float reserved2;
+  float ms_warmup;        /* device time of the warm-up passes */
 } ajg_guide_stats;
 
 /* ajg_render_pt with learned path guiding (no reference counterpart; the

@@ -60,9 +60,10 @@ def bf16(x):
 
 
 def encode(pos, nrm, box, f32=True):
-    """Input features: p_hat = 2 (p - min) / (max - min) - 1, then for each
-    normal component c and k = 0..15: sin(pi (k+1) n_c), cos(pi (k+1) n_c);
-    zero padding to 112."""
+    """Input features x[16j + 4g + i] (j = 0..6, g = 0..3, i = 0..3): for
+    j < 6, component c = j // 2 and harmonic h = 8 (j % 2) + 2g + i // 2 give
+    sin(pi (h+1) n_c) (even i) or cos(pi (h+1) n_c) (odd i); features 96..98
+    are p_hat = 2 (p - min) / (max - min) - 1, features 99..111 are zero."""
     box = np.asarray(box, np.float64)
     dt = np.float32 if f32 else np.float64
     pos = np.asarray(pos, dt)
@@ -71,13 +72,15 @@ def encode(pos, nrm, box, f32=True):
     x = np.zeros((n, N_IN), np.float64)
     lo = box[:3].astype(dt)
     hi = box[3:].astype(dt)
-    x[:, :3] = dt(2) * (pos - lo) / (hi - lo) - dt(1)
-    k = np.arange(1, N_FREQ + 1, dtype=dt)
+    x[:, 96:99] = dt(2) * (pos - lo) / (hi - lo) - dt(1)
     pi = dt(3.14159265) if f32 else np.pi
-    for c in range(3):
-        a = (pi * k)[None, :] * nrm[:, c:c + 1]
-        x[:, 3 + 32 * c:3 + 32 * c + 32:2] = np.sin(a)
-        x[:, 4 + 32 * c:3 + 32 * c + 32:2] = np.cos(a)
+    for j in range(6):
+        c = j // 2
+        for g in range(4):
+            for i in range(4):
+                h = 8 * (j % 2) + 2 * g + i // 2
+                a = dt(pi * dt(h + 1)) * nrm[:, c]
+                x[:, 16 * j + 4 * g + i] = np.sin(a) if i % 2 == 0 else np.cos(a)
     return x
 
 

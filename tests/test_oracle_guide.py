@@ -44,13 +44,25 @@ def test_encode_known_values():
     pos = np.array([[-1.0, -0.5, -2.0], [3.0, 2.5, 2.0], [1.0, 1.0, 0.0]])
     nrm = np.array([[0.0, 0.0, 1.0], [0.5, -0.5, 0.0], [1.0, 0.0, 0.0]])
     x = G.encode(pos, nrm, BOX, f32=False)
-    assert np.allclose(x[0, :3], -1) and np.allclose(x[1, :3], 1) and np.allclose(x[2, :3], 0)
-    # n_z = 1: sin(pi k) = 0, cos(pi k) = (-1)^k
-    k = np.arange(1, 17)
-    assert np.allclose(x[0, 3 + 64:3 + 96:2], 0, atol=1e-12)
-    assert np.allclose(x[0, 4 + 64:3 + 96:2], (-1.0) ** k)
-    # n_x = 0.5, k = 1: sin(pi/2) = 1, cos = 0
-    assert np.isclose(x[1, 3], 1) and np.isclose(x[1, 4], 0, atol=1e-12)
+    assert np.allclose(x[0, 96:99], -1) and np.allclose(x[1, 96:99], 1)
+    assert np.allclose(x[2, 96:99], 0)
+    # n_z = 1 (features 64..95): sin(pi m) = 0, cos(pi m) = (-1)^m for harmonic m = h + 1
+    for j in (4, 5):
+        for g in range(4):
+            for i in range(4):
+                m = 8 * (j % 2) + 2 * g + i // 2 + 1
+                want = 0.0 if i % 2 == 0 else (-1.0) ** m
+                assert abs(x[0, 16 * j + 4 * g + i] - want) < 1e-12
+    # n_x = 0.5, harmonic 1: sin(pi/2) = 1, cos = 0; harmonic 2: sin(pi) = 0, cos = -1
+    assert np.allclose(x[1, 0:4], [1, 0, 0, -1], atol=1e-12)
+    # every (component, harmonic, sin/cos) appears exactly once
+    rng = np.random.default_rng(2)
+    nn = rng.normal(size=(5, 3))
+    xs = G.encode(np.zeros((5, 3)), nn, BOX, f32=False)
+    ref = [np.sin(np.pi * k * nn[:, c]) for c in range(3) for k in range(1, 17)] + \
+          [np.cos(np.pi * k * nn[:, c]) for c in range(3) for k in range(1, 17)]
+    ref = np.sort(np.stack(ref, 1), axis=1)
+    assert np.allclose(np.sort(xs[:, :96], axis=1), ref)
     assert np.all(x[:, 99:] == 0)
 
 
