@@ -13,6 +13,15 @@ Prints one JSON line: paths/s over the timed steps (whole job), Mrays/s, PLOC
 build Mtris/s, the roofline of the dominant kernel (closest-hit traversal)
 and the CPU baseline (the oracle's all-core CPU restatement of the reference
 path tracer on a bounded sample of the same workload).
+
+The "guided" object is the equal-time comparison of learned path guiding
+against plain path tracing on the occluded room (4 x 3 x 4 room lit only
+through a 0.2 x 1.0 slit by the same HDR map, 204,866 triangles, 512 x 512,
+max 16 bounces): each method renders for the same wall-clock budget (the
+guided one's warm-up training included), and relMSE = mean over pixels of
+mean_c (I - R)^2 / (R_lum^2 + 0.01 mean(R)^2) against a high-spp reference R
+(guided by default, unbiased; --guided-ref plain for a plain-PT reference).
+"mlp" times the guide network's kernels alone (MFMA utilisation).
 """
 import argparse
 import json
@@ -42,12 +51,147 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--guided-budget", type=float, default=10.0,
+                    help="equal-time budget (s) of the guided vs plain comparison; 0 skips it")
+    ap.add_argument("--guided-res", type=int, default=512)
+    ap.add_argument("--guided-ref", choices=["guided", "plain"], default="guided")
+    ap.add_argument("--guided-ref-spp", type=int, default=4096)
     return ap.parse_args()
+
+
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOPS_PER_POINT = 2 * (112 * 64 + 64 * 64 * 2 + 64 * 48)  # forward multiply-adds x 2
+
+
+def mlp_bench(ctx):
+    """Guide network kernels alone: inference over 2^20 points and one
+    training step (forward + loss + backward + Adam) on 2^18 records."""
+    import numpy as np
+    import torch
+    from ajisai_render_amd.core import PathGuide
+    rng = np.random.default_rng(1)
+    box = np.array([0, 0, 0, 4, 3, 4], np.float32)
+    g = PathGuide(ctx, box, seed=3)
+    n = 1 << 20
+    pos = torch.from_numpy(rng.uniform(0, 3, (n, 3)).astype(np.float32)).cuda()
+    nrm = torch.nn.functional.normalize(torch.randn(n, 3), dim=1).cuda()
+    nb = 1 << 18
+    smp = torch.cat([pos[:nb], nrm[:nb], torch.nn.functional.normalize(torch.randn(nb, 3), dim=1).cuda(),
+                     torch.rand(nb, 1).cuda()], dim=1).contiguous()
+    for _ in range(3):
+        g.forward(pos, nrm, want_raw=False)
+        g.train_step(smp, lr=1e-3)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(20):
+        g.forward(pos, nrm, want_raw=False)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t) / 20
+    t = time.perf_counter()
+    for _ in range(20):
+        g.train_step(smp, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = (time.perf_counter() - t) / 20
+    g.close()
+    fwd_tf = MLP_FLOPS_PER_POINT * n / t_fwd / 1e12
+    train_tf = 3 * MLP_FLOPS_PER_POINT * nb / t_train / 1e12
+    return {"inference_points": n, "inference_ms": round(t_fwd * 1e3, 3),
+            "inference_tflops": round(fwd_tf, 1),
+            "train_batch": nb, "train_step_ms": round(t_train * 1e3, 3),
+            "train_tflops": round(train_tf, 1), "peak_bf16_tflops": BF16_PEAK_TFLOPS,
+            "inference_frac": round(fwd_tf / BF16_PEAK_TFLOPS, 4),
+            "train_frac": round(train_tf / BF16_PEAK_TFLOPS, 4)}
+
+
+def _img(out):
+    import numpy as np
+    n = out["count"].astype(np.float64)
+    return out["sum"].astype(np.float64) / np.maximum(n, 1)[:, None]
+
+
+def relmse(img, ref):
+    import numpy as np
+    eps = 1e-2 * float(np.mean(ref)) ** 2
+    return float(np.mean(np.mean((img - ref) ** 2, axis=1) / (np.mean(ref, axis=1) ** 2 + eps)))
+
+
+def guided_equal_time(ctx, a):
+    """Equal-time relMSE of guided vs plain path tracing on the occluded room."""
+    import numpy as np
+    import torch
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import GpuScene, PathGuide, PathTracer, guide_defaults
+    sc = scenes.occluded_room(res=(a.guided_res, a.guided_res), spp=16)
+    g = GpuScene(ctx, sc)
+    box = sc.scene_box()
+    ga = guide_defaults(warmup_passes=8, warmup_spp=16, warmup_steps_per_batch=64, train=False)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o = fn()
+        torch.cuda.synchronize()
+        return o, time.perf_counter() - t0
+
+    # reference
+    if a.guided_ref == "guided":
+        gr = PathGuide(ctx, box, seed=99)
+        ref, t_ref = timed(lambda: PathTracer(g, spp=a.guided_ref_spp, seed_mode=1, seed_base=901)
+                           .Render(guide=gr, guide_args=ga))
+        gr.close()
+    else:
+        acc, t_ref = None, 0.0
+        chunk = min(1024, a.guided_ref_spp)
+        for k in range(max(1, a.guided_ref_spp // chunk)):
+            o, dt = timed(lambda: PathTracer(g, spp=chunk, seed_mode=1, seed_base=901 + k).Render())
+            t_ref += dt
+            acc = o if acc is None else {"sum": acc["sum"] + o["sum"], "sum_sq": acc["sum_sq"] + o["sum_sq"],
+                                         "count": acc["count"] + o["count"]}
+            print(f"reference chunk {k}: {dt:.1f} s", file=sys.stderr, flush=True)
+        ref = acc
+    R = _img(ref)
+    n = ref["count"].astype(np.float64)
+    var_mean = np.maximum(ref["sum_sq"].astype(np.float64) / n[:, None] - R * R, 0) / n[:, None]
+    eps = 1e-2 * float(np.mean(R)) ** 2
+    ref_self = float(np.mean(np.mean(var_mean, axis=1) / (np.mean(R, axis=1) ** 2 + eps)))
+    # plain: calibrate, then render for the budget
+    _, t32 = timed(lambda: PathTracer(g, spp=32, seed_mode=1, seed_base=2).Render())
+    sp = max(1, int(a.guided_budget / (t32 / 32)))
+    op, tp = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=3).Render())
+    rp = relmse(_img(op), R)
+    # guided: calibrate warm-up time and per-spp time with a throw-away guide
+    gd = PathGuide(ctx, box, seed=5)
+    oc, tc = timed(lambda: PathTracer(g, spp=32, seed_mode=1, seed_base=4).Render(guide=gd, guide_args=ga))
+    gd.close()
+    per_spp = oc["stats"].ms_total * 1e-3 / 32
+    t_warm = tc - oc["stats"].ms_total * 1e-3
+    sg = max(1, int((a.guided_budget - t_warm) / per_spp))
+    gd = PathGuide(ctx, box, seed=6)
+    og, tg = timed(lambda: PathTracer(g, spp=sg, seed_mode=1, seed_base=5).Render(guide=gd, guide_args=ga))
+    gst = og["guide_stats"]
+    gd.close()
+    rg = relmse(_img(og), R)
+    g.close()
+    return {
+        "workload": f"occluded_room: 4x3x4 room, 0.2x1.0 slit, HDR env 1024x512 through the slit, "
+                    f"{sc.n_tri} triangles, {a.guided_res}x{a.guided_res}, max 16 bounces",
+        "budget_s": a.guided_budget,
+        "reference": {"kind": a.guided_ref, "spp": a.guided_ref_spp, "seconds": round(t_ref, 2),
+                      "mean": float(R.mean()), "relmse_self_estimate": round(ref_self, 5)},
+        "plain": {"spp": sp, "seconds": round(tp, 3), "relmse": round(rp, 5)},
+        "guided": {"spp": sg, "seconds": round(tg, 3), "relmse": round(rg, 5),
+                   "warmup_seconds": round(gst.ms_warmup * 1e-3, 3),
+                   "warmup": "8 passes x 16 spp, 64 Adam steps per 2^18-record batch, lr 1e-3",
+                   "train_steps": gst.train_steps, "records": gst.records,
+                   "last_loss": round(gst.last_loss, 4), "alpha": 0.5},
+        "equal_time_relmse_ratio": round((rp * tp) / (rg * tg), 3),
+    }
 
 
 def cpu_baseline(scene, target_s):
     """Oracle (CPU restatement, oracle/liborc.so) on the host cores over a
-    bounded band of image rows through the image centre."""
+    bounded sample of image rows spread evenly over the image (bit-reversed
+    row order, so any prefix samples every part of the image alike)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -55,14 +199,9 @@ def cpu_baseline(scene, target_s):
     cam = o.camera()
     args = o.args()
     wp, hp = o.traced_dims()
-    # render 8-row bands alternating outward from the image centre until the
-    # time budget is spent
-    order = []
-    mid = hp // 2
-    for k in range(hp):
-        r = mid + (k // 2 if k % 2 == 0 else -(k // 2 + 1))
-        if 0 <= r < hp:
-            order.append(r)
+    # rows in bit-reversed order until the time budget is spent
+    bits = max(1, (hp - 1).bit_length())
+    order = [r for r in (int(format(k, f"0{bits}b")[::-1], 2) for k in range(1 << bits)) if r < hp]
     rows, queries, dt = 0, 0, 0.0
     for i in range(0, len(order), 8):
         band = sorted(order[i:i + 8])
@@ -78,7 +217,7 @@ def cpu_baseline(scene, target_s):
     n_paths = rows * wp * args.spp
     o.close()
     return {"value": n_paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} image rows x {wp} px x {args.spp} spp around the image centre "
+            "sample": f"{rows} image rows (bit-reversed order) x {wp} px x {args.spp} spp "
                       f"({n_paths} paths, {dt:.1f} s, {queries / dt / 1e6:.2f} Mrays/s)"}
 
 
@@ -149,6 +288,9 @@ def main():
                 "nodes_per_query": round(cs.nodes_fetched / max(1, closest_q), 2),
                 "tris_per_query": round(cs.tris_tested / max(1, closest_q), 2)}
 
+    mlp = mlp_bench(ctx)
+    guided = guided_equal_time(ctx, a) if a.guided_budget > 0 else None
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, a.cpu_seconds)
@@ -181,7 +323,11 @@ def main():
                               "trace_any": round(sum(s.ms_trace_any for s in stats) / a.steps, 3),
                               "shade": round(sum(s.ms_shade for s in stats) / a.steps, 3)},
         "wavefront_iterations": stats[0].iterations,
-        "relmse_guided_vs_plain": None,
+        # relMSE(guided) / relMSE(plain) at equal time (time-normalised)
+        "relmse_guided_vs_plain": (round(1.0 / guided["equal_time_relmse_ratio"], 4)
+                                   if guided else None),
+        "guided": guided,
+        "mlp": mlp,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
