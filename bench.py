@@ -56,7 +56,20 @@ def parse():
     ap.add_argument("--guided-res", type=int, default=512)
     ap.add_argument("--guided-ref", choices=["guided", "plain"], default="guided")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel() * 4, "copies": 3}))
 
 
 BF16_PEAK_TFLOPS = 2500.0
@@ -188,6 +201,23 @@ def guided_equal_time(ctx, a):
     }
 
 
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_pq_closest_heavy_mesh_1024_64spp.txt")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the closest-hit traversal kernel as measured by
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a
+    plain copy); None when that summary is absent."""
+    try:
+        with open(PROFILE_TRAFFIC) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch k_pq_closest:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(scene, target_s):
     """Oracle (CPU restatement, oracle/liborc.so) on the host cores over a
     bounded sample of image rows spread evenly over the image (bit-reversed
@@ -223,6 +253,9 @@ def cpu_baseline(scene, target_s):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); "
               "this build measures a single MI355X", file=sys.stderr)
@@ -279,9 +312,11 @@ def main():
         alg_bytes = (closest_q * (RAY_BYTES + HIT_BYTES) + cs.nodes_fetched * FAT_NODE_BYTES +
                      cs.tris_tested * LEAF_TRI_BYTES)
         achieved = alg_bytes / (ms_closest * 1e-3) / 1e9
+        traffic = profiled_traffic()
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_q_closest (closest-hit traversal)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/k_pq_closest_heavy_mesh_1024_64spp.txt" if traffic else None,
+                "kernel": "k_pq_closest (closest-hit traversal)",
                 "launches_per_step": launches_closest,
                 "avg_launch_ms": round(ms_closest / max(1, launches_closest), 4),
                 "bytes_per_step": int(alg_bytes),
