@@ -281,7 +281,7 @@ int ajg_camera_setup(ajg_camera *cam, const float *pos, const float *look_at,
 
 /* ---------------------------------------------------------- path guiding
  * No counterpart in the reference: its path tracer samples continuation
- * directions from the BSDF only (integrator/pt.cpp:215-222). A small MLP
+ * directions from the BSDF only (renderer/pt.cpp:215-222). A small MLP
  * (x(112) -> 64 -> 64 -> 64 -> 48, ReLU, BF16 weights and activations, FP32
  * accumulation) maps a shading point (position, normal) to an 8-lobe
  * von Mises-Fisher mixture; it is trained online on the renderer's own
@@ -346,7 +346,7 @@ typedef struct ajg_guide_stats {
 } ajg_guide_stats;
 
 /* ajg_render_pt with learned path guiding (no reference counterpart; the
- * BSDF-only sampling of integrator/pt.cpp:215-222 and helper.h:209-246 is
+ * BSDF-only sampling of renderer/pt.cpp:215-222 and renderer/helper.h:209-246 is
  * replaced by the one-sample mixture of the guide and the BSDF for the
  * continuation and the MIS-BSDF ray at Lambertian vertices, and env-light
  * NEE weights use the mixture pdf there). Disney vertices keep BSDF-only

@@ -5,7 +5,7 @@ the input encoding, the MLP x(112) -> 64 -> 64 -> 64 -> 48 with ReLU, the
 decode of the outputs into an 8-lobe von Mises-Fisher mixture, the weighted
 negative log-likelihood loss and its gradient by hand-written backprop.
 The reference renderer has no guiding (it samples continuation directions
-from the BSDF only: integrator/pt.cpp:215-222), so there is no reference
+from the BSDF only: renderer/pt.cpp:215-222), so there is no reference
 call site to follow; the oracle is pinned by mathematics instead:
   * the analytic gradient against central finite differences (float64),
   * the vMF density against its closed-form normalisation (integral = 1)
