@@ -486,6 +486,7 @@ void destroy_accel(ajg_accel *a) {
   (void)hipFree(a->perm);
   (void)hipFree(a->leaf_tris);
   (void)hipFree(a->fat);
+  (void)hipFree(a->cursor);
   delete a;
 }
 
@@ -509,6 +510,7 @@ int build_accel(ajg_ctx *ctx, const float *tri, uint32_t n, const float *scene_b
   if ((st = dev_alloc(ctx, (void **)&a->final_nodes, sizeof(PlocNode) * nn, "final nodes"))) return bail(st);
   if ((st = dev_alloc(ctx, (void **)&a->perm, sizeof(uint32_t) * n, "perm"))) return bail(st);
   if ((st = dev_alloc(ctx, (void **)&a->leaf_tris, sizeof(float4) * 3 * (size_t)n, "leaf tris"))) return bail(st);
+  if ((st = dev_alloc(ctx, (void **)&a->cursor, 16, "trace cursor"))) return bail(st);
   if (n > 1 &&
       (st = dev_alloc(ctx, (void **)&a->fat, sizeof(FatNode) * (size_t)(n - 1), "fat nodes")))
     return bail(st);

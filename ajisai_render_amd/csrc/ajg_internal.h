@@ -31,6 +31,7 @@ struct ajg_accel {
   int root_ref = 0;                      // >= 0 fat index, < 0 ~leaf slot
   float root_box[6] = {0, 0, 0, 0, 0, 0};
   uint64_t bytes = 0;
+  uint32_t *cursor = nullptr;            // batch traversal fetch cursor (device)
 };
 
 namespace ajg {

@@ -338,9 +338,11 @@ __device__ __forceinline__ void wave_count_add(uint32_t v, unsigned long long *d
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-// Persistent traversal: a fixed grid of waves strides over 64-ray chunks of
-// the queue, whose size the shade stage wrote on the device; waves past the
-// end exit at once, so a nearly empty queue costs one short launch.
+// Persistent traversal of a queue whose size the shade stage wrote on the
+// device: a fixed grid of waves runs the lane-refill closest-hit traversal
+// of ajg_traverse.h (one atomic on the queue's fetch cursor per refill) or,
+// for any-hit queries, strides over 64-ray chunks; an empty queue costs one
+// short launch.
 static constexpr int kTravBlocksPerCU = 8;
 static constexpr int kItersPerCheck = 4;
 
@@ -351,19 +353,18 @@ k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
   const uint32_t n = *count;
-  const int lane = threadIdx.x & 63;
-  const uint32_t waves = gridDim.x * (kTravThreads / 64);
+  if (n == 0) return;
   uint32_t nn = 0, nt = 0;
-  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t i = c * 64 + lane;
-    if (i < n) {
-      const DevRay ray = load_ray(rays, i);
-      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-      const THit h = trace_closest<SPILL, COUNT>(bvh, r, s_node + threadIdx.x,
-                                                 s_entry + threadIdx.x, &nn, &nt);
-      hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
-    }
-  }
+  persistent_closest<SPILL, COUNT>(
+      bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t i) {
+        const DevRay ray = load_ray(rays, i);
+        return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      },
+      [&](uint32_t i, const THit &h) {
+        hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+      },
+      &nn, &nt);
   if (COUNT) {
     wave_count_add(nn, counters);
     wave_count_add(nt, counters + 1);
@@ -377,9 +378,11 @@ k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count, ui
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
   const uint32_t n = *count;
+  if (n == 0) return;
+  uint32_t nn = 0, nt = 0;
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (kTravThreads / 64);
-  uint32_t nn = 0, nt = 0;
+  (void)cursor;
   for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
     const uint32_t i = c * 64 + lane;
     if (i < n) {

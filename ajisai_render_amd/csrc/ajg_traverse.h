@@ -122,49 +122,42 @@ __device__ __forceinline__ void visit_leaf_closest(const DevBVH &bvh, const TRay
   }
 }
 
-// Traversal stack: the newest kLdsStack entries live in LDS (one column
-// per lane, lanes on consecutive banks); older entries spill to a private
-// array, which only deep paths ever touch. Entries are (node ref, entry t).
+// Traversal stack: entry k (k = 0 .. d-1, (node ref, entry t)) lives in the
+// lane's LDS ring slot k & (kLdsStack - 1) while k >= d - kLdsStack; older
+// entries are moved to a private spill array (index k), which only deep
+// paths touch. The depth d is a plain register; the ring is one LDS column
+// per lane (lanes on consecutive banks).
 static constexpr int kLdsStack = 8;
 static constexpr int kTravThreads = 256;
 
-template <int SPILL>
-struct TravStack {
-  int *ln;    // LDS node refs of this lane: element k at ln[k * kTravThreads]
+struct StackRef {
+  int *ln;    // LDS node refs: slot k at ln[k * kTravThreads]
   float *le;  // LDS entry distances
-  int top = 0, base = 0, spilled = 0;
-  int sn[SPILL];
-  float se[SPILL];
-  __device__ __forceinline__ TravStack(int *lds_node, float *lds_entry)
-      : ln(lds_node), le(lds_entry) {}
-  __device__ __forceinline__ void push(int n, float e) {
-    if (top == kLdsStack) {
-      sn[spilled] = ln[base * kTravThreads];
-      se[spilled] = le[base * kTravThreads];
-      ++spilled;
-      base = (base + 1) & (kLdsStack - 1);
-      --top;
-    }
-    const int i = (base + top) & (kLdsStack - 1);
-    ln[i * kTravThreads] = n;
-    le[i * kTravThreads] = e;
-    ++top;
-  }
-  __device__ __forceinline__ bool pop(int *n, float *e) {
-    if (top == 0) {
-      if (spilled == 0) return false;
-      --spilled;
-      *n = sn[spilled];
-      *e = se[spilled];
-      return true;
-    }
-    --top;
-    const int i = (base + top) & (kLdsStack - 1);
-    *n = ln[i * kTravThreads];
-    *e = le[i * kTravThreads];
-    return true;
-  }
+  int *sn;    // spill (private)
+  float *se;
 };
+
+__device__ __forceinline__ void stack_push(const StackRef &s, int &d, int n, float e) {
+  const int slot = (d & (kLdsStack - 1)) * kTravThreads;
+  if (d >= kLdsStack) {  // the slot holds entry d - kLdsStack: move it to the spill
+    s.sn[d - kLdsStack] = s.ln[slot];
+    s.se[d - kLdsStack] = s.le[slot];
+  }
+  s.ln[slot] = n;
+  s.le[slot] = e;
+  ++d;
+}
+
+__device__ __forceinline__ void stack_pop(const StackRef &s, int &d, int *n, float *e) {
+  --d;
+  const int slot = (d & (kLdsStack - 1)) * kTravThreads;
+  *n = s.ln[slot];
+  *e = s.le[slot];
+  if (d >= kLdsStack) {  // bring entry d - kLdsStack back into the freed slot
+    s.ln[slot] = s.sn[d - kLdsStack];
+    s.le[slot] = s.se[d - kLdsStack];
+  }
+}
 
 // Stack bound: entries <= tree depth - 1, so SPILL >= depth - kLdsStack.
 template <int SPILL, bool COUNT>
@@ -183,7 +176,10 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, 
       !(e < tmax))
     return h;
   int node = bvh.root_ref;
-  TravStack<SPILL> st(lds_node, lds_entry);
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0;
   for (;;) {
     if (node < 0) {
       if (COUNT) ++*n_tris;
@@ -196,16 +192,17 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, 
       const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
       // the right child is visited after the left subtree: keep its entry
       // distance and compare it with t_max when it is popped
-      if (s1) st.push(fn.d.y, e1);
+      if (s1) stack_push(st, d, fn.d.y, e1);
       if (s0 && e0 < tmax) {
         node = fn.d.x;
         continue;
       }
     }
     bool found = false;
-    int n;
-    float en;
-    while (st.pop(&n, &en)) {
+    while (d > 0) {
+      int n;
+      float en;
+      stack_pop(st, d, &n, &en);
       if (en < tmax) {
         node = n;
         found = true;
@@ -227,7 +224,10 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
       !(e < r.tmax))
     return false;
   int node = bvh.root_ref;
-  TravStack<SPILL> st(lds_node, lds_entry);
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0;
   for (;;) {
     if (node < 0) {
       if (COUNT) ++*n_tris;
@@ -244,7 +244,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
       const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1) &&
                       e1 < r.tmax;
       if (s0 && s1) {
-        st.push(fn.d.y, e1);
+        stack_push(st, d, fn.d.y, e1);
         node = fn.d.x;
         continue;
       }
@@ -257,12 +257,123 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
         continue;
       }
     }
+    if (d == 0) break;
     int n;
     float en;
-    if (!st.pop(&n, &en)) break;
+    stack_pop(st, d, &n, &en);
     node = n;
   }
   return false;
+}
+
+// ---- Persistent closest-hit traversal with lane refill.
+// A wave keeps one ray per lane and advances every active lane by one node
+// (or one leaf) per step; when at least kRefillMin lanes have finished, the
+// wave takes that many new rays from the batch with one atomic on a shared
+// cursor, so lanes stay busy whatever the spread of path lengths within a
+// wave. Each ray's traversal (order, culling, arithmetic) is the same as
+// trace_closest, so results are unchanged. (Any-hit rays are short and
+// uniform enough that a plain 64-ray chunk per wave is faster.)
+static constexpr int kRefillMin = 48;
+
+// Wave-uniform refill: idle lanes take consecutive batch indices. Returns the
+// lane's new index (kNone if none) and sets *drained when the batch is used up.
+__device__ __forceinline__ uint32_t refill(bool idle, uint32_t n, uint32_t *cursor, bool *drained) {
+  const uint64_t mask = __ballot(idle);
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const uint32_t cnt = (uint32_t)__popcll(mask);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(cursor, cnt);
+  base = __shfl(base, leader, 64);
+  if (base + cnt >= n) *drained = true;
+  if (!idle) return kNone;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t idx = base + (uint32_t)__popcll(mask & lt);
+  return idx < n ? idx : kNone;
+}
+
+// Drive closest-hit traversal over rays [0, n) of a batch. load(i) returns
+// the TRay of ray i, store(i, hit) writes its result. The per-lane state is
+// kept in plain locals (registers); only the stack spill array is private
+// memory.
+template <int SPILL, bool COUNT, class Load, class Store>
+__device__ __forceinline__ void persistent_closest(const DevBVH &bvh, uint32_t n, uint32_t *cursor,
+                                                   int *lds_node, float *lds_entry, Load load,
+                                                   Store store, uint32_t *nn, uint32_t *nt) {
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0;
+  TRay r;
+  THit h;
+  float tmax = 0.f;
+  int node = 0;
+  uint32_t ray = kNone;
+  bool drained = false;
+  for (;;) {
+    const bool idle = ray == kNone;
+    if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
+      const uint32_t idx = refill(idle, n, cursor, &drained);
+      if (idx != kNone) {
+        r = load(idx);
+        h.t = __int_as_float(0x7f800000);
+        h.u = 0.f;
+        h.v = 0.f;
+        h.slot = -1;
+        tmax = r.tmax;
+        d = 0;
+        float e;
+        if (slab_static(r, bvh.root_mn[0], bvh.root_mn[1], bvh.root_mn[2], bvh.root_mx[0],
+                        bvh.root_mx[1], bvh.root_mx[2], &e) &&
+            e < tmax) {
+          node = bvh.root_ref;
+          ray = idx;
+        } else {
+          store(idx, h);
+        }
+      }
+      continue;
+    }
+    if (__ballot(ray != kNone) == 0ull) {
+      if (drained) break;
+      continue;
+    }
+    if (ray == kNone) continue;
+    // one node visit of trace_closest
+    bool next = false;
+    if (node < 0) {
+      if (COUNT) ++*nt;
+      visit_leaf_closest(bvh, r, ~node, &h, &tmax);
+    } else {
+      if (COUNT) ++*nn;
+      const FatNode fn = bvh.fat[node];
+      float e0, e1;
+      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
+      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
+      if (s1) stack_push(st, d, fn.d.y, e1);
+      if (s0 && e0 < tmax) {
+        node = fn.d.x;
+        next = true;
+      }
+    }
+    if (!next) {
+      while (d > 0) {
+        int pn;
+        float en;
+        stack_pop(st, d, &pn, &en);
+        if (en < tmax) {
+          node = pn;
+          next = true;
+          break;
+        }
+      }
+    }
+    if (!next) {
+      store(ray, h);
+      ray = kNone;
+    }
+  }
 }
 
 // Spill capacity for a tree of the given depth (0: unsupported depth).
