@@ -107,6 +107,23 @@ class GuideStats(C.Structure):
                 ("ms_warmup", C.c_float)]
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("args", PTArgs), ("film", Film)]
+
+
+class BatchArgs(C.Structure):
+    _fields_ = [("max_slots", C.c_int32), ("samples_per_item", C.c_int32),
+                ("trav_blocks_per_cu", C.c_int32), ("lds_stack", C.c_int32),
+                ("iters_per_check", C.c_int32), ("use_graph", C.c_int32), ("flags", C.c_uint32),
+                ("pixel_tile", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("ms_total", C.c_float), ("iterations", C.c_uint32), ("slots", C.c_uint32),
+                ("graph_launches", C.c_uint32), ("paths", C.c_uint64),
+                ("closest_queries", C.c_uint64), ("any_queries", C.c_uint64)]
+
+
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("tri_id", "<u4"), ("u", "<f4"), ("v", "<f4")])
 NODE_DTYPE = np.dtype([("mn", "<f4", 3), ("mx", "<f4", 3), ("count", "<u4"), ("first", "<u4")])
@@ -140,6 +157,10 @@ _SIGS = {
     "ajg_render_pt_guided": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), _P, C.POINTER(GuideArgs),
                               C.POINTER(Film), C.POINTER(RenderStats), C.POINTER(GuideStats), _P],
                              C.c_int),
+    "ajg_render_batch": ([_P, C.POINTER(Job), C.c_uint32, C.POINTER(BatchArgs),
+                          C.POINTER(BatchStats), _P, _P], C.c_int),
+    "ajg_batch_autotune": ([_P, C.POINTER(Job), C.c_uint32, C.c_uint32, C.POINTER(BatchArgs),
+                            C.POINTER(C.c_float)], C.c_int),
     "ajg_guide_param_count": ([], C.c_int),
     "ajg_guide_create": ([_P, C.c_uint32, _P, C.POINTER(_P)], C.c_int),
     "ajg_guide_destroy": ([_P], C.c_int),

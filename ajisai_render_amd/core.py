@@ -338,6 +338,75 @@ class PathTracer:
         return rgb.cpu().numpy().reshape(h, w, 3)
 
 
+def batch_defaults(max_slots=0, samples_per_item=0, trav_blocks_per_cu=0, lds_stack=0,
+                   iters_per_check=0, use_graph=1, pixel_tile=0):
+    """ajg_batch_args (0 = the library's default for that knob)."""
+    return capi.BatchArgs(max_slots, samples_per_item, trav_blocks_per_cu, lds_stack,
+                          iters_per_check, use_graph, 0, pixel_tile)
+
+
+class BatchRenderer:
+    """Many (scene, camera, PTRendererArgs) jobs in one wavefront
+    (ajg_render_batch). Jobs are PathTracer objects; each keeps its own
+    PTRendererArgs and receives its own film."""
+
+    def __init__(self, ctx: Context, tracers):
+        self.ctx = ctx
+        self.tracers = list(tracers)
+        if not self.tracers:
+            raise ValueError("a batch needs at least one job")
+
+    def _jobs(self, films):
+        jobs = (capi.Job * len(self.tracers))()
+        for i, (pt, f) in enumerate(zip(self.tracers, films)):
+            jobs[i].scene = pt.g.h.value if isinstance(pt.g.h, C.c_void_p) else pt.g.h
+            jobs[i].camera = pt.camera()
+            C.memmove(C.byref(jobs[i].args), C.byref(pt.args), C.sizeof(capi.PTArgs))
+            jobs[i].film = capi.Film(_ptr(f["sum"]), _ptr(f["sum_sq"]), _ptr(f["count"]),
+                                     _ptr(f.get("per_sample")))
+        return jobs
+
+    def films(self, per_sample=False):
+        torch = _torch()
+        out = []
+        for pt in self.tracers:
+            wp, hp = pt.traced_dims()
+            n = wp * hp
+            f = dict(sum=torch.empty(n * 3, dtype=torch.float32, device="cuda"),
+                     sum_sq=torch.empty(n * 3, dtype=torch.float32, device="cuda"),
+                     count=torch.empty(n, dtype=torch.int32, device="cuda"))
+            if per_sample:
+                f["per_sample"] = torch.empty(n * pt.args.spp * 3, dtype=torch.float32,
+                                              device="cuda")
+            out.append(f)
+        return out
+
+    def render_device(self, bargs=None, films=None, per_sample=False, stream=None):
+        """Returns (films, BatchStats, per-job completion ms as numpy)."""
+        films = self.films(per_sample) if films is None else films
+        jobs = self._jobs(films)
+        ba = bargs if bargs is not None else batch_defaults()
+        st = capi.BatchStats()
+        done = np.zeros(len(self.tracers), np.float32)
+        self.ctx.check(self.ctx.lib.ajg_render_batch(self.ctx.h, jobs, len(self.tracers),
+                                                     C.byref(ba), C.byref(st), done.ctypes.data,
+                                                     stream), "ajg_render_batch")
+        return films, st, done
+
+    def autotune(self, trial_spp=2, start=None, films=None):
+        """Coordinate-descent autotuner; returns (BatchArgs, best paths/s)."""
+        films = self.films() if films is None else films
+        jobs = self._jobs(films)
+        ba = capi.BatchArgs()
+        C.memmove(C.byref(ba), C.byref(start if start is not None else batch_defaults()),
+                  C.sizeof(capi.BatchArgs))
+        best = C.c_float(0.0)
+        self.ctx.check(self.ctx.lib.ajg_batch_autotune(self.ctx.h, jobs, len(self.tracers),
+                                                       trial_spp, C.byref(ba), C.byref(best)),
+                       "ajg_batch_autotune")
+        return ba, float(best.value)
+
+
 def guide_defaults(alpha=0.5, train=True, batch=1 << 18, lr=1e-3, train_slots=0, max_steps=0,
                    steps_per_batch=4, warmup_passes=0, warmup_spp=16, warmup_steps_per_batch=64):
     """ajg_guide_args: one-sample mixture with probability alpha, online Adam

@@ -141,6 +141,7 @@ int ajg_ctx_create(int device, ajg_ctx **out) {
 }
 
 int ajg_ctx_destroy(ajg_ctx *ctx) {
+  if (ctx && ctx->batch_free) ctx->batch_free(ctx->batch_cache);
   delete ctx;
   return AJG_OK;
 }

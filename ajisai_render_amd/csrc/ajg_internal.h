@@ -16,6 +16,8 @@ struct ajg_ctx {
   std::string last_error;
   int inject_countdown = 0;
   int inject_status = 0;
+  void *batch_cache = nullptr;             // buffers of batched renders
+  void (*batch_free)(void *) = nullptr;    // frees batch_cache
 };
 
 struct ajg_accel {

@@ -135,7 +135,20 @@ struct RenderParams {
   int seed_mode;
   uint32_t seed_base;
   uint32_t slots;
+  uint32_t tile, tiles_x;  // work-list pixel order: T x T tiles (0: scanline)
 };
+
+// Pixel of the i-th entry of a work list's pixel order: scanline, or T x T
+// tiles in scanline order of tiles (T divides the traced area).
+__device__ __forceinline__ uint32_t pixel_of(uint32_t i, uint32_t tile, uint32_t tiles_x,
+                                             uint32_t Wp) {
+  if (tile == 0) return i;
+  const uint32_t tt = tile * tile;
+  const uint32_t t = i / tt, r = i - t * tt;
+  const uint32_t ty = t / tiles_x, tx = t - ty * tiles_x;
+  const uint32_t ry = r / tile, rx = r - ry * tile;
+  return (ty * tile + ry) * Wp + tx * tile + rx;
+}
 
 // pend_flags
 static constexpr uint32_t kPendActive = 1u;
@@ -161,6 +174,8 @@ struct PathState {
   // accumulators: 2 banks x {sum3, comp3, sq3, compsq3}
   float *acc;
   uint32_t *acc_n;      // 2 banks
+  // batched renders: job of the current work item and of the pending record
+  uint32_t *job, *pend_job;
 };
 
 struct Queues {
@@ -203,6 +218,50 @@ struct GuideParams {
 static constexpr int kGRec = 10;
 static constexpr int kGHist = 16;
 static constexpr int kGMix = 40;
+
+// Batched renders: many (scene, camera, args, film) jobs share one
+// wavefront. Global work item w belongs to the job whose [work_begin,
+// work_begin + P.total_work) holds it; every queued ray carries its job so
+// the traversal kernels trace it in that job's BVH.
+struct BatchJob {
+  DevScene sc;
+  RenderParams P;
+  float *partial;
+  uint32_t *partial_n;
+  float *per_sample;
+};
+
+struct BatchParams {
+  const BatchJob *jobs;
+  const uint32_t *job_begin;        // n_jobs + 1 prefix of work items
+  uint32_t n_jobs, slots, total_work;
+  uint32_t *remaining;              // per job: work items not finished
+  unsigned long long *done_clock;   // per job: device wall clock at completion
+  uint16_t *c_job_out, *a_job_out;  // job of each ray appended this iteration
+  const uint16_t *c_job_in, *a_job_in;
+};
+
+// Where a finished sample goes.
+struct AccTarget {
+  float *partial;
+  uint32_t *partial_n;
+  float *per_sample;
+  uint32_t n_pix, spp, Wp, tile, tiles_x;
+  uint32_t *remaining;              // batched: job completion counter
+  unsigned long long *done;
+};
+
+__device__ __forceinline__ uint32_t find_job(const BatchParams &B, uint32_t w) {
+  uint32_t lo = 0, hi = B.n_jobs;  // job_begin[lo] <= w < job_begin[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (B.job_begin[mid] <= w)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 __device__ __forceinline__ void kahan_add(float &sum, float &c, float x) {
   const float y = x - c;
@@ -357,7 +416,7 @@ k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count
   uint32_t nn = 0, nt = 0;
   persistent_closest<SPILL, COUNT>(
       bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
-      [&](uint32_t i) {
+      [&](uint32_t i, DevBVH &) {
         const DevRay ray = load_ray(rays, i);
         return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       },
@@ -398,6 +457,51 @@ k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count, ui
   }
 }
 
+// Batched renders: the BVH of each ray is its job's.
+template <int SPILL, int LDS>
+__global__ void __launch_bounds__(kTravThreads)
+k_pq_closest_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, float4 *hits,
+               const uint32_t *count, uint32_t *cursor) {
+  __shared__ int s_node[LDS * kTravThreads];
+  __shared__ float s_entry[LDS * kTravThreads];
+  const uint32_t n = *count;
+  if (n == 0) return;
+  const DevBVH first = jobs[0].sc.bvh;
+  uint32_t nn = 0, nt = 0;
+  persistent_closest<SPILL, false, true, LDS>(
+      first, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t i, DevBVH &b) {
+        b = jobs[job_of[i]].sc.bvh;
+        const DevRay ray = load_ray(rays, i);
+        return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      },
+      [&](uint32_t i, const THit &h) {
+        hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+      },
+      &nn, &nt);
+}
+
+template <int SPILL, int LDS>
+__global__ void __launch_bounds__(kTravThreads)
+k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uint8_t *occ,
+           const uint32_t *count) {
+  __shared__ int s_node[LDS * kTravThreads];
+  __shared__ float s_entry[LDS * kTravThreads];
+  const uint32_t n = *count;
+  if (n == 0) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (kTravThreads / 64);
+  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
+    const uint32_t i = c * 64 + lane;
+    if (i < n) {
+      const DevBVH b = jobs[job_of[i]].sc.bvh;
+      const DevRay ray = load_ray(rays, i);
+      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      occ[i] = trace_any<SPILL, false, LDS>(b, r, s_node + threadIdx.x, s_entry + threadIdx.x) ? 1 : 0;
+    }
+  }
+}
+
 template <bool COUNT>
 static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *rays, void *out,
                       const uint32_t *count, uint32_t *cursor, unsigned long long *ctr,
@@ -418,6 +522,27 @@ static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *
       hipLaunchKernelGGL((k_pq_any<1024, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
                          cursor, ctr);
   }
+}
+
+template <int SPILL>
+static void launch_pq_b(int lds, bool closest, const BatchJob *jobs, const uint16_t *job_of,
+                        const DevRay *rays, void *out, const uint32_t *count, uint32_t *cursor,
+                        uint32_t grid, hipStream_t s) {
+  const dim3 g(grid), b(kTravThreads);
+#define AJG_PQB(L)                                                                            \
+  if (closest)                                                                                \
+    hipLaunchKernelGGL((k_pq_closest_b<SPILL, L>), g, b, 0, s, jobs, job_of, rays, (float4 *)out, \
+                       count, cursor);                                                        \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_pq_any_b<SPILL, L>), g, b, 0, s, jobs, job_of, rays, (uint8_t *)out, count);
+  if (lds == 4) {
+    AJG_PQB(4)
+  } else if (lds == 16) {
+    AJG_PQB(16)
+  } else {
+    AJG_PQB(8)
+  }
+#undef AJG_PQB
 }
 
 // Event pairs around launches (AJG_RENDER_TIMING).
@@ -471,6 +596,12 @@ __global__ void k_iter_begin(uint32_t *ctr, int cur, int nxt, unsigned long long
   }
 }
 
+// Device wall clock at the start of a batched render (job completion times
+// are measured against it).
+__global__ void k_clock(unsigned long long *dst) {
+  if (threadIdx.x == 0) *dst = wall_clock64();
+}
+
 __device__ __forceinline__ f3 ld_f3(const float *p, uint32_t i) {
   return mk3(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]);
 }
@@ -508,10 +639,9 @@ __device__ void emit_history(const GuideParams &G, uint32_t hslot, f3 Lf) {
 }
 
 // Accumulate a finished sample into its work item's bank; flush the bank
-// when it was the item's last sample.
-__device__ void accumulate(const PathState &ps, const Queues &q, const RenderParams &P,
-                           uint32_t slot, uint32_t bank, uint32_t work, uint32_t sample, f3 L,
-                           bool last) {
+// when it was the item's last sample (and, in a batch, count the job down).
+__device__ void accumulate(const PathState &ps, const AccTarget &t, uint32_t slot, uint32_t bank,
+                           uint32_t work, uint32_t sample, f3 L, bool last) {
   float *a = ps.acc + ((size_t)slot * 2 + bank) * 12;
   kahan_add(a[0], a[3], L.x);
   kahan_add(a[1], a[4], L.y);
@@ -521,33 +651,73 @@ __device__ void accumulate(const PathState &ps, const Queues &q, const RenderPar
   kahan_add(a[8], a[11], L.z * L.z);
   uint32_t &n = ps.acc_n[(size_t)slot * 2 + bank];
   n += 1;
-  const uint32_t p = work % P.n_pix;
-  if (q.per_sample) {
-    float *o = q.per_sample + 3 * ((size_t)p * P.spp + sample);
+  const uint32_t chunk = work / t.n_pix;
+  const uint32_t p = pixel_of(work - chunk * t.n_pix, t.tile, t.tiles_x, t.Wp);
+  if (t.per_sample) {
+    float *o = t.per_sample + 3 * ((size_t)p * t.spp + sample);
     o[0] = L.x;
     o[1] = L.y;
     o[2] = L.z;
   }
   if (last) {
-    float *dst = q.partial + (size_t)work * 6;
+    const size_t cell = (size_t)chunk * t.n_pix + p;  // chunk-major, pixel-minor
+    float *dst = t.partial + cell * 6;
     dst[0] = a[0] - a[3];
     dst[1] = a[1] - a[4];
     dst[2] = a[2] - a[5];
     dst[3] = a[6] - a[9];
     dst[4] = a[7] - a[10];
     dst[5] = a[8] - a[11];
-    q.partial_n[work] = n;
+    t.partial_n[cell] = n;
     for (int k = 0; k < 12; ++k) a[k] = 0.f;
     n = 0;
+    if (t.remaining) {
+      __threadfence();
+      if (atomicSub(t.remaining, 1u) == 1u) *t.done = wall_clock64();
+    }
   }
+}
+
+__device__ __forceinline__ AccTarget acc_single(const Queues &q, const RenderParams &P) {
+  AccTarget t;
+  t.partial = q.partial;
+  t.partial_n = q.partial_n;
+  t.per_sample = q.per_sample;
+  t.n_pix = P.n_pix;
+  t.spp = P.spp;
+  t.Wp = P.Wp;
+  t.tile = P.tile;
+  t.tiles_x = P.tiles_x;
+  t.remaining = nullptr;
+  t.done = nullptr;
+  return t;
+}
+
+__device__ __forceinline__ AccTarget acc_job(const BatchParams &B, uint32_t j) {
+  const BatchJob &J = B.jobs[j];
+  AccTarget t;
+  t.partial = J.partial;
+  t.partial_n = J.partial_n;
+  t.per_sample = J.per_sample;
+  t.n_pix = J.P.n_pix;
+  t.spp = J.P.spp;
+  t.Wp = J.P.Wp;
+  t.tile = J.P.tile;
+  t.tiles_x = J.P.tiles_x;
+  t.remaining = B.remaining + j;
+  t.done = B.done_clock + j;
+  return t;
 }
 
 // Start the next sample of the slot (fetching a new work item when the
 // current one is exhausted) and emit its camera ray. Wave-uniform call.
-__device__ void next_sample(const PathState &ps, const Queues &q, const RenderParams &P,
-                            const DevScene &sc, uint32_t slot, bool need, Pcg32 &rng,
+// P0 is the render's parameters (single renders); a batch takes each job's.
+template <bool BATCH>
+__device__ void next_sample(const PathState &ps, const Queues &q, const RenderParams &P0,
+                            const BatchParams &B, uint32_t slot, bool need, Pcg32 &rng,
                             bool &live) {
   uint32_t work = need ? ps.work[slot] : kNone;
+  uint32_t job = BATCH && need ? ps.job[slot] : 0u;
   uint32_t sample = 0;
   bool fresh = false;
   if (need && work != kNone) {
@@ -557,10 +727,16 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   const bool fetch = need && work == kNone;
   const uint32_t w = wave_append(fetch, q.work_counter);
   if (fetch) {
-    if (w < P.total_work) {
-      work = w;
+    if (w < (BATCH ? B.total_work : P0.total_work)) {
+      uint32_t local = w;
+      if (BATCH) {
+        job = find_job(B, w);
+        local = w - B.job_begin[job];
+      }
+      const RenderParams &P = BATCH ? B.jobs[job].P : P0;
+      work = local;
       fresh = true;
-      const uint32_t c = w / P.n_pix;
+      const uint32_t c = local / P.n_pix;
       sample = c * P.chunk;
       ps.sample_end[slot] = min(P.spp, sample + P.chunk);
       ps.bank[slot] ^= 1u;
@@ -574,8 +750,10 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     ps.ext_idx[slot] = kNone;
     return;
   }
+  if (BATCH) ps.job[slot] = job;
+  const RenderParams &P = BATCH ? B.jobs[job].P : P0;
   ps.sample[slot] = sample;
-  const uint32_t p = work % P.n_pix;
+  const uint32_t p = pixel_of(work % P.n_pix, P.tile, P.tiles_x, P.Wp);
   const uint32_t x = p % P.Wp, y = p / P.Wp;
   if (P.seed_mode == 0) {
     if (fresh) pcg_seed(rng, (uint64_t)(int64_t)(int)(x + y * P.W));
@@ -602,11 +780,11 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     }
   }
   if (P.max_bounces <= 0) alive = false;
-  (void)alive;
   // A path that dies before its first intersection contributes zero.
   const uint32_t idx = wave_append(alive, q.c_count_out);
   if (alive) {
     store_ray(q.c_rays_out, idx, r);
+    if (BATCH) B.c_job_out[idx] = (uint16_t)job;
     ps.ext_idx[slot] = idx;
   } else {
     ps.ext_idx[slot] = kNone;
@@ -615,17 +793,18 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     // zero-radiance sample: accumulate immediately, then the next sample
     // starts in the following iteration (slot stays live).
     const bool last = sample + 1 >= ps.sample_end[slot];
-    accumulate(ps, q, P, slot, ps.bank[slot], work, sample, mk3(0.f, 0.f, 0.f), last);
+    accumulate(ps, BATCH ? acc_job(B, job) : acc_single(q, P), slot, ps.bank[slot], work, sample,
+               mk3(0.f, 0.f, 0.f), last);
     ps.ext_idx[slot] = kNone;
     ps.pend_flags[slot] = 0;
   }
 }
 
-template <bool GUIDED>
+template <bool GUIDED, bool BATCH>
 __global__ void __launch_bounds__(256)
-k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
+k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, BatchParams B) {
   const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-  const bool valid = slot < P.slots;
+  const bool valid = slot < (BATCH ? B.slots : P0.slots);
   const bool train = GUIDED && valid && G.train_stride && slot % G.train_stride == 0;
   const uint32_t hslot = train ? slot / G.train_stride : 0u;
   bool live = false;
@@ -637,6 +816,8 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
     // ---- 1. finish the previous bounce's direct lighting
     const uint32_t pf = ps.pend_flags[slot];
     if (pf & kPendActive) {
+      const uint32_t pj = BATCH ? ps.pend_job[slot] : 0u;
+      const DevScene &sc = BATCH ? B.jobs[pj].sc : sc0;
       f3 tpos, tnrm;
       if (train) {
         tpos = ld_f3(G.pend_pos, slot);
@@ -689,8 +870,8 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
         }
       }
       if (pf & kPendEnded) {
-        accumulate(ps, q, P, slot, ps.pend_bank[slot], ps.pend_work[slot], ps.pend_sample[slot],
-                   Lf, (pf & kPendLast) != 0);
+        accumulate(ps, BATCH ? acc_job(B, pj) : acc_single(q, P0), slot, ps.pend_bank[slot],
+                   ps.pend_work[slot], ps.pend_sample[slot], Lf, (pf & kPendLast) != 0);
       } else {
         st_f3(ps.L, slot, Lf);
       }
@@ -710,6 +891,9 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
     }
     if (ei != kNone) {
       live = true;
+      const uint32_t j = BATCH ? ps.job[slot] : 0u;
+      const DevScene &sc = BATCH ? B.jobs[j].sc : sc0;
+      const RenderParams &P = BATCH ? B.jobs[j].P : P0;
       const DevRay ray = load_ray(q.c_rays_in, ei);
       const float4 h = q.c_hits_in[ei];
       const int ls = __float_as_int(h.y);
@@ -721,7 +905,8 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
       if (ls < 0) {
         if (b == 0 && sc.env_index >= 0) L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
         if (train) emit_history(G, hslot, L);
-        accumulate(ps, q, P, slot, ps.bank[slot], work, sample, L, last);
+        accumulate(ps, BATCH ? acc_job(B, j) : acc_single(q, P), slot, ps.bank[slot], work, sample,
+                   L, last);
         need_new = true;
       } else {
         const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w);
@@ -743,6 +928,7 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
         ps.pend_work[slot] = work;
         ps.pend_sample[slot] = sample;
         ps.pend_bank[slot] = ps.bank[slot];
+        if (BATCH) ps.pend_job[slot] = j;
         if (train) {
           st_f3(G.pend_pos, slot, it.pos);
           st_f3(ps.pend_ns, slot, it.ns);
@@ -792,6 +978,7 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
           const uint32_t si = wave_append(emit, q.a_count_out);
           if (emit) {
             store_ray(q.a_rays_out, si, sr);
+            if (BATCH) B.a_job_out[si] = (uint16_t)j;
             ps.pend_shadow[(size_t)slot * kMaxLights + k] = si;
             pf2 |= kPendLight0 << k;
           }
@@ -827,6 +1014,7 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
           const uint32_t mi = wave_append(emit, q.a_count_out);
           if (emit) {
             store_ray(q.a_rays_out, mi, mr);
+            if (BATCH) B.a_job_out[mi] = (uint16_t)j;
             ps.pend_mis[slot] = mi;
             pf2 |= kPendMis;
           }
@@ -874,6 +1062,7 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
         const uint32_t ni = wave_append(cont, q.c_count_out);
         if (cont) {
           store_ray(q.c_rays_out, ni, nr);
+          if (BATCH) B.c_job_out[ni] = (uint16_t)j;
           ps.ext_idx[slot] = ni;
           ps.bounce[slot] = b + 1;
           st_f3(ps.beta, slot, beta);
@@ -902,7 +1091,7 @@ k_shade(DevScene sc, RenderParams P, PathState ps, Queues q, GuideParams G) {
   }
 
   // ---- 3. next sample / ray generation (wave-uniform control flow)
-  next_sample(ps, q, P, sc, slot, valid && need_new, rng, live);
+  next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live);
   if (valid) ps.rng[slot] = rng.state;
   const bool any_live = __ballot(valid && live) != 0ull;
   if ((threadIdx.x & 63) == 0 && any_live) atomicAdd(q.live, 1u);
@@ -1098,6 +1287,8 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&ps.pend_f, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_pdf, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.pend_ns, 12 * (size_t)S, "state")) || (st = al((void **)&ps.acc, 4 * 24 * (size_t)S, "state")) ||
       (st = al((void **)&ps.acc_n, 8 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.job, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_job, 4 * (size_t)S, "state")) ||
       (st = al((void **)&rb->c_rays[0], sizeof(DevRay) * cq_cap, "closest queue")) ||
       (st = al((void **)&rb->c_rays[1], sizeof(DevRay) * cq_cap, "closest queue")) ||
       (st = al((void **)&rb->c_hits, sizeof(float4) * cq_cap, "closest hits")) ||
@@ -1353,27 +1544,20 @@ int ajg_scene_env_tables(ajg_scene *s, uint32_t li, float *cond_cdf, float *cond
 
 }  // extern "C"
 
-static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
-                       ajg_guide *gd, const ajg_guide_args *ga, const ajg_film *film,
-                       ajg_render_stats *stats, ajg_guide_stats *gstats, void *stream) {
-  if (!sc || !camera || !args || !film || !film->sum || !film->sum_sq || !film->count)
-    return AJG_ERR_INVALID;
-  ajg_ctx *ctx = sc->ctx;
-  if (int st = injected(ctx)) return st;
-  const bool guided = gd != nullptr;
-  if (guided) {
-    if (!ga || !(ga->alpha >= 0.f && ga->alpha <= 1.f) || (ga->train && (ga->batch == 0 || !(ga->lr > 0.f))))
-      return fail(ctx, AJG_ERR_INVALID, "guided render: need 0 <= alpha <= 1, batch > 0 and lr > 0");
-  }
+// RenderParams of one render: PTRendererArgs semantics (renderer.h:37-45),
+// traced area = whole tiles (pt.cpp:43-49). Work items are (pixel, chunk of
+// samples): seed_mode 0 keeps a pixel's samples in one item (one sequential
+// stream); seed_mode 1 splits them into chunks of `samples_per_item`
+// (0: enough chunks to fill `target` slots).
+static int setup_params(ajg_ctx *ctx, const ajg_camera *camera, const ajg_pt_args *args,
+                        uint64_t target, uint32_t samples_per_item, RenderParams *out) {
   if (args->spp <= 0 || args->tile_size <= 0 || args->max_bounces < 0 || !args->use_mis)
     return fail(ctx, AJG_ERR_INVALID, "render: need spp > 0, tile_size > 0, max_bounces >= 0, use_mis");
   const float rx = camera->resolution[0], ry = camera->resolution[1];
   if (!(rx >= 1.f) || !(ry >= 1.f) || rx > 65536.f || ry > 65536.f)
     return fail(ctx, AJG_ERR_INVALID, "render: bad resolution");
-  hipStream_t s = (hipStream_t)stream;
-  const ajg_accel *a = sc->accel;
-
-  RenderParams P;
+  RenderParams &P = *out;
+  memset(&P, 0, sizeof(P));
   memcpy(P.cam.r2c, camera->raster2camera, sizeof(P.cam.r2c));
   memcpy(P.cam.c2w, camera->camera2world, sizeof(P.cam.c2w));
   P.cam.lens_radius = camera->lens_radius;
@@ -1390,10 +1574,12 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   P.cont_prob = args->cont_prob;
   P.seed_mode = args->seed_mode;
   P.seed_base = args->seed_base;
-  const uint64_t target = args->max_slots > 0 ? (uint64_t)args->max_slots : (1ull << 20);
   if (P.seed_mode == 0) {
     P.chunk = P.spp;
     P.n_chunks = 1;
+  } else if (samples_per_item > 0) {
+    P.chunk = std::min(P.spp, samples_per_item);
+    P.n_chunks = cdiv(P.spp, P.chunk);
   } else {
     uint32_t nc = (uint32_t)std::min<uint64_t>(P.spp, std::max<uint64_t>(1, cdiv(target, P.n_pix)));
     P.chunk = cdiv(P.spp, nc);
@@ -1402,10 +1588,13 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   const uint64_t total = (uint64_t)P.n_pix * P.n_chunks;
   if (total > 0xFFFFFFF0ull) return fail(ctx, AJG_ERR_INVALID, "render: too much work");
   P.total_work = (uint32_t)total;
-  P.slots = (uint32_t)std::min<uint64_t>(target, total);
-  const uint32_t S = P.slots;
+  return AJG_OK;
+}
 
+static DevScene dev_scene(const ajg_scene *sc) {
+  const ajg_accel *a = sc->accel;
   DevScene ds;
+  memset(&ds, 0, sizeof(ds));
   ds.bvh.fat = a->fat;
   ds.bvh.leaf_tris = a->leaf_tris;
   ds.bvh.root_ref = a->root_ref;
@@ -1420,6 +1609,30 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   ds.n_lights = sc->n_lights;
   ds.env_index = sc->env_index;
   ds.radius = sc->radius;
+  return ds;
+}
+
+static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
+                       ajg_guide *gd, const ajg_guide_args *ga, const ajg_film *film,
+                       ajg_render_stats *stats, ajg_guide_stats *gstats, void *stream) {
+  if (!sc || !camera || !args || !film || !film->sum || !film->sum_sq || !film->count)
+    return AJG_ERR_INVALID;
+  ajg_ctx *ctx = sc->ctx;
+  if (int st = injected(ctx)) return st;
+  const bool guided = gd != nullptr;
+  if (guided) {
+    if (!ga || !(ga->alpha >= 0.f && ga->alpha <= 1.f) || (ga->train && (ga->batch == 0 || !(ga->lr > 0.f))))
+      return fail(ctx, AJG_ERR_INVALID, "guided render: need 0 <= alpha <= 1, batch > 0 and lr > 0");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const ajg_accel *a = sc->accel;
+  const uint64_t target = args->max_slots > 0 ? (uint64_t)args->max_slots : (1ull << 20);
+  RenderParams P;
+  if (int st = setup_params(ctx, camera, args, target, 0, &P)) return st;
+  const uint64_t total = P.total_work;
+  P.slots = (uint32_t)std::min<uint64_t>(target, total);
+  const uint32_t S = P.slots;
+  const DevScene ds = dev_scene(sc);
 
   // ---- buffers (cached on the scene across renders)
   RenderBuffers *rb = (RenderBuffers *)sc->render_cache;
@@ -1431,6 +1644,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   PathState ps = rb->ps;
   GuideParams G;
   memset(&G, 0, sizeof(G));
+  BatchParams nob;
+  memset(&nob, 0, sizeof(nob));
   GuideBuffers *gb = nullptr;
   uint32_t batch = 0;
   if (guided) {
@@ -1559,9 +1774,11 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
       timer.begin(2, s);
       if (guided)
-        hipLaunchKernelGGL(k_shade<true>, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G);
+        hipLaunchKernelGGL((k_shade<true, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G,
+                           nob);
       else
-        hipLaunchKernelGGL(k_shade<false>, dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G);
+        hipLaunchKernelGGL((k_shade<false, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G,
+                           nob);
       timer.end(2, s);
       timer.begin(0, s);
       if (counting)
@@ -1680,6 +1897,325 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   return done(AJG_OK);
 }
 
+// ---------------------------------------------------------------- batches
+// Buffers of batched renders, kept on the context: the render buffers of
+// the shared wavefront, per-ray job ids of both queue generations, the
+// device job table, job work-list prefix, completion counters and clocks,
+// and the stream the iterations are captured on.
+struct BatchBuffers {
+  RenderBuffers rb;
+  std::vector<void *> bufs;
+  uint32_t slots = 0, n_jobs = 0;
+  uint16_t *c_job[2] = {nullptr, nullptr};
+  uint16_t *a_job[2] = {nullptr, nullptr};
+  BatchJob *jobs = nullptr;
+  uint32_t *job_begin = nullptr;
+  uint32_t *remaining = nullptr;
+  unsigned long long *clocks = nullptr;  // n_jobs completion clocks + start
+  hipStream_t stream = nullptr;
+  void release() {
+    for (void *p : bufs) (void)hipFree(p);
+    bufs.clear();
+    slots = n_jobs = 0;
+  }
+  ~BatchBuffers() {
+    release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static void batch_cache_free(void *p) { delete (BatchBuffers *)p; }
+
+static int ensure_batch_buffers(ajg_ctx *ctx, BatchBuffers *bb, uint32_t S, uint32_t n_jobs) {
+  if (!bb->stream) {
+    hipError_t e = hipStreamCreate(&bb->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate");
+  }
+  if (bb->slots >= S && bb->n_jobs >= n_jobs && !bb->bufs.empty()) return AJG_OK;
+  bb->release();
+  auto al = [&](void **p, size_t bytes, const char *what) {
+    int r = dev_alloc(ctx, p, bytes, what);
+    if (r == AJG_OK) bb->bufs.push_back(*p);
+    return r;
+  };
+  const size_t aq = (size_t)(kMaxLights + 1) * S;
+  int st = 0;
+  if ((st = al((void **)&bb->c_job[0], 2 * (size_t)S, "batch ray jobs")) ||
+      (st = al((void **)&bb->c_job[1], 2 * (size_t)S, "batch ray jobs")) ||
+      (st = al((void **)&bb->a_job[0], 2 * aq, "batch ray jobs")) ||
+      (st = al((void **)&bb->a_job[1], 2 * aq, "batch ray jobs")) ||
+      (st = al((void **)&bb->jobs, sizeof(BatchJob) * (size_t)n_jobs, "batch job table")) ||
+      (st = al((void **)&bb->job_begin, 4 * ((size_t)n_jobs + 1), "batch work prefix")) ||
+      (st = al((void **)&bb->remaining, 4 * (size_t)n_jobs, "batch job counters")) ||
+      (st = al((void **)&bb->clocks, 8 * ((size_t)n_jobs + 1), "batch job clocks"))) {
+    bb->release();
+    return st;
+  }
+  bb->slots = S;
+  bb->n_jobs = n_jobs;
+  return AJG_OK;
+}
+
+// Batch arguments with defaults filled in and ranges checked.
+static int batch_resolve(ajg_ctx *ctx, const ajg_batch_args *in, ajg_batch_args *out) {
+  ajg_batch_args a;
+  memset(&a, 0, sizeof(a));
+  if (in) a = *in;
+  if (a.max_slots <= 0) a.max_slots = 1 << 20;
+  if (a.samples_per_item < 0) a.samples_per_item = 0;
+  if (a.trav_blocks_per_cu <= 0) a.trav_blocks_per_cu = kTravBlocksPerCU;
+  if (a.lds_stack <= 0) a.lds_stack = kLdsStack;
+  if (a.iters_per_check <= 0) a.iters_per_check = kItersPerCheck;
+  if (a.lds_stack != 4 && a.lds_stack != 8 && a.lds_stack != 16)
+    return fail(ctx, AJG_ERR_INVALID, "batch: lds_stack must be 4, 8 or 16");
+  if (a.trav_blocks_per_cu > 64 || a.iters_per_check > 64 || a.max_slots > (1 << 24))
+    return fail(ctx, AJG_ERR_INVALID, "batch: trav_blocks_per_cu <= 64, iters_per_check <= 64, max_slots <= 2^24");
+  if (a.pixel_tile != 0 && a.pixel_tile != 4 && a.pixel_tile != 8 && a.pixel_tile != 16)
+    return fail(ctx, AJG_ERR_INVALID, "batch: pixel_tile must be 0, 4, 8 or 16");
+  if (a.flags & AJG_RENDER_COUNT) return fail(ctx, AJG_ERR_INVALID, "batch: AJG_RENDER_COUNT is not supported");
+  a.iters_per_check += a.iters_per_check & 1;  // even: a captured block ends on queue 0
+  *out = a;
+  return AJG_OK;
+}
+
+static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
+                             const ajg_batch_args *bargs, ajg_batch_stats *stats,
+                             float *job_done_ms, void *stream) {
+  if (!ctx || !jobs || n_jobs == 0) return AJG_ERR_INVALID;
+  if (int st = injected(ctx)) return st;
+  if (n_jobs > 65535) return fail(ctx, AJG_ERR_INVALID, "batch: at most 65535 jobs");
+  ajg_batch_args ba;
+  if (int st = batch_resolve(ctx, bargs, &ba)) return st;
+  const uint64_t target = (uint64_t)ba.max_slots;
+
+  // ---- job table: each job's render parameters and scene, work prefix
+  std::vector<BatchJob> hj(n_jobs);
+  std::vector<uint32_t> begin(n_jobs + 1, 0u);
+  std::vector<uint32_t> work_of(n_jobs);
+  uint32_t max_depth = 0;
+  uint64_t paths = 0, total = 0;
+  int cu = 256;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const ajg_job &J = jobs[j];
+    if (!J.scene || J.scene->ctx != ctx || !J.film.sum || !J.film.sum_sq || !J.film.count)
+      return fail(ctx, AJG_ERR_INVALID, "batch: every job needs a scene of this context and a film");
+    RenderParams P;
+    if (int st = setup_params(ctx, &J.camera, &J.args, target, (uint32_t)ba.samples_per_item, &P))
+      return st;
+    const uint32_t T = (uint32_t)ba.pixel_tile;
+    const uint32_t Hp = P.n_pix / P.Wp;
+    if (T && P.Wp % T == 0 && Hp % T == 0) {
+      P.tile = T;
+      P.tiles_x = P.Wp / T;
+    }
+    memset(&hj[j], 0, sizeof(BatchJob));
+    hj[j].sc = dev_scene(J.scene);
+    hj[j].P = P;
+    work_of[j] = P.total_work;
+    begin[j] = (uint32_t)total;
+    total += P.total_work;
+    if (total > 0xFFFFFFF0ull) return fail(ctx, AJG_ERR_INVALID, "batch: too much work");
+    max_depth = std::max(max_depth, J.scene->accel->depth);
+    paths += (uint64_t)P.n_pix * P.spp;
+    cu = J.scene->cu_count;
+  }
+  begin[n_jobs] = (uint32_t)total;
+  const uint32_t S = (uint32_t)std::min<uint64_t>(target, total);
+  const int spill = spill_for_depth(max_depth, ba.lds_stack);
+  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "batch: tree depth above 1024");
+
+  BatchBuffers *bb = (BatchBuffers *)ctx->batch_cache;
+  if (!bb) {
+    bb = new BatchBuffers();
+    ctx->batch_cache = bb;
+    ctx->batch_free = batch_cache_free;
+  }
+  if (int st = ensure_buffers(ctx, &bb->rb, S, total)) return st;
+  if (int st = ensure_batch_buffers(ctx, bb, S, n_jobs)) return st;
+  RenderBuffers *rb = &bb->rb;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    hj[j].partial = rb->partial + 6 * (size_t)begin[j];
+    hj[j].partial_n = rb->partial_n + begin[j];
+    hj[j].per_sample = jobs[j].film.per_sample;
+  }
+
+  // The batch runs on its own stream (capturable), after the caller's work.
+  hipStream_t us = (hipStream_t)stream;
+  hipStream_t s = bb->stream;
+  hipEvent_t ev0, ev1;
+  (void)hipEventCreate(&ev0);
+  (void)hipEventCreate(&ev1);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto done = [&](int code) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return code;
+  };
+#define AJG_BCHK(call)                                             \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) return done(hip_fail(ctx, e_, #call));   \
+  } while (0)
+  AJG_BCHK(hipStreamSynchronize(us));
+  PathState ps = rb->ps;
+  AJG_BCHK(hipMemcpyAsync(bb->jobs, hj.data(), sizeof(BatchJob) * n_jobs, hipMemcpyHostToDevice, s));
+  AJG_BCHK(hipMemcpyAsync(bb->job_begin, begin.data(), 4 * ((size_t)n_jobs + 1), hipMemcpyHostToDevice, s));
+  AJG_BCHK(hipMemcpyAsync(bb->remaining, work_of.data(), 4 * (size_t)n_jobs, hipMemcpyHostToDevice, s));
+  AJG_BCHK(hipMemsetAsync(bb->clocks, 0, 8 * ((size_t)n_jobs + 1), s));
+  AJG_BCHK(hipMemsetAsync(ps.work, 0xFF, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.sample, 0, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.sample_end, 0, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.ext_idx, 0xFF, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.bank, 0, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.rng, 0, 8 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.pend_flags, 0, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.acc, 0, 4 * 24 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.acc_n, 0, 8 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.job, 0, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(ps.pend_job, 0, 4 * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(rb->partial, 0, sizeof(float) * 6 * total, s));
+  AJG_BCHK(hipMemsetAsync(rb->partial_n, 0, 4 * total, s));
+  AJG_BCHK(hipMemsetAsync(rb->counters, 0, 4 * kCounterWords, s));
+  AJG_BCHK(hipMemsetAsync(rb->a_occ, 0, (size_t)(kMaxLights + 1) * S, s));
+  AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * (size_t)S, s));
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const ajg_film &f = jobs[j].film;
+    const size_t np = hj[j].P.n_pix;
+    AJG_BCHK(hipMemsetAsync(f.sum, 0, 12 * np, s));
+    AJG_BCHK(hipMemsetAsync(f.sum_sq, 0, 12 * np, s));
+    AJG_BCHK(hipMemsetAsync(f.count, 0, 4 * np, s));
+    if (f.per_sample) AJG_BCHK(hipMemsetAsync(f.per_sample, 0, 12 * np * hj[j].P.spp, s));
+  }
+
+  uint32_t *counters = rb->counters;  // word layout as in a single render
+  unsigned long long *totals = (unsigned long long *)(counters + 16);
+  DevScene sc0;
+  memset(&sc0, 0, sizeof(sc0));
+  RenderParams P0;
+  memset(&P0, 0, sizeof(P0));
+  GuideParams G;
+  memset(&G, 0, sizeof(G));
+  BatchParams B;
+  memset(&B, 0, sizeof(B));
+  B.jobs = bb->jobs;
+  B.job_begin = bb->job_begin;
+  B.n_jobs = n_jobs;
+  B.slots = S;
+  B.total_work = (uint32_t)total;
+  B.remaining = bb->remaining;
+  B.done_clock = bb->clocks;
+  Queues q;
+  memset(&q, 0, sizeof(q));
+  q.c_hits_in = rb->c_hits;
+  q.a_occ_in = rb->a_occ;
+  q.live = counters + 4;
+  q.work_counter = counters + 5;
+  const uint32_t shade_grid = cdiv(S, 256);
+  const uint32_t trav_grid = (uint32_t)std::max(1, cu) * (uint32_t)ba.trav_blocks_per_cu;
+  const int K = ba.iters_per_check;
+  // One block of K iterations starting and ending on queue generation 0.
+  auto enqueue_block = [&]() {
+    int cur = 0;
+    for (int k = 0; k < K; ++k) {
+      const int nxt = cur ^ 1;
+      q.c_rays_in = rb->c_rays[cur];
+      q.c_rays_out = rb->c_rays[nxt];
+      q.c_count_in = counters + cur;
+      q.c_count_out = counters + nxt;
+      q.a_rays_in = rb->a_rays[cur];
+      q.a_rays_out = rb->a_rays[nxt];
+      q.a_count_in = counters + 2 + cur;
+      q.a_count_out = counters + 2 + nxt;
+      B.c_job_out = bb->c_job[nxt];
+      B.a_job_out = bb->a_job[nxt];
+      B.c_job_in = bb->c_job[cur];
+      B.a_job_in = bb->a_job[cur];
+      hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
+      hipLaunchKernelGGL((k_shade<false, true>), dim3(shade_grid), dim3(256), 0, s, sc0, P0, ps, q, G,
+                         B);
+      if (spill == 48) {
+        launch_pq_b<48>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
+                        counters + nxt, counters + 6, trav_grid, s);
+        launch_pq_b<48>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
+                        counters + 2 + nxt, counters + 7, trav_grid, s);
+      } else {
+        launch_pq_b<1024>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
+                          counters + nxt, counters + 6, trav_grid, s);
+        launch_pq_b<1024>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
+                          counters + 2 + nxt, counters + 7, trav_grid, s);
+      }
+      cur = nxt;
+    }
+  };
+  if (ba.use_graph) {
+    AJG_BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_block();
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess) return done(hip_fail(ctx, e, "hipStreamEndCapture"));
+    AJG_BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, bb->clocks + n_jobs);
+  AJG_BCHK(hipEventRecord(ev0, s));
+  uint32_t iters = 0, graph_launches = 0;
+  uint32_t hc[8];
+  for (;;) {
+    if (exec) {
+      AJG_BCHK(hipGraphLaunch(exec, s));
+      ++graph_launches;
+    } else {
+      enqueue_block();
+    }
+    iters += (uint32_t)K;
+    AJG_BCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    AJG_BCHK(hipStreamSynchronize(s));
+    if (hc[4] == 0) break;
+    if (iters > 400000) return done(fail(ctx, AJG_ERR_INTERNAL, "batch: iteration limit"));
+  }
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const RenderParams &P = hj[j].P;
+    const ajg_film &f = jobs[j].film;
+    if (P.n_chunks > 1)
+      hipLaunchKernelGGL(k_reduce_chunks, dim3(cdiv(P.n_pix, 256)), dim3(256), 0, s,
+                         (const float *)hj[j].partial, (const uint32_t *)hj[j].partial_n, P.n_pix,
+                         P.n_chunks, f.sum, f.sum_sq, f.count);
+    else
+      hipLaunchKernelGGL(k_unpack_single, dim3(cdiv(P.n_pix, 256)), dim3(256), 0, s,
+                         (const float *)hj[j].partial, (const uint32_t *)hj[j].partial_n, P.n_pix,
+                         f.sum, f.sum_sq, f.count);
+  }
+  AJG_BCHK(hipEventRecord(ev1, s));
+  unsigned long long tot[2] = {0, 0};
+  std::vector<unsigned long long> clk(n_jobs + 1);
+  AJG_BCHK(hipMemcpyAsync(tot, totals, 16, hipMemcpyDeviceToHost, s));
+  AJG_BCHK(hipMemcpyAsync(clk.data(), bb->clocks, 8 * ((size_t)n_jobs + 1), hipMemcpyDeviceToHost, s));
+  AJG_BCHK(hipStreamSynchronize(s));
+  AJG_BCHK(hipGetLastError());
+  if (job_done_ms) {
+    int khz = 0;
+    AJG_BCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    const double per_ms = khz > 0 ? (double)khz : 1e5;
+    for (uint32_t j = 0; j < n_jobs; ++j)
+      job_done_ms[j] = clk[j] >= clk[n_jobs] ? (float)((double)(clk[j] - clk[n_jobs]) / per_ms) : -1.f;
+  }
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    stats->ms_total = ms;
+    stats->iterations = iters;
+    stats->slots = S;
+    stats->graph_launches = graph_launches;
+    stats->paths = paths;
+    stats->closest_queries = tot[0];
+    stats->any_queries = tot[1];
+  }
+#undef AJG_BCHK
+  return done(AJG_OK);
+}
+
 extern "C" {
 
 int ajg_render_pt(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_args *args,
@@ -1729,6 +2265,67 @@ int ajg_render_pt_guided(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_a
     gstats->ms_train += fs.ms_train;
     if (fs.train_steps) gstats->last_loss = fs.last_loss;
   }
+  return AJG_OK;
+}
+
+int ajg_render_batch(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
+                     const ajg_batch_args *bargs, ajg_batch_stats *stats, float *job_done_ms,
+                     void *stream) {
+  return render_batch_impl(ctx, jobs, n_jobs, bargs, stats, job_done_ms, stream);
+}
+
+int ajg_batch_autotune(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs, uint32_t trial_spp,
+                       ajg_batch_args *inout, float *best_paths_per_s) {
+  if (!ctx || !jobs || n_jobs == 0 || !inout || trial_spp == 0) return AJG_ERR_INVALID;
+  ajg_batch_args cur;
+  if (int st = batch_resolve(ctx, inout, &cur)) return st;
+  std::vector<ajg_job> tj(jobs, jobs + n_jobs);
+  for (auto &j : tj) j.args.spp = std::min<int32_t>(j.args.spp, (int32_t)trial_spp);
+  auto rate = [&](const ajg_batch_args &a, double *r) {
+    ajg_batch_stats st;
+    if (int e = render_batch_impl(ctx, tj.data(), n_jobs, &a, &st, nullptr, nullptr)) return e;
+    *r = st.ms_total > 0.f ? (double)st.paths / ((double)st.ms_total * 1e-3) : 0.0;
+    return (int)AJG_OK;
+  };
+  double best = 0.0;
+  if (int st = rate(cur, &best)) return st;  // first run also allocates
+  if (int st = rate(cur, &best)) return st;
+  // Coordinate descent, one pass: queue size (slots in flight), work-item
+  // shape (samples per item, pixel tile order), traversal stack in LDS,
+  // traversal grid, iterations per host check.
+  struct Knob {
+    int32_t ajg_batch_args::*field;
+    std::vector<int32_t> values;
+  };
+  const Knob knobs[] = {
+      {&ajg_batch_args::max_slots, {1 << 18, 1 << 19, 1 << 20, 1 << 21}},
+      {&ajg_batch_args::samples_per_item, {0, 1, 2, 4, 8}},
+      {&ajg_batch_args::pixel_tile, {0, 4, 8, 16}},
+      {&ajg_batch_args::lds_stack, {4, 8, 16}},
+      {&ajg_batch_args::trav_blocks_per_cu, {4, 6, 8, 12, 16}},
+      {&ajg_batch_args::iters_per_check, {2, 4, 8}},
+  };
+  bool any_chunked = false;  // samples_per_item only shapes seed_mode 1 jobs
+  for (const auto &j : tj) any_chunked |= j.args.seed_mode != 0;
+  for (const Knob &k : knobs) {
+    if (k.field == &ajg_batch_args::samples_per_item && !any_chunked) continue;
+    const int32_t keep = cur.*(k.field);
+    int32_t pick = keep;
+    for (int32_t v : k.values) {
+      if (v == keep) continue;
+      ajg_batch_args a = cur;
+      a.*(k.field) = v;
+      double r = 0.0;
+      if (int st = rate(a, &r)) return st;
+      if (r > best) {
+        best = r;
+        pick = v;
+      }
+    }
+    cur.*(k.field) = pick;
+  }
+  *inout = cur;
+  if (best_paths_per_s) *best_paths_per_s = (float)best;
   return AJG_OK;
 }
 

@@ -37,7 +37,7 @@ k_trace_closest(DevBVH bvh, const ajg_ray *rays, uint32_t n, ajg_hit *hits,
   uint32_t nn = 0, nt = 0;
   persistent_closest<SPILL, false>(
       bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
-      [&](uint32_t i) { return batch_ray(rays, i); },
+      [&](uint32_t i, DevBVH &) { return batch_ray(rays, i); },
       [&](uint32_t i, const THit &h) {
         ajg_hit o;
         o.t = h.t;

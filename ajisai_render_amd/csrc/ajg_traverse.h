@@ -137,25 +137,28 @@ struct StackRef {
   float *se;
 };
 
+// LDS: ring entries per lane (a power of two; kLdsStack unless tuned).
+template <int LDS = kLdsStack>
 __device__ __forceinline__ void stack_push(const StackRef &s, int &d, int n, float e) {
-  const int slot = (d & (kLdsStack - 1)) * kTravThreads;
-  if (d >= kLdsStack) {  // the slot holds entry d - kLdsStack: move it to the spill
-    s.sn[d - kLdsStack] = s.ln[slot];
-    s.se[d - kLdsStack] = s.le[slot];
+  const int slot = (d & (LDS - 1)) * kTravThreads;
+  if (d >= LDS) {  // the slot holds entry d - LDS: move it to the spill
+    s.sn[d - LDS] = s.ln[slot];
+    s.se[d - LDS] = s.le[slot];
   }
   s.ln[slot] = n;
   s.le[slot] = e;
   ++d;
 }
 
+template <int LDS = kLdsStack>
 __device__ __forceinline__ void stack_pop(const StackRef &s, int &d, int *n, float *e) {
   --d;
-  const int slot = (d & (kLdsStack - 1)) * kTravThreads;
+  const int slot = (d & (LDS - 1)) * kTravThreads;
   *n = s.ln[slot];
   *e = s.le[slot];
-  if (d >= kLdsStack) {  // bring entry d - kLdsStack back into the freed slot
-    s.ln[slot] = s.sn[d - kLdsStack];
-    s.le[slot] = s.se[d - kLdsStack];
+  if (d >= LDS) {  // bring entry d - LDS back into the freed slot
+    s.ln[slot] = s.sn[d - LDS];
+    s.le[slot] = s.se[d - LDS];
   }
 }
 
@@ -214,7 +217,7 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, 
   return h;
 }
 
-template <int SPILL, bool COUNT = false>
+template <int SPILL, bool COUNT = false, int LDS = kLdsStack>
 __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int *lds_node,
                                           float *lds_entry, uint32_t *n_nodes = nullptr,
                                           uint32_t *n_tris = nullptr) {
@@ -244,7 +247,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
       const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1) &&
                       e1 < r.tmax;
       if (s0 && s1) {
-        stack_push(st, d, fn.d.y, e1);
+        stack_push<LDS>(st, d, fn.d.y, e1);
         node = fn.d.x;
         continue;
       }
@@ -260,7 +263,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
     if (d == 0) break;
     int n;
     float en;
-    stack_pop(st, d, &n, &en);
+    stack_pop<LDS>(st, d, &n, &en);
     node = n;
   }
   return false;
@@ -293,14 +296,19 @@ __device__ __forceinline__ uint32_t refill(bool idle, uint32_t n, uint32_t *curs
   return idx < n ? idx : kNone;
 }
 
-// Drive closest-hit traversal over rays [0, n) of a batch. load(i) returns
-// the TRay of ray i, store(i, hit) writes its result. The per-lane state is
-// kept in plain locals (registers); only the stack spill array is private
-// memory.
-template <int SPILL, bool COUNT, class Load, class Store>
-__device__ __forceinline__ void persistent_closest(const DevBVH &bvh, uint32_t n, uint32_t *cursor,
-                                                   int *lds_node, float *lds_entry, Load load,
-                                                   Store store, uint32_t *nn, uint32_t *nt) {
+// Drive closest-hit traversal over rays [0, n) of a batch. load(i, bvh)
+// returns the TRay of ray i and, when PER_RAY_BVH, sets the BVH it is traced
+// in (batched renders); store(i, hit) writes its result. The per-lane state
+// is kept in plain locals (registers); only the stack spill array is
+// private memory.
+template <int SPILL, bool COUNT, bool PER_RAY_BVH = false, int LDS = kLdsStack, class Load,
+          class Store>
+__device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, uint32_t n,
+                                                   uint32_t *cursor, int *lds_node,
+                                                   float *lds_entry, Load load, Store store,
+                                                   uint32_t *nn, uint32_t *nt) {
+  DevBVH bvh_lane = bvh_uniform;
+  const DevBVH &bvh = PER_RAY_BVH ? bvh_lane : bvh_uniform;
   int sn[SPILL];
   float se[SPILL];
   const StackRef st = {lds_node, lds_entry, sn, se};
@@ -316,7 +324,7 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh, uint32_t n
     if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
       const uint32_t idx = refill(idle, n, cursor, &drained);
       if (idx != kNone) {
-        r = load(idx);
+        r = load(idx, bvh_lane);
         h.t = __int_as_float(0x7f800000);
         h.u = 0.f;
         h.v = 0.f;
@@ -351,7 +359,7 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh, uint32_t n
       float e0, e1;
       const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
       const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
-      if (s1) stack_push(st, d, fn.d.y, e1);
+      if (s1) stack_push<LDS>(st, d, fn.d.y, e1);
       if (s0 && e0 < tmax) {
         node = fn.d.x;
         next = true;
@@ -361,7 +369,7 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh, uint32_t n
       while (d > 0) {
         int pn;
         float en;
-        stack_pop(st, d, &pn, &en);
+        stack_pop<LDS>(st, d, &pn, &en);
         if (en < tmax) {
           node = pn;
           next = true;
@@ -376,9 +384,10 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh, uint32_t n
   }
 }
 
-// Spill capacity for a tree of the given depth (0: unsupported depth).
-inline int spill_for_depth(uint32_t depth) {
-  if (depth <= (uint32_t)kLdsStack + 48) return 48;
+// Spill capacity for a tree of the given depth with an LDS ring of `lds`
+// entries per lane (0: unsupported depth).
+inline int spill_for_depth(uint32_t depth, int lds = kLdsStack) {
+  if (depth <= (uint32_t)lds + 48) return 48;
   if (depth <= 1024) return 1024;
   return 0;
 }

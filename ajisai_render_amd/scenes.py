@@ -423,6 +423,90 @@ def occluded_room(seed: int = 20261206, res=(512, 512), spp=64, env_size=(1024, 
                  PTSpec(spp=spp, tile_size=16, min_bounces=3, max_bounces=16, cont_prob=0.9))
 
 
+_ICO_CACHE = {}
+
+
+def _icosphere_cached(subdiv: int):
+    if subdiv not in _ICO_CACHE:
+        _ICO_CACHE[subdiv] = icosphere(subdiv)
+    return _ICO_CACHE[subdiv]
+
+
+def wavy_icosphere(subdiv: int, rng, radius=1.0, center=(0.0, 0.0, 0.0), amplitude=0.15):
+    """MeshGeo buffers of an icosphere displaced along its normals by a
+    seeded sum of 6 sinusoidal waves (frequencies doubling, amplitudes
+    halving: an fBm-like spectrum that costs O(vertices) to evaluate)."""
+    v, f = _icosphere_cached(subdiv)
+    disp = np.zeros(v.shape[0])
+    amp, freq = 1.0, 1.5
+    for _ in range(6):
+        k = rng.normal(size=3)
+        k *= freq / np.linalg.norm(k)
+        disp += amp * np.sin(np.pi * (v @ k) + rng.uniform(0, 2 * np.pi))
+        amp *= 0.5
+        freq *= 2.0
+    disp /= np.abs(disp).max() + 1e-12
+    p = v * (radius * (1.0 + amplitude * disp))[:, None] + np.asarray(center)
+    fn = np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]])
+    n = np.stack([sum(np.bincount(f[:, k], weights=fn[:, c], minlength=p.shape[0])
+                      for k in range(3)) for c in range(3)], axis=1)
+    n /= np.linalg.norm(n, axis=1, keepdims=True) + 1e-30
+    pos4 = np.concatenate([p, np.zeros((p.shape[0], 1))], axis=1).astype(np.float32)
+    nrm4 = np.concatenate([n, np.zeros((p.shape[0], 1))], axis=1).astype(np.float32)
+    return pos4, nrm4, f.astype(np.uint32)
+
+
+BATCH_RES = (128, 256, 512, 1024)
+BATCH_SPP = (4, 16, 64)
+
+
+def batch_scene(rng, env, tri_target: float, res: int, spp: int, name="batch") -> Scene:
+    """One job of the batched uneven workload: a set of 1-8 displaced
+    icospheres whose triangle total approaches tri_target (each sphere's
+    subdivision picked for an even share of what is left), 1-8 Disney
+    materials, a 2-triangle ground quad, the shared env map, res x res,
+    spp samples, max 12 bounces (Russian roulette after bounce 5, p 0.9)."""
+    n_sph = int(rng.integers(1, 9))
+    n_mat = int(rng.integers(1, 9))
+    mats = [_disney_from(rng) for _ in range(n_mat)]
+    left = float(tri_target)
+    prims = []
+    for i in range(n_sph):
+        share = max(left / (n_sph - i), 20.0)
+        sub = int(np.clip(np.round(np.log(share / 20.0) / np.log(4.0)), 0, 8))
+        ang = rng.uniform(0, 2 * np.pi)
+        dist = 0.0 if i == 0 else rng.uniform(0.8, 2.2)
+        rad = rng.uniform(0.3, 0.8) if i else 0.9
+        c = (dist * np.cos(ang), rad - 1.0, dist * np.sin(ang))
+        pos4, nrm4, f = wavy_icosphere(sub, rng, radius=rad, center=c)
+        prims.append(mesh(pos4, nrm4, f, i % n_mat, two_sided=True))
+        left -= f.shape[0]
+    prims.append(quad((-8, -1.0, -8), (-8, -1.0, 8), (8, -1.0, 8), (8, -1.0, -8), 0))
+    lights = [Light(LIGHT_ENV, env=env, rot=0.0, scaler=1.0)]
+    az = rng.uniform(0, 2 * np.pi)
+    cam = CameraSpec(pos=(5.0 * np.cos(az), 1.2, 5.0 * np.sin(az)), look_at=(0.0, -0.3, 0.0),
+                     up=(0.0, 1.0, 0.0), fov=45.0, res=(res, res))
+    return Scene(name, prims, mats, lights, cam,
+                 PTSpec(spp=spp, tile_size=16, min_bounces=5, max_bounces=12, cont_prob=0.9))
+
+
+def batched_jobs(n_jobs: int = 64, seed: int = 20261205, env_size=(1024, 512),
+                 tri_range=(1e3, 2e6), res_choices=BATCH_RES, spp_choices=BATCH_SPP):
+    """The batched uneven workload: n_jobs independent scenes with triangle
+    totals log-uniform in tri_range, resolution uniform among res_choices
+    (square), spp uniform among spp_choices, max 12 bounces, all lit by the
+    heavy-mesh scene's env map."""
+    rng = np.random.default_rng(seed)
+    env = hdr_env(env_size[0], env_size[1], 20261204 + 2)
+    out = []
+    for j in range(n_jobs):
+        t = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        res = int(rng.choice(res_choices))
+        spp = int(rng.choice(spp_choices))
+        out.append(batch_scene(rng, env, t, res, spp, name=f"batch{j}"))
+    return out
+
+
 def random_soup(n: int, seed: int, extent=1.0, size=0.05) -> np.ndarray:
     """(n, 9) float32 random triangle soup in [-extent, extent]^3."""
     rng = np.random.default_rng(seed)

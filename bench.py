@@ -22,6 +22,14 @@ guided one's warm-up training included), and relMSE = mean over pixels of
 mean_c (I - R)^2 / (R_lum^2 + 0.01 mean(R)^2) against a high-spp reference R
 (guided by default, unbiased; --guided-ref plain for a plain-PT reference).
 "mlp" times the guide network's kernels alone (MFMA utilisation).
+
+The "batched" object is the batched uneven workload: 64 independent scenes
+(displaced-icosphere sets, triangle totals log-uniform in [1e3, 2e6], 1-8
+Disney materials, resolution among 128^2 ... 1024^2, 4/16/64 spp, max 12
+bounces, the same HDR env map) rendered as one wavefront whose iterations
+are replayed from a captured hipGraph, with the autotuner's setting; it
+reports aggregate paths/s, each job's completion time, and the same jobs
+rendered one after another for comparison.
 """
 import argparse
 import json
@@ -56,6 +64,9 @@ def parse():
     ap.add_argument("--guided-res", type=int, default=512)
     ap.add_argument("--guided-ref", choices=["guided", "plain"], default="guided")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
+    ap.add_argument("--batch-jobs", type=int, default=64,
+                    help="jobs of the batched uneven workload (0: skip)")
+    ap.add_argument("--batch-trial-spp", type=int, default=1)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -204,6 +215,82 @@ def guided_equal_time(ctx, a):
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_pq_closest_heavy_mesh_1024_64spp.txt")
 
 
+def batched_bench(ctx, a):
+    """Batched uneven workload: autotune, then one timed batched render, then
+    the same jobs rendered alone one after another."""
+    import numpy as np
+    import torch
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import BatchRenderer, GpuScene, PathTracer, batch_defaults
+
+    t = time.perf_counter()
+    specs = scenes.batched_jobs(a.batch_jobs)
+    t_gen = time.perf_counter() - t
+    t = time.perf_counter()
+    gss = [GpuScene(ctx, sc) for sc in specs]
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t
+    trs = [PathTracer(g, seed_mode=0) for g in gss]
+    br = BatchRenderer(ctx, trs)
+    films = br.films()
+    # untuned batch (library defaults), also the warm-up of the buffers
+    _, st0, _ = br.render_device(batch_defaults(), films=films)
+    t = time.perf_counter()
+    tuned, tuned_rate = br.autotune(trial_spp=a.batch_trial_spp, films=films)
+    t_tune = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st, done = br.render_device(tuned, films=films)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    # the same jobs alone, one after another (each job's own film)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    seq_ms = 0.0
+    for pt, f in zip(trs, films):
+        _, s1 = pt.render_device(film=f)
+        seq_ms += s1.ms_total
+    torch.cuda.synchronize()
+    seq_wall = time.perf_counter() - t
+    paths = int(st.paths)
+    order = np.argsort(done)
+    res = [int(sc.camera.res[0]) for sc in specs]
+    out = {
+        "jobs": len(specs),
+        "triangles": int(sum(sc.n_tri for sc in specs)),
+        "paths": paths,
+        "seconds": round(wall, 3),
+        "paths_per_s": round(paths / wall, 1),
+        "device_ms": round(st.ms_total, 2),
+        "mrays_per_s": round((st.closest_queries + st.any_queries) / wall / 1e6, 2),
+        "iterations": st.iterations,
+        "graph_launches": st.graph_launches,
+        "slots": st.slots,
+        "untuned_paths_per_s": round(paths / (st0.ms_total * 1e-3), 1),
+        "autotuned": {k: getattr(tuned, k) for k, _ in tuned._fields_},
+        "autotune_trial_spp": a.batch_trial_spp,
+        "autotune_seconds": round(t_tune, 2),
+        "autotune_best_trial_paths_per_s": round(tuned_rate, 1),
+        "sequential_seconds": round(seq_wall, 3),
+        "sequential_paths_per_s": round(paths / seq_wall, 1),
+        "speedup_vs_sequential": round(seq_wall / wall, 3),
+        "job_done_ms": [round(float(x), 2) for x in done],
+        "job_done_ms_summary": {"min": round(float(done.min()), 2),
+                                "median": round(float(np.median(done)), 2),
+                                "max": round(float(done.max()), 2)},
+        "job_paths": [int(sc.camera.res[0] * sc.camera.res[1] * sc.pt.spp) for sc in specs],
+        "job_triangles": [int(sc.n_tri) for sc in specs],
+        "job_res": res,
+        "job_spp": [int(sc.pt.spp) for sc in specs],
+        "first_done_job": int(order[0]),
+        "scene_generation_s": round(t_gen, 2),
+        "scene_build_s": round(t_build, 2),
+    }
+    for g in gss:
+        g.close()
+    return out
+
+
 def profiled_traffic():
     """HBM bytes per launch of the closest-hit traversal kernel as measured by
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a
@@ -325,6 +412,7 @@ def main():
 
     mlp = mlp_bench(ctx)
     guided = guided_equal_time(ctx, a) if a.guided_budget > 0 else None
+    batched = batched_bench(ctx, a) if a.batch_jobs > 0 else None
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -362,6 +450,7 @@ def main():
         "relmse_guided_vs_plain": (round(1.0 / guided["equal_time_relmse_ratio"], 4)
                                    if guided else None),
         "guided": guided,
+        "batched": batched,
         "mlp": mlp,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -361,6 +361,64 @@ int ajg_render_pt_guided(ajg_scene *scene, const ajg_camera *camera,
                          ajg_render_stats *stats, ajg_guide_stats *gstats,
                          void *stream);
 
+/* ---------------------------------------------------------- batched jobs
+ * No counterpart in the reference, which renders one (scene, camera) per
+ * Renderer::Render call (renderer.h:64-65, pt.cpp:40-73). A batch runs many
+ * jobs of uneven size through one wavefront: every job's work items (pixel
+ * x chunk of samples) join one work list, the path slots take items from
+ * any job, and every queued ray carries its job so traversal uses that
+ * job's BVH. Each job's film ends as ajg_render_pt leaves it: for seed_mode 0
+ * and for seed_mode 1 with the same samples per work item the sums are
+ * bitwise those of the job rendered alone; per-sample radiance
+ * (film.per_sample) is bitwise equal in every case. */
+typedef struct ajg_job {
+  ajg_scene *scene;     /* all jobs of a batch share one ajg_ctx */
+  ajg_camera camera;
+  ajg_pt_args args;     /* max_slots and flags are ignored (batch-wide) */
+  ajg_film film;
+} ajg_job;
+
+typedef struct ajg_batch_args {
+  int32_t max_slots;          /* path slots in flight (0: 1 << 20) */
+  int32_t samples_per_item;   /* seed_mode 1 jobs: samples per work item
+                                 (0: as ajg_render_pt chooses for max_slots) */
+  int32_t trav_blocks_per_cu; /* persistent traversal grid (0: 8) */
+  int32_t lds_stack;          /* LDS ring entries per lane: 4, 8 or 16 (0: 8) */
+  int32_t iters_per_check;    /* iterations per host check, even (0: 4) */
+  int32_t use_graph;          /* 1: replay the iterations from a hipGraph */
+  uint32_t flags;             /* AJG_RENDER_COUNT is not supported */
+  int32_t pixel_tile;         /* work-list pixel order: 0 scanline, or T x T
+                                 tiles (4, 8, 16) where T divides the traced area */
+} ajg_batch_args;
+
+typedef struct ajg_batch_stats {
+  float ms_total;            /* device time of the batch (events) */
+  uint32_t iterations;
+  uint32_t slots;
+  uint32_t graph_launches;   /* hipGraph replays (0 without use_graph) */
+  uint64_t paths;            /* sum over jobs of W*H*spp (traced area) */
+  uint64_t closest_queries;
+  uint64_t any_queries;
+} ajg_batch_stats;
+
+/* Renders n_jobs jobs in one wavefront. job_done_ms (optional, n_jobs
+ * floats) receives each job's completion time in ms after the batch start
+ * (device wall clock at the last work item's final accumulate). Films are
+ * cleared and filled like ajg_render_pt. Synchronous. */
+int ajg_render_batch(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
+                     const ajg_batch_args *bargs, ajg_batch_stats *stats,
+                     float *job_done_ms, void *stream);
+
+/* Autotuner: coordinate descent over max_slots, samples_per_item,
+ * pixel_tile, lds_stack, trav_blocks_per_cu and iters_per_check, each
+ * candidate timed
+ * on the batch with every job's spp cut to trial_spp (the films receive
+ * the trial renders). Starts from *inout and leaves the fastest setting
+ * there; best_paths_per_s (optional) gets its rate. Synchronous. */
+int ajg_batch_autotune(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
+                       uint32_t trial_spp, ajg_batch_args *inout,
+                       float *best_paths_per_s);
+
 #ifdef __cplusplus
 }
 #endif
