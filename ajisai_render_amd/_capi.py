@@ -107,6 +107,13 @@ class GuideStats(C.Structure):
                 ("ms_warmup", C.c_float)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("material", C.c_uint32), ("object2world", C.c_float * 12)]
+
+
+INSTANCE_KEEP_MATERIAL = 0xFFFFFFFF
+
+
 class Job(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("args", PTArgs), ("film", Film)]
 
@@ -147,6 +154,12 @@ _SIGS = {
                           C.POINTER(Light), C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)],
                          C.c_int),
     "ajg_scene_destroy": ([_P], C.c_int),
+    "ajg_scene_create_instanced": ([_P, C.POINTER(Primitive), C.c_uint32, C.POINTER(Instance),
+                                    C.c_uint32, C.POINTER(Material), C.c_uint32, C.POINTER(Light),
+                                    C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(_P)], C.c_int),
+    "ajg_scene_instances": ([_P, _P, _P], C.c_int),
+    "ajg_scene_trace_closest": ([_P, _P, C.c_uint32, _P, _P, _P], C.c_int),
+    "ajg_scene_trace_any": ([_P, _P, C.c_uint32, _P, _P], C.c_int),
     "ajg_scene_get_accel": ([_P, C.POINTER(_P)], C.c_int),
     "ajg_scene_env_tables": ([_P, C.c_uint32, _P, _P, _P, _P], C.c_int),
     "ajg_render_pt": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), C.POINTER(Film),

@@ -219,15 +219,56 @@ class GpuScene:
         mats = c_materials(scene.materials)
         lights, keep2 = c_lights(scene.lights)
         h = C.c_void_p()
-        ctx.check(ctx.lib.ajg_scene_create(ctx.h, prims, len(scene.primitives), mats,
-                                           len(scene.materials), lights, len(scene.lights),
-                                           scene.search_radius, env_cdf_fp64, C.byref(h)),
-                  "ajg_scene_create")
+        if scene.instances is not None:
+            insts = (capi.Instance * max(1, len(scene.instances)))()
+            for i, ins in enumerate(scene.instances):
+                m = np.asarray(ins.o2w, np.float32).reshape(12)
+                insts[i] = capi.Instance(ins.mesh, capi.INSTANCE_KEEP_MATERIAL if ins.material < 0
+                                         else ins.material, (C.c_float * 12)(*m))
+            ctx.check(ctx.lib.ajg_scene_create_instanced(
+                ctx.h, prims, len(scene.primitives), insts, len(scene.instances), mats,
+                len(scene.materials), lights, len(scene.lights), scene.search_radius,
+                env_cdf_fp64, C.byref(h)), "ajg_scene_create_instanced")
+        else:
+            ctx.check(ctx.lib.ajg_scene_create(ctx.h, prims, len(scene.primitives), mats,
+                                               len(scene.materials), lights, len(scene.lights),
+                                               scene.search_radius, env_cdf_fp64, C.byref(h)),
+                      "ajg_scene_create")
         self.h = h
         del keep, keep2
         acc = C.c_void_p()
         ctx.check(ctx.lib.ajg_scene_get_accel(self.h, C.byref(acc)))
         self.accel = PLOCAggregate(ctx, scene.search_radius, handle=acc, owner=False)
+
+    # Scene::Intersect / Occlude batched (default_scene.cpp:63-70)
+    def Intersect(self, rays_dev, stream=None):
+        """Returns (hits n x 16 B as uint8 device tensor viewable as
+        ajg_hit, instance ids)."""
+        torch = _torch()
+        n = rays_dev.numel() // 32
+        hits = torch.empty(n * 16, dtype=torch.uint8, device=rays_dev.device)
+        inst = torch.empty(n, dtype=torch.int32, device=rays_dev.device)
+        self.ctx.check(self.ctx.lib.ajg_scene_trace_closest(self.h, _ptr(rays_dev), n, _ptr(hits),
+                                                            _ptr(inst), stream), "scene trace")
+        return hits, inst
+
+    def Occlude(self, rays_dev, stream=None):
+        torch = _torch()
+        n = rays_dev.numel() // 32
+        occ = torch.empty(n, dtype=torch.uint8, device=rays_dev.device)
+        self.ctx.check(self.ctx.lib.ajg_scene_trace_any(self.h, _ptr(rays_dev), n, _ptr(occ),
+                                                        stream), "scene occlude")
+        return occ
+
+    def instances(self):
+        """(world->object maps (n, 12), padded world boxes (n, 6)) as used
+        on the device."""
+        n = len(self.scene.instances)
+        w2o = np.zeros((n, 12), np.float32)
+        box = np.zeros((n, 6), np.float32)
+        self.ctx.check(self.ctx.lib.ajg_scene_instances(self.h, w2o.ctypes.data, box.ctypes.data),
+                       "instances")
+        return w2o, box
 
     def env_tables(self, li):
         l = self.scene.lights[li]

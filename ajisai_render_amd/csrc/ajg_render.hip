@@ -40,6 +40,15 @@ struct ajg_scene {
   int cu_count = 256;
   void *render_cache = nullptr;  // RenderBuffers reused across renders
   void *guide_cache = nullptr;   // GuideBuffers reused across guided renders
+  // instanced scenes: `accel` is the top-level tree over instance boxes
+  std::vector<ajg_accel *> blas;   // one bottom-level tree per mesh
+  ajg::DevInst *insts = nullptr;
+  ajg::DevBlas *blases = nullptr;
+  uint32_t n_inst = 0;
+  uint32_t depth2 = 0;             // top depth + deepest bottom depth
+  std::vector<float> inst_w2o;     // 12 floats per instance (as on the device)
+  std::vector<float> inst_box;     // 6 floats per instance (padded world box)
+  uint32_t *trace_ctr = nullptr;   // cursor of scene trace calls
 };
 
 namespace ajg {
@@ -116,9 +125,11 @@ __global__ void k_env_marginal(const float *cond_int, uint32_t h, float *marg_fu
 
 // --------------------------------------------------------------- render
 struct DevScene {
-  DevBVH bvh;
+  DevBVH bvh;                  // instanced: the top-level tree
   const uint4 *tri_shade;
   const float4 *normals;
+  const DevInst *inst;         // instanced scenes only (else null)
+  const DevBlas *blas;
   const DevMaterial *mats;
   DevLight lights[kMaxLights];
   uint32_t n_lights;
@@ -181,6 +192,7 @@ struct PathState {
 struct Queues {
   DevRay *c_rays_in, *c_rays_out;
   float4 *c_hits_in;   // t, leaf slot (int bits), u, v
+  const uint32_t *c_inst_in;  // instanced scenes: instance of each hit
   uint32_t *c_count_in, *c_count_out;
   DevRay *a_rays_in, *a_rays_out;
   uint8_t *a_occ_in;
@@ -291,14 +303,28 @@ struct Inct {
 // MT normal + PostIntersect (ploc_aggregate.cpp:199-211; quad.cpp:73-90,
 // cube.cpp:96-108 (tuple order reversed in the index buffer), mesh.cpp:183-204,
 // two_sided.cpp:44-53).
+// Instanced scenes: the triangle is read in its mesh's object space; the
+// hit point comes from the world ray, and normals go to world space through
+// the instance's normal map (for a rotation with uniform scale that map is the
+// rotation itself, so the shading normal has the length it would have in a
+// world-space copy of the mesh).
 __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay &r, float t, int ls,
-                                               float u, float v) {
+                                               float u, float v, uint32_t inst = 0) {
   Inct it;
   f3 a, b, c;
-  load_leaf(sc.bvh.leaf_tris, ls, &a, &b, &c);
-  const uint4 info = sc.tri_shade[ls];
+  const float4 *leaf = sc.bvh.leaf_tris;
+  const uint4 *shade = sc.tri_shade;
+  const DevInst *I = nullptr;
+  if (sc.inst) {
+    I = sc.inst + inst;
+    const DevBlas &B = sc.blas[I->info.x];
+    leaf = B.bvh.leaf_tris;
+    shade = B.tri_shade;
+  }
+  load_leaf(leaf, ls, &a, &b, &c);
+  const uint4 info = shade[ls];
   const uint32_t kind = info.x >> kKindShift;
-  it.material = info.x & kMatMask;
+  it.material = (I && I->info.y != kKeepMaterial) ? I->info.y : (info.x & kMatMask);
   it.pos = add(r.o, scale(r.d, t));
   it.wr = neg(r.d);
   if (kind == AJG_GEOM_CUBE) {
@@ -314,6 +340,11 @@ __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay 
     } else {
       it.ns = it.ng;
     }
+  }
+  if (I) {
+    const bool interp = kind == AJG_GEOM_MESH;
+    it.ng = normalize(xform_vec(I->nrm, it.ng));
+    it.ns = interp ? xform_vec(I->nrm, it.ns) : it.ng;
   }
   if ((info.x & kTwoSidedBit) && dot(it.ng, it.wr) < 0) {
     it.ng = scale(it.ng, -1.f);
@@ -499,6 +530,117 @@ k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uin
       const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       occ[i] = trace_any<SPILL, false, LDS>(b, r, s_node + threadIdx.x, s_entry + threadIdx.x) ? 1 : 0;
     }
+  }
+}
+
+// Instanced scenes: two-level traversal (top-level tree over instances).
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
+k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
+              float4 *hits, uint32_t *hit_inst, const uint32_t *count, uint32_t *cursor) {
+  __shared__ int s_node[kLdsStack * kTravThreads];
+  __shared__ float s_entry[kLdsStack * kTravThreads];
+  const uint32_t n = *count;
+  if (n == 0) return;
+  persistent_closest_2l<SPILL, kLdsStack>(
+      tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t i) {
+        const DevRay ray = load_ray(rays, i);
+        return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      },
+      [&](uint32_t i, const THit &h, uint32_t inst) {
+        hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+        hit_inst[i] = inst;
+      });
+}
+
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
+k_pq_any2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
+          uint8_t *occ, const uint32_t *count) {
+  __shared__ int s_node[kLdsStack * kTravThreads];
+  __shared__ float s_entry[kLdsStack * kTravThreads];
+  const uint32_t n = *count;
+  if (n == 0) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (kTravThreads / 64);
+  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
+    const uint32_t i = c * 64 + lane;
+    if (i < n) {
+      const DevRay ray = load_ray(rays, i);
+      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      occ[i] = trace_any_2l<SPILL, kLdsStack>(tlas, insts, blases, r, s_node + threadIdx.x,
+                                             s_entry + threadIdx.x) ? 1 : 0;
+    }
+  }
+}
+
+// C-ABI scene traces of instanced scenes (rays in the ajg_ray layout).
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
+k_scene_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const ajg_ray *rays,
+                 uint32_t n, ajg_hit *hits, uint32_t *hit_inst, uint32_t *cursor) {
+  __shared__ int s_node[kLdsStack * kTravThreads];
+  __shared__ float s_entry[kLdsStack * kTravThreads];
+  persistent_closest_2l<SPILL, kLdsStack>(
+      tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t i) {
+        const ajg_ray ray = rays[i];
+        return make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
+      },
+      [&](uint32_t i, const THit &h, uint32_t inst) {
+        ajg_hit o;
+        o.t = h.t;
+        o.tri_id = h.slot >= 0 ? blases[insts[inst].info.x].perm[h.slot] : kNone;
+        o.u = h.u;
+        o.v = h.v;
+        hits[i] = o;
+        if (hit_inst) hit_inst[i] = inst;
+      });
+}
+
+template <int SPILL>
+__global__ void __launch_bounds__(kTravThreads)
+k_scene_any2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const ajg_ray *rays,
+             uint32_t n, uint8_t *occ) {
+  __shared__ int s_node[kLdsStack * kTravThreads];
+  __shared__ float s_entry[kLdsStack * kTravThreads];
+  const uint32_t i = blockIdx.x * kTravThreads + threadIdx.x;
+  if (i >= n) return;
+  const ajg_ray ray = rays[i];
+  const TRay r = make_tray(ld3(ray.o), ld3(ray.d), ray.tmin, ray.tmax);
+  occ[i] = trace_any_2l<SPILL, kLdsStack>(tlas, insts, blases, r, s_node + threadIdx.x,
+                                         s_entry + threadIdx.x) ? 1 : 0;
+}
+
+__global__ void k_clear_word(uint32_t *w) {
+  if (threadIdx.x == 0) *w = 0u;
+}
+
+__global__ void __launch_bounds__(256)
+k_one_level_instance(const ajg_hit *hits, uint32_t n, uint32_t *inst) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) inst[i] = hits[i].tri_id == kNone ? kNone : 0u;
+}
+
+static void launch_pq2(int stack, const DevScene &ds, bool closest, const DevRay *rays, void *out,
+                       uint32_t *hit_inst, const uint32_t *count, uint32_t *cursor, uint32_t grid,
+                       hipStream_t s) {
+  const dim3 g(grid), b(kTravThreads);
+  if (closest) {
+    if (stack == 48)
+      hipLaunchKernelGGL((k_pq_closest2<48>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+                         (float4 *)out, hit_inst, count, cursor);
+    else
+      hipLaunchKernelGGL((k_pq_closest2<1024>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+                         (float4 *)out, hit_inst, count, cursor);
+  } else {
+    if (stack == 48)
+      hipLaunchKernelGGL((k_pq_any2<48>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+                         (uint8_t *)out, count);
+    else
+      hipLaunchKernelGGL((k_pq_any2<1024>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+                         (uint8_t *)out, count);
   }
 }
 
@@ -909,7 +1051,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
                    L, last);
         need_new = true;
       } else {
-        const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w);
+        const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w, sc.inst ? q.c_inst_in[ei] : 0u);
         const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
         const _Float16 *gm = GUIDED ? G.mix + (size_t)ei * kGMix : nullptr;
         // The guide is mixed in only where the BSDF's pdf is the density of
@@ -1100,8 +1242,8 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
 // Guided rendering: position and unit shading normal of each closest-queue
 // hit (the guide network's input); misses get a zero record.
 __global__ void __launch_bounds__(256)
-k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t *count,
-             float *pts) {
+k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t *hit_inst,
+             const uint32_t *count, float *pts) {
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const float4 h = hits[i];
@@ -1109,7 +1251,7 @@ k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t
     f3 p = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 1.f);
     if (ls >= 0) {
       const DevRay r = load_ray(rays, i);
-      const Inct it = post_intersect(sc, r, h.x, ls, h.z, h.w);
+      const Inct it = post_intersect(sc, r, h.x, ls, h.z, h.w, sc.inst ? hit_inst[i] : 0u);
       p = it.pos;
       nn = normalize(it.ns);
     }
@@ -1205,6 +1347,7 @@ struct RenderBuffers {
   DevRay *c_rays[2] = {nullptr, nullptr};
   DevRay *a_rays[2] = {nullptr, nullptr};
   float4 *c_hits = nullptr;
+  uint32_t *c_inst = nullptr;  // instance of each closest hit (instanced scenes)
   uint8_t *a_occ = nullptr;
   uint32_t *counters = nullptr;
   float *partial = nullptr;
@@ -1292,6 +1435,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&rb->c_rays[0], sizeof(DevRay) * cq_cap, "closest queue")) ||
       (st = al((void **)&rb->c_rays[1], sizeof(DevRay) * cq_cap, "closest queue")) ||
       (st = al((void **)&rb->c_hits, sizeof(float4) * cq_cap, "closest hits")) ||
+      (st = al((void **)&rb->c_inst, 4 * cq_cap, "closest hit instances")) ||
       (st = al((void **)&rb->a_rays[0], sizeof(DevRay) * aq_cap, "shadow queue")) ||
       (st = al((void **)&rb->a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
       (st = al((void **)&rb->a_occ, aq_cap, "shadow results")) ||
@@ -1310,6 +1454,8 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
 
 using namespace ajg;
 
+static DevScene dev_scene(const ajg_scene *sc);
+
 static void scene_free(ajg_scene *s) {
   if (!s) return;
   destroy_accel(s->accel);
@@ -1319,29 +1465,27 @@ static void scene_free(ajg_scene *s) {
   for (void *p : s->env_bufs) (void)hipFree(p);
   delete (RenderBuffers *)s->render_cache;
   delete (GuideBuffers *)s->guide_cache;
+  for (ajg_accel *b : s->blas) destroy_accel(b);
+  (void)hipFree(s->insts);
+  (void)hipFree(s->blases);
+  (void)hipFree(s->trace_ctr);
   delete s;
 }
 
-extern "C" {
 
-int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
-                     const ajg_material *mats, uint32_t n_mats, const ajg_light *lights,
-                     uint32_t n_lights, uint32_t search_radius, int32_t env_cdf_fp64,
-                     ajg_scene **out) {
-  if (!ctx || !out || (n_prims && !prims) || (n_mats && !mats) || (n_lights && !lights))
-    return AJG_ERR_INVALID;
-  *out = nullptr;
-  if (int st = injected(ctx)) return st;
+// Argument checks shared by one-level and instanced scenes; counts the
+// mesh normals the primitives bring.
+static int check_scene_inputs(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
+                              const ajg_material *mats, uint32_t n_mats, const ajg_light *lights,
+                              uint32_t n_lights, size_t *n_normals) {
   if (n_lights > (uint32_t)kMaxLights)
     return fail(ctx, AJG_ERR_INVALID, "scene: at most 4 lights are supported");
   if (n_mats == 0) return fail(ctx, AJG_ERR_INVALID, "scene: no materials");
   for (uint32_t i = 0; i < n_mats; ++i)
     if (mats[i].type != AJG_MAT_DIFFUSE && mats[i].type != AJG_MAT_DISNEY)
       return fail(ctx, AJG_ERR_INVALID, "scene: unsupported material type");
-  std::vector<ajg_mesh_view> views(n_prims);
-  size_t n_normals = 0;
+  *n_normals = 0;
   for (uint32_t i = 0; i < n_prims; ++i) {
-    views[i] = prims[i].view;
     if (prims[i].kind > AJG_GEOM_MESH)
       return fail(ctx, AJG_ERR_INVALID, "scene: unsupported geometry kind");
     if (prims[i].material >= n_mats)
@@ -1349,7 +1493,7 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
     if (prims[i].kind == AJG_GEOM_MESH) {
       if (!prims[i].normal_and_vs)
         return fail(ctx, AJG_ERR_INVALID, "scene: mesh without normal_and_vs");
-      n_normals += prims[i].view.position_size;
+      *n_normals += prims[i].view.position_size;
     }
   }
   int env_count = 0;
@@ -1364,69 +1508,72 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
   }
   if (env_count > 1) return fail(ctx, AJG_ERR_INVALID, "scene: more than one env light");
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, AJG_ERR_HIP, "hipSetDevice");
+  return AJG_OK;
+}
 
-  std::vector<float> tri;
-  float box[6];
-  if (int st = flatten_views(ctx, views.data(), n_prims, &tri, box)) return st;
-  ajg_scene *s = new ajg_scene();
-  s->ctx = ctx;
-  {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
-      s->cu_count = prop.multiProcessorCount;
-  }
-  int st = upload_and_build(ctx, tri, box, search_radius, nullptr, &s->accel);
-  if (st) {
-    scene_free(s);
-    return st;
-  }
-  const uint32_t n = s->accel->n_tri;
-  for (int i = 0; i < 6; ++i) s->world[i] = box[i];
-  // DirectionalLight/EnvLight radius = |max - center| of the world bound
-  {
-    box3 wb;
-    wb.mn = ld3(box);
-    wb.mx = ld3(box + 3);
-    s->radius = length(sub(wb.mx, box_center(wb)));
-  }
-  // per-triangle shading records, permuted into leaf-slot order
-  std::vector<uint32_t> perm(n);
-  if (hipMemcpy(perm.data(), s->accel->perm, 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
-    scene_free(s);
-    return fail(ctx, AJG_ERR_HIP, "perm download");
-  }
-  std::vector<uint4> by_id(n);
-  std::vector<float4> normals(std::max<size_t>(n_normals, 1));
-  {
-    size_t t = 0, noff = 0;
-    for (uint32_t p = 0; p < n_prims; ++p) {
-      const ajg_primitive &pr = prims[p];
-      const uint32_t flags = (pr.kind << kKindShift) | (pr.two_sided ? kTwoSidedBit : 0u) |
-                             (pr.material & kMatMask);
-      for (size_t i = 0; i < pr.view.index_size; ++i, ++t) {
-        uint4 rec = make_uint4(flags, 0, 0, 0);
-        if (pr.kind == AJG_GEOM_MESH) {
-          const char *base = (const char *)pr.view.index_buffer + pr.view.index_offset +
-                             i * pr.view.index_stride;
-          uint32_t w3[3];
-          memcpy(w3, base, 12);
-          rec.y = (uint32_t)(noff + w3[0]);
-          rec.z = (uint32_t)(noff + w3[1]);
-          rec.w = (uint32_t)(noff + w3[2]);
-        }
-        by_id[t] = rec;
-      }
+// PostIntersect records of primitives [p0, p1) in flatten order (flags |
+// material, then the three normal indices of a mesh triangle into the
+// shared normal array, whose mesh normals are appended to `normals`).
+static void shade_records(const ajg_primitive *prims, uint32_t p0, uint32_t p1,
+                          std::vector<uint4> *by_id, std::vector<float4> *normals) {
+  for (uint32_t p = p0; p < p1; ++p) {
+    const ajg_primitive &pr = prims[p];
+    const size_t noff = normals->size();
+    const uint32_t flags = (pr.kind << kKindShift) | (pr.two_sided ? kTwoSidedBit : 0u) |
+                           (pr.material & kMatMask);
+    for (size_t i = 0; i < pr.view.index_size; ++i) {
+      uint4 rec = make_uint4(flags, 0, 0, 0);
       if (pr.kind == AJG_GEOM_MESH) {
-        for (size_t v = 0; v < pr.view.position_size; ++v) {
-          const float *src = pr.normal_and_vs + 4 * v;
-          normals[noff + v] = make_float4(src[0], src[1], src[2], src[3]);
-        }
-        noff += pr.view.position_size;
+        const char *base = (const char *)pr.view.index_buffer + pr.view.index_offset +
+                           i * pr.view.index_stride;
+        uint32_t w3[3];
+        memcpy(w3, base, 12);
+        rec.y = (uint32_t)(noff + w3[0]);
+        rec.z = (uint32_t)(noff + w3[1]);
+        rec.w = (uint32_t)(noff + w3[2]);
+      }
+      by_id->push_back(rec);
+    }
+    if (pr.kind == AJG_GEOM_MESH) {
+      for (size_t v = 0; v < pr.view.position_size; ++v) {
+        const float *src = pr.normal_and_vs + 4 * v;
+        normals->push_back(make_float4(src[0], src[1], src[2], src[3]));
       }
     }
   }
-  std::vector<uint4> by_slot(n);
-  for (uint32_t i = 0; i < n; ++i) by_slot[i] = by_id[perm[i]];
+}
+
+static int download_perm(ajg_ctx *ctx, const ajg_accel *a, std::vector<uint32_t> *perm) {
+  perm->resize(a->n_tri);
+  if (hipMemcpy(perm->data(), a->perm, 4 * (size_t)a->n_tri, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(ctx, AJG_ERR_HIP, "perm download");
+  return AJG_OK;
+}
+
+static ajg_scene *new_scene(ajg_ctx *ctx) {
+  ajg_scene *s = new ajg_scene();
+  s->ctx = ctx;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+    s->cu_count = prop.multiProcessorCount;
+  return s;
+}
+
+// World bound and the light radius |max - center| (DirectionalLight/EnvLight
+// Process, default_scene.cpp:40-48).
+static void set_world(ajg_scene *s, const float *box) {
+  for (int i = 0; i < 6; ++i) s->world[i] = box[i];
+  box3 wb;
+  wb.mn = ld3(box);
+  wb.mx = ld3(box + 3);
+  s->radius = length(sub(wb.mx, box_center(wb)));
+}
+
+// Materials and lights of a scene whose geometry is in place (the env
+// Distribution2D tables are built on the device, EnvLight ctor,
+// env_light.cpp:29-48).
+static int finish_scene(ajg_ctx *ctx, ajg_scene *s, const ajg_material *mats, uint32_t n_mats,
+                        const ajg_light *lights, uint32_t n_lights, int32_t env_cdf_fp64) {
   std::vector<DevMaterial> dm(n_mats);
   for (uint32_t i = 0; i < n_mats; ++i) {
     const ajg_material &m = mats[i];
@@ -1443,22 +1590,11 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
     dm[i].clearcoat = m.clearcoat;
     dm[i].clearcoat_gloss = m.clearcoat_gloss;
   }
-  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(uint4) * n, "tri shade")) ||
-      (st = dev_alloc(ctx, (void **)&s->normals, sizeof(float4) * normals.size(), "normals")) ||
-      (st = dev_alloc(ctx, (void **)&s->mats, sizeof(DevMaterial) * n_mats, "materials"))) {
-    scene_free(s);
-    return st;
-  }
-  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(uint4) * n, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(s->mats, dm.data(), sizeof(DevMaterial) * n_mats, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    scene_free(s);
-    return hip_fail(ctx, e, "scene upload");
-  }
+  int st = 0;
+  if ((st = dev_alloc(ctx, (void **)&s->mats, sizeof(DevMaterial) * n_mats, "materials"))) return st;
+  hipError_t e = hipMemcpy(s->mats, dm.data(), sizeof(DevMaterial) * n_mats, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "material upload");
   s->n_mats = n_mats;
-  // lights (Light::Process with the world bound, default_scene.cpp:40-48)
   s->n_lights = n_lights;
   for (uint32_t i = 0; i < n_lights; ++i) {
     DevLight &l = s->lights[i];
@@ -1482,15 +1618,10 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
     if ((st = al(&tex, 3 * (size_t)w * h, "env texels")) || (st = al(&wts, (size_t)w * h, "env weights")) ||
         (st = al(&cf, (size_t)w * h, "env cond func")) || (st = al(&cc, (size_t)(w + 1) * h, "env cond cdf")) ||
         (st = al(&ci, h, "env cond int")) || (st = al(&mf, h, "env marg func")) ||
-        (st = al(&mc, h + 1, "env marg cdf")) || (st = al(&mi, 1, "env marg int"))) {
-      scene_free(s);
+        (st = al(&mc, h + 1, "env marg cdf")) || (st = al(&mi, 1, "env marg int")))
       return st;
-    }
     e = hipMemcpy(tex, lights[i].env_texels, sizeof(float) * 3 * (size_t)w * h, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      scene_free(s);
-      return hip_fail(ctx, e, "env upload");
-    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "env upload");
     l.env.w = w;
     l.env.h = h;
     l.env.tex = tex;
@@ -1508,13 +1639,284 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(64), 0, nullptr, (const float *)ci, h, mf, mc, mi,
                        (int)env_cdf_fp64);
     e = hipMemcpy(&l.env.marg_int, mi, 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) {
-      scene_free(s);
-      return hip_fail(ctx, e, "env tables");
+    if (e != hipSuccess) return hip_fail(ctx, e, "env tables");
+  }
+  if ((st = dev_alloc(ctx, (void **)&s->trace_ctr, 16, "trace cursor"))) return st;
+  return AJG_OK;
+}
+
+// Instance boxes and maps (host, once per scene). The world->object map is
+// the inverse of the given object->world map, computed in double and rounded
+// to float; the normal map is cbrt|det A| (A^-1)^T for the linear part A, so a
+// rotation with uniform scale maps normals by the rotation alone. The world
+// box of an instance is the box of its mesh's root box corners mapped to world
+// space, widened by 2^-18 of its largest coordinate magnitude (so that the
+// rounding of corner and ray transforms cannot cull a hit).
+static bool instance_maps(const float *o2w, const float *root_box, float *w2o, float *nrm,
+                          float *box) {
+  double A[3][3], t[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) A[r][c] = o2w[4 * r + c];
+    t[r] = o2w[4 * r + 3];
+  }
+  const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                     A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                     A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) return false;
+  double inv[3][3];
+  inv[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+  inv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+  inv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+  inv[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+  inv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+  inv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+  inv[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+  inv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+  inv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+  const double sc = std::cbrt(std::fabs(det));
+  for (int r = 0; r < 3; ++r) {
+    double tr = 0;
+    for (int c = 0; c < 3; ++c) {
+      w2o[4 * r + c] = (float)inv[r][c];
+      tr -= inv[r][c] * t[c];
+      nrm[4 * r + c] = (float)(inv[c][r] * sc);
+    }
+    w2o[4 * r + 3] = (float)tr;
+    nrm[4 * r + 3] = 0.f;
+  }
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = 0; k < 8; ++k) {
+    const float p[3] = {root_box[(k & 1) ? 3 : 0], root_box[(k & 2) ? 4 : 1], root_box[(k & 4) ? 5 : 2]};
+    for (int r = 0; r < 3; ++r) {
+      const float v = ((o2w[4 * r] * p[0] + o2w[4 * r + 1] * p[1]) + o2w[4 * r + 2] * p[2]) + o2w[4 * r + 3];
+      mn[r] = std::min(mn[r], v);
+      mx[r] = std::max(mx[r], v);
     }
   }
+  float mag = 0.f;
+  for (int r = 0; r < 3; ++r) mag = std::max(mag, std::max(std::fabs(mn[r]), std::fabs(mx[r])));
+  const float pad = std::ldexp(mag, -18) + 1e-30f;
+  for (int r = 0; r < 3; ++r) {
+    box[r] = mn[r] - pad;
+    box[3 + r] = mx[r] + pad;
+    if (!std::isfinite(box[r]) || !std::isfinite(box[3 + r])) return false;
+  }
+  return true;
+}
+
+extern "C" {
+
+int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
+                     const ajg_material *mats, uint32_t n_mats, const ajg_light *lights,
+                     uint32_t n_lights, uint32_t search_radius, int32_t env_cdf_fp64,
+                     ajg_scene **out) {
+  if (!ctx || !out || (n_prims && !prims) || (n_mats && !mats) || (n_lights && !lights))
+    return AJG_ERR_INVALID;
+  *out = nullptr;
+  if (int st = injected(ctx)) return st;
+  size_t n_normals = 0;
+  if (int st = check_scene_inputs(ctx, prims, n_prims, mats, n_mats, lights, n_lights, &n_normals))
+    return st;
+  std::vector<ajg_mesh_view> views(n_prims);
+  for (uint32_t i = 0; i < n_prims; ++i) views[i] = prims[i].view;
+  std::vector<float> tri;
+  float box[6];
+  if (int st = flatten_views(ctx, views.data(), n_prims, &tri, box)) return st;
+  ajg_scene *s = new_scene(ctx);
+  auto bail = [&](int st) {
+    scene_free(s);
+    return st;
+  };
+  if (int st = upload_and_build(ctx, tri, box, search_radius, nullptr, &s->accel)) return bail(st);
+  set_world(s, box);
+  // per-triangle shading records, permuted into leaf-slot order
+  const uint32_t n = s->accel->n_tri;
+  std::vector<uint32_t> perm;
+  if (int st = download_perm(ctx, s->accel, &perm)) return bail(st);
+  std::vector<uint4> by_id;
+  std::vector<float4> normals;
+  by_id.reserve(n);
+  normals.reserve(n_normals);
+  shade_records(prims, 0, n_prims, &by_id, &normals);
+  if (normals.empty()) normals.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+  std::vector<uint4> by_slot(n);
+  for (uint32_t i = 0; i < n; ++i) by_slot[i] = by_id[perm[i]];
+  int st = 0;
+  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(uint4) * n, "tri shade")) ||
+      (st = dev_alloc(ctx, (void **)&s->normals, sizeof(float4) * normals.size(), "normals")))
+    return bail(st);
+  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(uint4) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(hip_fail(ctx, e, "scene upload"));
+  if ((st = finish_scene(ctx, s, mats, n_mats, lights, n_lights, env_cdf_fp64))) return bail(st);
   *out = s;
   return AJG_OK;
+}
+
+int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes, uint32_t n_meshes,
+                               const ajg_instance *instances, uint32_t n_instances,
+                               const ajg_material *mats, uint32_t n_mats, const ajg_light *lights,
+                               uint32_t n_lights, uint32_t search_radius, int32_t env_cdf_fp64,
+                               ajg_scene **out) {
+  if (!ctx || !out || !meshes || n_meshes == 0 || !instances || n_instances == 0 ||
+      (n_mats && !mats) || (n_lights && !lights))
+    return AJG_ERR_INVALID;
+  *out = nullptr;
+  if (int st = injected(ctx)) return st;
+  size_t n_normals = 0;
+  if (int st = check_scene_inputs(ctx, meshes, n_meshes, mats, n_mats, lights, n_lights, &n_normals))
+    return st;
+  for (uint32_t i = 0; i < n_instances; ++i) {
+    if (instances[i].mesh >= n_meshes)
+      return fail(ctx, AJG_ERR_INVALID, "instanced scene: mesh index out of range");
+    if (instances[i].material != AJG_INSTANCE_KEEP_MATERIAL && instances[i].material >= n_mats)
+      return fail(ctx, AJG_ERR_INVALID, "instanced scene: material index out of range");
+  }
+  ajg_scene *s = new_scene(ctx);
+  auto bail = [&](int st) {
+    scene_free(s);
+    return st;
+  };
+  // bottom level: one PLOC tree per mesh, in the mesh's own space
+  std::vector<uint4> by_slot;
+  std::vector<float4> normals;
+  normals.reserve(n_normals);
+  std::vector<size_t> shade_off(n_meshes);
+  uint32_t max_depth = 0;
+  for (uint32_t m = 0; m < n_meshes; ++m) {
+    std::vector<float> tri;
+    float box[6];
+    if (int st = flatten_views(ctx, &meshes[m].view, 1, &tri, box)) return bail(st);
+    ajg_accel *acc = nullptr;
+    if (int st = upload_and_build(ctx, tri, box, search_radius, nullptr, &acc)) return bail(st);
+    s->blas.push_back(acc);
+    std::vector<uint32_t> perm;
+    if (int st = download_perm(ctx, acc, &perm)) return bail(st);
+    std::vector<uint4> by_id;
+    by_id.reserve(acc->n_tri);
+    shade_records(meshes, m, m + 1, &by_id, &normals);
+    shade_off[m] = by_slot.size();
+    for (uint32_t i = 0; i < acc->n_tri; ++i) by_slot.push_back(by_id[perm[i]]);
+    max_depth = std::max(max_depth, acc->depth);
+  }
+  if (normals.empty()) normals.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+  // top level: PLOC over the instance boxes, each given to the builder as the
+  // degenerate triangle (min, max, min), whose box is exactly the instance box
+  s->n_inst = n_instances;
+  s->inst_w2o.resize(12 * (size_t)n_instances);
+  s->inst_box.resize(6 * (size_t)n_instances);
+  std::vector<DevInst> di(n_instances);
+  std::vector<float> tl(9 * (size_t)n_instances);
+  float wbox[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < n_instances; ++i) {
+    const ajg_instance &in = instances[i];
+    float w2o[12], nrm[12], box[6];
+    if (!instance_maps(in.object2world, s->blas[in.mesh]->root_box, w2o, nrm, box))
+      return bail(fail(ctx, AJG_ERR_INVALID, "instanced scene: singular or non-finite object2world"));
+    memcpy(&s->inst_w2o[12 * (size_t)i], w2o, sizeof(w2o));
+    memcpy(&s->inst_box[6 * (size_t)i], box, sizeof(box));
+    for (int r = 0; r < 3; ++r) {
+      di[i].w2o[r] = make_float4(w2o[4 * r], w2o[4 * r + 1], w2o[4 * r + 2], w2o[4 * r + 3]);
+      di[i].nrm[r] = make_float4(nrm[4 * r], nrm[4 * r + 1], nrm[4 * r + 2], 0.f);
+    }
+    di[i].info = make_uint4(in.mesh, in.material, 0u, 0u);
+    float *t9 = &tl[9 * (size_t)i];
+    for (int k = 0; k < 3; ++k) {
+      t9[k] = box[k];
+      t9[3 + k] = box[3 + k];
+      t9[6 + k] = box[k];
+      wbox[k] = std::min(wbox[k], box[k]);
+      wbox[3 + k] = std::max(wbox[3 + k], box[3 + k]);
+    }
+  }
+  if (int st = upload_and_build(ctx, tl, wbox, search_radius, nullptr, &s->accel)) return bail(st);
+  s->depth2 = s->accel->depth + max_depth;
+  set_world(s, wbox);
+  std::vector<DevBlas> db(n_meshes);
+  int st = 0;
+  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(uint4) * by_slot.size(), "tri shade")) ||
+      (st = dev_alloc(ctx, (void **)&s->normals, sizeof(float4) * normals.size(), "normals")) ||
+      (st = dev_alloc(ctx, (void **)&s->insts, sizeof(DevInst) * n_instances, "instances")) ||
+      (st = dev_alloc(ctx, (void **)&s->blases, sizeof(DevBlas) * n_meshes, "mesh trees")))
+    return bail(st);
+  for (uint32_t m = 0; m < n_meshes; ++m) {
+    const ajg_accel *a = s->blas[m];
+    db[m].bvh.fat = a->fat;
+    db[m].bvh.leaf_tris = a->leaf_tris;
+    db[m].bvh.root_ref = a->root_ref;
+    for (int k = 0; k < 3; ++k) {
+      db[m].bvh.root_mn[k] = a->root_box[k];
+      db[m].bvh.root_mx[k] = a->root_box[3 + k];
+    }
+    db[m].tri_shade = s->tri_shade + shade_off[m];
+    db[m].perm = a->perm;
+  }
+  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(uint4) * by_slot.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->insts, di.data(), sizeof(DevInst) * n_instances, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->blases, db.data(), sizeof(DevBlas) * n_meshes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return bail(hip_fail(ctx, e, "instanced scene upload"));
+  if ((st = finish_scene(ctx, s, mats, n_mats, lights, n_lights, env_cdf_fp64))) return bail(st);
+  *out = s;
+  return AJG_OK;
+}
+
+int ajg_scene_instances(ajg_scene *scene, float *world2object, float *boxes) {
+  if (!scene || !scene->insts) return AJG_ERR_INVALID;
+  if (world2object) memcpy(world2object, scene->inst_w2o.data(), 4 * scene->inst_w2o.size());
+  if (boxes) memcpy(boxes, scene->inst_box.data(), 4 * scene->inst_box.size());
+  return AJG_OK;
+}
+
+int ajg_scene_trace_closest(ajg_scene *scene, const ajg_ray *rays, uint32_t n, ajg_hit *hits,
+                            uint32_t *instance, void *stream) {
+  if (!scene || (n && (!rays || !hits))) return AJG_ERR_INVALID;
+  if (int st = injected(scene->ctx)) return st;
+  if (n == 0) return AJG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (!scene->insts) {
+    if (int st = launch_trace_closest(scene->accel, rays, n, hits, s)) return st;
+    if (instance)
+      hipLaunchKernelGGL(k_one_level_instance, dim3(cdiv(n, 256)), dim3(256), 0, s,
+                         (const ajg_hit *)hits, n, instance);
+  } else {
+    const int spill = spill_for_depth(scene->depth2);
+    if (!spill) return fail(scene->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+    const DevScene ds = dev_scene(scene);
+    const uint32_t grid = std::min<uint32_t>(cdiv(n, kTravThreads),
+                                             (uint32_t)scene->cu_count * kTravBlocksPerCU);
+    hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, s, scene->trace_ctr);
+    if (spill == 48)
+      hipLaunchKernelGGL(k_scene_closest2<48>, dim3(grid), dim3(kTravThreads), 0, s, ds.bvh, ds.inst,
+                         ds.blas, rays, n, hits, instance, scene->trace_ctr);
+    else
+      hipLaunchKernelGGL(k_scene_closest2<1024>, dim3(grid), dim3(kTravThreads), 0, s, ds.bvh, ds.inst,
+                         ds.blas, rays, n, hits, instance, scene->trace_ctr);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(scene->ctx, e, "scene trace closest");
+}
+
+int ajg_scene_trace_any(ajg_scene *scene, const ajg_ray *rays, uint32_t n, uint8_t *occluded,
+                        void *stream) {
+  if (!scene || (n && (!rays || !occluded))) return AJG_ERR_INVALID;
+  if (int st = injected(scene->ctx)) return st;
+  if (n == 0) return AJG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (!scene->insts) return launch_trace_any(scene->accel, rays, n, occluded, s);
+  const int spill = spill_for_depth(scene->depth2);
+  if (!spill) return fail(scene->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+  const DevScene ds = dev_scene(scene);
+  if (spill == 48)
+    hipLaunchKernelGGL(k_scene_any2<48>, dim3(cdiv(n, kTravThreads)), dim3(kTravThreads), 0, s, ds.bvh,
+                       ds.inst, ds.blas, rays, n, occluded);
+  else
+    hipLaunchKernelGGL(k_scene_any2<1024>, dim3(cdiv(n, kTravThreads)), dim3(kTravThreads), 0, s, ds.bvh,
+                       ds.inst, ds.blas, rays, n, occluded);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(scene->ctx, e, "scene trace any");
 }
 
 int ajg_scene_destroy(ajg_scene *scene) {
@@ -1609,6 +2011,8 @@ static DevScene dev_scene(const ajg_scene *sc) {
   ds.n_lights = sc->n_lights;
   ds.env_index = sc->env_index;
   ds.radius = sc->radius;
+  ds.inst = sc->insts;
+  ds.blas = sc->blases;
   return ds;
 }
 
@@ -1733,16 +2137,20 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   if (film->per_sample) AJG_RCHK(hipMemsetAsync(film->per_sample, 0, 12 * n_pix_sz * P.spp, s));
 
   q.c_hits_in = c_hits;
+  q.c_inst_in = rb->c_inst;
   q.a_occ_in = a_occ;
   q.live = counters + 4;
   q.work_counter = counters + 5;
   q.partial = partial;
   q.partial_n = partial_n;
   q.per_sample = film->per_sample;
-  const int stack = spill_for_depth(a->depth);
+  const bool inst = ds.inst != nullptr;
+  const int stack = spill_for_depth(inst ? sc->depth2 : a->depth);
   if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree depth above 1024"));
 
   const bool counting = (args->flags & AJG_RENDER_COUNT) != 0;
+  if (inst && counting)
+    return done(fail(ctx, AJG_ERR_INVALID, "render: AJG_RENDER_COUNT is not supported for instanced scenes"));
   StageTimer timer;
   timer.on = (args->flags & AJG_RENDER_TIMING) != 0;
   // traversal counters: closest (nodes, tris) at +8, any (nodes, tris) at +12
@@ -1781,7 +2189,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
                            nob);
       timer.end(2, s);
       timer.begin(0, s);
-      if (counting)
+      if (inst)
+        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, counters + nxt, counters + 6,
+                   trav_grid, s);
+      else if (counting)
         launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, counters + 6,
                         trav_ctr, trav_grid, s);
       else
@@ -1789,7 +2200,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
                          trav_ctr, trav_grid, s);
       timer.end(0, s);
       timer.begin(1, s);
-      if (counting)
+      if (inst)
+        launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, counters + 2 + nxt, nullptr,
+                   trav_grid, s);
+      else if (counting)
         launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, counters + 7,
                         trav_ctr + 2, trav_grid, s);
       else
@@ -1801,7 +2215,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
         timer.begin(3, s);
         hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(S, 256), 4096)), dim3(256), 0, s,
                            ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
-                           (const uint32_t *)(counters + nxt), gb->pts);
+                           (const uint32_t *)rb->c_inst, (const uint32_t *)(counters + nxt), gb->pts);
         if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, S, counters + nxt, 6, nullptr,
                                           gb->mix, nullptr, s))
           return done(st);
@@ -1999,6 +2413,8 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
     const ajg_job &J = jobs[j];
     if (!J.scene || J.scene->ctx != ctx || !J.film.sum || !J.film.sum_sq || !J.film.count)
       return fail(ctx, AJG_ERR_INVALID, "batch: every job needs a scene of this context and a film");
+    if (J.scene->insts)
+      return fail(ctx, AJG_ERR_INVALID, "batch: instanced scenes render with ajg_render_pt");
     RenderParams P;
     if (int st = setup_params(ctx, &J.camera, &J.args, target, (uint32_t)ba.samples_per_item, &P))
       return st;
@@ -2110,6 +2526,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   Queues q;
   memset(&q, 0, sizeof(q));
   q.c_hits_in = rb->c_hits;
+  q.c_inst_in = rb->c_inst;
   q.a_occ_in = rb->a_occ;
   q.live = counters + 4;
   q.work_counter = counters + 5;

@@ -384,6 +384,236 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
   }
 }
 
+// ---- Two-level traversal (instancing; no counterpart in the reference).
+// The top-level tree's leaves are instances; an instance holds the
+// world->object affine map of its mesh and the index of the mesh's own
+// (bottom-level) tree. Entering an instance transforms the ray into object
+// space with the direction left unnormalised, so the ray parameter t (and
+// with it t_max and every stacked entry distance) means the same in both
+// spaces and one stack serves both levels: `base` is the stack depth at which
+// the bottom-level traversal started; popping back to it restores the world
+// ray and the top-level arrays. Visit order, culling and arithmetic per level
+// are those of the one-level traversal.
+struct DevInst {
+  float4 w2o[3];  // rows of the world->object map (3x4, row-major)
+  float4 nrm[3];  // rows of the normal map (xyz): cbrt|det M| (M^-1)^T of the linear part
+  uint4 info;     // x: mesh (bottom-level tree), y: material or kKeepMaterial
+};
+static constexpr uint32_t kKeepMaterial = 0xFFFFFFFFu;
+
+struct DevBlas {
+  DevBVH bvh;
+  const uint4 *tri_shade;  // per leaf slot (as the one-level scene)
+  const uint32_t *perm;    // leaf slot -> triangle index within the mesh
+};
+
+// ((m0 x + m1 y) + m2 z) + m3 per row: the same order on every path.
+__device__ __forceinline__ f3 xform_point(const float4 *m, f3 p) {
+  return mk3(((m[0].x * p.x + m[0].y * p.y) + m[0].z * p.z) + m[0].w,
+             ((m[1].x * p.x + m[1].y * p.y) + m[1].z * p.z) + m[1].w,
+             ((m[2].x * p.x + m[2].y * p.y) + m[2].z * p.z) + m[2].w);
+}
+__device__ __forceinline__ f3 xform_vec(const float4 *m, f3 v) {
+  return mk3((m[0].x * v.x + m[0].y * v.y) + m[0].z * v.z,
+             (m[1].x * v.x + m[1].y * v.y) + m[1].z * v.z,
+             (m[2].x * v.x + m[2].y * v.y) + m[2].z * v.z);
+}
+
+__device__ __forceinline__ uint32_t tlas_leaf_instance(const DevBVH &tlas, int slot) {
+  return __float_as_uint(tlas.leaf_tris[3 * slot].w);
+}
+
+// Closest hit over a two-level scene; store(i, hit, instance).
+template <int SPILL, int LDS, class Load, class Store>
+__device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const DevInst *insts,
+                                                      const DevBlas *blases, uint32_t n,
+                                                      uint32_t *cursor, int *lds_node,
+                                                      float *lds_entry, Load load, Store store) {
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0;
+  TRay r, rw;  // ray of the current level; world ray
+  THit h;
+  uint32_t h_inst = kNone, cur_inst = 0;
+  float tmax = 0.f;
+  int node = 0, base = -1;
+  const FatNode *fat = tlas.fat;
+  const float4 *leaf = tlas.leaf_tris;
+  uint32_t ray = kNone;
+  bool drained = false;
+  for (;;) {
+    const bool idle = ray == kNone;
+    if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
+      const uint32_t idx = refill(idle, n, cursor, &drained);
+      if (idx != kNone) {
+        rw = load(idx);
+        r = rw;
+        h.t = __int_as_float(0x7f800000);
+        h.u = 0.f;
+        h.v = 0.f;
+        h.slot = -1;
+        h_inst = kNone;
+        tmax = rw.tmax;
+        d = 0;
+        base = -1;
+        fat = tlas.fat;
+        leaf = tlas.leaf_tris;
+        float e;
+        if (slab_static(rw, tlas.root_mn[0], tlas.root_mn[1], tlas.root_mn[2], tlas.root_mx[0],
+                        tlas.root_mx[1], tlas.root_mx[2], &e) &&
+            e < tmax) {
+          node = tlas.root_ref;
+          ray = idx;
+        } else {
+          store(idx, h, h_inst);
+        }
+      }
+      continue;
+    }
+    if (__ballot(ray != kNone) == 0ull) {
+      if (drained) break;
+      continue;
+    }
+    if (ray == kNone) continue;
+    bool next = false;
+    if (node < 0) {
+      if (base < 0) {  // an instance: enter its mesh's tree in object space
+        cur_inst = tlas_leaf_instance(tlas, ~node);
+        const DevInst *I = insts + cur_inst;
+        const DevBlas *B = blases + I->info.x;
+        const TRay ro = make_tray(xform_point(I->w2o, rw.o), xform_vec(I->w2o, rw.d), rw.tmin,
+                                  rw.tmax);
+        float e;
+        if (slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
+                        B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
+            e < tmax) {
+          r = ro;
+          base = d;
+          fat = B->bvh.fat;
+          leaf = B->bvh.leaf_tris;
+          node = B->bvh.root_ref;
+          next = true;
+        }
+      } else {
+        f3 a, b, c;
+        load_leaf(leaf, ~node, &a, &b, &c);
+        float t, u, v;
+        if (mt_test(a, b, c, r, tmax, &t, &u, &v)) {
+          tmax = t;
+          h.t = t;
+          h.u = u;
+          h.v = v;
+          h.slot = ~node;
+          h_inst = cur_inst;
+        }
+      }
+    } else {
+      const FatNode fn = fat[node];
+      float e0, e1;
+      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
+      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
+      if (s1) stack_push<LDS>(st, d, fn.d.y, e1);
+      if (s0 && e0 < tmax) {
+        node = fn.d.x;
+        next = true;
+      }
+    }
+    if (!next) {
+      for (;;) {
+        if (d == base) {  // the mesh's tree is done: back to the top level
+          base = -1;
+          r = rw;
+          fat = tlas.fat;
+          leaf = tlas.leaf_tris;
+        }
+        if (d == 0) break;
+        int pn;
+        float en;
+        stack_pop<LDS>(st, d, &pn, &en);
+        if (en < tmax) {
+          node = pn;
+          next = true;
+          break;
+        }
+      }
+    }
+    if (!next) {
+      store(ray, h, h_inst);
+      ray = kNone;
+    }
+  }
+}
+
+// Any hit over a two-level scene (one ray).
+template <int SPILL, int LDS>
+__device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *insts,
+                                             const DevBlas *blases, const TRay &rw, int *lds_node,
+                                             float *lds_entry) {
+  float e;
+  if (!slab_static(rw, tlas.root_mn[0], tlas.root_mn[1], tlas.root_mn[2], tlas.root_mx[0],
+                   tlas.root_mx[1], tlas.root_mx[2], &e) ||
+      !(e < rw.tmax))
+    return false;
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0, base = -1;
+  TRay r = rw;
+  const FatNode *fat = tlas.fat;
+  const float4 *leaf = tlas.leaf_tris;
+  int node = tlas.root_ref;
+  for (;;) {
+    bool next = false;
+    if (node < 0) {
+      if (base < 0) {
+        const DevInst *I = insts + tlas_leaf_instance(tlas, ~node);
+        const DevBlas *B = blases + I->info.x;
+        const TRay ro = make_tray(xform_point(I->w2o, rw.o), xform_vec(I->w2o, rw.d), rw.tmin,
+                                  rw.tmax);
+        if (slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
+                        B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
+            e < rw.tmax) {
+          r = ro;
+          base = d;
+          fat = B->bvh.fat;
+          leaf = B->bvh.leaf_tris;
+          node = B->bvh.root_ref;
+          next = true;
+        }
+      } else {
+        f3 a, b, c;
+        load_leaf(leaf, ~node, &a, &b, &c);
+        float t, u, v;
+        if (mt_test(a, b, c, r, rw.tmax, &t, &u, &v)) return true;
+      }
+    } else {
+      const FatNode fn = fat[node];
+      float e0, e1;
+      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0) &&
+                      e0 < rw.tmax;
+      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1) &&
+                      e1 < rw.tmax;
+      if (s0 && s1) stack_push<LDS>(st, d, fn.d.y, e1);
+      if (s0 || s1) {
+        node = s0 ? fn.d.x : fn.d.y;
+        next = true;
+      }
+    }
+    if (!next) {
+      if (d == base) {
+        base = -1;
+        r = rw;
+        fat = tlas.fat;
+        leaf = tlas.leaf_tris;
+      }
+      if (d == 0) return false;
+      float en;
+      stack_pop<LDS>(st, d, &node, &en);
+    }
+  }
+}
+
 // Spill capacity for a tree of the given depth with an LDS ring of `lds`
 // entries per lane (0: unsupported depth).
 inline int spill_for_depth(uint32_t depth, int lds = kLdsStack) {

@@ -92,6 +92,25 @@ class PTSpec:
     cont_prob: float = 0.9
 
 
+KEEP_MATERIAL = -1
+
+
+@dataclasses.dataclass
+class Instance:
+    """A mesh placed in the world: row-major 3x4 object->world affine map;
+    material KEEP_MATERIAL keeps the mesh primitive's own."""
+    mesh: int
+    o2w: np.ndarray
+    material: int = KEEP_MATERIAL
+
+
+def normal_map(o2w) -> np.ndarray:
+    """cbrt|det A| (A^-1)^T of the linear part A: maps normals to world space
+    and is the rotation itself for a rotation with uniform scale."""
+    a = np.asarray(o2w, np.float64)[:, :3]
+    return np.linalg.inv(a).T * np.cbrt(abs(np.linalg.det(a)))
+
+
 @dataclasses.dataclass
 class Scene:
     name: str
@@ -101,9 +120,12 @@ class Scene:
     camera: CameraSpec
     pt: PTSpec
     search_radius: int = 14
+    instances: Optional[List[Instance]] = None  # set: primitives are the meshes
 
     @property
     def n_tri(self) -> int:
+        if self.instances is not None:
+            return sum(self.primitives[i.mesh].n_tri for i in self.instances)
         return sum(p.n_tri for p in self.primitives)
 
     def soup(self) -> np.ndarray:
@@ -116,6 +138,13 @@ class Scene:
         return np.ascontiguousarray(np.concatenate(out, axis=0).astype(np.float32))
 
     def scene_box(self) -> np.ndarray:
+        if self.instances is not None:
+            mn, mx = np.full(3, np.inf), np.full(3, -np.inf)
+            for ins in self.instances:
+                pos = self.primitives[ins.mesh].positions[:, :3].astype(np.float64)
+                w = pos @ np.asarray(ins.o2w, np.float64)[:, :3].T + np.asarray(ins.o2w)[:, 3]
+                mn, mx = np.minimum(mn, w.min(axis=0)), np.maximum(mx, w.max(axis=0))
+            return np.concatenate([mn, mx]).astype(np.float32)
         mn = np.full(3, np.finfo(np.float32).max, np.float32)
         mx = np.full(3, -np.finfo(np.float32).max, np.float32)
         for p in self.primitives:
@@ -505,6 +534,79 @@ def batched_jobs(n_jobs: int = 64, seed: int = 20261205, env_size=(1024, 512),
         spp = int(rng.choice(spp_choices))
         out.append(batch_scene(rng, env, t, res, spp, name=f"batch{j}"))
     return out
+
+
+def flatten_instances(scene: Scene) -> Scene:
+    """World-space copy of an instanced scene (one primitive per instance):
+    positions through the object->world map, mesh normals through the normal
+    map (not renormalised: interpolating them equals mapping the interpolated
+    object-space normal), instance materials applied."""
+    prims = []
+    for ins in scene.instances:
+        p = scene.primitives[ins.mesh]
+        m = np.asarray(ins.o2w, np.float64)
+        pos = p.positions.astype(np.float64).copy()
+        pos[:, :3] = pos[:, :3] @ m[:, :3].T + m[:, 3]
+        nrm = None
+        if p.normal_and_vs is not None:
+            nrm = p.normal_and_vs.astype(np.float64).copy()
+            nrm[:, :3] = nrm[:, :3] @ normal_map(m).T
+            nrm = nrm.astype(np.float32)
+        mat = p.material if ins.material == KEEP_MATERIAL else ins.material
+        prims.append(Primitive(p.kind, pos.astype(np.float32), p.index_words, p.two_sided, mat,
+                               nrm))
+    return dataclasses.replace(scene, primitives=prims, instances=None)
+
+
+def random_rotation(rng) -> np.ndarray:
+    """Uniform random rotation (normalised Gaussian quaternion)."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_field(seed: int = 20261207, n_instances: int = 2000, n_meshes: int = 8,
+                    tri_range=(5e4, 5e5), ground=200.0, res=(1920, 1080), spp=256,
+                    env_size=(2048, 1024), max_bounces=24) -> Scene:
+    """Large instanced scene: n_meshes seeded displaced icospheres (triangle
+    counts log-uniform in tri_range, rounded to the nearest icosphere level
+    20*4^s), n_instances instances with uniform random rotations, scale
+    log-uniform in [0.2, 5] and positions on a ground x ground quad (resting
+    on it), one Disney material per mesh; HDR env map (sigma 2.5) plus a
+    directional light; 1920x1080, 256 spp, max 24 bounces."""
+    rng = np.random.default_rng(seed)
+    mats = [_disney_from(rng) for _ in range(n_meshes)]
+    mats.append(Material(MAT_DIFFUSE, (0.45, 0.42, 0.38)))
+    prims = []
+    for m in range(n_meshes):
+        t = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        sub = int(np.clip(np.round(np.log(t / 20.0) / np.log(4.0)), 0, 8))
+        pos4, nrm4, f = wavy_icosphere(sub, rng, radius=1.0)
+        prims.append(mesh(pos4, nrm4, f, m, two_sided=True))
+    g = ground / 2
+    prims.append(quad((-g, 0.0, -g), (-g, 0.0, g), (g, 0.0, g), (g, 0.0, -g), n_meshes))
+    insts = []
+    for _ in range(n_instances):
+        m = int(rng.integers(0, n_meshes))
+        sc = float(np.exp(rng.uniform(np.log(0.2), np.log(5.0))))
+        rot = random_rotation(rng) * sc
+        x, z = rng.uniform(-g * 0.95, g * 0.95, 2)
+        o2w = np.concatenate([rot, np.array([[x], [sc * 0.9], [z]])], axis=1)
+        insts.append(Instance(m, o2w))
+    insts.append(Instance(n_meshes, np.concatenate([np.eye(3), np.zeros((3, 1))], axis=1)))
+    env = hdr_env(env_size[0], env_size[1], seed + 1, sigma=2.5)
+    d = rng.normal(size=3)
+    d[1] = abs(d[1]) + 0.5
+    lights = [Light(LIGHT_ENV, env=env, rot=0.0, scaler=1.0),
+              Light(LIGHT_DIRECTIONAL, dir=tuple(d / np.linalg.norm(d)), radiance=(3.0, 2.9, 2.7))]
+    cam = CameraSpec(pos=(0.0, 14.0, -g * 0.9), look_at=(0.0, 2.0, 0.0), up=(0.0, 1.0, 0.0),
+                     fov=50.0, res=res)
+    return Scene("instanced_field", prims, mats, lights, cam,
+                 PTSpec(spp=spp, tile_size=16, min_bounces=5, max_bounces=max_bounces,
+                        cont_prob=0.9), instances=insts)
 
 
 def random_soup(n: int, seed: int, extent=1.0, size=0.05) -> np.ndarray:

@@ -257,6 +257,48 @@ int ajg_scene_env_tables(ajg_scene *scene, uint32_t li, float *cond_cdf,
                          float *cond_func_int, float *marg_cdf,
                          float *marg_func_int);
 
+/* ------------------------------------------------------------- instancing
+ * No counterpart in the reference, which flattens every primitive into one
+ * PLOC tree (ploc_aggregate.cpp:252-327). An instanced scene keeps one PLOC
+ * tree per mesh in the mesh's own space (bottom level) and a PLOC tree over
+ * the instances' world boxes (top level); a ray entering an instance is
+ * mapped into the mesh's space with its direction left unnormalised, so the
+ * ray parameter t is the same at both levels. */
+#define AJG_INSTANCE_KEEP_MATERIAL 0xFFFFFFFFu
+
+typedef struct ajg_instance {
+  uint32_t mesh;           /* index into the scene's mesh list */
+  uint32_t material;       /* material of the instance, or
+                              AJG_INSTANCE_KEEP_MATERIAL for the mesh's own */
+  float object2world[12];  /* row-major 3x4 affine map (non-singular) */
+} ajg_instance;
+
+/* Instanced DefaultScene: each of the n_meshes primitives (any kind; its
+ * material and two-sided flag as in ajg_scene_create) becomes one
+ * bottom-level tree; instances place meshes in the world. Lights are
+ * processed against the union of the instance boxes. The scene renders
+ * with ajg_render_pt / ajg_render_pt_guided (not in batches).
+ * Synchronous. */
+int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes,
+                               uint32_t n_meshes, const ajg_instance *instances,
+                               uint32_t n_instances, const ajg_material *mats,
+                               uint32_t n_mats, const ajg_light *lights,
+                               uint32_t n_lights, uint32_t search_radius,
+                               int32_t env_cdf_fp64, ajg_scene **out);
+/* Host copies of the per-instance world->object maps (12 floats, row-major
+ * 3x4, as used on the device) and padded world boxes (6 floats); either may
+ * be NULL. */
+int ajg_scene_instances(ajg_scene *scene, float *world2object, float *boxes);
+/* Scene::Intersect / Occlude batched (default_scene.cpp:63-70) for one-level
+ * and instanced scenes. hits[i].tri_id is the triangle index within the hit
+ * mesh (one-level scenes: the flattened id, as ajg_accel_trace_closest);
+ * instance[i] (optional) receives the instance (0 for one-level scenes,
+ * 0xFFFFFFFF on a miss). Device pointers; enqueued on `stream`. */
+int ajg_scene_trace_closest(ajg_scene *scene, const ajg_ray *rays, uint32_t n,
+                            ajg_hit *hits, uint32_t *instance, void *stream);
+int ajg_scene_trace_any(ajg_scene *scene, const ajg_ray *rays, uint32_t n,
+                        uint8_t *occluded, void *stream);
+
 /* Renderer::Render for PathTracing::TraceMIS (renderer.h:64-65,
  * pt.cpp:40-73 and 98-227) as a wavefront on the device: ray-gen,
  * closest/any-hit traversal, shade/sample, shadow and accumulate stages.

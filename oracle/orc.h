@@ -20,6 +20,9 @@
  *                          level) and brute-force NN on small inputs.
  *   Slab + MT traversal  — brute-force MT over all triangles (same t/id except
  *                          exact-t ties), hand-computed special cases.
+ *   Two-level traversal  — identity instances equal the one-level traversal
+ *                          bit for bit; brute force over world-space copies of
+ *                          the instances (ids except near-ties, t to 1e-4 rel).
  *   Warps / frames       — closed forms (unit length, orthonormality, pdfs).
  *   Lambert              — closed forms (f = a/pi, furnace integral, pdf).
  *   Disney               — sample/pdf consistency and MC integrals only:
@@ -70,6 +73,14 @@ typedef struct orc_mesh_view {
   const void *index_buffer;
   size_t index_offset, index_stride, index_size;
 } orc_mesh_view;
+
+/* One bottom-level tree of an instanced scene: nodes, leaf order and the
+ * mesh's triangles (object space). */
+typedef struct orc_blas {
+  const orc_node *nodes;
+  const uint32_t *perm;
+  const float *tri;
+} orc_blas;
 
 /* ---- PLOC build (ploc_aggregate.cpp) ---------------------------------- */
 uint32_t orc_morton_split(uint32_t x);
@@ -243,6 +254,18 @@ uint64_t orc_render_mt(const orc_scene *s, const orc_camera *cam,
  * per-traced-pixel sums to the W x H image (RGB float). */
 void orc_film_resolve(uint32_t w, uint32_t h, uint32_t tile_size,
                       const double *sum, const uint32_t *count, float *rgb);
+
+/* Two-level closest / any hit (instancing, no reference counterpart): the
+ * top-level tree's leaf primitives are instances; w2o holds 12 floats
+ * (row-major 3x4) per instance, inst_mesh its bottom-level tree. hits[i].id
+ * is the triangle index within the mesh, inst_out[i] the instance
+ * (0xFFFFFFFF on a miss). */
+void orc_trace_closest_2l(const orc_node *tlas, const uint32_t *tlas_perm, const float *w2o,
+                          const uint32_t *inst_mesh, const orc_blas *blas, const orc_ray *rays,
+                          uint32_t n, orc_hit *hits, uint32_t *inst_out);
+void orc_trace_any_2l(const orc_node *tlas, const uint32_t *tlas_perm, const float *w2o,
+                      const uint32_t *inst_mesh, const orc_blas *blas, const orc_ray *rays,
+                      uint32_t n, uint8_t *occluded);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,8 @@ def load():
         "orc_trace_closest": ([_P, _P, _P, _P, C.c_uint32, _P], None),
         "orc_trace_any": ([_P, _P, _P, _P, C.c_uint32, _P], None),
         "orc_brute_closest": ([_P, C.c_uint32, _P, C.c_uint32, _P], None),
+        "orc_trace_closest_2l": ([_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P], None),
+        "orc_trace_any_2l": ([_P, _P, _P, _P, _P, _P, C.c_uint32, _P], None),
         "orc_trace_closest_counts": ([_P, _P, _P, _P, C.c_uint32, _P, _P], None),
         "orc_pcg32_floats": ([C.c_uint64, C.c_uint32, _P], None),
         "orc_scene_create": ([_P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.c_uint32,
@@ -179,6 +181,37 @@ def trace_any(nodes, perm, tri, rays):
     occ = np.zeros(rays.shape[0], np.uint8)
     lib.orc_trace_any(_p(nodes), _p(perm), _p(np.ascontiguousarray(tri, np.float32)),
                       _p(rays), rays.shape[0], _p(occ))
+    return occ
+
+
+class _Blas(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("perm", C.c_void_p), ("tri", C.c_void_p)]
+
+
+def trace_2l(tlas_nodes, tlas_perm, w2o, inst_mesh, blases, rays, closest=True):
+    """Two-level traversal. blases: list of (nodes, perm, tri) per mesh;
+    w2o: (n_inst, 12) float32; inst_mesh: (n_inst,) uint32. Returns
+    (hits, instance) for closest, occluded flags otherwise."""
+    lib = load()
+    keep = [(np.ascontiguousarray(nd), np.ascontiguousarray(pm, np.uint32),
+             np.ascontiguousarray(t, np.float32)) for nd, pm, t in blases]
+    arr = (_Blas * len(keep))()
+    for i, (nd, pm, t) in enumerate(keep):
+        arr[i].nodes, arr[i].perm, arr[i].tri = _p(nd), _p(pm), _p(t)
+    tn = np.ascontiguousarray(tlas_nodes)
+    tp = np.ascontiguousarray(tlas_perm, np.uint32)
+    m = np.ascontiguousarray(w2o, np.float32)
+    im = np.ascontiguousarray(inst_mesh, np.uint32)
+    rays = np.ascontiguousarray(rays)
+    n = rays.shape[0]
+    if closest:
+        hits = np.zeros(n, HIT_DTYPE)
+        inst = np.zeros(n, np.uint32)
+        lib.orc_trace_closest_2l(_p(tn), _p(tp), _p(m), _p(im), C.addressof(arr), _p(rays), n,
+                                 _p(hits), _p(inst))
+        return hits, inst
+    occ = np.zeros(n, np.uint8)
+    lib.orc_trace_any_2l(_p(tn), _p(tp), _p(m), _p(im), C.addressof(arr), _p(rays), n, _p(occ))
     return occ
 
 

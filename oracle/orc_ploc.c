@@ -430,3 +430,92 @@ void orc_brute_closest(const float *tri, uint32_t n_tri, const orc_ray *rays,
     hits[i] = h;
   }
 }
+
+/* ---- two-level traversal (instancing) -----------------------------------
+ * No reference counterpart (the reference flattens all primitives into one
+ * tree). Restates the device's two-level traversal in the recursive form of
+ * IntersectAux/OccludeAux: the top-level tree's leaves are instances; at an
+ * instance the ray is mapped into the mesh's space by the instance's
+ * world->object map with per-row order ((m0 x + m1 y) + m2 z) + m3 and the
+ * direction left unnormalised, then the mesh's tree is traversed with the
+ * same t_max. Pinned by: an identity instance equals the one-level traversal
+ * bit for bit; against a brute-force MT over the world-space copy of every
+ * instance, hit ids agree except at near-ties and t within a relative
+ * tolerance. */
+typedef struct ctx2 {
+  const orc_node *tlas;
+  const uint32_t *tlas_perm;
+  const float *w2o;
+  const uint32_t *inst_mesh;
+  const orc_blas *blas;
+} ctx2;
+
+static tray map_ray(const ctx2 *c, uint32_t inst, const tray *rw) {
+  const float *m = c->w2o + 12 * (size_t)inst;
+  orc_ray rr;
+  for (int k = 0; k < 3; ++k) {
+    const float *row = m + 4 * k;
+    rr.o[k] = ((row[0] * rw->o.x + row[1] * rw->o.y) + row[2] * rw->o.z) + row[3];
+    rr.d[k] = (row[0] * rw->d.x + row[1] * rw->d.y) + row[2] * rw->d.z;
+  }
+  rr.tmin = rw->tmin;
+  rr.tmax = rw->tmax;
+  return make_tray(&rr);
+}
+
+static int intersect2_aux(const ctx2 *c, tray *rw, orc_hit *h, uint32_t *hi, uint32_t ni) {
+  const orc_node *nd = &c->tlas[ni];
+  if (!slab(nd, rw)) return 0;
+  if (nd->count != 0) {
+    const uint32_t inst = c->tlas_perm[nd->first];
+    tray ro = map_ray(c, inst, rw);
+    const orc_blas *b = &c->blas[c->inst_mesh[inst]];
+    trav_ctx bc = {b->nodes, b->perm, b->tri, 0, 0};
+    if (intersect_aux(&bc, &ro, h, 0)) {
+      rw->tmax = ro.tmax;
+      *hi = inst;
+      return 1;
+    }
+    return 0;
+  }
+  int left = intersect2_aux(c, rw, h, hi, nd->first + 0);
+  int right = intersect2_aux(c, rw, h, hi, nd->first + 1);
+  return left || right;
+}
+
+static int occlude2_aux(const ctx2 *c, const tray *rw, uint32_t ni) {
+  const orc_node *nd = &c->tlas[ni];
+  if (!slab(nd, rw)) return 0;
+  if (nd->count != 0) {
+    const uint32_t inst = c->tlas_perm[nd->first];
+    tray ro = map_ray(c, inst, rw);
+    const orc_blas *b = &c->blas[c->inst_mesh[inst]];
+    trav_ctx bc = {b->nodes, b->perm, b->tri, 0, 0};
+    return occlude_aux(&bc, &ro, 0);
+  }
+  return occlude2_aux(c, rw, nd->first + 0) || occlude2_aux(c, rw, nd->first + 1);
+}
+
+void orc_trace_closest_2l(const orc_node *tlas, const uint32_t *tlas_perm, const float *w2o,
+                          const uint32_t *inst_mesh, const orc_blas *blas, const orc_ray *rays,
+                          uint32_t n, orc_hit *hits, uint32_t *inst_out) {
+  const ctx2 c = {tlas, tlas_perm, w2o, inst_mesh, blas};
+  for (uint32_t i = 0; i < n; ++i) {
+    tray r = make_tray(&rays[i]);
+    orc_hit h = {INFINITY, 0xFFFFFFFFu, 0.f, 0.f};
+    uint32_t hi = 0xFFFFFFFFu;
+    intersect2_aux(&c, &r, &h, &hi, 0);
+    hits[i] = h;
+    inst_out[i] = hi;
+  }
+}
+
+void orc_trace_any_2l(const orc_node *tlas, const uint32_t *tlas_perm, const float *w2o,
+                      const uint32_t *inst_mesh, const orc_blas *blas, const orc_ray *rays,
+                      uint32_t n, uint8_t *occluded) {
+  const ctx2 c = {tlas, tlas_perm, w2o, inst_mesh, blas};
+  for (uint32_t i = 0; i < n; ++i) {
+    tray r = make_tray(&rays[i]);
+    occluded[i] = (uint8_t)occlude2_aux(&c, &r, 0);
+  }
+}
