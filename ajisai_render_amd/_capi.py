@@ -76,6 +76,7 @@ class PTArgs(C.Structure):
 
 RENDER_TIMING = 1
 RENDER_COUNT = 2
+RENDER_FINITE_PDF = 4
 
 
 class Film(C.Structure):

@@ -147,6 +147,7 @@ struct RenderParams {
   uint32_t seed_base;
   uint32_t slots;
   uint32_t tile, tiles_x;  // work-list pixel order: T x T tiles (0: scanline)
+  uint32_t flags;          // AJG_RENDER_* bits
 };
 
 // Pixel of the i-th entry of a work list's pixel order: scanline, or T x T
@@ -1182,6 +1183,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
             sam.z = pcg_next_float(rng);
             const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
             cont = s.valid && !(is_black(s.f) || s.pdf < 0.f);
+            if ((P.flags & AJG_RENDER_FINITE_PDF) && s.pdf == 0.f) cont = false;
             dir = s.dir;
             fv = s.f;
             pdf = s.pdf;
@@ -1976,6 +1978,7 @@ static int setup_params(ajg_ctx *ctx, const ajg_camera *camera, const ajg_pt_arg
   P.cont_prob = args->cont_prob;
   P.seed_mode = args->seed_mode;
   P.seed_base = args->seed_base;
+  P.flags = args->flags;
   if (P.seed_mode == 0) {
     P.chunk = P.spp;
     P.n_chunks = 1;

@@ -30,6 +30,13 @@ bounces, the same HDR env map) rendered as one wavefront whose iterations
 are replayed from a captured hipGraph, with the autotuner's setting; it
 reports aggregate paths/s, each job's completion time, and the same jobs
 rendered one after another for comparison.
+
+The "instanced" object is the large instanced scene: 8 displaced meshes
+(81,920 or 327,680 triangles each) placed by 2000 instances (random
+rotations, scale log-uniform in [0.2, 5]) on a 200 x 200 ground, two-level
+BVH (PLOC per mesh + PLOC over instance boxes), Disney materials, a
+2048 x 1024 HDR env map (sigma 2.5) plus a directional light, 1920 x 1080,
+256 spp, max 24 bounces: paths/s and rays/s of one full render.
 """
 import argparse
 import json
@@ -67,6 +74,11 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=64,
                     help="jobs of the batched uneven workload (0: skip)")
     ap.add_argument("--batch-trial-spp", type=int, default=1)
+    ap.add_argument("--inst-spp", type=int, default=256,
+                    help="spp of the instanced-scene render (0: skip)")
+    ap.add_argument("--inst-guided-budget", type=float, default=5.0,
+                    help="equal-time budget (s) of guided vs plain on the instanced scene (0: skip)")
+    ap.add_argument("--inst-guided-res", type=int, default=480)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -134,18 +146,63 @@ def _img(out):
 
 
 def relmse(img, ref):
+    """relMSE over the pixels finite in both images (the reference's
+    estimator can overflow to inf/NaN on rare heavy-tailed Disney paths;
+    such pixels are counted separately by nonfinite())."""
     import numpy as np
-    eps = 1e-2 * float(np.mean(ref)) ** 2
-    return float(np.mean(np.mean((img - ref) ** 2, axis=1) / (np.mean(ref, axis=1) ** 2 + eps)))
+    ok = np.isfinite(img).all(axis=1) & np.isfinite(ref).all(axis=1)
+    eps = 1e-2 * float(np.mean(ref[ok])) ** 2
+    return float(np.mean(np.mean((img[ok] - ref[ok]) ** 2, axis=1) /
+                         (np.mean(ref[ok], axis=1) ** 2 + eps)))
+
+
+def nonfinite(img):
+    import numpy as np
+    return int((~np.isfinite(img).all(axis=1)).sum())
 
 
 def guided_equal_time(ctx, a):
     """Equal-time relMSE of guided vs plain path tracing on the occluded room."""
+    from ajisai_render_amd import scenes
+    sc = scenes.occluded_room(res=(a.guided_res, a.guided_res), spp=16)
+    label = (f"occluded_room: 4x3x4 room, 0.2x1.0 slit, HDR env 1024x512 through the slit, "
+             f"{sc.n_tri} triangles, {a.guided_res}x{a.guided_res}, max 16 bounces")
+    return guided_compare(ctx, sc, a.guided_budget, a.guided_ref, a.guided_ref_spp, label)
+
+
+def guided_instanced(ctx, a):
+    """The same comparison on the large instanced scene at reduced resolution
+    (the guide acts at the Lambert ground; Disney vertices sample the BSDF),
+    and on a variant whose materials are all Lambertian (albedo = the Disney
+    base colour), where the estimator has finite variance: the reference's
+    Disney sampler reports a pdf that is not its sampling density, so single
+    Disney bounces reach weights of 1e5 and relMSE at equal time is decided
+    by the largest sample either run happens to draw."""
+    import dataclasses
+    from ajisai_render_amd import scenes
+    w = a.inst_guided_res
+    h = max(16, (w * 9) // 16)
+    sc = scenes.instanced_field(res=(w, h), spp=16)
+    label = (f"instanced_field: 2000 instances of 8 meshes ({sc.n_tri} instanced triangles), "
+             f"HDR env 2048x1024 + directional, {w}x{h}, max 24 bounces")
+    out = {"disney": guided_compare(ctx, sc, a.inst_guided_budget, "guided", a.guided_ref_spp, label)}
+    lam = dataclasses.replace(sc, materials=[
+        scenes.Material(scenes.MAT_DIFFUSE, tuple(float(c) for c in m.base_color))
+        for m in sc.materials])
+    out["lambert_variant"] = guided_compare(ctx, lam, a.inst_guided_budget, "guided",
+                                            a.guided_ref_spp, label + ", all materials Lambertian")
+    out["note"] = ("disney: relMSE of both runs is set by a few samples of weight >= 1e5 from "
+                   "the Disney sampler (pdf not its sampling density); the ratio is not a "
+                   "measure of the guide. lambert_variant: finite-variance comparison.")
+    return out
+
+
+def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
+    """Equal-time relMSE of guided vs plain path tracing on one scene."""
     import numpy as np
     import torch
-    from ajisai_render_amd import scenes
+    from ajisai_render_amd import _capi as capi
     from ajisai_render_amd.core import GpuScene, PathGuide, PathTracer, guide_defaults
-    sc = scenes.occluded_room(res=(a.guided_res, a.guided_res), spp=16)
     g = GpuScene(ctx, sc)
     box = sc.scene_box()
     ga = guide_defaults(warmup_passes=8, warmup_spp=16, warmup_steps_per_batch=64, train=False)
@@ -158,15 +215,15 @@ def guided_equal_time(ctx, a):
         return o, time.perf_counter() - t0
 
     # reference
-    if a.guided_ref == "guided":
+    if ref_kind == "guided":
         gr = PathGuide(ctx, box, seed=99)
-        ref, t_ref = timed(lambda: PathTracer(g, spp=a.guided_ref_spp, seed_mode=1, seed_base=901)
+        ref, t_ref = timed(lambda: PathTracer(g, spp=ref_spp, seed_mode=1, seed_base=901)
                            .Render(guide=gr, guide_args=ga))
         gr.close()
     else:
         acc, t_ref = None, 0.0
-        chunk = min(1024, a.guided_ref_spp)
-        for k in range(max(1, a.guided_ref_spp // chunk)):
+        chunk = min(1024, ref_spp)
+        for k in range(max(1, ref_spp // chunk)):
             o, dt = timed(lambda: PathTracer(g, spp=chunk, seed_mode=1, seed_base=901 + k).Render())
             t_ref += dt
             acc = o if acc is None else {"sum": acc["sum"] + o["sum"], "sum_sq": acc["sum_sq"] + o["sum_sq"],
@@ -175,13 +232,18 @@ def guided_equal_time(ctx, a):
         ref = acc
     R = _img(ref)
     n = ref["count"].astype(np.float64)
-    var_mean = np.maximum(ref["sum_sq"].astype(np.float64) / n[:, None] - R * R, 0) / n[:, None]
-    eps = 1e-2 * float(np.mean(R)) ** 2
-    ref_self = float(np.mean(np.mean(var_mean, axis=1) / (np.mean(R, axis=1) ** 2 + eps)))
-    # plain: calibrate, then render for the budget
-    _, t32 = timed(lambda: PathTracer(g, spp=32, seed_mode=1, seed_base=2).Render())
-    sp = max(1, int(a.guided_budget / (t32 / 32)))
-    op, tp = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=3).Render())
+    with np.errstate(invalid="ignore", over="ignore"):
+        var_mean = np.maximum(ref["sum_sq"].astype(np.float64) / n[:, None] - R * R, 0) / n[:, None]
+    ok = np.isfinite(var_mean).all(axis=1) & np.isfinite(R).all(axis=1)
+    eps = 1e-2 * float(np.mean(R[ok])) ** 2
+    ref_self = float(np.mean(np.mean(var_mean[ok], axis=1) / (np.mean(R[ok], axis=1) ** 2 + eps)))
+    # plain: calibrate, then render for the budget. Plain paths end on a
+    # zero-pdf continuation sample as guided ones do (AJG_RENDER_FINITE_PDF),
+    # so the two estimators differ only where the guide is mixed in.
+    fin = capi.RENDER_FINITE_PDF
+    _, t32 = timed(lambda: PathTracer(g, spp=32, seed_mode=1, seed_base=2, flags=fin).Render())
+    sp = max(1, int(budget / (t32 / 32)))
+    op, tp = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=3, flags=fin).Render())
     rp = relmse(_img(op), R)
     # guided: calibrate warm-up time and per-spp time with a throw-away guide
     gd = PathGuide(ctx, box, seed=5)
@@ -189,7 +251,7 @@ def guided_equal_time(ctx, a):
     gd.close()
     per_spp = oc["stats"].ms_total * 1e-3 / 32
     t_warm = tc - oc["stats"].ms_total * 1e-3
-    sg = max(1, int((a.guided_budget - t_warm) / per_spp))
+    sg = max(1, int((budget - t_warm) / per_spp))
     gd = PathGuide(ctx, box, seed=6)
     og, tg = timed(lambda: PathTracer(g, spp=sg, seed_mode=1, seed_base=5).Render(guide=gd, guide_args=ga))
     gst = og["guide_stats"]
@@ -197,13 +259,16 @@ def guided_equal_time(ctx, a):
     rg = relmse(_img(og), R)
     g.close()
     return {
-        "workload": f"occluded_room: 4x3x4 room, 0.2x1.0 slit, HDR env 1024x512 through the slit, "
-                    f"{sc.n_tri} triangles, {a.guided_res}x{a.guided_res}, max 16 bounces",
-        "budget_s": a.guided_budget,
-        "reference": {"kind": a.guided_ref, "spp": a.guided_ref_spp, "seconds": round(t_ref, 2),
-                      "mean": float(R.mean()), "relmse_self_estimate": round(ref_self, 5)},
-        "plain": {"spp": sp, "seconds": round(tp, 3), "relmse": round(rp, 5)},
+        "workload": label,
+        "budget_s": budget,
+        "reference": {"kind": ref_kind, "spp": ref_spp, "seconds": round(t_ref, 2),
+                      "mean": float(np.mean(R[ok])), "relmse_self_estimate": round(ref_self, 5),
+                      "nonfinite_pixels": nonfinite(R)},
+        "plain": {"spp": sp, "seconds": round(tp, 3), "relmse": round(rp, 5),
+                  "flags": "AJG_RENDER_FINITE_PDF",
+                  "nonfinite_pixels": nonfinite(_img(op))},
         "guided": {"spp": sg, "seconds": round(tg, 3), "relmse": round(rg, 5),
+                   "nonfinite_pixels": nonfinite(_img(og)),
                    "warmup_seconds": round(gst.ms_warmup * 1e-3, 3),
                    "warmup": "8 passes x 16 spp, 64 Adam steps per 2^18-record batch, lr 1e-3",
                    "train_steps": gst.train_steps, "records": gst.records,
@@ -288,6 +353,57 @@ def batched_bench(ctx, a):
     }
     for g in gss:
         g.close()
+    return out
+
+
+def instanced_bench(ctx, a):
+    """One full render of the large instanced scene."""
+    import torch
+    from ajisai_render_amd import _capi as capi
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import GpuScene, PathTracer
+
+    t = time.perf_counter()
+    sc = scenes.instanced_field(spp=a.inst_spp)
+    t_gen = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    g = GpuScene(ctx, sc)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t
+    pt = PathTracer(g, seed_mode=0, flags=capi.RENDER_TIMING)
+    w, h = sc.camera.res
+    wp, hp = pt.traced_dims()
+    warm = PathTracer(g, spp=1, seed_mode=0)
+    film, _ = warm.render_device()
+    film, _ = pt.render_device()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = pt.render_device(film=film)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    paths = w * h * sc.pt.spp
+    q = st.closest_queries + st.any_queries
+    out = {
+        "instances": len(sc.instances),
+        "meshes": len(sc.primitives),
+        "instanced_triangles": int(sc.n_tri),
+        "unique_triangles": int(sum(p.n_tri for p in sc.primitives)),
+        "resolution": [w, h],
+        "traced_pixels": wp * hp,
+        "spp": sc.pt.spp,
+        "max_bounces": sc.pt.max_bounces,
+        "seconds": round(wall, 3),
+        "paths_per_s": round(paths / wall, 1),
+        "mrays_per_s": round(q / wall / 1e6, 2),
+        "rays_per_path": round(q / paths, 3),
+        "stage_ms": {"trace_closest": round(st.ms_trace_closest, 2),
+                     "trace_any": round(st.ms_trace_any, 2), "shade": round(st.ms_shade, 2)},
+        "iterations": st.iterations,
+        "scene_generation_s": round(t_gen, 2),
+        "scene_build_s": round(t_build, 3),
+    }
+    g.close()
     return out
 
 
@@ -413,6 +529,9 @@ def main():
     mlp = mlp_bench(ctx)
     guided = guided_equal_time(ctx, a) if a.guided_budget > 0 else None
     batched = batched_bench(ctx, a) if a.batch_jobs > 0 else None
+    instanced = instanced_bench(ctx, a) if a.inst_spp > 0 else None
+    if instanced is not None and a.inst_guided_budget > 0:
+        instanced["guided"] = guided_instanced(ctx, a)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -451,6 +570,7 @@ def main():
                                    if guided else None),
         "guided": guided,
         "batched": batched,
+        "instanced": instanced,
         "mlp": mlp,
         "roofline": roof,
         "cpu_baseline": cpu,

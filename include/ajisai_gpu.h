@@ -212,6 +212,10 @@ typedef struct ajg_pt_args {
 #define AJG_RENDER_TIMING 1u   /* per-stage device times via HIP events */
 #define AJG_RENDER_COUNT 2u    /* count fat nodes / triangles per query (slower;
                                   for the byte model, not for timing) */
+#define AJG_RENDER_FINITE_PDF 4u /* end a path whose continuation sample has
+                                  pdf 0 (the reference continues it with an
+                                  infinite throughput, pt.cpp:216-221; guided
+                                  sampling always ends it) */
 
 /* Film accumulators per traced pixel (traced area = whole tiles,
  * pt.cpp:43-49): Kahan-compensated float sums of L and L^2 per channel and
