@@ -440,6 +440,28 @@ k_build_fat(const PlocNode *fin, const uint32_t *fat_idx, uint32_t nn, FatNode *
 }
 
 __global__ void __launch_bounds__(256)
+k_build_wide(const FatNode *fat, uint32_t n_int, WideNode *wide) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_int) return;
+  const FatNode f = fat[i];
+  WideNode w;
+  w.a = f.a;
+  w.b = f.b;
+  w.c = f.c;
+  w.r = make_int4(f.d.x, f.d.y, -1, -1);
+  w.la = w.lb = w.lc = w.pad = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (f.d.x >= 0) {
+    const FatNode l = fat[f.d.x];
+    w.la = l.a;
+    w.lb = l.b;
+    w.lc = l.c;
+    w.r.z = l.d.x;
+    w.r.w = l.d.y;
+  }
+  wide[i] = w;
+}
+
+__global__ void __launch_bounds__(256)
 k_set_parent(FatNode *fat, const int *parent_of_fat, uint32_t n_int) {
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < n_int) fat[i].d.z = i == 0 ? -1 : parent_of_fat[i];
@@ -486,6 +508,7 @@ void destroy_accel(ajg_accel *a) {
   (void)hipFree(a->perm);
   (void)hipFree(a->leaf_tris);
   (void)hipFree(a->fat);
+  (void)hipFree(a->wide);
   (void)hipFree(a->cursor);
   delete a;
 }
@@ -512,9 +535,11 @@ int build_accel(ajg_ctx *ctx, const float *tri, uint32_t n, const float *scene_b
   if ((st = dev_alloc(ctx, (void **)&a->leaf_tris, sizeof(float4) * 3 * (size_t)n, "leaf tris"))) return bail(st);
   if ((st = dev_alloc(ctx, (void **)&a->cursor, 16, "trace cursor"))) return bail(st);
   if (n > 1 &&
-      (st = dev_alloc(ctx, (void **)&a->fat, sizeof(FatNode) * (size_t)(n - 1), "fat nodes")))
+      ((st = dev_alloc(ctx, (void **)&a->fat, sizeof(FatNode) * (size_t)(n - 1), "fat nodes")) ||
+       (st = dev_alloc(ctx, (void **)&a->wide, sizeof(WideNode) * (size_t)(n - 1), "traversal nodes"))))
     return bail(st);
-  a->bytes = sizeof(PlocNode) * (uint64_t)nn + 4ull * n + 48ull * n + 64ull * (n - 1);
+  a->bytes = sizeof(PlocNode) * (uint64_t)nn + 4ull * n + 48ull * n +
+             (uint64_t)sizeof(WideNode) * (n - 1);
 
   DevBuf tmp(ctx);
   // ---- scene box
@@ -630,6 +655,8 @@ int build_accel(ajg_ctx *ctx, const float *tri, uint32_t n, const float *scene_b
     scan_u32(flag, fat_idx, nn, false, scan_tmp, s);
     hipLaunchKernelGGL(k_build_fat, dim3(gn), dim3(256), 0, s, a->final_nodes, fat_idx, nn, a->fat, pof, pol);
     hipLaunchKernelGGL(k_set_parent, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, a->fat, pof, n - 1);
+    hipLaunchKernelGGL(k_build_wide, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, (const FatNode *)a->fat,
+                       n - 1, a->wide);
     uint32_t *ddepth;
     if ((st = tmp.alloc(&ddepth, 1, "depth"))) return bail(st);
     AJG_HIP(ctx, hipMemsetAsync(ddepth, 0, 4, s));
@@ -637,6 +664,9 @@ int build_accel(ajg_ctx *ctx, const float *tri, uint32_t n, const float *scene_b
     hipError_t e = hipMemcpyAsync(&a->depth, ddepth, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(hip_fail(ctx, e, "fat layout"));
+    // traversal reads the wide nodes only
+    (void)hipFree(a->fat);
+    a->fat = nullptr;
     a->root_ref = 0;
   }
   if (n == 1) a->depth = 1;

@@ -137,6 +137,19 @@ struct FatNode {
   int4 d;    // child0, child1, parent (fat index, -1 at root), unused
 };
 
+// Traversal node (128 bytes, one per interior node, same index as the fat
+// node): the node's two child boxes and refs and, when its left child is
+// interior, that child's two child boxes and refs. A traversal that enters
+// the left child tests its children from the same fetch, so a left descent
+// costs one dependent load per two tree levels; the tests, their order and
+// t_max at each test are those of the binary traversal.
+struct WideNode {
+  float4 a, b, c;     // child boxes (FatNode layout)
+  int4 r;             // left, right, left-left, left-right (fat index, or ~leaf slot)
+  float4 la, lb, lc;  // child boxes of the left child (when r.x >= 0)
+  float4 pad;
+};
+
 static constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 }  // namespace ajg

@@ -30,6 +30,7 @@ struct ajg_accel {
   uint32_t *perm = nullptr;              // n: leaf slot -> global triangle id
   float4 *leaf_tris = nullptr;           // 3n: a.xyz|id, b.xyz, c.xyz per leaf slot
   ajg::FatNode *fat = nullptr;           // n-1 interior nodes (root at 0)
+  ajg::WideNode *wide = nullptr;         // n-1 traversal nodes (fat index)
   int root_ref = 0;                      // >= 0 fat index, < 0 ~leaf slot
   float root_box[6] = {0, 0, 0, 0, 0, 0};
   uint64_t bytes = 0;

@@ -9,7 +9,7 @@ namespace ajg {
 
 static DevBVH dev_bvh(const ajg_accel *a) {
   DevBVH b;
-  b.fat = a->fat;
+  b.wide = a->wide;
   b.leaf_tris = a->leaf_tris;
   b.root_ref = a->root_ref;
   for (int i = 0; i < 3; ++i) {

@@ -6,12 +6,14 @@
 // t_max shrinks to each accepted hit. The device loop keeps the same visit
 // order with an explicit stack. The slab test of a node (IntersectP,
 // :154-162) splits into a part that does not depend on t_max
-// (entry <= exit and exit > t_min) and the comparison entry < t_max; a fat
-// node evaluates both child boxes when it is fetched, pushes the right child
+// (entry <= exit and exit > t_min) and the comparison entry < t_max; a node
+// evaluates both child boxes when it is fetched, pushes the right child
 // with its entry distance, and compares that entry with the t_max current at
-// pop time — exactly when the recursive reference would test it. Leaf boxes
-// are exact (the PLOC leaf AABB), so ids, t, u, v are bit-identical to the
-// reference, ties included.
+// pop time — exactly when the recursive reference would test it. A fetched
+// WideNode also carries its left child's child boxes, so entering an interior
+// left child performs that child's tests without another fetch (same tests,
+// same order, same t_max). Leaf boxes are exact (the PLOC leaf AABB), so ids,
+// t, u, v are bit-identical to the reference, ties included.
 // Any hit restates OccludeAux (:345-365); its result does not depend on the
 // visit order because t_max never changes.
 #pragma once
@@ -21,7 +23,7 @@
 namespace ajg {
 
 struct DevBVH {
-  const FatNode *fat;
+  const WideNode *wide;
   const float4 *leaf_tris;
   int root_ref;
   float root_mn[3], root_mx[3];
@@ -162,6 +164,70 @@ __device__ __forceinline__ void stack_pop(const StackRef &s, int &d, int *n, flo
   }
 }
 
+// Closest-hit visit of interior node `node` from one fetch: the child tests
+// of the node (right child pushed with its entry distance, left child entered
+// if its entry is below t_max) and, when the entered left child is interior,
+// that child's tests too, with the same t_max, exactly as the binary
+// traversal does at its next fetch. Returns false when the stack must be
+// popped; otherwise `node` is the next node (interior or ~leaf).
+template <int LDS>
+__device__ __forceinline__ bool closest_step(const WideNode *wide, int &node, const TRay &r,
+                                             float tmax, const StackRef &st, int &d) {
+  const WideNode *w = wide + node;
+  const float4 a = w->a, b = w->b, c = w->c;
+  const int4 ref = w->r;
+  const float4 la = w->la, lb = w->lb, lc = w->lc;
+  float e0, e1;
+  const bool s0 = slab_static(r, a.x, a.y, a.z, a.w, b.x, b.y, &e0);
+  const bool s1 = slab_static(r, b.z, b.w, c.x, c.y, c.z, c.w, &e1);
+  if (s1) stack_push<LDS>(st, d, ref.y, e1);
+  if (!(s0 && e0 < tmax)) return false;
+  if (ref.x < 0) {
+    node = ref.x;
+    return true;
+  }
+  const bool t0 = slab_static(r, la.x, la.y, la.z, la.w, lb.x, lb.y, &e0);
+  const bool t1 = slab_static(r, lb.z, lb.w, lc.x, lc.y, lc.z, lc.w, &e1);
+  if (t1) stack_push<LDS>(st, d, ref.w, e1);
+  if (t0 && e0 < tmax) {
+    node = ref.z;
+    return true;
+  }
+  return false;
+}
+
+// Any-hit visit (t_max fixed, so culling happens at push time and the visit
+// order does not change the result).
+template <int LDS>
+__device__ __forceinline__ bool any_step(const WideNode *wide, int &node, const TRay &r,
+                                         float tmax, const StackRef &st, int &d) {
+  const WideNode *w = wide + node;
+  const float4 a = w->a, b = w->b, c = w->c;
+  const int4 ref = w->r;
+  float e0, e1;
+  const bool s0 = slab_static(r, a.x, a.y, a.z, a.w, b.x, b.y, &e0) && e0 < tmax;
+  const bool s1 = slab_static(r, b.z, b.w, c.x, c.y, c.z, c.w, &e1) && e1 < tmax;
+  if (!s0) {
+    node = ref.y;
+    return s1;
+  }
+  if (s1) stack_push<LDS>(st, d, ref.y, e1);
+  if (ref.x < 0) {
+    node = ref.x;
+    return true;
+  }
+  const float4 la = w->la, lb = w->lb, lc = w->lc;
+  const bool t0 = slab_static(r, la.x, la.y, la.z, la.w, lb.x, lb.y, &e0) && e0 < tmax;
+  const bool t1 = slab_static(r, lb.z, lb.w, lc.x, lc.y, lc.z, lc.w, &e1) && e1 < tmax;
+  if (!t0) {
+    node = ref.w;
+    return t1;
+  }
+  if (t1) stack_push<LDS>(st, d, ref.w, e1);
+  node = ref.z;
+  return true;
+}
+
 // Stack bound: entries <= tree depth - 1, so SPILL >= depth - kLdsStack.
 template <int SPILL, bool COUNT>
 __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, int *lds_node,
@@ -189,17 +255,9 @@ __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, 
       visit_leaf_closest(bvh, r, ~node, &h, &tmax);
     } else {
       if (COUNT) ++*n_nodes;
-      const FatNode fn = bvh.fat[node];
-      float e0, e1;
-      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
-      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
-      // the right child is visited after the left subtree: keep its entry
-      // distance and compare it with t_max when it is popped
-      if (s1) stack_push(st, d, fn.d.y, e1);
-      if (s0 && e0 < tmax) {
-        node = fn.d.x;
-        continue;
-      }
+      // the right child is visited after the left subtree: its entry
+      // distance is kept and compared with t_max when it is popped
+      if (closest_step<kLdsStack>(bvh.wide, node, r, tmax, st, d)) continue;
     }
     bool found = false;
     while (d > 0) {
@@ -240,25 +298,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
       if (mt_test(a, b, c, r, r.tmax, &t, &u, &v)) return true;
     } else {
       if (COUNT) ++*n_nodes;
-      const FatNode fn = bvh.fat[node];
-      float e0, e1;
-      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0) &&
-                      e0 < r.tmax;
-      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1) &&
-                      e1 < r.tmax;
-      if (s0 && s1) {
-        stack_push<LDS>(st, d, fn.d.y, e1);
-        node = fn.d.x;
-        continue;
-      }
-      if (s0) {
-        node = fn.d.x;
-        continue;
-      }
-      if (s1) {
-        node = fn.d.y;
-        continue;
-      }
+      if (any_step<LDS>(bvh.wide, node, r, r.tmax, st, d)) continue;
     }
     if (d == 0) break;
     int n;
@@ -355,15 +395,7 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
       visit_leaf_closest(bvh, r, ~node, &h, &tmax);
     } else {
       if (COUNT) ++*nn;
-      const FatNode fn = bvh.fat[node];
-      float e0, e1;
-      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
-      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
-      if (s1) stack_push<LDS>(st, d, fn.d.y, e1);
-      if (s0 && e0 < tmax) {
-        node = fn.d.x;
-        next = true;
-      }
+      next = closest_step<LDS>(bvh.wide, node, r, tmax, st, d);
     }
     if (!next) {
       while (d > 0) {
@@ -423,7 +455,9 @@ __device__ __forceinline__ uint32_t tlas_leaf_instance(const DevBVH &tlas, int s
   return __float_as_uint(tlas.leaf_tris[3 * slot].w);
 }
 
-// Closest hit over a two-level scene; store(i, hit, instance).
+// Closest hit over a two-level scene; store(i, hit, instance). The world
+// ray is kept as origin and direction only (registers); its reciprocal
+// direction is recomputed (same operations) when a mesh's tree is left.
 template <int SPILL, int LDS, class Load, class Store>
 __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const DevInst *insts,
                                                       const DevBlas *blases, uint32_t n,
@@ -433,12 +467,13 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
   float se[SPILL];
   const StackRef st = {lds_node, lds_entry, sn, se};
   int d = 0;
-  TRay r, rw;  // ray of the current level; world ray
+  TRay r;        // ray of the current level
+  f3 wo, wd;     // world ray
   THit h;
   uint32_t h_inst = kNone, cur_inst = 0;
   float tmax = 0.f;
   int node = 0, base = -1;
-  const FatNode *fat = tlas.fat;
+  const WideNode *wn = tlas.wide;
   const float4 *leaf = tlas.leaf_tris;
   uint32_t ray = kNone;
   bool drained = false;
@@ -447,20 +482,21 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
     if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
       const uint32_t idx = refill(idle, n, cursor, &drained);
       if (idx != kNone) {
-        rw = load(idx);
-        r = rw;
+        r = load(idx);
+        wo = r.o;
+        wd = r.d;
         h.t = __int_as_float(0x7f800000);
         h.u = 0.f;
         h.v = 0.f;
         h.slot = -1;
         h_inst = kNone;
-        tmax = rw.tmax;
+        tmax = r.tmax;
         d = 0;
         base = -1;
-        fat = tlas.fat;
+        wn = tlas.wide;
         leaf = tlas.leaf_tris;
         float e;
-        if (slab_static(rw, tlas.root_mn[0], tlas.root_mn[1], tlas.root_mn[2], tlas.root_mx[0],
+        if (slab_static(r, tlas.root_mn[0], tlas.root_mn[1], tlas.root_mn[2], tlas.root_mx[0],
                         tlas.root_mx[1], tlas.root_mx[2], &e) &&
             e < tmax) {
           node = tlas.root_ref;
@@ -482,15 +518,14 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         cur_inst = tlas_leaf_instance(tlas, ~node);
         const DevInst *I = insts + cur_inst;
         const DevBlas *B = blases + I->info.x;
-        const TRay ro = make_tray(xform_point(I->w2o, rw.o), xform_vec(I->w2o, rw.d), rw.tmin,
-                                  rw.tmax);
+        const TRay ro = make_tray(xform_point(I->w2o, wo), xform_vec(I->w2o, wd), r.tmin, tmax);
         float e;
         if (slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
                         B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
             e < tmax) {
           r = ro;
           base = d;
-          fat = B->bvh.fat;
+          wn = B->bvh.wide;
           leaf = B->bvh.leaf_tris;
           node = B->bvh.root_ref;
           next = true;
@@ -509,22 +544,14 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         }
       }
     } else {
-      const FatNode fn = fat[node];
-      float e0, e1;
-      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0);
-      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1);
-      if (s1) stack_push<LDS>(st, d, fn.d.y, e1);
-      if (s0 && e0 < tmax) {
-        node = fn.d.x;
-        next = true;
-      }
+      next = closest_step<LDS>(wn, node, r, tmax, st, d);
     }
     if (!next) {
       for (;;) {
         if (d == base) {  // the mesh's tree is done: back to the top level
           base = -1;
-          r = rw;
-          fat = tlas.fat;
+          r = make_tray(wo, wd, r.tmin, tmax);
+          wn = tlas.wide;
           leaf = tlas.leaf_tris;
         }
         if (d == 0) break;
@@ -560,7 +587,7 @@ __device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *
   const StackRef st = {lds_node, lds_entry, sn, se};
   int d = 0, base = -1;
   TRay r = rw;
-  const FatNode *fat = tlas.fat;
+  const WideNode *wn = tlas.wide;
   const float4 *leaf = tlas.leaf_tris;
   int node = tlas.root_ref;
   for (;;) {
@@ -576,7 +603,7 @@ __device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *
             e < rw.tmax) {
           r = ro;
           base = d;
-          fat = B->bvh.fat;
+          wn = B->bvh.wide;
           leaf = B->bvh.leaf_tris;
           node = B->bvh.root_ref;
           next = true;
@@ -588,23 +615,13 @@ __device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *
         if (mt_test(a, b, c, r, rw.tmax, &t, &u, &v)) return true;
       }
     } else {
-      const FatNode fn = fat[node];
-      float e0, e1;
-      const bool s0 = slab_static(r, fn.a.x, fn.a.y, fn.a.z, fn.a.w, fn.b.x, fn.b.y, &e0) &&
-                      e0 < rw.tmax;
-      const bool s1 = slab_static(r, fn.b.z, fn.b.w, fn.c.x, fn.c.y, fn.c.z, fn.c.w, &e1) &&
-                      e1 < rw.tmax;
-      if (s0 && s1) stack_push<LDS>(st, d, fn.d.y, e1);
-      if (s0 || s1) {
-        node = s0 ? fn.d.x : fn.d.y;
-        next = true;
-      }
+      next = any_step<LDS>(wn, node, r, rw.tmax, st, d);
     }
     if (!next) {
       if (d == base) {
         base = -1;
         r = rw;
-        fat = tlas.fat;
+        wn = tlas.wide;
         leaf = tlas.leaf_tris;
       }
       if (d == 0) return false;

@@ -48,7 +48,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FAT_NODE_BYTES = 64      # traversal node: two child boxes + two child refs
+NODE_BYTES = 112         # traversal node fetch: two child boxes, four refs, the left child's two child boxes
 LEAF_TRI_BYTES = 48      # leaf triangle: 3 x float4
 RAY_BYTES = 32           # o, d, tmin, tmax
 HIT_BYTES = 16           # t, leaf, u, v
@@ -74,11 +74,16 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=64,
                     help="jobs of the batched uneven workload (0: skip)")
     ap.add_argument("--batch-trial-spp", type=int, default=1)
+    ap.add_argument("--batch-fixed", type=str, default="",
+                    help="skip the autotuner and the one-by-one renders; use this setting "
+                         "(e.g. 'max_slots=2097152,pixel_tile=8')")
     ap.add_argument("--inst-spp", type=int, default=256,
                     help="spp of the instanced-scene render (0: skip)")
     ap.add_argument("--inst-guided-budget", type=float, default=5.0,
                     help="equal-time budget (s) of guided vs plain on the instanced scene (0: skip)")
     ap.add_argument("--inst-guided-res", type=int, default=480)
+    ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp"],
+                    help="run one section alone and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -301,7 +306,11 @@ def batched_bench(ctx, a):
     # untuned batch (library defaults), also the warm-up of the buffers
     _, st0, _ = br.render_device(batch_defaults(), films=films)
     t = time.perf_counter()
-    tuned, tuned_rate = br.autotune(trial_spp=a.batch_trial_spp, films=films)
+    if a.batch_fixed:
+        kw = {k: int(v) for k, v in (x.split("=") for x in a.batch_fixed.split(","))}
+        tuned, tuned_rate = batch_defaults(**kw), 0.0
+    else:
+        tuned, tuned_rate = br.autotune(trial_spp=a.batch_trial_spp, films=films)
     t_tune = time.perf_counter() - t
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -311,12 +320,11 @@ def batched_bench(ctx, a):
     # the same jobs alone, one after another (each job's own film)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    seq_ms = 0.0
-    for pt, f in zip(trs, films):
-        _, s1 = pt.render_device(film=f)
-        seq_ms += s1.ms_total
+    if not a.batch_fixed:
+        for pt, f in zip(trs, films):
+            pt.render_device(film=f)
     torch.cuda.synchronize()
-    seq_wall = time.perf_counter() - t
+    seq_wall = time.perf_counter() - t if not a.batch_fixed else float("nan")
     paths = int(st.paths)
     order = np.argsort(done)
     res = [int(sc.camera.res[0]) for sc in specs]
@@ -469,8 +477,13 @@ def main():
     from ajisai_render_amd import scenes
     from ajisai_render_amd.core import Context, GpuScene, PathTracer, PLOCAggregate
 
-    scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
     ctx = Context(0)
+    if a.only:
+        fn = {"batched": batched_bench, "instanced": instanced_bench,
+              "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c)}[a.only]
+        print(json.dumps({a.only: fn(ctx, a)}))
+        return
+    scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
 
     # --- PLOC build throughput from a device-resident soup
     tri = torch.from_numpy(scene.soup()).cuda()
@@ -512,7 +525,7 @@ def main():
         cpt = PathTracer(gs, seed_mode=a.seed_mode, flags=capi.RENDER_COUNT)
         _, cs = cpt.render_device(film=film)
         assert cs.closest_queries == closest_q, "counting pass diverged from the timed pass"
-        alg_bytes = (closest_q * (RAY_BYTES + HIT_BYTES) + cs.nodes_fetched * FAT_NODE_BYTES +
+        alg_bytes = (closest_q * (RAY_BYTES + HIT_BYTES) + cs.nodes_fetched * NODE_BYTES +
                      cs.tris_tested * LEAF_TRI_BYTES)
         achieved = alg_bytes / (ms_closest * 1e-3) / 1e9
         traffic = profiled_traffic()

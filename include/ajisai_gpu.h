@@ -136,7 +136,8 @@ int ajg_accel_trace_closest(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
 int ajg_accel_trace_any(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
                         uint8_t *occluded, void *stream);
 /* Counting pass of trace_closest for the byte model: per ray, the number of
- * fat nodes fetched and triangles tested. Device pointers; not timed. */
+ * traversal nodes fetched (128-byte wide nodes, each covering up to two
+ * tree levels) and triangles tested. Device pointers; not timed. */
 int ajg_accel_trace_counts(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
                            uint32_t *nodes_fetched, uint32_t *tris_tested,
                            void *stream);
@@ -210,7 +211,7 @@ typedef struct ajg_pt_args {
 
 /* ajg_pt_args.flags */
 #define AJG_RENDER_TIMING 1u   /* per-stage device times via HIP events */
-#define AJG_RENDER_COUNT 2u    /* count fat nodes / triangles per query (slower;
+#define AJG_RENDER_COUNT 2u    /* count node fetches / triangles per query (slower;
                                   for the byte model, not for timing) */
 #define AJG_RENDER_FINITE_PDF 4u /* end a path whose continuation sample has
                                   pdf 0 (the reference continues it with an
@@ -237,7 +238,7 @@ typedef struct ajg_render_stats {
   float ms_trace_closest, ms_trace_any, ms_shade; /* AJG_RENDER_TIMING */
   uint32_t launches_closest, launches_any, launches_shade;
   uint32_t reserved;
-  uint64_t nodes_fetched;   /* AJG_RENDER_COUNT: fat nodes fetched, closest */
+  uint64_t nodes_fetched;   /* AJG_RENDER_COUNT: node fetches, closest */
   uint64_t tris_tested;     /* AJG_RENDER_COUNT: triangles tested, closest */
   uint64_t nodes_fetched_any, tris_tested_any; /* same for shadow rays */
 } ajg_render_stats;
