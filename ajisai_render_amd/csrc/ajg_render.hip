@@ -1356,14 +1356,31 @@ struct RenderBuffers {
   uint32_t *counters = nullptr;
   float *partial = nullptr;
   uint32_t *partial_n = nullptr;
+  // shadow rays are traced on a second stream, concurrently with the
+  // closest-hit queue (fork after shade, join before the next shade)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   void release() {
     for (void *p : bufs) (void)hipFree(p);
     bufs.clear();
     slots = 0;
     total = 0;
   }
-  ~RenderBuffers() { release(); }
+  ~RenderBuffers() {
+    release();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (aux) (void)hipStreamDestroy(aux);
+  }
 };
+
+static int ensure_aux_stream(ajg_ctx *ctx, RenderBuffers *rb) {
+  if (rb->aux) return AJG_OK;
+  hipError_t e = hipStreamCreateWithFlags(&rb->aux, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&rb->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&rb->ev_join, hipEventDisableTiming);
+  return e == hipSuccess ? AJG_OK : hip_fail(ctx, e, "render streams");
+}
 
 // Guided-rendering buffers (kept on the scene next to RenderBuffers).
 struct GuideBuffers {
@@ -2050,6 +2067,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     sc->render_cache = rb;
   }
   if (int st = ensure_buffers(ctx, rb, S, total)) return st;
+  if (int st = ensure_aux_stream(ctx, rb)) return st;
   PathState ps = rb->ps;
   GuideParams G;
   memset(&G, 0, sizeof(G));
@@ -2193,6 +2211,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
         hipLaunchKernelGGL((k_shade<false, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G,
                            nob);
       timer.end(2, s);
+      AJG_RCHK(hipEventRecord(rb->ev_fork, s));
+      AJG_RCHK(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       timer.begin(0, s);
       if (inst)
         launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, counters + nxt, counters + 6,
@@ -2204,17 +2224,21 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
         launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, counters + 6,
                          trav_ctr, trav_grid, s);
       timer.end(0, s);
-      timer.begin(1, s);
-      if (inst)
-        launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, counters + 2 + nxt, nullptr,
-                   trav_grid, s);
-      else if (counting)
-        launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt, counters + 7,
-                        trav_ctr + 2, trav_grid, s);
-      else
-        launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
-                         counters + 7, trav_ctr + 2, trav_grid, s);
-      timer.end(1, s);
+      {
+        hipStream_t sa = rb->aux;
+        timer.begin(1, sa);
+        if (inst)
+          launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, counters + 2 + nxt, nullptr,
+                     trav_grid, sa);
+        else if (counting)
+          launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
+                          counters + 7, trav_ctr + 2, trav_grid, sa);
+        else
+          launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
+                           counters + 7, trav_ctr + 2, trav_grid, sa);
+        timer.end(1, sa);
+        AJG_RCHK(hipEventRecord(rb->ev_join, sa));
+      }
       if (guided) {
         // guide mixtures of the new closest hits, read by the next shade
         timer.begin(3, s);
@@ -2226,6 +2250,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
           return done(st);
         timer.end(3, s);
       }
+      AJG_RCHK(hipStreamWaitEvent(s, rb->ev_join, 0));
       cur = nxt;
       ++iters;
     }

@@ -388,16 +388,13 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
       continue;
     }
     if (ray == kNone) continue;
-    // one node visit of trace_closest
-    bool next = false;
+    // a leaf test (if the lane stands on a leaf) and then one interior
+    // visit, in trace_closest's order
+    bool next = true;
     if (node < 0) {
       if (COUNT) ++*nt;
       visit_leaf_closest(bvh, r, ~node, &h, &tmax);
-    } else {
-      if (COUNT) ++*nn;
-      next = closest_step<LDS>(bvh.wide, node, r, tmax, st, d);
-    }
-    if (!next) {
+      next = false;
       while (d > 0) {
         int pn;
         float en;
@@ -406,6 +403,22 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
           node = pn;
           next = true;
           break;
+        }
+      }
+    }
+    if (next && node >= 0) {
+      if (COUNT) ++*nn;
+      next = closest_step<LDS>(bvh.wide, node, r, tmax, st, d);
+      if (!next) {
+        while (d > 0) {
+          int pn;
+          float en;
+          stack_pop<LDS>(st, d, &pn, &en);
+          if (en < tmax) {
+            node = pn;
+            next = true;
+            break;
+          }
         }
       }
     }
