@@ -2478,6 +2478,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   }
   if (int st = ensure_buffers(ctx, &bb->rb, S, total)) return st;
   if (int st = ensure_batch_buffers(ctx, bb, S, n_jobs)) return st;
+  if (int st = ensure_aux_stream(ctx, &bb->rb)) return st;
   RenderBuffers *rb = &bb->rb;
   for (uint32_t j = 0; j < n_jobs; ++j) {
     hj[j].partial = rb->partial + 6 * (size_t)begin[j];
@@ -2563,7 +2564,13 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   const uint32_t shade_grid = cdiv(S, 256);
   const uint32_t trav_grid = (uint32_t)std::max(1, cu) * (uint32_t)ba.trav_blocks_per_cu;
   const int K = ba.iters_per_check;
-  // One block of K iterations starting and ending on queue generation 0.
+  // One block of K iterations starting and ending on queue generation 0;
+  // shadow rays are traced on the aux stream concurrently with the closest
+  // queue (under capture the fork/join becomes two graph branches).
+  hipError_t qerr = hipSuccess;
+  auto q_ok = [&](hipError_t e) {
+    if (qerr == hipSuccess) qerr = e;
+  };
   auto enqueue_block = [&]() {
     int cur = 0;
     for (int k = 0; k < K; ++k) {
@@ -2583,17 +2590,21 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
       hipLaunchKernelGGL((k_shade<false, true>), dim3(shade_grid), dim3(256), 0, s, sc0, P0, ps, q, G,
                          B);
+      q_ok(hipEventRecord(rb->ev_fork, s));
+      q_ok(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       if (spill == 48) {
         launch_pq_b<48>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
                         counters + nxt, counters + 6, trav_grid, s);
         launch_pq_b<48>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
-                        counters + 2 + nxt, counters + 7, trav_grid, s);
+                        counters + 2 + nxt, counters + 7, trav_grid, rb->aux);
       } else {
         launch_pq_b<1024>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
                           counters + nxt, counters + 6, trav_grid, s);
         launch_pq_b<1024>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
-                          counters + 2 + nxt, counters + 7, trav_grid, s);
+                          counters + 2 + nxt, counters + 7, trav_grid, rb->aux);
       }
+      q_ok(hipEventRecord(rb->ev_join, rb->aux));
+      q_ok(hipStreamWaitEvent(s, rb->ev_join, 0));
       cur = nxt;
     }
   };
@@ -2602,6 +2613,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
     enqueue_block();
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (e != hipSuccess) return done(hip_fail(ctx, e, "hipStreamEndCapture"));
+    if (qerr != hipSuccess) return done(hip_fail(ctx, qerr, "batch stream fork/join"));
     AJG_BCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
   hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, bb->clocks + n_jobs);
@@ -2614,6 +2626,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       ++graph_launches;
     } else {
       enqueue_block();
+      if (qerr != hipSuccess) return done(hip_fail(ctx, qerr, "batch stream fork/join"));
     }
     iters += (uint32_t)K;
     AJG_BCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
@@ -2745,7 +2758,7 @@ int ajg_batch_autotune(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs, uint3
     std::vector<int32_t> values;
   };
   const Knob knobs[] = {
-      {&ajg_batch_args::max_slots, {1 << 18, 1 << 19, 1 << 20, 1 << 21}},
+      {&ajg_batch_args::max_slots, {1 << 19, 1 << 20, 1 << 21, 1 << 22}},
       {&ajg_batch_args::samples_per_item, {0, 1, 2, 4, 8}},
       {&ajg_batch_args::pixel_tile, {0, 4, 8, 16}},
       {&ajg_batch_args::lds_stack, {4, 8, 16}},

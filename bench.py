@@ -9,6 +9,13 @@ p = 0.9), MIS path tracing with one closest-hit continuation ray, one
 closest-hit MIS-BSDF ray and one shadow ray per bounce. Inputs (scene, BVH,
 env tables) are resident in HBM before the timed region.
 
+Random numbers: by default one counter-based PCG32 stream per (pixel,
+sample) (seed_mode 1), so the 64 samples of a pixel run in parallel and 4M
+paths are in flight; the reference's per-pixel sequential stream (seed_mode
+0: a pixel's samples one after another, 1M paths in flight at 1024x1024) is
+timed as well (paths_per_s_seed_mode0). Both trace the same paths per pixel
+with bit-exact hits; only which random numbers a sample draws differs.
+
 Prints one JSON line: paths/s over the timed steps (whole job), Mrays/s, PLOC
 build Mtris/s, the roofline of the dominant kernel (closest-hit traversal)
 and the CPU baseline (the oracle's all-core CPU restatement of the reference
@@ -62,7 +69,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--res", type=int, default=1024)
-    ap.add_argument("--seed-mode", type=int, default=0)
+    ap.add_argument("--seed-mode", type=int, default=1,
+                    help="1: one PCG32 stream per (pixel, sample), samples of a pixel in parallel; "
+                         "0: the reference's per-pixel sequential stream (one path per pixel in flight)")
+    ap.add_argument("--slots", type=int, default=1 << 22, help="paths in flight (seed_mode 1)")
+    ap.add_argument("--no-seed0", action="store_true",
+                    help="skip the per-pixel-stream (seed_mode 0) comparison render")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -73,7 +85,7 @@ def parse():
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--batch-jobs", type=int, default=64,
                     help="jobs of the batched uneven workload (0: skip)")
-    ap.add_argument("--batch-trial-spp", type=int, default=1)
+    ap.add_argument("--batch-trial-spp", type=int, default=4)
     ap.add_argument("--batch-fixed", type=str, default="",
                     help="skip the autotuner and the one-by-one renders; use this setting "
                          "(e.g. 'max_slots=2097152,pixel_tile=8')")
@@ -500,7 +512,7 @@ def main():
     del agg, tri
 
     gs = GpuScene(ctx, scene)
-    pt = PathTracer(gs, seed_mode=a.seed_mode, flags=capi.RENDER_TIMING)
+    pt = PathTracer(gs, seed_mode=a.seed_mode, max_slots=a.slots, flags=capi.RENDER_TIMING)
     film, _ = pt.render_device()          # allocates the film once
     for _ in range(a.warmup):
         pt.render_device(film=film)
@@ -520,9 +532,21 @@ def main():
     launches_closest = stats[0].launches_closest
     closest_q = stats[0].closest_queries
 
+    # the reference's seeding: per-pixel sequential streams (one path per pixel)
+    pps0, slots0 = None, None
+    if not a.no_seed0:
+        p0 = PathTracer(gs, seed_mode=0, flags=capi.RENDER_TIMING)
+        p0.render_device(film=film)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, st0 = p0.render_device(film=film)
+        torch.cuda.synchronize()
+        pps0 = round(w * h * a.spp / (time.perf_counter() - t), 1)
+        slots0 = st0.slots
+
     roof = None
     if not a.no_count:
-        cpt = PathTracer(gs, seed_mode=a.seed_mode, flags=capi.RENDER_COUNT)
+        cpt = PathTracer(gs, seed_mode=a.seed_mode, max_slots=a.slots, flags=capi.RENDER_COUNT)
         _, cs = cpt.render_device(film=film)
         assert cs.closest_queries == closest_q, "counting pass diverged from the timed pass"
         alg_bytes = (closest_q * (RAY_BYTES + HIT_BYTES) + cs.nodes_fetched * NODE_BYTES +
@@ -569,6 +593,8 @@ def main():
                                f"{w}x{h}, {a.spp} spp, max 16 bounces",
                    "resolution": [w, h], "spp": a.spp, "max_bounces": 16, "triangles": n_tri,
                    "seed_mode": a.seed_mode, "slots": stats[0].slots},
+        "paths_per_s_seed_mode0": pps0,
+        "slots_seed_mode0": slots0,
         "mrays_per_s": round(queries / elapsed / 1e6, 2),
         "rays_per_path": round(queries / paths, 3),
         "build_mtris_per_s": round(n_tri / (min(build_ms) * 1e-3) / 1e6, 2),
