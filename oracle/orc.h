@@ -25,9 +25,14 @@
  *                          the instances (ids except near-ties, t to 1e-4 rel).
  *   Warps / frames       — closed forms (unit length, orthonormality, pdfs).
  *   Lambert              — closed forms (f = a/pi, furnace integral, pdf).
- *   Disney               — sample/pdf consistency and MC integrals only:
- *                          exact values are parity unpinned (see header of
- *                          orc_pt.c).
+ *   Disney               — GGX D normalisation, the visible-normal identity
+ *                          of Smith G1 with D, the visible-normal sampler's
+ *                          histogram against its density, the conductor
+ *                          Fresnel (closed forms and PBRT-2 FrCond), Eval in
+ *                          closed form at normal incidence and in
+ *                          retro-reflection, sample/pdf MC consistency. Values
+ *                          off those configurations follow from the pinned
+ *                          pieces; the world-wo coat pdf quirk is stated.
  *   Env Distribution2D   — pdf integrates to 1; sample histogram vs pdf;
  *                          1x1 map gives 1/(4 pi).
  *   Film resolve         — closed form of the 2x2 clamped box splat.
@@ -254,6 +259,15 @@ uint64_t orc_render_mt(const orc_scene *s, const orc_camera *cam,
  * per-traced-pixel sums to the W x H image (RGB float). */
 void orc_film_resolve(uint32_t w, uint32_t h, uint32_t tile_size,
                       const double *sum, const uint32_t *count, float *rgb);
+
+/* Disney microfacet pieces (test hooks): per i, D(h_i), Smith G1(v_i, h_i)
+ * and the visible-normal pdf of h_i given v_i; the visible-normal sampler
+ * (u: 2 numbers per sample); the conductor Fresnel. Local shading frame. */
+void orc_ggx_parts(uint32_t n, const float *v, const float *h, float alpha, float *d, float *g1,
+                   float *pdf_vn);
+void orc_ggx_sample_vn_batch(uint32_t n, const float *v, const float *u2, float alpha, float *h,
+                             float *pdf);
+float orc_fresnel_conductor(float cosi, float eta, float k);
 
 /* Two-level closest / any hit (instancing, no reference counterpart): the
  * top-level tree's leaf primitives are instances; w2o holds 12 floats

@@ -88,6 +88,9 @@ def load():
         "orc_trace_any": ([_P, _P, _P, _P, C.c_uint32, _P], None),
         "orc_brute_closest": ([_P, C.c_uint32, _P, C.c_uint32, _P], None),
         "orc_trace_closest_2l": ([_P, _P, _P, _P, _P, _P, C.c_uint32, _P, _P], None),
+        "orc_ggx_parts": ([C.c_uint32, _P, _P, C.c_float, _P, _P, _P], None),
+        "orc_ggx_sample_vn_batch": ([C.c_uint32, _P, _P, C.c_float, _P, _P], None),
+        "orc_fresnel_conductor": ([C.c_float, C.c_float, C.c_float], C.c_float),
         "orc_trace_any_2l": ([_P, _P, _P, _P, _P, _P, C.c_uint32, _P], None),
         "orc_trace_closest_counts": ([_P, _P, _P, _P, C.c_uint32, _P, _P], None),
         "orc_pcg32_floats": ([C.c_uint64, C.c_uint32, _P], None),
@@ -213,6 +216,28 @@ def trace_2l(tlas_nodes, tlas_perm, w2o, inst_mesh, blases, rays, closest=True):
     occ = np.zeros(n, np.uint8)
     lib.orc_trace_any_2l(_p(tn), _p(tp), _p(m), _p(im), C.addressof(arr), _p(rays), n, _p(occ))
     return occ
+
+
+def ggx_parts(v, h, alpha):
+    """(D(h), Smith G1(v, h), visible-normal pdf of h given v), shading frame."""
+    lib = load()
+    v = np.ascontiguousarray(v, np.float32)
+    h = np.ascontiguousarray(h, np.float32)
+    n = h.shape[0]
+    d, g1, pv = (np.zeros(n, np.float32) for _ in range(3))
+    lib.orc_ggx_parts(n, _p(v), _p(h), float(alpha), _p(d), _p(g1), _p(pv))
+    return d, g1, pv
+
+
+def ggx_sample_vn(v, u, alpha):
+    lib = load()
+    v = np.ascontiguousarray(v, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    n = v.shape[0]
+    h = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib.orc_ggx_sample_vn_batch(n, _p(v), _p(u), float(alpha), _p(h), _p(pdf))
+    return h, pdf
 
 
 def brute_closest(tri, rays):

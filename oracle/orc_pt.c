@@ -887,6 +887,27 @@ float orc_bsdf_pdf(const orc_material *m, const float *ng, const float *ns,
   bsdf b = make_bsdf(m, vload(ng), vload(ns));
   return bsdf_pdf_all(&b, vload(wi), vload(wo));
 }
+/* Microfacet pieces of the Disney BSDF, exported for the tests that pin them
+ * (GGX D, Smith G1 and the visible-normal pdf, disney.cpp:289-299, 350-360,
+ * 435-453; the visible-normal sampler, :301-348, 390-412; the conductor
+ * Fresnel, :427-436). */
+void orc_ggx_parts(uint32_t n, const float *v, const float *h, float alpha, float *d, float *g1,
+                   float *pdf_vn) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const v3 vv = vload(v + 3 * (size_t)i), hh = vload(h + 3 * (size_t)i);
+    d[i] = ggx_d(hh, alpha);
+    g1[i] = smith_g(vv, hh, alpha);
+    pdf_vn[i] = ggx_pdf_vn(vv, hh, alpha);
+  }
+}
+void orc_ggx_sample_vn_batch(uint32_t n, const float *v, const float *u2, float alpha, float *h,
+                             float *pdf) {
+  for (uint32_t i = 0; i < n; ++i)
+    vstore(h + 3 * (size_t)i, ggx_sample_vn(vload(v + 3 * (size_t)i), u2[2 * (size_t)i],
+                                            u2[2 * (size_t)i + 1], &pdf[i], alpha));
+}
+float orc_fresnel_conductor(float cosi, float eta, float k) { return fresnel_conductor(cosi, eta, k); }
+
 void orc_square_to_disk(const float *s2, float *out2) {
   v2 s = {s2[0], s2[1]};
   v2 r = disk_concentric(s);
