@@ -5,11 +5,15 @@
  * Disney BSDFs, directional and environment lights, the thin-lens camera,
  * the PCG32 sampler and the box-filtered film.
  *
- * Parity status of the Disney BSDF: its exact values (including the
- * world-space wo used by the coat term of Pdf, disney.cpp:208-220) follow
- * disney.cpp line by line, but no reference output exists to pin them; the
- * tests pin it only statistically (sample/pdf consistency, MC integrals,
- * reciprocity of Eval). Parity unpinned for exact Disney values.
+ * Parity status of the Disney BSDF: its pieces are pinned by identities the
+ * reference's formulas satisfy (GGX D normalisation, the visible-normal
+ * identity of Smith G1 with D, the visible-normal sampler's density, the
+ * conductor Fresnel) and Eval by closed forms at normal incidence and in
+ * retro-reflection (tests/test_oracle_disney.py); sample/pdf consistency, MC
+ * integrals, reciprocity and the world-space wo of the coat pdf
+ * (disney.cpp:208-220) are tested in tests/test_oracle_pt.py. No reference
+ * output exists, so values between those configurations rest on the pinned
+ * pieces.
  *
  * Environment sampling draws its two numbers as the GCC build of the
  * reference evaluates SampleContinuous(Next1D(), Next1D(), ...)
