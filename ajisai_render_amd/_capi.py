@@ -115,6 +115,13 @@ class Instance(C.Structure):
 INSTANCE_KEEP_MATERIAL = 0xFFFFFFFF
 
 
+class Filter(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("radius", C.c_float * 2), ("alpha", C.c_float)]
+
+
+FILTER_BOX, FILTER_GAUSSIAN = 0, 1
+
+
 class Job(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("args", PTArgs), ("film", Film)]
 
@@ -166,6 +173,9 @@ _SIGS = {
     "ajg_render_pt": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), C.POINTER(Film),
                        C.POINTER(RenderStats), _P], C.c_int),
     "ajg_film_resolve": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P], C.c_int),
+    "ajg_film_resolve_filter": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Filter), _P,
+                                 _P, _P, _P], C.c_int),
+    "ajg_image_srgb8": ([_P, C.c_uint32, C.c_uint32, _P, C.c_int32, _P, _P], C.c_int),
     "ajg_camera_setup": ([C.POINTER(Camera), _P, _P, _P, C.c_float, C.c_float, C.c_float,
                           C.c_float, C.c_float], C.c_int),
     "ajg_render_pt_guided": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), _P, C.POINTER(GuideArgs),

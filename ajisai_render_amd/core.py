@@ -367,6 +367,31 @@ class PathTracer:
             out["guide_stats"] = res[2]
         return out
 
+    def resolve_filtered(self, film_dev, ftype="box", radius=None, alpha=2.0):
+        """Film resolve with the reference's box (radius 0.5) or gaussian
+        (radius 2, alpha 2) filter -> (H, W, 3) device tensor."""
+        torch = _torch()
+        w, h = self.g.scene.camera.res
+        t = capi.FILTER_BOX if ftype == "box" else capi.FILTER_GAUSSIAN
+        r = radius if radius is not None else ((0.5, 0.5) if t == capi.FILTER_BOX else (2.0, 2.0))
+        f = capi.Filter(t, (C.c_float * 2)(*r), alpha)
+        rgb = torch.empty(h * w * 3, dtype=torch.float32, device="cuda")
+        self.g.ctx.check(self.g.ctx.lib.ajg_film_resolve_filter(
+            self.g.ctx.h, w, h, self.args.tile_size, C.byref(f), _ptr(film_dev["sum"]),
+            _ptr(film_dev["count"]), _ptr(rgb), None), "ajg_film_resolve_filter")
+        return rgb.view(h, w, 3)
+
+    def write_srgb8(self, rgb_dev, flip=True):
+        """ImageWriter::Write pixel path -> (H, W, 3) uint8 numpy array (rows
+        bottom to top when flip, as the reference stores its PNG)."""
+        torch = _torch()
+        h, w = rgb_dev.shape[:2]
+        out = torch.empty(h * w * 3, dtype=torch.uint8, device="cuda")
+        self.g.ctx.check(self.g.ctx.lib.ajg_image_srgb8(self.g.ctx.h, w, h, _ptr(rgb_dev),
+                                                        1 if flip else 0, _ptr(out), None),
+                         "ajg_image_srgb8")
+        return out.cpu().numpy().reshape(h, w, 3)
+
     def resolve(self, film_dev):
         """Film resolve on the device -> (H, W, 3) numpy image."""
         torch = _torch()

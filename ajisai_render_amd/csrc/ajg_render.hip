@@ -1338,6 +1338,68 @@ k_resolve(uint32_t w, uint32_t h, uint32_t Wp, uint32_t Hp, const float *sum,
   for (int k = 0; k < 3; ++k) rgb[3 * (size_t)q + k] = wt != 0 ? (float)(acc[k] / wt) : 0.f;
 }
 
+// Filtered resolve (film.cpp:27-63 with any Filter): a traced pixel p adds
+// its sums into targets j = p + o, o in [omin, omax] per axis, with weight
+// w[o - omin] (the filter at offset o + 0.5), clamped to the film. Output x
+// gathers from every p whose footprint clamps onto x; per axis that weight is
+// the sum of w over the offsets of p that land on x.
+struct FilterAxis {
+  int omin, omax;
+  float w[16];
+};
+
+__device__ __forceinline__ float axis_weight(const FilterAxis &f, int p, int x, int n) {
+  float a = 0.f;
+  for (int o = f.omin; o <= f.omax; ++o)
+    if (clampi(p + o, 0, n - 1) == x) a += f.w[o - f.omin];
+  return a;
+}
+
+__global__ void __launch_bounds__(256)
+k_resolve_filter(uint32_t w, uint32_t h, uint32_t Wp, uint32_t Hp, FilterAxis fx, FilterAxis fy,
+                 const float *sum, const uint32_t *count, float *rgb) {
+  const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= w * h) return;
+  const int qx = (int)(q % w), qy = (int)(q / w);
+  const int x_lo = max(0, qx - fx.omax);
+  const int x_hi = qx == (int)w - 1 ? (int)Wp - 1 : min((int)Wp - 1, qx - fx.omin);
+  const int y_lo = max(0, qy - fy.omax);
+  const int y_hi = qy == (int)h - 1 ? (int)Hp - 1 : min((int)Hp - 1, qy - fy.omin);
+  double acc[3] = {0, 0, 0}, wt = 0;
+  for (int py = y_lo; py <= y_hi; ++py) {
+    const float ay = axis_weight(fy, py, qy, (int)h);
+    if (ay == 0.f) continue;
+    for (int px = x_lo; px <= x_hi; ++px) {
+      const float ax = axis_weight(fx, px, qx, (int)w);
+      if (ax == 0.f) continue;
+      const double k = (double)ax * (double)ay;
+      const size_t p = (size_t)px + (size_t)py * Wp;
+      for (int c = 0; c < 3; ++c) acc[c] += k * sum[3 * p + c];
+      wt += k * count[p];
+    }
+  }
+  for (int c = 0; c < 3; ++c) rgb[3 * (size_t)q + c] = wt != 0 ? (float)(acc[c] / wt) : 0.f;
+}
+
+// ImageWriter::Write pixel path (image.cpp:32-62, image.h:73-84,
+// color.h:54-64): sRGB encode, clamp to [0, 1], round(v * 255.5) in double,
+// rows bottom to top when flipped; NaN components become 0.
+__global__ void __launch_bounds__(256)
+k_srgb8(uint32_t w, uint32_t h, const float *rgb, int flip, uint8_t *out) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= w * h) return;
+  const uint32_t x = i % w, y = i / w;
+  const uint32_t sy = flip ? h - 1 - y : y;
+  for (int k = 0; k < 3; ++k) {
+    const float v = rgb[3 * ((size_t)x + (size_t)sy * w) + k];
+    float sv = v > 0.0031308f ? 1.055f * powf(v, 1.0f / 2.4f) - 0.055f : v * 12.92f;
+    sv = sv < 0.f ? 0.f : (sv > 1.f ? 1.f : sv);
+    if (sv != sv) sv = 0.f;
+    const double r = round((double)sv * 255.5);
+    out[3 * (size_t)i + k] = (uint8_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
+  }
+}
+
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // Device buffers of a render: path state, double-buffered ray queues,
@@ -2787,6 +2849,49 @@ int ajg_batch_autotune(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs, uint3
   *inout = cur;
   if (best_paths_per_s) *best_paths_per_s = (float)best;
   return AJG_OK;
+}
+
+int ajg_film_resolve_filter(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size,
+                            const ajg_filter *filter, const float *sum, const uint32_t *count,
+                            float *rgb, void *stream) {
+  if (!ctx || !filter || !sum || !count || !rgb || w == 0 || h == 0 || tile_size == 0)
+    return AJG_ERR_INVALID;
+  if (int st = injected(ctx)) return st;
+  if (filter->type != AJG_FILTER_BOX && filter->type != AJG_FILTER_GAUSSIAN)
+    return fail(ctx, AJG_ERR_INVALID, "film: unknown filter type");
+  FilterAxis ax[2];
+  for (int a = 0; a < 2; ++a) {
+    const float r = filter->radius[a];
+    if (!(r > 0.f) || r > 7.5f) return fail(ctx, AJG_ERR_INVALID, "film: filter radius must be in (0, 7.5]");
+    // targets of pixel p: [ceil(p - 0.5 - r), floor(p - 0.5 + r)] (film.cpp:30-33)
+    ax[a].omin = (int)std::ceil(-0.5f - r);
+    ax[a].omax = (int)std::floor(-0.5f + r);
+    if (ax[a].omax - ax[a].omin + 1 > 16) return fail(ctx, AJG_ERR_INVALID, "film: filter too wide");
+    // Filter::Eval per axis at o + 0.5, in float with the host's expf
+    // (box_filter.cpp:35; gaussian_filter.cpp:38-45)
+    const float ev = std::exp(-filter->alpha * r * r);
+    for (int o = ax[a].omin; o <= ax[a].omax; ++o) {
+      const float d = (float)o + 0.5f;
+      ax[a].w[o - ax[a].omin] =
+          filter->type == AJG_FILTER_BOX ? 1.f : std::max(0.f, (float)std::exp(-filter->alpha * d * d) - ev);
+    }
+  }
+  const uint32_t Wp = (w + tile_size - 1) / tile_size * tile_size;
+  const uint32_t Hp = (h + tile_size - 1) / tile_size * tile_size;
+  hipLaunchKernelGGL(k_resolve_filter, dim3(cdiv((uint64_t)w * h, 256)), dim3(256), 0, (hipStream_t)stream,
+                     w, h, Wp, Hp, ax[0], ax[1], sum, count, rgb);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(ctx, e, "k_resolve_filter");
+}
+
+int ajg_image_srgb8(ajg_ctx *ctx, uint32_t w, uint32_t h, const float *rgb, int32_t flip,
+                    uint8_t *out, void *stream) {
+  if (!ctx || !rgb || !out || w == 0 || h == 0) return AJG_ERR_INVALID;
+  if (int st = injected(ctx)) return st;
+  hipLaunchKernelGGL(k_srgb8, dim3(cdiv((uint64_t)w * h, 256)), dim3(256), 0, (hipStream_t)stream, w, h,
+                     rgb, (int)flip, out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? AJG_OK : hip_fail(ctx, e, "k_srgb8");
 }
 
 int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size, const float *sum,

@@ -427,6 +427,39 @@ def instanced_bench(ctx, a):
     return out
 
 
+def film_bench(pt, film):
+    """Film resolve (box, gaussian) and 8-bit sRGB encode of the heavy-mesh
+    film: device time per call and algorithmic bytes (sums 28 B per traced
+    pixel read, 12 B per output pixel written; encode 12 B read, 3 B written)."""
+    import torch
+    w, h = pt.g.scene.camera.res
+    wp, hp = pt.traced_dims()
+    out = {}
+    for name, fn in [("resolve_box", lambda: pt.resolve_filtered(film, "box")),
+                     ("resolve_gaussian", lambda: pt.resolve_filtered(film, "gaussian"))]:
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            rgb = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        b = 28 * wp * hp + 12 * w * h
+        out[name] = {"ms": round(ms, 4), "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1)}
+    pt.write_srgb8(rgb)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        pt.write_srgb8(rgb)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20  # includes the 3 MB copy to the host
+    out["srgb8_with_host_copy"] = {"ms": round(ms, 4)}
+    out["pixels"] = w * h
+    return out
+
+
 def profiled_traffic():
     """HBM bytes per launch of the closest-hit traversal kernel as measured by
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a
@@ -563,6 +596,7 @@ def main():
                 "nodes_per_query": round(cs.nodes_fetched / max(1, closest_q), 2),
                 "tris_per_query": round(cs.tris_tested / max(1, closest_q), 2)}
 
+    film_perf = film_bench(pt, film)
     mlp = mlp_bench(ctx)
     guided = guided_equal_time(ctx, a) if a.guided_budget > 0 else None
     batched = batched_bench(ctx, a) if a.batch_jobs > 0 else None
@@ -611,6 +645,7 @@ def main():
         "batched": batched,
         "instanced": instanced,
         "mlp": mlp,
+        "film": film_perf,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

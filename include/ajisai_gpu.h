@@ -320,6 +320,36 @@ int ajg_film_resolve(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size,
                      const float *sum, const uint32_t *count, float *rgb,
                      void *stream);
 
+/* Film filters (Filter, core/filter/filter.h): BoxFilter (box_filter.cpp:28-42,
+ * weight 1; the reference's default radius 0.5) and GaussianFilter
+ * (gaussian_filter.cpp:28-50, max(0, exp(-alpha d^2) - exp(-alpha r^2)) per
+ * axis, product; defaults radius 2, alpha 2, filter_creators.cpp:39-50). */
+enum ajg_filter_type { AJG_FILTER_BOX = 0, AJG_FILTER_GAUSSIAN = 1 };
+typedef struct ajg_filter {
+  uint32_t type;
+  float radius[2];  /* x, y; in (0, 7.5] */
+  float alpha;      /* gaussian only */
+} ajg_filter;
+
+/* Tile::AddSample + Film::MergeTile + WriteImage's division (film.cpp:27-63,
+ * film.h:119-147, image.h:38-48) for any of the filters above: each traced
+ * pixel p adds its sums into [ceil(p - 0.5 - r), floor(p - 0.5 + r)] per axis
+ * with the filter's weight at the target's offset, clamped to the film; the
+ * image is radiance / weight. The filter weights are evaluated on the host in
+ * float (as the reference does); accumulation is in double on the device.
+ * Device pointers; enqueued. */
+int ajg_film_resolve_filter(ajg_ctx *ctx, uint32_t w, uint32_t h, uint32_t tile_size,
+                            const ajg_filter *filter, const float *sum,
+                            const uint32_t *count, float *rgb, void *stream);
+
+/* ImageWriter::Write's pixel path (image.cpp:32-62, color.h:54-64): linear
+ * RGB (w*h*3 floats) -> 8-bit sRGB (w*h*3 bytes): 12.92 x below 0.0031308,
+ * else 1.055 x^(1/2.4) - 0.055; clamp to [0, 1]; round(v * 255.5); flip != 0
+ * stores rows bottom to top (stbi_flip_vertically_on_write). NaN -> 0.
+ * Device pointers; enqueued. */
+int ajg_image_srgb8(ajg_ctx *ctx, uint32_t w, uint32_t h, const float *rgb,
+                    int32_t flip, uint8_t *out, void *stream);
+
 /* ThinLensCamera matrices (thin_lens.cpp:32-55) from look-at parameters;
  * fov in degrees. Host only. */
 int ajg_camera_setup(ajg_camera *cam, const float *pos, const float *look_at,

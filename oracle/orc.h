@@ -257,6 +257,12 @@ uint64_t orc_render_mt(const orc_scene *s, const orc_camera *cam,
                        uint32_t *count, int n_threads);
 /* Film resolve (film.cpp:27-63, film.h:119-129, image.h:38-48) from
  * per-traced-pixel sums to the W x H image (RGB float). */
+/* Filtered film resolve: type 0 box, 1 gaussian (radius rx, ry; alpha). */
+void orc_film_resolve_filter(uint32_t w, uint32_t h, uint32_t tile_size, int type, float rx,
+                             float ry, float alpha, const double *sum, const uint32_t *count,
+                             float *rgb);
+/* Linear RGB (w*h*3) -> 8-bit sRGB as the image writer stores it. */
+void orc_image_srgb8(uint32_t w, uint32_t h, const float *rgb, int flip, uint8_t *out);
 void orc_film_resolve(uint32_t w, uint32_t h, uint32_t tile_size,
                       const double *sum, const uint32_t *count, float *rgb);
 

@@ -119,6 +119,9 @@ def load():
         "orc_render_mt": ([_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_int],
                           C.c_uint64),
         "orc_film_resolve": ([C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P], None),
+        "orc_film_resolve_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_float,
+                                     C.c_float, C.c_float, _P, _P, _P], None),
+        "orc_image_srgb8": ([C.c_uint32, C.c_uint32, _P, C.c_int, _P], None),
     }
     for name, (a, r) in sig.items():
         f = getattr(lib, name)
@@ -403,6 +406,24 @@ def film_resolve(w, h, ts, sums, counts):
     c = np.ascontiguousarray(counts, np.uint32)
     out = np.zeros((h, w, 3), np.float32)
     lib.orc_film_resolve(w, h, ts, _p(s), _p(c), _p(out))
+    return out
+
+
+def film_resolve_filter(w, h, ts, sums, counts, ftype=0, radius=(0.5, 0.5), alpha=2.0):
+    lib = load()
+    s = np.ascontiguousarray(sums, np.float64)
+    c = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros((h, w, 3), np.float32)
+    lib.orc_film_resolve_filter(w, h, ts, ftype, radius[0], radius[1], alpha, _p(s), _p(c), _p(out))
+    return out
+
+
+def image_srgb8(rgb, flip=True):
+    lib = load()
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    out = np.zeros((h, w, 3), np.uint8)
+    lib.orc_image_srgb8(w, h, _p(rgb), 1 if flip else 0, _p(out))
     return out
 
 

@@ -1371,6 +1371,64 @@ uint64_t orc_render_mt(const orc_scene *s, const orc_camera *cam,
 /* Film resolve: a sample at traced pixel p splats weight 1 into the pixels
  * {p-1, p} x {p-1, p} (box radius 0.5, film.cpp:27-48); film writes clamp to
  * the edge (image.h:38-48); I = radiance / weight (film.h:131-152). */
+/* Filtered film (film.cpp:27-63 Tile::AddSample + Film::MergeTile, image.h:38-48
+ * clamped access, film.h:130-147 WriteImage without splats): every traced
+ * pixel p (whole tiles) adds filter-weighted sums into j in
+ * [ceil(p - 0.5 - r), floor(p - 0.5 + r)] per axis, clamped to the film;
+ * box_filter.cpp:28-42 (weight 1) or gaussian_filter.cpp:28-50
+ * (max(0, exp(-alpha d^2) - exp(-alpha r^2)) per axis, product). The sums of
+ * a pixel's samples are weighted at once (the reference weights sample by
+ * sample; same value up to float rounding). */
+static float filter_axis(int type, float d, float r, float alpha) {
+  if (type == 0) return 1.f;
+  const float ev = expf(-alpha * r * r);
+  return std_max(0.f, (float)expf(-alpha * d * d) - ev);
+}
+void orc_film_resolve_filter(uint32_t w, uint32_t h, uint32_t ts, int type, float rx, float ry,
+                             float alpha, const double *sum, const uint32_t *count, float *rgb) {
+  uint32_t Wp = (w + ts - 1) / ts * ts, Hp = (h + ts - 1) / ts * ts;
+  float *rad = (float *)calloc((size_t)w * h * 3, sizeof(float));
+  float *wt = (float *)calloc((size_t)w * h, sizeof(float));
+  for (uint32_t py = 0; py < Hp; ++py)
+    for (uint32_t px = 0; px < Wp; ++px) {
+      const size_t p = (size_t)px + (size_t)py * Wp;
+      const int x0 = (int)ceilf((float)px - 0.5f - rx), x1 = (int)floorf((float)px - 0.5f + rx);
+      const int y0 = (int)ceilf((float)py - 0.5f - ry), y1 = (int)floorf((float)py - 0.5f + ry);
+      for (int i = y0; i <= y1; ++i)
+        for (int j = x0; j <= x1; ++j) {
+          const float fw = filter_axis(type, (float)j - (float)px + 0.5f, rx, alpha) *
+                           filter_axis(type, (float)i - (float)py + 0.5f, ry, alpha);
+          const size_t q = (size_t)clampi(j, 0, (int)w - 1) + (size_t)clampi(i, 0, (int)h - 1) * w;
+          wt[q] += fw * (float)count[p];
+          for (int k = 0; k < 3; ++k) rad[3 * q + k] += fw * (float)sum[3 * p + k];
+        }
+    }
+  for (size_t q = 0; q < (size_t)w * h; ++q)
+    for (int k = 0; k < 3; ++k) rgb[3 * q + k] = wt[q] != 0 ? rad[3 * q + k] / wt[q] : 0.f;
+  free(rad);
+  free(wt);
+}
+
+/* ImageWriter::Write pixel path (image.cpp:32-62, image.h:73-84,
+ * color.h:54-64): linear RGB -> sRGB (12.92 x below 0.0031308, else
+ * 1.055 x^(1/2.4) - 0.055, per component), clamp to [0, 1],
+ * round(v * 255.5) in double clamped to [0, 255]; flip = rows bottom to top
+ * (stbi_flip_vertically_on_write). NaN components become 0. */
+void orc_image_srgb8(uint32_t w, uint32_t h, const float *rgb, int flip, uint8_t *out) {
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t x = 0; x < w; ++x) {
+      const uint32_t sy = flip ? h - 1 - y : y;
+      for (int k = 0; k < 3; ++k) {
+        const float v = rgb[3 * ((size_t)x + (size_t)sy * w) + k];
+        float s = v > 0.0031308f ? 1.055f * powf(v, 1.0f / 2.4f) - 0.055f : v * 12.92f;
+        s = s < 0.f ? 0.f : (s > 1.f ? 1.f : s);
+        if (s != s) s = 0.f;
+        double r = round((double)s * 255.5);
+        out[3 * ((size_t)x + (size_t)y * w) + k] = (uint8_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
+      }
+    }
+}
+
 void orc_film_resolve(uint32_t w, uint32_t h, uint32_t ts, const double *sum,
                       const uint32_t *count, float *rgb) {
   uint32_t Wp = (w + ts - 1) / ts * ts, Hp = (h + ts - 1) / ts * ts;
