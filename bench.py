@@ -94,7 +94,7 @@ def parse():
     ap.add_argument("--inst-guided-budget", type=float, default=5.0,
                     help="equal-time budget (s) of guided vs plain on the instanced scene (0: skip)")
     ap.add_argument("--inst-guided-res", type=int, default=480)
-    ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp"],
+    ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp", "guided"],
                     help="run one section alone and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -525,7 +525,8 @@ def main():
     ctx = Context(0)
     if a.only:
         fn = {"batched": batched_bench, "instanced": instanced_bench,
-              "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c)}[a.only]
+              "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c),
+              "guided": guided_equal_time}[a.only]
         print(json.dumps({a.only: fn(ctx, a)}))
         return
     scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
