@@ -44,7 +44,8 @@ class AccelInfo(C.Structure):
 class Primitive(C.Structure):
     _fields_ = [("view", MeshView), ("kind", C.c_uint32), ("two_sided", C.c_uint32),
                 ("material", C.c_uint32), ("reserved", C.c_uint32),
-                ("normal_and_vs", C.c_void_p)]
+                ("normal_and_vs", C.c_void_p), ("emission", C.c_float * 3),
+                ("reserved2", C.c_uint32)]
 
 
 class Material(C.Structure):

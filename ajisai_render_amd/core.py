@@ -89,7 +89,8 @@ def c_primitives(prims: Sequence[Primitive]):
             nrm = np.ascontiguousarray(p.normal_and_vs, np.float32)
             keep.append(nrm)
         arr[i] = capi.Primitive(views[i], p.kind, int(p.two_sided), p.material, 0,
-                                nrm.ctypes.data if nrm is not None else None)
+                                nrm.ctypes.data if nrm is not None else None,
+                                (C.c_float * 3)(*p.emission), 0)
     return arr, keep
 
 

@@ -39,6 +39,7 @@ class Primitive:
     two_sided: bool
     material: int
     normal_and_vs: Optional[np.ndarray] = None  # (n, 4) float32, MESH only
+    emission: tuple = (0.0, 0.0, 0.0)           # non-black: an area light
 
     @property
     def n_tri(self) -> int:

@@ -152,7 +152,10 @@ enum ajg_light_type { AJG_LIGHT_DIRECTIONAL = 0, AJG_LIGHT_ENV = 1 };
  * reads: the geometry's MeshView, its kind (selects the PostIntersect rule:
  * quad.cpp:73-90, cube.cpp:96-108, mesh.cpp:183-204), the TwoSided wrapper
  * (two_sided.cpp:44-53), the material index and, for meshes, the per-vertex
- * normal_and_vs buffer (mesh.cpp:190-193; 4 floats per vertex). */
+ * normal_and_vs buffer (mesh.cpp:190-193; 4 floats per vertex). A non-black
+ * emission makes the primitive an area light (GeometricPrimitive,
+ * geometric.cpp:35-51; AreaLight, area_light.cpp:27-85), listed before the
+ * scene's own lights (default_scene.cpp:34-48). */
 typedef struct ajg_primitive {
   ajg_mesh_view view;
   uint32_t kind;
@@ -160,6 +163,8 @@ typedef struct ajg_primitive {
   uint32_t material;
   uint32_t reserved;
   const float *normal_and_vs;
+  float emission[3];
+  uint32_t reserved2;
 } ajg_primitive;
 
 /* Material: Diffuse (material/diffuse.cpp:35-48; base_color = albedo) or

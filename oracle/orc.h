@@ -143,6 +143,8 @@ typedef struct orc_primitive {
   uint32_t material;   /* index into materials */
   uint32_t pad;
   const float *normal_and_vs; /* MESH: per vertex (nx,ny,nz,v) */
+  float emission[3];   /* non-black: an AreaLight (geometric.cpp:35-51) */
+  uint32_t pad2;
 } orc_primitive;
 
 /* Material record: DIFFUSE uses base_color as albedo. DISNEY parameter order

@@ -30,7 +30,8 @@ class MeshView(C.Structure):
 
 class Prim(C.Structure):
     _fields_ = [("view", MeshView), ("kind", C.c_uint32), ("two_sided", C.c_uint32),
-                ("material", C.c_uint32), ("pad", C.c_uint32), ("normal_and_vs", C.c_void_p)]
+                ("material", C.c_uint32), ("pad", C.c_uint32), ("normal_and_vs", C.c_void_p),
+                ("emission", C.c_float * 3), ("pad2", C.c_uint32)]
 
 
 class Mat(C.Structure):
@@ -287,7 +288,8 @@ class OracleScene:
             mv = MeshView(pos.ctypes.data, 0, pos.strides[0], pos.shape[0], idx.ctypes.data, 0,
                           12, idx.shape[0])
             prims[i] = Prim(mv, p.kind, int(p.two_sided), p.material, 0,
-                            nrm.ctypes.data if nrm is not None else None)
+                            nrm.ctypes.data if nrm is not None else None,
+                            (C.c_float * 3)(*getattr(p, "emission", (0.0, 0.0, 0.0))), 0)
         mats = (Mat * max(1, len(scene.materials)))()
         for i, m in enumerate(scene.materials):
             mats[i] = Mat(m.type, (C.c_float * 3)(*m.base_color), m.subsurface, m.metallic,

@@ -596,9 +596,18 @@ static float distrib_sample_cont(const distrib *d, float u, float *pdf, int *off
 }
 
 /* ---- scene -------------------------------------------------------------- */
+enum { ORC_LIGHT_AREA = 2 }; /* emissive primitive (internal light type) */
+
 typedef struct light_rt {
   uint32_t type;
   v3 dir, radiance;
+  /* area light: the geometry's Sample triangles in the order the geometry's
+   * Sample indexes them (9 floats each), its kind and TwoSided wrapper,
+   * surface area, the quad's first-triangle probability, the cube's pdf of
+   * its first triangle */
+  uint32_t kind, two_sided, n_stri;
+  float *stris;
+  float area, prob, cube_pdf0;
   frame fr;
   float angle_cos;
   /* env */
@@ -611,6 +620,7 @@ typedef struct light_rt {
 } light_rt;
 
 struct orc_scene {
+  int *prim_light; /* per primitive: its area light, or -1 */
   uint32_t n_tri;
   float *tri;
   orc_node *nodes;
@@ -734,6 +744,123 @@ static light_sample dir_sample(const light_rt *l, v3 ref, v2 u) {
 
 static int v_is_point(const orc_mesh_view *mv) { (void)mv; return 0; }
 
+static inline uint32_t idx_word(const orc_mesh_view *mv, uint32_t local, int k) {
+  uint32_t v;
+  memcpy(&v, (const char *)mv->index_buffer + mv->index_offset +
+                 (size_t)local * mv->index_stride + 4 * k, 4);
+  return v;
+}
+static inline v3 view_pos(const orc_mesh_view *mv, uint32_t i) {
+  float f[3];
+  memcpy(f, (const char *)mv->position_buffer + mv->position_offset +
+                (size_t)i * mv->position_stride, 12);
+  return V3(f[0], f[1], f[2]);
+}
+
+/* ---- area lights (AreaLight, area_light.cpp:27-85; Geometry::Sample and
+ * Pdf of quad.cpp:129-171, cube.cpp:145-191, mesh.cpp:251-297, TwoSided
+ * two_sided.cpp:62-115). Restated as written, including:
+ *  - MeshGeo::Sample indexes indices_ with the triangle number itself
+ *    (indices_[idx], [idx+1], [idx+2], idx < triangle count), so its
+ *    "triangles" are consecutive index entries, and its pdf is 1/area;
+ *  - the Cube picks one of its 12 triangles uniformly with the pdf of the
+ *    picked triangle, and its Pdf uses the first triangle;
+ *  - the solid-angle pdf divides dist^2 by |n . (ref - pos)| (not by the
+ *    cosine), here and in Pdf. */
+static float vlen_ref(v3 v) { return sqrtf(vdot(v, v)); }
+
+static void area_light_init(light_rt *l, const orc_primitive *pr, const float *flat) {
+  l->type = ORC_LIGHT_AREA;
+  l->radiance = vload(pr->emission);
+  l->kind = pr->kind;
+  l->two_sided = pr->two_sided;
+  const uint32_t n = (uint32_t)pr->view.index_size;
+  if (pr->kind == ORC_GEOM_QUAD) {
+    const v3 a = view_pos(&pr->view, 0), b = view_pos(&pr->view, 1),
+             c = view_pos(&pr->view, 2), d = view_pos(&pr->view, 3);
+    l->n_stri = 2;
+    l->stris = (float *)malloc(sizeof(float) * 18);
+    vstore(l->stris, a), vstore(l->stris + 3, b), vstore(l->stris + 6, c);
+    vstore(l->stris + 9, a), vstore(l->stris + 12, c), vstore(l->stris + 15, d);
+    const float abc = vlen_ref(vcross(vsub(b, a), vsub(c, a))) * 0.5f;
+    const float acd = vlen_ref(vcross(vsub(c, a), vsub(d, a))) * 0.5f;
+    l->area = abc + acd;
+    l->prob = abc / l->area;
+  } else if (pr->kind == ORC_GEOM_CUBE) {
+    l->n_stri = n;
+    l->stris = (float *)malloc(sizeof(float) * 9 * n);
+    for (uint32_t t = 0; t < n; ++t)   /* logical (a, b, c): the flattened order reversed */
+      for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c) l->stris[9 * t + 3 * k + c] = flat[9 * t + 3 * (2 - k) + c];
+    const float *t0 = l->stris;
+    l->cube_pdf0 = 1.f / (12.f * vlen_ref(vcross(vsub(vload(t0 + 3), vload(t0)),
+                                                 vsub(vload(t0 + 6), vload(t0)))) * 0.5f);
+  } else {
+    l->n_stri = n;
+    l->stris = (float *)malloc(sizeof(float) * 9 * n);
+    for (uint32_t t = 0; t < n; ++t)
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t flat_i = t + (uint32_t)k;  /* the index array read at the triangle number */
+        const uint32_t w = idx_word(&pr->view, flat_i / 3, flat_i % 3);
+        vstore(l->stris + 9 * t + 3 * k, view_pos(&pr->view, w));
+      }
+    float area = 0.f;
+    for (uint32_t t = 0; t < n; ++t) {
+      const float *p = flat + 9 * (size_t)t;
+      const v3 e0 = vsub(vload(p + 3), vload(p)), e1 = vsub(vload(p + 6), vload(p));
+      area = (float)((double)area + 0.5 * (double)vlen_ref(vcross(e0, e1)));
+    }
+    l->area = area;
+  }
+}
+
+/* Geometry::Sample of the light's primitive (with the TwoSided wrapper) */
+static void area_geom_sample(const light_rt *l, v3 sam, v3 *pos, v3 *ng, float *pdf) {
+  int back = 0;
+  if (l->two_sided) {
+    back = sam.x < 0.5f;
+    sam.x = back ? 2 * sam.x : 2 * (sam.x - 0.5f);
+  }
+  v3 sn = sam;
+  if (sn.x + sn.y > 1.f) {
+    sn.x = 1.f - sn.x;
+    sn.y = 1.f - sn.y;
+  }
+  const float *t;
+  if (l->kind == ORC_GEOM_QUAD) {
+    t = sam.z < l->prob ? l->stris : l->stris + 9;
+    *pdf = 1.f / l->area;
+  } else {
+    const int cnt = (int)l->n_stri;
+    int idx = (int)(sam.z * (float)cnt);
+    if (idx > cnt - 1) idx = cnt - 1;
+    t = l->stris + 9 * (size_t)idx;
+    if (l->kind == ORC_GEOM_CUBE)
+      *pdf = 1.f / (12.f * vlen_ref(vcross(vsub(vload(t + 3), vload(t)),
+                                           vsub(vload(t + 6), vload(t)))) * 0.5f);
+    else
+      *pdf = 1.f / l->area;
+  }
+  const v3 a = vload(t), b = vload(t + 3), c = vload(t + 6);
+  *pos = vadd(vadd(vscale(a, (1 - sn.x) - sn.y), vscale(b, sn.x)), vscale(c, sn.y));
+  *ng = vnormalize(vcross(vsub(b, a), vsub(c, a)));
+  if (l->two_sided) {
+    *pdf = *pdf * 0.5f;
+    if (back) *ng = vscale(*ng, -1.f);
+  }
+}
+
+/* AreaLight::Pdf(ref, pos, normal) (area_light.cpp:67-78) */
+static float area_light_pdf(const light_rt *l, v3 ref, v3 pos, v3 n) {
+  if (vdot(vsub(ref, pos), n) <= 0.f) return 0.f;
+  float apdf = l->kind == ORC_GEOM_CUBE ? l->cube_pdf0 : 1.f / l->area;
+  if (l->two_sided) apdf = apdf * 0.5f;
+  const v3 l2r = vsub(ref, pos);
+  const float dist2 = vdot(l2r, l2r);
+  const float abscos = fabsf(vdot(l2r, n));
+  return apdf * (dist2 / abscos);
+}
+
 orc_scene *orc_scene_create(const orc_primitive *prims, uint32_t n_prims,
                             const orc_material *mats, uint32_t n_mats,
                             const orc_light *lights, uint32_t n_lights,
@@ -771,25 +898,37 @@ orc_scene *orc_scene_create(const orc_primitive *prims, uint32_t n_prims,
   s->mats = (orc_material *)malloc(sizeof(orc_material) * (n_mats ? n_mats : 1));
   memcpy(s->mats, mats, sizeof(orc_material) * n_mats);
   s->n_mats = n_mats;
-  s->n_lights = n_lights;
-  s->lights = (light_rt *)calloc(n_lights ? n_lights : 1, sizeof(light_rt));
+  /* DefaultScene (default_scene.cpp:34-48): emissive primitives' area lights
+   * first, in primitive order, then the scene's lights */
+  uint32_t n_area = 0;
+  s->prim_light = (int *)malloc(sizeof(int) * (n_prims ? n_prims : 1));
+  for (uint32_t p = 0; p < n_prims; ++p)
+    s->prim_light[p] = is_black(vload(prims[p].emission)) ? -1 : (int)n_area++;
+  s->n_lights = n_area + n_lights;
+  s->lights = (light_rt *)calloc(s->n_lights ? s->n_lights : 1, sizeof(light_rt));
   s->env_index = -1;
-  for (uint32_t i = 0; i < n_lights; ++i) {
+  for (uint32_t p = 0, t0 = 0; p < n_prims; t0 += (uint32_t)prims[p].view.index_size, ++p) {
+    if (s->prim_light[p] < 0) continue;
+    light_rt *l = &s->lights[s->prim_light[p]];
+    area_light_init(l, &prims[p], s->tri + 9 * (size_t)t0);
+  }
+  for (uint32_t i0 = 0; i0 < n_lights; ++i0) {
+    const uint32_t i = n_area + i0;
     light_rt *l = &s->lights[i];
-    l->type = lights[i].type;
+    l->type = lights[i0].type;
     l->aabb = s->world; /* Light::Process(world_bound) (default_scene.cpp:47) */
     if (l->type == ORC_LIGHT_DIRECTIONAL) {
-      l->dir = vnormalize(vload(lights[i].dir));
+      l->dir = vnormalize(vload(lights[i0].dir));
       l->fr = make_frame(l->dir);
-      l->angle_cos = cosf(lights[i].angle);
-      l->radiance = vload(lights[i].radiance);
+      l->angle_cos = cosf(lights[i0].angle);
+      l->radiance = vload(lights[i0].radiance);
     } else {
       s->env_index = (int)i;
-      l->w = lights[i].env_w;
-      l->h = lights[i].env_h;
-      l->tex = lights[i].env_texels;
-      l->rot = lights[i].rot;
-      l->scaler = lights[i].scaler;
+      l->w = lights[i0].env_w;
+      l->h = lights[i0].env_h;
+      l->tex = lights[i0].env_texels;
+      l->rot = lights[i0].rot;
+      l->scaler = lights[i0].scaler;
       /* EnvLight ctor (env_light.cpp:29-48) */
       float *wts = (float *)malloc(sizeof(float) * l->w * l->h);
       for (uint32_t y = 0; y < l->h; ++y) {
@@ -824,7 +963,10 @@ void orc_scene_destroy(orc_scene *s) {
       distrib_free(&l->marg);
     }
   }
+  for (uint32_t i = 0; i < s->n_lights; ++i)
+    if (s->lights[i].type == ORC_LIGHT_AREA) free(s->lights[i].stris);
   free(s->lights);
+  free(s->prim_light);
   free(s->tri);
   free(s->nodes);
   free(s->perm);
@@ -1104,19 +1246,6 @@ static int scene_occlude(const orc_scene *s, const orc_ray *r, uint64_t *q) {
   return o;
 }
 
-static inline uint32_t idx_word(const orc_mesh_view *mv, uint32_t local, int k) {
-  uint32_t v;
-  memcpy(&v, (const char *)mv->index_buffer + mv->index_offset +
-                 (size_t)local * mv->index_stride + 4 * k, 4);
-  return v;
-}
-static inline v3 view_pos(const orc_mesh_view *mv, uint32_t i) {
-  float f[3];
-  memcpy(f, (const char *)mv->position_buffer + mv->position_offset +
-                (size_t)i * mv->position_stride, 12);
-  return V3(f[0], f[1], f[2]);
-}
-
 /* MT normal + PostIntersect (ploc_aggregate.cpp:199-211; quad.cpp:73-90,
  * cube.cpp:96-108, mesh.cpp:183-204, two_sided.cpp:44-53). */
 static inct post_intersect(const orc_scene *s, const orc_ray *r, const orc_hit *h) {
@@ -1175,6 +1304,31 @@ static v3 mis_sample_light(const orc_scene *s, const light_rt *l, const inct *it
                            const bsdf *b, sampler *smp, uint64_t *q) {
   light_sample ls;
   v3 zero = V3(0, 0, 0);
+  if (l->type == ORC_LIGHT_AREA) {
+    /* MISSampleAreaLight (helper.h:44-71) with AreaLight::Sample */
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    v3 pos, ng;
+    float pdf_area;
+    area_geom_sample(l, sam, &pos, &ng, &pdf_area);
+    if (vdot(ng, vsub(it->pos, pos)) <= 0) return zero;
+    const v3 i2r = vsub(it->pos, pos);
+    const float dist2 = vdot(i2r, i2r);
+    const float pdf = pdf_area * dist2 / fabsf(vdot(ng, i2r));
+    if (is_black(l->radiance) || pdf <= 0.f) return zero;
+    const float len = vlength(vsub(pos, it->pos)) - ORC_EPS;
+    if (len < ORC_EPS) return zero;
+    const v3 to_light = vnormalize(vsub(pos, it->pos));
+    orc_ray sr = make_ray(it->pos, to_light, ORC_EPS, 0.99f * len);
+    if (scene_occlude(s, &sr, q)) return zero;
+    v3 bf = bsdf_eval_all(b, to_light, it->wr);
+    if (is_black(bf)) return zero;
+    v3 f = vscale(vmul(l->radiance, bf), fabsf(vdot(to_light, it->ns)));
+    float bpdf = bsdf_pdf_all(b, to_light, it->wr);
+    return vscale(vdivs(f, pdf), power_heuristic(pdf, bpdf));
+  }
   if (l->type == ORC_LIGHT_ENV) {
     float first = next1(smp);
     float second = next1(smp);
@@ -1227,7 +1381,16 @@ static v3 mis_sample_bsdf(const orc_scene *s, const inct *it, const bsdf *b,
     }
     return env;
   }
-  return zero; /* no emissive primitives */
+  const int li = s->prim_light[s->tri_prim[h.id]];
+  if (li < 0) return zero;
+  const light_rt *l = &s->lights[li];
+  inct hi = post_intersect(s, &r, &h);
+  if (!(vdot(hi.ng, hi.wr) > 0)) return zero;  /* AreaLight::Radiance */
+  if (is_black(l->radiance)) return zero;
+  const v3 d = vload(r.d);
+  v3 f = vscale(vmul(l->radiance, bs.f), fabsf(vdot(it->ns, d)));
+  const float lpdf = area_light_pdf(l, vload(r.o), hi.pos, hi.ng);
+  return vscale(vdivs(f, bs.pdf), power_heuristic(bs.pdf, lpdf));
 }
 
 typedef struct path_stats {
@@ -1249,6 +1412,11 @@ static v3 trace_mis(const orc_scene *s, const orc_pt_args *a, orc_ray ray,
       return L;
     }
     inct it = post_intersect(s, &ray, &h);
+    if (bounce == 0) {  /* emitted radiance at the first hit (pt.cpp:188-194) */
+      const int li = s->prim_light[s->tri_prim[h.id]];
+      if (li >= 0 && vdot(it.ng, it.wr) > 0)
+        L = vadd(L, vmul(beta, s->lights[li].radiance));
+    }
     uint32_t mi = s->prims[s->tri_prim[h.id]].material;
     bsdf b = make_bsdf(&s->mats[mi], it.ng, it.ns);
     v3 direct = V3(0, 0, 0);
