@@ -33,8 +33,9 @@ struct ajg_scene {
   uint32_t n_mats = 0;
   ajg::DevLight lights[ajg::kMaxLights];
   uint32_t n_lights = 0;
+  uint32_t n_area = 0;               // lights[0, n_area): emissive primitives
   int env_index = -1;
-  std::vector<void *> env_bufs;
+  std::vector<void *> light_bufs;    // env tables, area-light triangles
   float world[6] = {0, 0, 0, 0, 0, 0};
   float radius = 0.f;
   int cu_count = 256;
@@ -62,7 +63,12 @@ int upload_and_build(ajg_ctx *ctx, const std::vector<float> &tri, const float *b
 // tri_shade flags
 static constexpr uint32_t kKindShift = 28;  // 2 bits
 static constexpr uint32_t kTwoSidedBit = 1u << 27;
-static constexpr uint32_t kMatMask = (1u << 27) - 1;
+static constexpr uint32_t kLightShift = 24;  // 3 bits: area light index + 1 (0: none)
+static constexpr uint32_t kMatMask = (1u << 24) - 1;
+
+// Closest-queue entries per path slot: the extension ray and, in scenes with
+// area lights, the MIS-BSDF ray (which must find the emitter it hits).
+static constexpr size_t kCqPerSlot = 2;
 
 // ------------------------------------------------------------ env tables
 // EnvLight constructor weights (env_light.cpp:33-45): luminance of the
@@ -133,6 +139,7 @@ struct DevScene {
   const DevMaterial *mats;
   DevLight lights[kMaxLights];
   uint32_t n_lights;
+  uint32_t n_area;             // area lights (lights[0, n_area))
   int env_index;
   float radius;
 };
@@ -169,6 +176,7 @@ static constexpr uint32_t kPendLast = 4u;
 static constexpr uint32_t kPendMis = 8u;
 static constexpr uint32_t kPendHist = 16u;    // guided: the vertex has a history entry
 static constexpr uint32_t kPendLight0 = 32u;  // + k
+static constexpr uint32_t kPendMisClosest = 1u << 12;  // the MIS-BSDF ray is in the closest queue
 
 struct PathState {
   uint32_t *work;       // current work item (kNone = none)
@@ -183,6 +191,7 @@ struct PathState {
   uint32_t *pend_flags, *pend_mis, *pend_shadow;  // shadow: kMaxLights per slot
   uint32_t *pend_work, *pend_sample, *pend_bank;
   float *pend_L, *pend_beta, *pend_c, *pend_f, *pend_pdf, *pend_ns;
+  float *pend_pb;       // the BSDF's own pdf of the MIS-BSDF direction
   // accumulators: 2 banks x {sum3, comp3, sq3, compsq3}
   float *acc;
   uint32_t *acc_n;      // 2 banks
@@ -299,6 +308,7 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
 struct Inct {
   f3 pos, ng, ns, wr;
   uint32_t material;
+  uint32_t light;  // area light index + 1 of an emissive primitive, else 0
 };
 
 // MT normal + PostIntersect (ploc_aggregate.cpp:199-211; quad.cpp:73-90,
@@ -326,6 +336,7 @@ __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay 
   const uint4 info = shade[ls];
   const uint32_t kind = info.x >> kKindShift;
   it.material = (I && I->info.y != kKeepMaterial) ? I->info.y : (info.x & kMatMask);
+  it.light = (info.x >> kLightShift) & 7u;
   it.pos = add(r.o, scale(r.d, t));
   it.wr = neg(r.d);
   if (kind == AJG_GEOM_CUBE) {
@@ -375,7 +386,8 @@ __device__ __forceinline__ f3 bsdf_eval_sampled(const Bsdf &b, f3 wi, f3 wo) {
 
 struct GSample {
   f3 dir, f;
-  float pdf;
+  float pdf;  // density of the mixture the direction was drawn from
+  float pb;   // the BSDF's own pdf of the direction
   bool ok;
 };
 
@@ -401,6 +413,7 @@ __device__ __forceinline__ GSample guided_sample(const Bsdf &bs, f3 wr, const _F
   if (!g.ok) return g;
   g.f = bsdf_eval_sampled(bs, g.dir, wr);
   const float pb = bsdf_pdf_all(bs, g.dir, wr);
+  g.pb = pb;
   g.pdf = alpha > 0.f ? alpha * gmix_pdf(gm, g.dir) + (1.f - alpha) * pb : pb;
   g.ok = g.pdf > 0.f && isfinite(g.pdf) && !is_black(g.f);
   return g;
@@ -984,14 +997,41 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
       }
       f3 mis = mk3(0.f, 0.f, 0.f);
       if (pf & kPendMis) {
-        // The MIS-BSDF ray contributes only when it escapes to the
-        // environment (the scene has no emissive primitives, helper.h:248),
-        // so it is traced as an any-hit query: it escapes iff the
-        // reference's closest-hit query misses.
+        // MISSampleBsdf (helper.h:208-262). Without area lights the ray
+        // contributes only when it escapes to the environment, so it is
+        // traced as an any-hit query (it escapes iff the reference's
+        // closest-hit query misses); with area lights it is a closest-hit
+        // query, and a hit on the emitting side of an emissive primitive
+        // adds that light.
         const uint32_t mi = ps.pend_mis[slot];
-        if (!q.a_occ_in[mi] && sc.env_index >= 0) {
+        const bool closest = (pf & kPendMisClosest) != 0;
+        const DevRay mr = load_ray(closest ? q.c_rays_in : q.a_rays_in, mi);
+        float4 mh = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (closest) mh = q.c_hits_in[mi];
+        const bool escaped = closest ? __float_as_int(mh.y) < 0 : !q.a_occ_in[mi];
+        if (!escaped && closest) {
+          const Inct hi = post_intersect(sc, mr, mh.x, __float_as_int(mh.y), mh.z, mh.w,
+                                         sc.inst ? q.c_inst_in[mi] : 0u);
+          if (hi.light && dot(hi.ng, hi.wr) > 0) {  // AreaLight::Radiance
+            const DevLight &l = sc.lights[hi.light - 1];
+            if (!is_black(l.radiance)) {
+              const f3 f = scale(mul(l.radiance, ld_f3(ps.pend_f, slot)),
+                                 fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
+              const float pdf = ps.pend_pdf[slot];
+              const float lpdf = area_pdf(l, mr.o, hi.pos, hi.ng);
+              // guided: the weight keeps the BSDF's own pdf (the area
+              // light's pdf is not a density of the same measure, so the
+              // two weights do not sum to one; keeping the plain path's
+              // weight functions keeps the value the estimator converges to)
+              const float wb = power_heuristic(GUIDED ? ps.pend_pb[slot] : pdf, lpdf);
+              mis = scale(divs(f, pdf), wb);
+              if (train)
+                emit_record(G, tpos, tnrm, mr.d, lum_f(l.radiance) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
+            }
+          }
+        }
+        if (escaped && sc.env_index >= 0) {
           const DevEnv &e = sc.lights[sc.env_index].env;
-          const DevRay mr = load_ray(q.a_rays_in, mi);
           const f3 le = env_radiance(e, mr.d);
           if (!is_black(le)) {
             const f3 f = scale(mul(le, ld_f3(ps.pend_f, slot)),
@@ -1067,6 +1107,9 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
         // value the render converges to. Disney vertices therefore sample
         // the BSDF alone (alpha 0), exactly as the plain path does.
         const float alpha = (GUIDED && bs.m.type != AJG_MAT_DISNEY) ? G.alpha : 0.f;
+        // emitted radiance at the first hit (pt.cpp:188-194)
+        if (b == 0 && it.light && dot(it.ng, it.wr) > 0)
+          L = add(L, mul(beta, sc.lights[it.light - 1].radiance));
         uint32_t pf2 = kPendActive;
         st_f3(ps.pend_L, slot, L);
         st_f3(ps.pend_beta, slot, beta);
@@ -1086,6 +1129,12 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
             const float first = pcg_next_float(rng);
             const float second = pcg_next_float(rng);
             lsmp = env_sample(l.env, it.pos, second, first, sc.radius);
+          } else if (l.type == kLightArea) {
+            f3 sam;
+            sam.x = pcg_next_float(rng);
+            sam.y = pcg_next_float(rng);
+            sam.z = pcg_next_float(rng);
+            lsmp = area_sample(l, it.pos, sam);
           } else {
             const float u0 = pcg_next_float(rng);
             const float u1 = pcg_next_float(rng);
@@ -1099,8 +1148,11 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
               const f3 to_light = normalize(sub(lsmp.pos, it.pos));
               const f3 bf = bsdf_eval_all(bs, to_light, it.wr);
               if (!is_black(bf)) {
+                // MISSampleAreaLight takes the shading normal as it is,
+                // the other lights normalise it (helper.h:64, 92, 125)
                 const f3 nrm = normalize(it.ns);
-                const f3 f = scale(mul(lsmp.radiance, bf), fabsf(dot(to_light, nrm)));
+                const f3 f = scale(mul(lsmp.radiance, bf),
+                                   fabsf(dot(to_light, l.type == kLightArea ? it.ns : nrm)));
                 float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
                 // guided: the env light's MIS partner is the mixture density
                 // (a delta light is never reached by directional sampling, so
@@ -1112,7 +1164,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
                 st_f3(ps.pend_c, slot * kMaxLights + k, c);
                 if (train)
                   G.pend_tw[(size_t)slot * kMaxLights + k] =
-                      l.type == AJG_LIGHT_ENV
+                      l.type != AJG_LIGHT_DIRECTIONAL
                           ? lum_f(lsmp.radiance) * fabsf(dot(to_light, nrm)) / lsmp.pdf * wl
                           : 0.f;
                 sr = make_ray(it.pos, to_light, AJG_EPS, 0.99f * len);
@@ -1132,12 +1184,14 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
         {
           bool emit = false;
           DevRay mr;
+          const bool lit = sc.env_index >= 0 || sc.n_area > 0;
           if (GUIDED) {
             const GSample g = guided_sample(bs, it.wr, gm, alpha, rng);
-            if (g.ok && sc.env_index >= 0) {
+            if (g.ok && lit) {
               mr = make_ray(eps_offset(it, g.dir), g.dir, AJG_EPS, __int_as_float(0x7f800000));
               st_f3(ps.pend_f, slot, g.f);
               ps.pend_pdf[slot] = g.pdf;
+              ps.pend_pb[slot] = g.pb;
               st_f3(ps.pend_ns, slot, it.ns);
               emit = true;
             }
@@ -1147,7 +1201,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
             sam.y = pcg_next_float(rng);
             sam.z = pcg_next_float(rng);
             BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
-            if (s.valid && !(is_black(s.f) || s.pdf == 0.f) && sc.env_index >= 0) {
+            if (s.valid && !(is_black(s.f) || s.pdf == 0.f) && lit) {
               const f3 dn = normalize(s.dir);
               mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
               st_f3(ps.pend_f, slot, s.f);
@@ -1156,11 +1210,19 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
               emit = true;
             }
           }
-          const uint32_t mi = wave_append(emit, q.a_count_out);
-          if (emit) {
-            store_ray(q.a_rays_out, mi, mr);
-            if (BATCH) B.a_job_out[mi] = (uint16_t)j;
-            ps.pend_mis[slot] = mi;
+          // with area lights: a closest-hit query (the emitter it hits)
+          const bool to_c = emit && sc.n_area > 0;
+          const uint32_t mc = wave_append(to_c, q.c_count_out);
+          const uint32_t ma = wave_append(emit && !to_c, q.a_count_out);
+          if (to_c) {
+            store_ray(q.c_rays_out, mc, mr);
+            if (BATCH) B.c_job_out[mc] = (uint16_t)j;
+            ps.pend_mis[slot] = mc;
+            pf2 |= kPendMis | kPendMisClosest;
+          } else if (emit) {
+            store_ray(q.a_rays_out, ma, mr);
+            if (BATCH) B.a_job_out[ma] = (uint16_t)j;
+            ps.pend_mis[slot] = ma;
             pf2 |= kPendMis;
           }
         }
@@ -1471,8 +1533,8 @@ static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint
     return r;
   };
   int st = 0;
-  if ((st = al((void **)&gb->pts, 24 * (size_t)S, "guide hit points")) ||
-      (st = al((void **)&gb->mix, 2 * (size_t)kGMix * S, "guide mixtures")) ||
+  if ((st = al((void **)&gb->pts, 24 * kCqPerSlot * S, "guide hit points")) ||
+      (st = al((void **)&gb->mix, 2 * (size_t)kGMix * kCqPerSlot * S, "guide mixtures")) ||
       (st = al((void **)&gb->pend_pos, 12 * (size_t)S, "guide pending positions")) ||
       (st = al((void **)&gb->pend_tw, 4 * (size_t)kMaxLights * S, "guide NEE weights")) ||
       (st = al((void **)&gb->hist, 4 * (size_t)kGHist * T * D, "guide vertex history")) ||
@@ -1497,7 +1559,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
     return r;
   };
   PathState &ps = rb->ps;
-  const size_t cq_cap = (size_t)S, aq_cap = (size_t)(kMaxLights + 1) * S;
+  const size_t cq_cap = kCqPerSlot * S, aq_cap = (size_t)(kMaxLights + 1) * S;
   int st = 0;
   if ((st = al((void **)&ps.work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.sample, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.sample_end, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bounce, 4 * (size_t)S, "state")) ||
@@ -1511,6 +1573,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&ps.pend_beta, 12 * (size_t)S, "state")) ||
       (st = al((void **)&ps.pend_c, 12 * (size_t)S * kMaxLights, "state")) ||
       (st = al((void **)&ps.pend_f, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_pdf, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.pend_pb, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.pend_ns, 12 * (size_t)S, "state")) || (st = al((void **)&ps.acc, 4 * 24 * (size_t)S, "state")) ||
       (st = al((void **)&ps.acc_n, 8 * (size_t)S, "state")) ||
       (st = al((void **)&ps.job, 4 * (size_t)S, "state")) ||
@@ -1545,7 +1608,7 @@ static void scene_free(ajg_scene *s) {
   (void)hipFree(s->tri_shade);
   (void)hipFree(s->normals);
   (void)hipFree(s->mats);
-  for (void *p : s->env_bufs) (void)hipFree(p);
+  for (void *p : s->light_bufs) (void)hipFree(p);
   delete (RenderBuffers *)s->render_cache;
   delete (GuideBuffers *)s->guide_cache;
   for (ajg_accel *b : s->blas) destroy_accel(b);
@@ -1558,12 +1621,24 @@ static void scene_free(ajg_scene *s) {
 
 // Argument checks shared by one-level and instanced scenes; counts the
 // mesh normals the primitives bring.
+// An emissive primitive (non-black emission, GeometricPrimitive,
+// geometric.cpp:35-51) carries an area light.
+static bool emissive(const ajg_primitive &p) {
+  return std::max(std::max(p.emission[0], p.emission[1]), p.emission[2]) > 0.f;
+}
+
 static int check_scene_inputs(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
                               const ajg_material *mats, uint32_t n_mats, const ajg_light *lights,
-                              uint32_t n_lights, size_t *n_normals) {
-  if (n_lights > (uint32_t)kMaxLights)
-    return fail(ctx, AJG_ERR_INVALID, "scene: at most 4 lights are supported");
+                              uint32_t n_lights, size_t *n_normals, bool allow_emission = true) {
+  uint32_t n_area = 0;
+  for (uint32_t i = 0; i < n_prims; ++i) n_area += emissive(prims[i]) ? 1u : 0u;
+  if (n_area && !allow_emission)
+    return fail(ctx, AJG_ERR_INVALID, "instanced scene: emissive meshes are not supported");
+  if (n_lights + n_area > (uint32_t)kMaxLights)
+    return fail(ctx, AJG_ERR_INVALID,
+                "scene: at most 4 lights (emissive primitives included) are supported");
   if (n_mats == 0) return fail(ctx, AJG_ERR_INVALID, "scene: no materials");
+  if (n_mats > kMatMask + 1) return fail(ctx, AJG_ERR_INVALID, "scene: too many materials");
   for (uint32_t i = 0; i < n_mats; ++i)
     if (mats[i].type != AJG_MAT_DIFFUSE && mats[i].type != AJG_MAT_DISNEY)
       return fail(ctx, AJG_ERR_INVALID, "scene: unsupported material type");
@@ -1577,6 +1652,15 @@ static int check_scene_inputs(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t
       if (!prims[i].normal_and_vs)
         return fail(ctx, AJG_ERR_INVALID, "scene: mesh without normal_and_vs");
       *n_normals += prims[i].view.position_size;
+    }
+    if (emissive(prims[i])) {
+      const ajg_mesh_view &v = prims[i].view;
+      if (v.index_size == 0 || !v.index_buffer || !v.position_buffer)
+        return fail(ctx, AJG_ERR_INVALID, "scene: emissive primitive without triangles");
+      if (prims[i].kind == AJG_GEOM_QUAD && (v.position_size < 4 || v.index_size != 2))
+        return fail(ctx, AJG_ERR_INVALID, "scene: emissive quad needs 4 positions, 2 triangles");
+      if (prims[i].kind == AJG_GEOM_CUBE && v.index_size != 12)
+        return fail(ctx, AJG_ERR_INVALID, "scene: emissive cube needs 12 triangles");
     }
   }
   int env_count = 0;
@@ -1599,11 +1683,14 @@ static int check_scene_inputs(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t
 // shared normal array, whose mesh normals are appended to `normals`).
 static void shade_records(const ajg_primitive *prims, uint32_t p0, uint32_t p1,
                           std::vector<uint4> *by_id, std::vector<float4> *normals) {
+  uint32_t area = 0;  // emissive primitives before p0 (none in instanced scenes)
+  for (uint32_t p = 0; p < p0; ++p) area += emissive(prims[p]) ? 1u : 0u;
   for (uint32_t p = p0; p < p1; ++p) {
     const ajg_primitive &pr = prims[p];
     const size_t noff = normals->size();
+    const uint32_t light = emissive(pr) ? ++area : 0u;  // light index + 1
     const uint32_t flags = (pr.kind << kKindShift) | (pr.two_sided ? kTwoSidedBit : 0u) |
-                           (pr.material & kMatMask);
+                           (light << kLightShift) | (pr.material & kMatMask);
     for (size_t i = 0; i < pr.view.index_size; ++i) {
       uint4 rec = make_uint4(flags, 0, 0, 0);
       if (pr.kind == AJG_GEOM_MESH) {
@@ -1652,11 +1739,91 @@ static void set_world(ajg_scene *s, const float *box) {
   s->radius = length(sub(wb.mx, box_center(wb)));
 }
 
+static f3 view_position(const ajg_mesh_view &v, size_t i) {
+  float p[3];
+  memcpy(p, (const char *)v.position_buffer + v.position_offset + i * v.position_stride, 12);
+  return mk3(p[0], p[1], p[2]);
+}
+static uint32_t view_word(const ajg_mesh_view &v, size_t tri, int k) {
+  uint32_t w;
+  memcpy(&w, (const char *)v.index_buffer + v.index_offset + tri * v.index_stride + 4 * k, 4);
+  return w;
+}
+
+// AreaLight of an emissive primitive: the triangles its geometry's Sample
+// draws from, in the order it indexes them, and the areas its pdfs use
+// (Quad ctor, quad.cpp:35-50; Cube, cube.cpp:31-75 with the tuple words
+// stored in reverse; MeshGeo ctor surface area, mesh.cpp:53-70, accumulated
+// in a float with a double addend). `flat` holds the primitive's triangles
+// in index-word order.
+static int area_light_init(ajg_ctx *ctx, ajg_scene *s, DevLight *l, const ajg_primitive &pr,
+                           const float *flat) {
+  memset(l, 0, sizeof(*l));
+  l->type = kLightArea;
+  l->radiance = ld3(pr.emission);
+  l->kind = pr.kind;
+  l->two_sided = pr.two_sided ? 1u : 0u;
+  const ajg_mesh_view &v = pr.view;
+  const uint32_t n = (uint32_t)v.index_size;
+  std::vector<float> st;
+  auto push = [&](f3 p) {
+    st.push_back(p.x);
+    st.push_back(p.y);
+    st.push_back(p.z);
+  };
+  if (pr.kind == AJG_GEOM_QUAD) {
+    const f3 a = view_position(v, 0), b = view_position(v, 1), c = view_position(v, 2),
+             d = view_position(v, 3);
+    push(a), push(b), push(c), push(a), push(c), push(d);
+    const float abc = length(cross(sub(b, a), sub(c, a))) * 0.5f;
+    const float acd = length(cross(sub(c, a), sub(d, a))) * 0.5f;
+    l->area = abc + acd;
+    l->prob = abc / l->area;
+    l->n_stri = 2;
+  } else if (pr.kind == AJG_GEOM_CUBE) {
+    for (uint32_t t = 0; t < n; ++t)
+      for (int k = 0; k < 3; ++k) push(ld3(flat + 9 * (size_t)t + 3 * (2 - k)));
+    const f3 a = ld3(st.data()), b = ld3(st.data() + 3), c = ld3(st.data() + 6);
+    l->cube_pdf0 = 1.f / (12.f * length(cross(sub(b, a), sub(c, a))) * 0.5f);
+    l->n_stri = n;
+  } else {
+    for (uint32_t t = 0; t < n; ++t)
+      for (uint32_t k = 0; k < 3; ++k) {
+        const uint32_t f = t + k;  // the index array read at the triangle number
+        const uint32_t w = view_word(v, f / 3, (int)(f % 3));
+        if (w >= v.position_size)
+          return fail(ctx, AJG_ERR_INVALID, "scene: emissive mesh index out of range");
+        push(view_position(v, w));
+      }
+    float area = 0.f;
+    for (uint32_t t = 0; t < n; ++t) {
+      const float *p = flat + 9 * (size_t)t;
+      const f3 e0 = sub(ld3(p + 3), ld3(p)), e1 = sub(ld3(p + 6), ld3(p));
+      area = (float)((double)area + 0.5 * (double)length(cross(e0, e1)));
+    }
+    l->area = area;
+    l->n_stri = n;
+  }
+  float *d = nullptr;
+  if (int r = dev_alloc(ctx, (void **)&d, sizeof(float) * st.size(), "area light triangles")) return r;
+  s->light_bufs.push_back(d);
+  const hipError_t e = hipMemcpy(d, st.data(), sizeof(float) * st.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "area light upload");
+  l->stris = d;
+  return AJG_OK;
+}
+
 // Materials and lights of a scene whose geometry is in place (the env
 // Distribution2D tables are built on the device, EnvLight ctor,
-// env_light.cpp:29-48).
+// env_light.cpp:29-48). The light list is DefaultScene's
+// (default_scene.cpp:34-48): the emissive primitives' area lights in
+// primitive order, then the scene's lights. `flat` holds the primitives'
+// triangles in flatten order (one-level scenes; instanced scenes have no
+// emissive primitives).
 static int finish_scene(ajg_ctx *ctx, ajg_scene *s, const ajg_material *mats, uint32_t n_mats,
-                        const ajg_light *lights, uint32_t n_lights, int32_t env_cdf_fp64) {
+                        const ajg_light *lights_in, uint32_t n_lights_in, int32_t env_cdf_fp64,
+                        const ajg_primitive *prims = nullptr, uint32_t n_prims = 0,
+                        const float *flat = nullptr) {
   std::vector<DevMaterial> dm(n_mats);
   for (uint32_t i = 0; i < n_mats; ++i) {
     const ajg_material &m = mats[i];
@@ -1678,24 +1845,33 @@ static int finish_scene(ajg_ctx *ctx, ajg_scene *s, const ajg_material *mats, ui
   hipError_t e = hipMemcpy(s->mats, dm.data(), sizeof(DevMaterial) * n_mats, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(ctx, e, "material upload");
   s->n_mats = n_mats;
+  s->n_area = 0;
+  for (uint32_t p = 0, t0 = 0; p < n_prims; t0 += (uint32_t)prims[p].view.index_size, ++p) {
+    if (!emissive(prims[p])) continue;
+    if ((st = area_light_init(ctx, s, &s->lights[s->n_area], prims[p], flat + 9 * (size_t)t0)))
+      return st;
+    ++s->n_area;
+  }
+  const uint32_t n_lights = s->n_area + n_lights_in;
   s->n_lights = n_lights;
-  for (uint32_t i = 0; i < n_lights; ++i) {
+  for (uint32_t i = s->n_area; i < n_lights; ++i) {
     DevLight &l = s->lights[i];
+    const ajg_light &src = lights_in[i - s->n_area];
     memset(&l, 0, sizeof(l));
-    l.type = lights[i].type;
+    l.type = src.type;
     if (l.type == AJG_LIGHT_DIRECTIONAL) {
-      const f3 d = normalize(ld3(lights[i].dir));
+      const f3 d = normalize(ld3(src.dir));
       l.fr = make_frame(d);
-      l.angle_cos = std::cos(lights[i].angle);
-      l.radiance = ld3(lights[i].radiance);
+      l.angle_cos = std::cos(src.angle);
+      l.radiance = ld3(src.radiance);
       continue;
     }
     s->env_index = (int)i;
-    const uint32_t w = lights[i].env_w, h = lights[i].env_h;
+    const uint32_t w = src.env_w, h = src.env_h;
     float *tex, *wts, *cf, *cc, *ci, *mf, *mc, *mi;
     auto al = [&](float **p, size_t cnt, const char *what) {
       int r = dev_alloc(ctx, (void **)p, sizeof(float) * cnt, what);
-      if (r == AJG_OK) s->env_bufs.push_back(*p);
+      if (r == AJG_OK) s->light_bufs.push_back(*p);
       return r;
     };
     if ((st = al(&tex, 3 * (size_t)w * h, "env texels")) || (st = al(&wts, (size_t)w * h, "env weights")) ||
@@ -1703,7 +1879,7 @@ static int finish_scene(ajg_ctx *ctx, ajg_scene *s, const ajg_material *mats, ui
         (st = al(&ci, h, "env cond int")) || (st = al(&mf, h, "env marg func")) ||
         (st = al(&mc, h + 1, "env marg cdf")) || (st = al(&mi, 1, "env marg int")))
       return st;
-    e = hipMemcpy(tex, lights[i].env_texels, sizeof(float) * 3 * (size_t)w * h, hipMemcpyHostToDevice);
+    e = hipMemcpy(tex, src.env_texels, sizeof(float) * 3 * (size_t)w * h, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(ctx, e, "env upload");
     l.env.w = w;
     l.env.h = h;
@@ -1713,8 +1889,8 @@ static int finish_scene(ajg_ctx *ctx, ajg_scene *s, const ajg_material *mats, ui
     l.env.cond_int = ci;
     l.env.marg_cdf = mc;
     l.env.marg_func = mf;
-    l.env.rot = lights[i].rot;
-    l.env.scaler = lights[i].scaler;
+    l.env.rot = src.rot;
+    l.env.scaler = src.scaler;
     hipLaunchKernelGGL(k_env_weights, dim3(cdiv((uint64_t)w * h, 256)), dim3(256), 0, nullptr,
                        l.env, wts);
     hipLaunchKernelGGL(k_env_rows, dim3(cdiv(h, 64)), dim3(64), 0, nullptr, (const float *)wts, w, h,
@@ -1832,7 +2008,9 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
   if (e == hipSuccess)
     e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(hip_fail(ctx, e, "scene upload"));
-  if ((st = finish_scene(ctx, s, mats, n_mats, lights, n_lights, env_cdf_fp64))) return bail(st);
+  if ((st = finish_scene(ctx, s, mats, n_mats, lights, n_lights, env_cdf_fp64, prims, n_prims,
+                          tri.data())))
+    return bail(st);
   *out = s;
   return AJG_OK;
 }
@@ -1848,7 +2026,8 @@ int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes, uint32
   *out = nullptr;
   if (int st = injected(ctx)) return st;
   size_t n_normals = 0;
-  if (int st = check_scene_inputs(ctx, meshes, n_meshes, mats, n_mats, lights, n_lights, &n_normals))
+  if (int st = check_scene_inputs(ctx, meshes, n_meshes, mats, n_mats, lights, n_lights, &n_normals,
+                                  false))
     return st;
   for (uint32_t i = 0; i < n_instances; ++i) {
     if (instances[i].mesh >= n_meshes)
@@ -2015,7 +2194,9 @@ int ajg_scene_get_accel(ajg_scene *scene, ajg_accel **out) {
 
 int ajg_scene_env_tables(ajg_scene *s, uint32_t li, float *cond_cdf, float *cond_func_int,
                          float *marg_cdf, float *marg_func_int) {
-  if (!s || li >= s->n_lights || s->lights[li].type != AJG_LIGHT_ENV) return AJG_ERR_INVALID;
+  if (!s || li >= s->n_lights - s->n_area) return AJG_ERR_INVALID;
+  li += s->n_area;  // the scene's lights follow the emissive primitives' area lights
+  if (s->lights[li].type != AJG_LIGHT_ENV) return AJG_ERR_INVALID;
   const DevEnv &e = s->lights[li].env;
   hipError_t r = hipSuccess;
   if (cond_cdf) r = hipMemcpy(cond_cdf, e.cond_cdf, sizeof(float) * (e.w + 1) * e.h, hipMemcpyDeviceToHost);
@@ -2093,6 +2274,7 @@ static DevScene dev_scene(const ajg_scene *sc) {
   ds.mats = sc->mats;
   for (int i = 0; i < kMaxLights; ++i) ds.lights[i] = sc->lights[i];
   ds.n_lights = sc->n_lights;
+  ds.n_area = sc->n_area;
   ds.env_index = sc->env_index;
   ds.radius = sc->radius;
   ds.inst = sc->insts;
@@ -2171,7 +2353,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   }
   Queues q;
   memset(&q, 0, sizeof(q));
-  const size_t cq_cap = (size_t)S, aq_cap = (size_t)(kMaxLights + 1) * S;
+  const size_t cq_cap = kCqPerSlot * S, aq_cap = (size_t)(kMaxLights + 1) * S;
   DevRay **c_rays = rb->c_rays, **a_rays = rb->a_rays;
   float4 *c_hits = rb->c_hits;
   uint8_t *a_occ = rb->a_occ;
@@ -2307,7 +2489,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
         hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(S, 256), 4096)), dim3(256), 0, s,
                            ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
                            (const uint32_t *)rb->c_inst, (const uint32_t *)(counters + nxt), gb->pts);
-        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, S, counters + nxt, 6, nullptr,
+        const uint32_t n_cq = ds.n_area ? (uint32_t)(kCqPerSlot * S) : S;
+        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, n_cq, counters + nxt, 6, nullptr,
                                           gb->mix, nullptr, s))
           return done(st);
         timer.end(3, s);
@@ -2446,8 +2629,8 @@ static int ensure_batch_buffers(ajg_ctx *ctx, BatchBuffers *bb, uint32_t S, uint
   };
   const size_t aq = (size_t)(kMaxLights + 1) * S;
   int st = 0;
-  if ((st = al((void **)&bb->c_job[0], 2 * (size_t)S, "batch ray jobs")) ||
-      (st = al((void **)&bb->c_job[1], 2 * (size_t)S, "batch ray jobs")) ||
+  if ((st = al((void **)&bb->c_job[0], 2 * kCqPerSlot * S, "batch ray jobs")) ||
+      (st = al((void **)&bb->c_job[1], 2 * kCqPerSlot * S, "batch ray jobs")) ||
       (st = al((void **)&bb->a_job[0], 2 * aq, "batch ray jobs")) ||
       (st = al((void **)&bb->a_job[1], 2 * aq, "batch ray jobs")) ||
       (st = al((void **)&bb->jobs, sizeof(BatchJob) * (size_t)n_jobs, "batch job table")) ||
@@ -2589,7 +2772,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   AJG_BCHK(hipMemsetAsync(rb->partial_n, 0, 4 * total, s));
   AJG_BCHK(hipMemsetAsync(rb->counters, 0, 4 * kCounterWords, s));
   AJG_BCHK(hipMemsetAsync(rb->a_occ, 0, (size_t)(kMaxLights + 1) * S, s));
-  AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * (size_t)S, s));
+  AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * kCqPerSlot * S, s));
   for (uint32_t j = 0; j < n_jobs; ++j) {
     const ajg_film &f = jobs[j].film;
     const size_t np = hj[j].P.n_pix;

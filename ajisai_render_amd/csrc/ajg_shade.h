@@ -518,12 +518,23 @@ struct DevEnv {
   float rot, scaler;
 };
 
+// Light type of an emissive primitive's AreaLight (the public light types
+// are directional and environment).
+static constexpr uint32_t kLightArea = 2;
+
 struct DevLight {
   uint32_t type;
   f3 radiance;
   Frame fr;
   float angle_cos;
   DevEnv env;
+  // area light: the triangles its geometry's Sample draws from (9 floats
+  // each, in the order Sample indexes them), the geometry kind, the TwoSided
+  // wrapper, surface area, the quad's first-triangle probability and the
+  // cube's pdf of its first triangle (what Cube::Pdf returns)
+  const float *stris;
+  uint32_t kind, two_sided, n_stri;
+  float area, prob, cube_pdf0;
 };
 
 static constexpr int kMaxLights = 4;
@@ -659,6 +670,67 @@ __device__ __forceinline__ LightSample dir_sample(const DevLight &l, f3 ref, flo
   r.radiance = l.radiance;
   r.pdf = uniform_cone_pdf(l.angle_cos);
   return r;
+}
+
+// AreaLight::Sample (area_light.cpp:37-58) over its geometry's Sample
+// (quad.cpp:129-160, cube.cpp:145-172, mesh.cpp:251-282) and the TwoSided
+// wrapper (two_sided.cpp:62-99), as the reference writes them: the mesh
+// draws the vertices indices[t], [t+1], [t+2] for triangle number t (the
+// triangles in `stris` are those), the cube reports the pdf of the triangle
+// it picked, and the solid-angle pdf divides dist^2 by |n . (ref - pos)|.
+// A sample whose emitting side faces away from ref has pdf 0.
+__device__ __forceinline__ LightSample area_sample(const DevLight &l, f3 ref, f3 sam) {
+  bool back = false;
+  if (l.two_sided) {
+    back = sam.x < 0.5f;
+    sam.x = back ? 2 * sam.x : 2 * (sam.x - 0.5f);
+  }
+  float sx = sam.x, sy = sam.y;
+  if (sx + sy > 1.f) {
+    sx = 1.f - sx;
+    sy = 1.f - sy;
+  }
+  const float *t;
+  bool cube_pdf = false;
+  if (l.kind == AJG_GEOM_QUAD) {
+    t = sam.z < l.prob ? l.stris : l.stris + 9;
+  } else {
+    const int cnt = (int)l.n_stri;
+    const int idx = min((int)(sam.z * (float)cnt), cnt - 1);
+    t = l.stris + 9 * (size_t)idx;
+    cube_pdf = l.kind == AJG_GEOM_CUBE;
+  }
+  const f3 a = mk3(t[0], t[1], t[2]), b = mk3(t[3], t[4], t[5]), c = mk3(t[6], t[7], t[8]);
+  float pdf = cube_pdf ? 1.f / (12.f * length(cross(sub(b, a), sub(c, a))) * 0.5f) : 1.f / l.area;
+  LightSample r;
+  r.pos = add(add(scale(a, (1 - sx) - sy), scale(b, sx)), scale(c, sy));
+  f3 ng = normalize(cross(sub(b, a), sub(c, a)));
+  if (l.two_sided) {
+    pdf = pdf * 0.5f;
+    if (back) ng = scale(ng, -1.f);
+  }
+  const f3 i2r = sub(ref, r.pos);
+  if (dot(ng, i2r) <= 0) {
+    r.radiance = mk3(0.f, 0.f, 0.f);
+    r.pdf = 0.f;
+    return r;
+  }
+  const float dist2 = dot(i2r, i2r);
+  r.pdf = pdf * dist2 / fabsf(dot(ng, i2r));
+  r.radiance = l.radiance;
+  return r;
+}
+
+// AreaLight::Pdf (area_light.cpp:67-78) with Geometry::Pdf (quad.cpp:163-166,
+// cube.cpp:175-183, mesh.cpp:290-293, two_sided.cpp:107-115).
+__device__ __forceinline__ float area_pdf(const DevLight &l, f3 ref, f3 pos, f3 n) {
+  if (dot(sub(ref, pos), n) <= 0.f) return 0.f;
+  float apdf = l.kind == AJG_GEOM_CUBE ? l.cube_pdf0 : 1.f / l.area;
+  if (l.two_sided) apdf = apdf * 0.5f;
+  const f3 l2r = sub(ref, pos);
+  const float dist2 = dot(l2r, l2r);
+  const float abscos = fabsf(dot(l2r, n));
+  return apdf * (dist2 / abscos);
 }
 
 // ---------------------------------------------------------------- camera

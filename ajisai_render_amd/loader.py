@@ -4,12 +4,13 @@ Mirrors the reference's factory (ajisai/factory/factory.h:96-115: every node
 is created by its "type" key) and creators (factory/creator/*.cpp) for what
 the GPU path renders: the root keys "scene", "camera", "renderer"
 (app/main.cpp:74-80); primitives of type "geometry" with "quad", "cube",
-"mesh" (OBJ) and "twosided" geometries; "diffuse" and "disney" materials with
-"constant" textures; "directional" and "envmap" lights ("constant" or "hdr"
-textures, .pfm or Radiance .hdr files); the "thin_lens" camera with "box" or
-"gaussian" filters; the "pt" renderer; the "ploc" aggregate. Keys and
-defaults are the creators'. Anything else (spheres, emission, media, other
-materials, textures, integrators) raises SceneError naming it.
+"mesh" (OBJ) and "twosided" geometries, emissive ones ("emission", an area
+light, primitive_creators.cpp:65-72) included; "diffuse" and "disney"
+materials with "constant" textures; "directional" and "envmap" lights
+("constant" or "hdr" textures, .pfm or Radiance .hdr files); the "thin_lens"
+camera with "box" or "gaussian" filters; the "pt" renderer; the "ploc"
+aggregate. Keys and defaults are the creators'. Anything else (spheres, media,
+other materials, textures, integrators) raises SceneError naming it.
 
 OBJ files are re-indexed as MeshGeoCreatorImpl does (geometry_creators.cpp:
 104-240): polygons fan-triangulated, one vertex per distinct (position,
@@ -352,15 +353,15 @@ def load(source, base_dir: Optional[str] = None) -> LoadedScene:
     for p in sc_node.get("primitives", []) or []:
         if p.get("type") != "geometry":
             raise SceneError(f"primitive type '{p.get('type')}' is not supported (geometry)")
-        emission = _vec(p.get("emission"), 3, (0, 0, 0), "primitive.emission")
-        if any(e != 0 for e in emission):
-            raise SceneError("emissive primitives (area lights) are not supported on the GPU path")
+        emission = tuple(float(e) for e in _vec(p.get("emission"), 3, (0, 0, 0),
+                                                  "primitive.emission"))
         if "med_in" in p or "med_out" in p:
             raise SceneError("participating media are not supported on the GPU path")
         if "material" not in p:
             raise SceneError("a primitive without material (null material) is not supported")
         mats.append(_material(p["material"]))
-        prims.append(_geometry(p["geometry"], len(mats) - 1, base_dir))
+        prims.append(dataclasses.replace(_geometry(p["geometry"], len(mats) - 1, base_dir),
+                                         emission=emission))
     lights = [_light(l, base_dir) for l in (sc_node.get("lights", []) or [])]
     agg = sc_node.get("aggregate") or {"type": "ploc"}
     if agg.get("type") != "ploc":

@@ -365,6 +365,58 @@ def cornell(seed: int = 20261203, res=(256, 256), spp=16) -> Scene:
                  PTSpec(spp=spp, tile_size=16, min_bounces=3, max_bounces=8, cont_prob=0.9))
 
 
+def cornell_area(emitters=("quad",), res=(64, 64), spp=4, env=0.0, seed: int = 20261203) -> Scene:
+    """The Cornell box of `cornell` lit by emissive primitives (area lights)
+    instead of the directional light; `env` > 0 adds a constant environment.
+    Emitters (each one primitive, Lambert 0.5 grey underneath):
+      "quad"      ceiling lamp [0.35, 0.65]^2 at y = 0.998, one-sided, facing
+                  down, radiance (12, 10, 8)
+      "twosided"  a vertical two-sided quad near the left wall, radiance 3
+      "mesh"      three separate small triangles above the right box
+                  (vertices 0..8 in order), radiance (2, 6, 3), two-sided.
+                  MeshGeo::Sample reads the index array at the triangle
+                  number, so the reference samples the vertex windows
+                  (0 1 2), (1 2 3), (2 3 4); a window that repeats a
+                  position (a closed mesh sharing vertices) has a NaN normal
+                  and gives NaN samples, which this mesh avoids
+      "cube"      a small cube (side 0.12) on the floor at the front,
+                  radiance (6, 3, 1)"""
+    base = cornell(seed=seed, res=res, spp=spp)
+    prims = list(base.primitives)
+    mats = list(base.materials) + [Material(MAT_DIFFUSE, (0.5, 0.5, 0.5))]
+    lamp_mat = len(mats) - 1
+    for kind in emitters:
+        if kind == "quad":
+            p = quad((0.35, 0.998, 0.35), (0.65, 0.998, 0.35), (0.65, 0.998, 0.65),
+                     (0.35, 0.998, 0.65), lamp_mat, two_sided=False)
+            p.emission = (12.0, 10.0, 8.0)
+        elif kind == "twosided":
+            p = quad((0.08, 0.45, 0.3), (0.08, 0.75, 0.3), (0.08, 0.75, 0.6), (0.08, 0.45, 0.6),
+                     lamp_mat, two_sided=True)
+            p.emission = (3.0, 3.0, 3.0)
+        elif kind == "mesh":
+            v = np.array([[0.60, 0.62, 0.55], [0.70, 0.62, 0.55], [0.65, 0.64, 0.66],
+                          [0.72, 0.66, 0.60], [0.82, 0.64, 0.62], [0.76, 0.70, 0.72],
+                          [0.66, 0.74, 0.70], [0.74, 0.80, 0.74], [0.62, 0.78, 0.80]], np.float32)
+            faces = np.arange(9, dtype=np.uint32).reshape(3, 3)
+            nrm = np.zeros((9, 3), np.float32)
+            for f in faces:
+                n = np.cross(v[f[1]] - v[f[0]], v[f[2]] - v[f[0]])
+                nrm[f] = n / np.linalg.norm(n)
+            pos4 = np.concatenate([v, np.zeros((9, 1), np.float32)], axis=1)
+            nrm4 = np.concatenate([nrm, np.zeros((9, 1), np.float32)], axis=1)
+            p = mesh(pos4, nrm4, faces, lamp_mat, two_sided=True)
+            p.emission = (2.0, 6.0, 3.0)
+        elif kind == "cube":
+            p = cube(translate(0.3, 0.06, 0.85) @ scale3(0.12, 0.12, 0.12), lamp_mat)
+            p.emission = (6.0, 3.0, 1.0)
+        else:
+            raise ValueError(kind)
+        prims.append(p)
+    lights = [Light(LIGHT_ENV, env=constant_env(env), rot=0.0, scaler=1.0)] if env > 0 else []
+    return Scene("cornell_area", prims, mats, lights, base.camera, base.pt)
+
+
 def _disney_from(rng) -> Material:
     return Material(MAT_DISNEY, tuple(float(x) for x in rng.uniform(0, 1, 3)),
                     subsurface=0.0, metallic=float(rng.uniform()), specular=float(rng.uniform()),

@@ -261,8 +261,9 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims,
 int ajg_scene_destroy(ajg_scene *scene);
 /* Borrowed accelerator of the scene (owned by the scene). */
 int ajg_scene_get_accel(ajg_scene *scene, ajg_accel **out);
-/* Env Distribution2D tables of light li as built on the device (host
- * copies): cond_cdf[h][w+1], cond_func_int[h], marg_cdf[h+1], marg_func_int. */
+/* Env Distribution2D tables of light li (an index into the `lights` given to
+ * ajg_scene_create) as built on the device (host copies): cond_cdf[h][w+1],
+ * cond_func_int[h], marg_cdf[h+1], marg_func_int. */
 int ajg_scene_env_tables(ajg_scene *scene, uint32_t li, float *cond_cdf,
                          float *cond_func_int, float *marg_cdf,
                          float *marg_func_int);

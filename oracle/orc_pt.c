@@ -767,8 +767,6 @@ static inline v3 view_pos(const orc_mesh_view *mv, uint32_t i) {
  *    picked triangle, and its Pdf uses the first triangle;
  *  - the solid-angle pdf divides dist^2 by |n . (ref - pos)| (not by the
  *    cosine), here and in Pdf. */
-static float vlen_ref(v3 v) { return sqrtf(vdot(v, v)); }
-
 static void area_light_init(light_rt *l, const orc_primitive *pr, const float *flat) {
   l->type = ORC_LIGHT_AREA;
   l->radiance = vload(pr->emission);
@@ -782,8 +780,8 @@ static void area_light_init(light_rt *l, const orc_primitive *pr, const float *f
     l->stris = (float *)malloc(sizeof(float) * 18);
     vstore(l->stris, a), vstore(l->stris + 3, b), vstore(l->stris + 6, c);
     vstore(l->stris + 9, a), vstore(l->stris + 12, c), vstore(l->stris + 15, d);
-    const float abc = vlen_ref(vcross(vsub(b, a), vsub(c, a))) * 0.5f;
-    const float acd = vlen_ref(vcross(vsub(c, a), vsub(d, a))) * 0.5f;
+    const float abc = vlength(vcross(vsub(b, a), vsub(c, a))) * 0.5f;
+    const float acd = vlength(vcross(vsub(c, a), vsub(d, a))) * 0.5f;
     l->area = abc + acd;
     l->prob = abc / l->area;
   } else if (pr->kind == ORC_GEOM_CUBE) {
@@ -793,7 +791,7 @@ static void area_light_init(light_rt *l, const orc_primitive *pr, const float *f
       for (int k = 0; k < 3; ++k)
         for (int c = 0; c < 3; ++c) l->stris[9 * t + 3 * k + c] = flat[9 * t + 3 * (2 - k) + c];
     const float *t0 = l->stris;
-    l->cube_pdf0 = 1.f / (12.f * vlen_ref(vcross(vsub(vload(t0 + 3), vload(t0)),
+    l->cube_pdf0 = 1.f / (12.f * vlength(vcross(vsub(vload(t0 + 3), vload(t0)),
                                                  vsub(vload(t0 + 6), vload(t0)))) * 0.5f);
   } else {
     l->n_stri = n;
@@ -808,7 +806,7 @@ static void area_light_init(light_rt *l, const orc_primitive *pr, const float *f
     for (uint32_t t = 0; t < n; ++t) {
       const float *p = flat + 9 * (size_t)t;
       const v3 e0 = vsub(vload(p + 3), vload(p)), e1 = vsub(vload(p + 6), vload(p));
-      area = (float)((double)area + 0.5 * (double)vlen_ref(vcross(e0, e1)));
+      area = (float)((double)area + 0.5 * (double)vlength(vcross(e0, e1)));
     }
     l->area = area;
   }
@@ -836,7 +834,7 @@ static void area_geom_sample(const light_rt *l, v3 sam, v3 *pos, v3 *ng, float *
     if (idx > cnt - 1) idx = cnt - 1;
     t = l->stris + 9 * (size_t)idx;
     if (l->kind == ORC_GEOM_CUBE)
-      *pdf = 1.f / (12.f * vlen_ref(vcross(vsub(vload(t + 3), vload(t)),
+      *pdf = 1.f / (12.f * vlength(vcross(vsub(vload(t + 3), vload(t)),
                                            vsub(vload(t + 6), vload(t)))) * 0.5f);
     else
       *pdf = 1.f / l->area;

@@ -10,8 +10,9 @@ import yaml
 from ajisai_render_amd import loader, scenes
 
 
-def _cornell_yaml(tmp, obj=True, env="constant"):
-    """A Cornell-style scene in the reference's file format."""
+def _cornell_yaml(tmp, obj=True, env="constant", lamp=False):
+    """A Cornell-style scene in the reference's file format; `lamp` adds an
+    emissive one-sided ceiling quad (an area light)."""
     mat = lambda c: {"type": "diffuse", "albedo": {"type": "constant", "texel": list(c)}}  # noqa: E731
     quad = lambda a, b, c, d, col: {"type": "geometry", "material": mat(col),  # noqa: E731
                                     "geometry": {"type": "twosided", "internal":
@@ -35,6 +36,10 @@ def _cornell_yaml(tmp, obj=True, env="constant"):
                       "geometry": {"type": "mesh", "filename": "tet.obj",
                                    "transform": [{"type": "matrix", "matrix":
                                                   [1, 0, 0, 0.1, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]}]}})
+    if lamp:
+        prims.append({"type": "geometry", "material": mat([0.5, 0.5, 0.5]), "emission": [9, 8, 6],
+                      "geometry": {"type": "quad", "a": [0.35, 0.99, 0.35], "b": [0.65, 0.99, 0.35],
+                                   "c": [0.65, 0.99, 0.65], "d": [0.35, 0.99, 0.65]}})
     lights = [{"type": "directional", "direction": [0.3, 0.8, 0.5], "emission": [5, 5, 5]}]
     if env == "constant":
         lights.append({"type": "envmap", "texture": {"type": "constant", "texel": 0.1}})
@@ -129,8 +134,7 @@ def test_pfm_and_hdr_readers(tmp_path):
 
 def test_unsupported_inputs_are_named(tmp_path):
     cfg = yaml.safe_load(open(_cornell_yaml(str(tmp_path), obj=False)))
-    bad = [("scene", "primitives", 0, "emission", [1, 1, 1]),
-           ("scene", "primitives", 0, "material", {"type": "glass"}),
+    bad = [("scene", "primitives", 0, "material", {"type": "glass"}),
            ("scene", "primitives", 0, "geometry", {"type": "sphere", "radius": 1, "center": [0, 0, 0]})]
     for path in bad:
         c = yaml.safe_load(yaml.safe_dump(cfg))
@@ -144,3 +148,13 @@ def test_unsupported_inputs_are_named(tmp_path):
     c["renderer"]["use_mis"] = False
     with pytest.raises(loader.SceneError):
         loader.load(c)
+
+
+def test_emissive_primitive_is_an_area_light(tmp_path):
+    """"emission" on a geometry primitive (primitive_creators.cpp:65-72) makes
+    it an area light; the primitive keeps its geometry and material."""
+    sc = loader.load(_cornell_yaml(str(tmp_path), obj=False, lamp=True)).scene
+    lamp = sc.primitives[-1]
+    assert lamp.emission == (9.0, 8.0, 6.0) and lamp.kind == scenes.GEOM_QUAD
+    assert not lamp.two_sided
+    assert all(max(p.emission) == 0 for p in sc.primitives[:-1])
