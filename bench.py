@@ -94,7 +94,11 @@ def parse():
     ap.add_argument("--inst-guided-budget", type=float, default=5.0,
                     help="equal-time budget (s) of guided vs plain on the instanced scene (0: skip)")
     ap.add_argument("--inst-guided-res", type=int, default=480)
-    ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp", "guided"],
+    ap.add_argument("--area-spp", type=int, default=64,
+                    help="spp of the area-lit Cornell box render (0: skip)")
+    ap.add_argument("--area-cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp", "guided",
+                                       "area"],
                     help="run one section alone and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -427,6 +431,50 @@ def instanced_bench(ctx, a):
     return out
 
 
+def area_bench(ctx, a):
+    """Cornell box lit by four emissive primitives (one-sided quad lamp,
+    two-sided quad, mesh of three triangles, cube), 1024x1024: one full
+    render, with the oracle timed on a bounded sample of rows beside it."""
+    import torch
+    from ajisai_render_amd import _capi as capi
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import GpuScene, PathTracer
+
+    sc = scenes.cornell_area(("quad", "twosided", "mesh", "cube"), res=(1024, 1024),
+                             spp=a.area_spp)
+    g = GpuScene(ctx, sc)
+    pt = PathTracer(g, seed_mode=1, max_slots=a.slots, flags=capi.RENDER_TIMING)
+    PathTracer(g, spp=1, seed_mode=1, max_slots=a.slots).render_device()
+    film, _ = pt.render_device()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, st = pt.render_device(film=film)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    w, h = sc.camera.res
+    paths = w * h * sc.pt.spp
+    q = st.closest_queries + st.any_queries
+    out = {
+        "workload": "cornell_area: Cornell box, 4 area lights (quad, two-sided quad, "
+                    "3-triangle mesh, cube), Lambert, max 8 bounces",
+        "resolution": [w, h], "spp": sc.pt.spp, "lights": 4,
+        "seconds": round(wall, 3),
+        "paths_per_s": round(paths / wall, 1),
+        "mrays_per_s": round(q / wall / 1e6, 2),
+        "rays_per_path": round(q / paths, 3),
+        "stage_ms": {"trace_closest": round(st.ms_trace_closest, 2),
+                     "trace_any": round(st.ms_trace_any, 2), "shade": round(st.ms_shade, 2)},
+    }
+    g.close()
+    if not a.no_cpu_baseline and a.area_cpu_seconds > 0:
+        cpu = cpu_baseline(scenes.cornell_area(("quad", "twosided", "mesh", "cube"),
+                                               res=(1024, 1024), spp=a.area_spp),
+                           a.area_cpu_seconds)
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu_paths"] = round(out["paths_per_s"] / cpu["value"], 1)
+    return out
+
+
 def film_bench(pt, film):
     """Film resolve (box, gaussian) and 8-bit sRGB encode of the heavy-mesh
     film: device time per call and algorithmic bytes (sums 28 B per traced
@@ -526,7 +574,7 @@ def main():
     if a.only:
         fn = {"batched": batched_bench, "instanced": instanced_bench,
               "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c),
-              "guided": guided_equal_time}[a.only]
+              "guided": guided_equal_time, "area": area_bench}[a.only]
         print(json.dumps({a.only: fn(ctx, a)}))
         return
     scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
@@ -602,6 +650,7 @@ def main():
     guided = guided_equal_time(ctx, a) if a.guided_budget > 0 else None
     batched = batched_bench(ctx, a) if a.batch_jobs > 0 else None
     instanced = instanced_bench(ctx, a) if a.inst_spp > 0 else None
+    area = area_bench(ctx, a) if a.area_spp > 0 else None
     if instanced is not None and a.inst_guided_budget > 0:
         instanced["guided"] = guided_instanced(ctx, a)
 
@@ -645,6 +694,7 @@ def main():
         "guided": guided,
         "batched": batched,
         "instanced": instanced,
+        "area_lights": area,
         "mlp": mlp,
         "film": film_perf,
         "roofline": roof,
