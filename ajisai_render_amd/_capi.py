@@ -96,6 +96,26 @@ class RenderStats(C.Structure):
                 ("nodes_fetched_any", C.c_uint64), ("tris_tested_any", C.c_uint64)]
 
 
+class SppmArgs(C.Structure):
+    _fields_ = [("forward_max_depth", C.c_int32), ("init_radius", C.c_float),
+                ("iteration_count", C.c_int32), ("photons_per_iteration", C.c_int32),
+                ("photon_min_depth", C.c_int32), ("photon_max_depth", C.c_int32),
+                ("photon_cont_prob", C.c_float), ("update_alpha", C.c_float),
+                ("photon_seed_base", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class SppmStats(C.Structure):
+    _fields_ = [("photons", C.c_uint64), ("deposits", C.c_uint64), ("grid_entries", C.c_uint64),
+                ("visible_points", C.c_uint32), ("pad", C.c_uint32), ("ms_total", C.c_float),
+                ("ms_visible_points", C.c_float), ("ms_photons", C.c_float),
+                ("ms_grid", C.c_float)]
+
+
+SPPM_PIXEL_DTYPE = np.dtype([("pos", "<f4", 3), ("beta", "<f4", 3), ("direct", "<f4", 3),
+                             ("tau", "<f4", 3), ("radius", "<f4"), ("n", "<f4"),
+                             ("valid", "<u4"), ("pad", "<u4")])
+
+
 class GuideArgs(C.Structure):
     _fields_ = [("alpha", C.c_float), ("train", C.c_int32), ("batch", C.c_uint32),
                 ("lr", C.c_float), ("train_slots", C.c_uint32), ("max_steps", C.c_uint32),
@@ -173,6 +193,8 @@ _SIGS = {
     "ajg_scene_env_tables": ([_P, C.c_uint32, _P, _P, _P, _P], C.c_int),
     "ajg_render_pt": ([_P, C.POINTER(Camera), C.POINTER(PTArgs), C.POINTER(Film),
                        C.POINTER(RenderStats), _P], C.c_int),
+    "ajg_render_sppm": ([_P, C.POINTER(Camera), C.POINTER(SppmArgs), _P, _P,
+                         C.POINTER(SppmStats), _P], C.c_int),
     "ajg_film_resolve": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P], C.c_int),
     "ajg_film_resolve_filter": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Filter), _P,
                                  _P, _P, _P], C.c_int),

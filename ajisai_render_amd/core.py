@@ -295,6 +295,66 @@ class GpuScene:
             pass
 
 
+class SPPMRenderer:
+    """CreateSPPMRenderer(SPPMRendererArgs) (renderer.h:82-97, 113) /
+    SPPMRenderer::Render (sppm.cpp:184-294) on the device. The per-pixel
+    radiance is the value the reference adds to its film, one sample per
+    pixel; `resolve_filtered` applies the film's filter."""
+
+    def __init__(self, gscene: GpuScene, spec=None, timing=False):
+        from .scenes import SPPMSpec
+        self.g = gscene
+        sp = spec if spec is not None else SPPMSpec()
+        self.spec = sp
+        self.args = capi.SppmArgs(sp.forward_max_depth, sp.init_radius, sp.iteration_count,
+                                  sp.photons_per_iteration, sp.photon_min_depth,
+                                  sp.photon_max_depth, sp.photon_cont_prob, sp.update_alpha,
+                                  sp.photon_seed_base, capi.RENDER_TIMING if timing else 0)
+
+    def render_device(self, cam_override=None, state=False, stream=None):
+        """-> (radiance (H, W, 3) device tensor, state (H*W*64 bytes) or None,
+        SppmStats)."""
+        torch = _torch()
+        w, h = self.g.scene.camera.res
+        if cam_override is None:
+            cam = camera_setup(self.g.scene.camera)
+        else:
+            cam = capi.Camera()
+            C.memmove(C.byref(cam), C.byref(cam_override), C.sizeof(capi.Camera))
+        rad = torch.empty(h * w * 3, dtype=torch.float32, device="cuda")
+        st = (torch.empty(h * w * capi.SPPM_PIXEL_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+              if state else None)
+        stats = capi.SppmStats()
+        self.g.ctx.check(self.g.ctx.lib.ajg_render_sppm(
+            self.g.h, C.byref(cam), C.byref(self.args), _ptr(rad), _ptr(st), C.byref(stats),
+            stream), "ajg_render_sppm")
+        return rad.view(h, w, 3), st, stats
+
+    # Renderer::Render (renderer.h:64-65)
+    def Render(self, cam_override=None, state=False):
+        rad, st, stats = self.render_device(cam_override, state)
+        out = {"radiance": rad.cpu().numpy(), "stats": stats}
+        if st is not None:
+            w, h = self.g.scene.camera.res
+            out["state"] = st.cpu().numpy().view(capi.SPPM_PIXEL_DTYPE).reshape(h, w)
+        return out
+
+    def resolve_filtered(self, rad_dev, ftype="box", radius=None, alpha=2.0):
+        """Film::AddSample of one sample per pixel then MergeTile with the
+        film's filter -> (H, W, 3) device tensor."""
+        torch = _torch()
+        h, w = rad_dev.shape[:2]
+        t = capi.FILTER_BOX if ftype == "box" else capi.FILTER_GAUSSIAN
+        r = radius if radius is not None else ((0.5, 0.5) if t == capi.FILTER_BOX else (2.0, 2.0))
+        f = capi.Filter(t, (C.c_float * 2)(*r), alpha)
+        ones = torch.ones(h * w, dtype=torch.int32, device="cuda")
+        rgb = torch.empty(h * w * 3, dtype=torch.float32, device="cuda")
+        self.g.ctx.check(self.g.ctx.lib.ajg_film_resolve_filter(
+            self.g.ctx.h, w, h, 64, C.byref(f), _ptr(rad_dev.reshape(-1)), _ptr(ones), _ptr(rgb),
+            None), "ajg_film_resolve_filter")
+        return rgb.view(h, w, 3)
+
+
 class PathTracer:
     """CreatePTRenderer(PTRendererArgs) (renderer.h:37-45, 80) for TraceMIS on
     the device. Renders into per-traced-pixel film sums."""

@@ -23,34 +23,8 @@
 #include "ajg_internal.h"
 #include "ajg_shade.h"
 #include "ajg_traverse.h"
+#include "ajg_scene.h"
 
-struct ajg_scene {
-  ajg_ctx *ctx = nullptr;
-  ajg_accel *accel = nullptr;
-  uint4 *tri_shade = nullptr;  // per leaf slot: flags|material, n_i, n_j, n_k
-  float4 *normals = nullptr;   // mesh normal_and_vs, all meshes concatenated
-  ajg::DevMaterial *mats = nullptr;
-  uint32_t n_mats = 0;
-  ajg::DevLight lights[ajg::kMaxLights];
-  uint32_t n_lights = 0;
-  uint32_t n_area = 0;               // lights[0, n_area): emissive primitives
-  int env_index = -1;
-  std::vector<void *> light_bufs;    // env tables, area-light triangles
-  float world[6] = {0, 0, 0, 0, 0, 0};
-  float radius = 0.f;
-  int cu_count = 256;
-  void *render_cache = nullptr;  // RenderBuffers reused across renders
-  void *guide_cache = nullptr;   // GuideBuffers reused across guided renders
-  // instanced scenes: `accel` is the top-level tree over instance boxes
-  std::vector<ajg_accel *> blas;   // one bottom-level tree per mesh
-  ajg::DevInst *insts = nullptr;
-  ajg::DevBlas *blases = nullptr;
-  uint32_t n_inst = 0;
-  uint32_t depth2 = 0;             // top depth + deepest bottom depth
-  std::vector<float> inst_w2o;     // 12 floats per instance (as on the device)
-  std::vector<float> inst_box;     // 6 floats per instance (padded world box)
-  uint32_t *trace_ctr = nullptr;   // cursor of scene trace calls
-};
 
 namespace ajg {
 
@@ -59,12 +33,6 @@ int flatten_views(ajg_ctx *ctx, const ajg_mesh_view *views, uint32_t n_views,
                   std::vector<float> *tri, float *box);
 int upload_and_build(ajg_ctx *ctx, const std::vector<float> &tri, const float *box,
                      uint32_t r, hipStream_t s, ajg_accel **out);
-
-// tri_shade flags
-static constexpr uint32_t kKindShift = 28;  // 2 bits
-static constexpr uint32_t kTwoSidedBit = 1u << 27;
-static constexpr uint32_t kLightShift = 24;  // 3 bits: area light index + 1 (0: none)
-static constexpr uint32_t kMatMask = (1u << 24) - 1;
 
 // Closest-queue entries per path slot: the extension ray and, in scenes with
 // area lights, the MIS-BSDF ray (which must find the emitter it hits).
@@ -130,19 +98,6 @@ __global__ void k_env_marginal(const float *cond_int, uint32_t h, float *marg_fu
 }
 
 // --------------------------------------------------------------- render
-struct DevScene {
-  DevBVH bvh;                  // instanced: the top-level tree
-  const uint4 *tri_shade;
-  const float4 *normals;
-  const DevInst *inst;         // instanced scenes only (else null)
-  const DevBlas *blas;
-  const DevMaterial *mats;
-  DevLight lights[kMaxLights];
-  uint32_t n_lights;
-  uint32_t n_area;             // area lights (lights[0, n_area))
-  int env_index;
-  float radius;
-};
 
 struct RenderParams {
   DevCamera cam;
@@ -303,76 +258,6 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
   base = __shfl(base, leader, 64);
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   return want ? base + (uint32_t)__popcll(mask & lt) : kNone;
-}
-
-struct Inct {
-  f3 pos, ng, ns, wr;
-  uint32_t material;
-  uint32_t light;  // area light index + 1 of an emissive primitive, else 0
-};
-
-// MT normal + PostIntersect (ploc_aggregate.cpp:199-211; quad.cpp:73-90,
-// cube.cpp:96-108 (tuple order reversed in the index buffer), mesh.cpp:183-204,
-// two_sided.cpp:44-53).
-// Instanced scenes: the triangle is read in its mesh's object space; the
-// hit point comes from the world ray, and normals go to world space through
-// the instance's normal map (for a rotation with uniform scale that map is the
-// rotation itself, so the shading normal has the length it would have in a
-// world-space copy of the mesh).
-__device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay &r, float t, int ls,
-                                               float u, float v, uint32_t inst = 0) {
-  Inct it;
-  f3 a, b, c;
-  const float4 *leaf = sc.bvh.leaf_tris;
-  const uint4 *shade = sc.tri_shade;
-  const DevInst *I = nullptr;
-  if (sc.inst) {
-    I = sc.inst + inst;
-    const DevBlas &B = sc.blas[I->info.x];
-    leaf = B.bvh.leaf_tris;
-    shade = B.tri_shade;
-  }
-  load_leaf(leaf, ls, &a, &b, &c);
-  const uint4 info = shade[ls];
-  const uint32_t kind = info.x >> kKindShift;
-  it.material = (I && I->info.y != kKeepMaterial) ? I->info.y : (info.x & kMatMask);
-  it.light = (info.x >> kLightShift) & 7u;
-  it.pos = add(r.o, scale(r.d, t));
-  it.wr = neg(r.d);
-  if (kind == AJG_GEOM_CUBE) {
-    it.ng = normalize(cross(sub(b, c), sub(a, c)));
-    it.ns = it.ng;
-  } else {
-    it.ng = normalize(cross(sub(b, a), sub(c, a)));
-    if (kind == AJG_GEOM_MESH) {
-      const float w = 1 - u - v;
-      const float4 ni = sc.normals[info.y], nj = sc.normals[info.z], nk = sc.normals[info.w];
-      it.ns = add(add(scale(mk3(ni.x, ni.y, ni.z), w), scale(mk3(nj.x, nj.y, nj.z), u)),
-                  scale(mk3(nk.x, nk.y, nk.z), v));
-    } else {
-      it.ns = it.ng;
-    }
-  }
-  if (I) {
-    const bool interp = kind == AJG_GEOM_MESH;
-    it.ng = normalize(xform_vec(I->nrm, it.ng));
-    it.ns = interp ? xform_vec(I->nrm, it.ns) : it.ng;
-  }
-  if ((info.x & kTwoSidedBit) && dot(it.ng, it.wr) < 0) {
-    it.ng = scale(it.ng, -1.f);
-    it.ns = scale(it.ns, -1.f);
-  }
-  return it;
-}
-
-// Intersection::EpsOffset (intersection.h:42-47)
-__device__ __forceinline__ f3 eps_offset(const Inct &it, f3 dir) {
-  if (dot(dir, it.ng) > 0.f) return add(it.pos, scale(it.ng, AJG_EPS));
-  return sub(it.pos, scale(it.ng, AJG_EPS));
-}
-
-__device__ __forceinline__ float lum_f(f3 c) {
-  return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
 }
 
 // BSDF value as the sampling routine reports it: LambertianReflection's
@@ -1600,7 +1485,6 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
 
 using namespace ajg;
 
-static DevScene dev_scene(const ajg_scene *sc);
 
 static void scene_free(ajg_scene *s) {
   if (!s) return;
@@ -1611,6 +1495,7 @@ static void scene_free(ajg_scene *s) {
   for (void *p : s->light_bufs) (void)hipFree(p);
   delete (RenderBuffers *)s->render_cache;
   delete (GuideBuffers *)s->guide_cache;
+  sppm_free(s->sppm_cache);
   for (ajg_accel *b : s->blas) destroy_accel(b);
   (void)hipFree(s->insts);
   (void)hipFree(s->blases);
@@ -2258,7 +2143,7 @@ static int setup_params(ajg_ctx *ctx, const ajg_camera *camera, const ajg_pt_arg
   return AJG_OK;
 }
 
-static DevScene dev_scene(const ajg_scene *sc) {
+DevScene dev_scene(const ajg_scene *sc) {
   const ajg_accel *a = sc->accel;
   DevScene ds;
   memset(&ds, 0, sizeof(ds));

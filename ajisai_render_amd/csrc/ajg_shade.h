@@ -679,7 +679,10 @@ __device__ __forceinline__ LightSample dir_sample(const DevLight &l, f3 ref, flo
 // triangles in `stris` are those), the cube reports the pdf of the triangle
 // it picked, and the solid-angle pdf divides dist^2 by |n . (ref - pos)|.
 // A sample whose emitting side faces away from ref has pdf 0.
-__device__ __forceinline__ LightSample area_sample(const DevLight &l, f3 ref, f3 sam) {
+// area_geom_sample is the geometry's Sample(pdf, sam) with the TwoSided
+// wrapper: a point, its (flipped) normal and the area pdf.
+__device__ __forceinline__ void area_geom_sample(const DevLight &l, f3 sam, f3 *pos, f3 *n,
+                                                 float *pdf_area) {
   bool back = false;
   if (l.two_sided) {
     back = sam.x < 0.5f;
@@ -702,13 +705,20 @@ __device__ __forceinline__ LightSample area_sample(const DevLight &l, f3 ref, f3
   }
   const f3 a = mk3(t[0], t[1], t[2]), b = mk3(t[3], t[4], t[5]), c = mk3(t[6], t[7], t[8]);
   float pdf = cube_pdf ? 1.f / (12.f * length(cross(sub(b, a), sub(c, a))) * 0.5f) : 1.f / l.area;
-  LightSample r;
-  r.pos = add(add(scale(a, (1 - sx) - sy), scale(b, sx)), scale(c, sy));
+  *pos = add(add(scale(a, (1 - sx) - sy), scale(b, sx)), scale(c, sy));
   f3 ng = normalize(cross(sub(b, a), sub(c, a)));
   if (l.two_sided) {
     pdf = pdf * 0.5f;
     if (back) ng = scale(ng, -1.f);
   }
+  *n = ng;
+  *pdf_area = pdf;
+}
+__device__ __forceinline__ LightSample area_sample(const DevLight &l, f3 ref, f3 sam) {
+  LightSample r;
+  f3 ng;
+  float pdf;
+  area_geom_sample(l, sam, &r.pos, &ng, &pdf);
   const f3 i2r = sub(ref, r.pos);
   if (dot(ng, i2r) <= 0) {
     r.radiance = mk3(0.f, 0.f, 0.f);

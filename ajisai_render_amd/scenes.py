@@ -93,6 +93,21 @@ class PTSpec:
     cont_prob: float = 0.9
 
 
+@dataclasses.dataclass
+class SPPMSpec:
+    """SPPMRendererArgs (renderer.h:82-97) with the creator's defaults
+    (renderer_creators.cpp:63-71); photon_seed_base seeds the photon streams."""
+    forward_max_depth: int = 8
+    init_radius: float = 0.1
+    iteration_count: int = 100
+    photons_per_iteration: int = 100000
+    photon_min_depth: int = 5
+    photon_max_depth: int = 10
+    photon_cont_prob: float = 0.9
+    update_alpha: float = 2.0 / 3.0
+    photon_seed_base: int = 0
+
+
 KEEP_MATERIAL = -1
 
 

@@ -502,6 +502,63 @@ int ajg_batch_autotune(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
                        uint32_t trial_spp, ajg_batch_args *inout,
                        float *best_paths_per_s);
 
+/* ------------------------------------------------------------------ SPPM
+ * Replaces CreateSPPMRenderer (renderer/renderer.h:82-97, 113) /
+ * SPPMRenderer::Render (renderer/sppm.cpp:184-294) over the GPU scene
+ * (one-level scenes). Fields and defaults of SPPMRendererArgs
+ * (renderer_creators.cpp:63-71), plus photon_seed_base: photon i of
+ * iteration k draws from its own PCG32 stream seeded with
+ * mix(photon_seed_base, i, k), where the reference hands all photon threads
+ * the renderer's one shared sampler (sppm.cpp:255-260), whose draws are not
+ * reproducible. Visible points (TracerVP, sppm.cpp:35-106) use the
+ * reference's per-pixel seeding x + y*W, which every iteration repeats, so
+ * they are traced once and their direct light is accumulated once per
+ * iteration in the reference's order of additions. The hash grid is
+ * VisiblePointSearcher's (photon_mapping.h:70-151, 40960 entries, cells
+ * [min, max) of each radius box). The film width and height must be
+ * multiples of 64 (the reference's 64-pixel tiles are not clipped to the
+ * film, sppm.cpp:213-234). */
+typedef struct ajg_sppm_args {
+  int32_t forward_max_depth;      /* 8 */
+  float init_radius;              /* 0.1; < 0: world-box diagonal / 1000 */
+  int32_t iteration_count;        /* 100 */
+  int32_t photons_per_iteration;  /* 100000 */
+  int32_t photon_min_depth;       /* 5 */
+  int32_t photon_max_depth;       /* 10 */
+  float photon_cont_prob;         /* 0.9 */
+  float update_alpha;             /* 2/3 */
+  uint32_t photon_seed_base;
+  uint32_t flags;                 /* AJG_RENDER_TIMING */
+} ajg_sppm_args;
+
+/* Per-pixel state after the last iteration (test and debugging output):
+ * the visible point (position, throughput as traced; valid = a visible
+ * point was found), the accumulated direct light, tau, radius and N. */
+typedef struct ajg_sppm_pixel {
+  float pos[3], beta[3];
+  float direct[3];
+  float tau[3];
+  float radius, n;
+  uint32_t valid, pad;
+} ajg_sppm_pixel;
+
+typedef struct ajg_sppm_stats {
+  uint64_t photons;          /* iteration_count x photons_per_iteration */
+  uint64_t deposits;         /* photon-visible point contributions */
+  uint64_t grid_entries;     /* registrations summed over iterations */
+  uint32_t visible_points;   /* pixels with a visible point */
+  uint32_t pad;
+  float ms_total, ms_visible_points, ms_photons, ms_grid;  /* AJG_RENDER_TIMING */
+} ajg_sppm_stats;
+
+/* radiance: device float[W*H*3] receiving ComputePixelRadiance per pixel
+ * (the value Film::AddSample then filters, sppm.cpp:282-293; resolve it with
+ * ajg_film_resolve_filter, tile size 64, count 1 per pixel). state (device
+ * ajg_sppm_pixel[W*H]) and stats may be NULL. Synchronous. */
+int ajg_render_sppm(ajg_scene *scene, const ajg_camera *camera, const ajg_sppm_args *args,
+                    float *radiance, ajg_sppm_pixel *state, ajg_sppm_stats *stats,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -289,6 +289,47 @@ void orc_trace_any_2l(const orc_node *tlas, const uint32_t *tlas_perm, const flo
                       const uint32_t *inst_mesh, const orc_blas *blas, const orc_ray *rays,
                       uint32_t n, uint8_t *occluded);
 
+/* SPPM (SPPMRenderer, sppm.cpp:35-303; VisiblePointSearcher,
+ * photon_mapping.h:29-153). Fields and defaults of SPPMRendererArgs
+ * (renderer.h:82-97, renderer_creators.cpp:63-71) plus the photon streams'
+ * seed: photon i of iteration k draws from PCG32 seeded with
+ * mix(photon_seed_base, i, k) (the reference hands every photon thread the
+ * one shared sampler, whose draws are not reproducible). */
+typedef struct orc_sppm_args {
+  int32_t forward_max_depth;
+  float init_radius;
+  int32_t iteration_count;
+  int32_t photons_per_iteration;
+  int32_t photon_min_depth;
+  int32_t photon_max_depth;
+  float photon_cont_prob;
+  float update_alpha;
+  uint32_t photon_seed_base;
+  uint32_t pad;
+} orc_sppm_args;
+
+/* Per-pixel SPPM state after the last iteration run. */
+typedef struct orc_sppm_pixel {
+  float pos[3], beta[3];   /* visible point (beta: throughput) */
+  float direct[3];         /* accumulated direct illumination */
+  float tau[3];
+  float radius, n;
+  uint32_t valid, pad;
+} orc_sppm_pixel;
+
+/* Runs `iterations` of args->iteration_count iterations (all if 0) and
+ * writes the radiance ComputePixelRadiance gives after them (w*h*3, the
+ * per-pixel value the film then filters) and optionally the per-pixel state.
+ * Returns the number of photon deposits, or -1 if the film size is not a
+ * multiple of 64. */
+int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
+                        int32_t iterations, float *radiance, orc_sppm_pixel *state);
+/* Test hook: the visible-point grid alone (VisiblePointSearcher AddVp and
+ * AddPhoton's lookup and distance test, no BSDF): per visible point, the
+ * number of the photon positions matched to it. */
+void orc_sppm_grid_probe(const float *bmin, float side, uint32_t n_vp, const float *vp_pos,
+                         const float *vp_r, uint32_t n_ph, const float *ph_pos, uint32_t *counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,18 @@ class Args(C.Structure):
                 ("env_cdf_fp64", C.c_int32), ("pad", C.c_int32)]
 
 
+class SppmArgs(C.Structure):
+    _fields_ = [("forward_max_depth", C.c_int32), ("init_radius", C.c_float),
+                ("iteration_count", C.c_int32), ("photons_per_iteration", C.c_int32),
+                ("photon_min_depth", C.c_int32), ("photon_max_depth", C.c_int32),
+                ("photon_cont_prob", C.c_float), ("update_alpha", C.c_float),
+                ("photon_seed_base", C.c_uint32), ("pad", C.c_uint32)]
+
+
+SPPM_PIXEL_DTYPE = np.dtype([("pos", "<f4", 3), ("beta", "<f4", 3), ("direct", "<f4", 3),
+                             ("tau", "<f4", 3), ("radius", "<f4"), ("n", "<f4"),
+                             ("valid", "<u4"), ("pad", "<u4")])
+
 _P = C.c_void_p
 _lib = None
 
@@ -123,6 +135,8 @@ def load():
         "orc_film_resolve_filter": ([C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_float,
                                      C.c_float, C.c_float, _P, _P, _P], None),
         "orc_image_srgb8": ([C.c_uint32, C.c_uint32, _P, C.c_int, _P], None),
+        "orc_render_sppm": ([_P, _P, _P, C.c_int32, _P, _P], C.c_int64),
+        "orc_sppm_grid_probe": ([_P, C.c_float, C.c_uint32, _P, _P, C.c_uint32, _P, _P], None),
     }
     for name, (a, r) in sig.items():
         f = getattr(lib, name)
@@ -390,6 +404,22 @@ class OracleScene:
                                     _p(s), _p(s2), _p(cnt), _p(ps) if ps is not None else None)
         return dict(sum=s, sum_sq=s2, count=cnt, per_sample=ps, queries=int(q))
 
+    def render_sppm(self, spec, iterations=0, cam=None):
+        """SPPM (the scene's SPPMSpec): per-pixel radiance (h, w, 3), the
+        per-pixel state (SPPM_PIXEL_DTYPE) and the photon deposit count."""
+        cam = cam or self.camera()
+        w, h = self.scene.camera.res
+        a = SppmArgs(spec.forward_max_depth, spec.init_radius, spec.iteration_count,
+                     spec.photons_per_iteration, spec.photon_min_depth, spec.photon_max_depth,
+                     spec.photon_cont_prob, spec.update_alpha, spec.photon_seed_base, 0)
+        rad = np.zeros((h, w, 3), np.float32)
+        st = np.zeros(w * h, SPPM_PIXEL_DTYPE)
+        n = self.lib.orc_render_sppm(self.h, C.byref(cam), C.byref(a), iterations, _p(rad),
+                                     st.ctypes.data)
+        if n < 0:
+            raise ValueError("sppm: film width and height must be multiples of 64")
+        return rad, st.reshape(h, w), int(n)
+
     def close(self):
         if self.h:
             self.lib.orc_scene_destroy(self.h)
@@ -400,6 +430,19 @@ class OracleScene:
             self.close()
         except Exception:
             pass
+
+
+def sppm_grid_probe(bmin, side, vp_pos, vp_r, ph_pos):
+    """Per visible point, the photons the SPPM grid lookup matches to it."""
+    lib = load()
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    vp_pos = np.ascontiguousarray(vp_pos, np.float32).reshape(-1, 3)
+    vp_r = np.ascontiguousarray(vp_r, np.float32)
+    ph_pos = np.ascontiguousarray(ph_pos, np.float32).reshape(-1, 3)
+    out = np.zeros(vp_pos.shape[0], np.uint32)
+    lib.orc_sppm_grid_probe(_p(bmin), side, vp_pos.shape[0], _p(vp_pos), _p(vp_r),
+                            ph_pos.shape[0], _p(ph_pos), _p(out))
+    return out
 
 
 def film_resolve(w, h, ts, sums, counts):

@@ -1618,3 +1618,425 @@ void orc_film_resolve(uint32_t w, uint32_t h, uint32_t ts, const double *sum,
   free(rad);
   free(wt);
 }
+
+/* ---- SPPM (SPPMRenderer, sppm.cpp:35-303) -------------------------------
+ * Restated as written: every iteration reseeds each pixel's sampler with
+ * x + y*W (sppm.cpp:222), so the visible points and their direct light are
+ * the same in every iteration and the direct light is accumulated once per
+ * iteration; visible points register in the hash-grid cells [min, max) of
+ * their radius box (the upper cell excluded, photon_mapping.h:88-90); the
+ * entry index keeps the reference's low bits (z & 2 on bit 1,
+ * photon_mapping.h:133-138); the next iteration's grid cell is 1.05 x the
+ * largest radius among valid visible points before the update
+ * (sppm.cpp:263-271). Photon i of iteration k draws from its own PCG32
+ * stream seeded with mix(photon_seed_base, i, k): the reference passes the
+ * renderer's one shared sampler to every photon thread (sppm.cpp:255-260),
+ * whose draws are not reproducible. */
+typedef struct sppm_px {
+  int valid;
+  v3 pos, wr, beta;
+  v3 beta_traced; /* the throughput as traced (UpdatePixel zeroes beta) */
+  bsdf b;
+  v3 direct, tau;
+  float radius, n;
+  float phi[3];
+  int32_t m;
+} sppm_px;
+
+/* TracerVP (sppm.cpp:35-106) */
+static void sppm_tracer_vp(const orc_scene *s, int max_depth, orc_ray ray, sampler *smp,
+                           sppm_px *px, uint64_t *q) {
+  v3 beta = V3(1.f, 1.f, 1.f);
+  px->valid = 0;
+  for (int depth = 0; depth < max_depth; ++depth) {
+    orc_hit h;
+    if (!scene_intersect(s, &ray, &h, q)) {
+      if (depth == 0 && s->env_index >= 0)
+        px->direct = vadd(px->direct,
+                          vmul(beta, env_radiance(&s->lights[s->env_index], vload(ray.d))));
+      return;
+    }
+    inct it = post_intersect(s, &ray, &h);
+    uint32_t mi = s->prims[s->tri_prim[h.id]].material;
+    bsdf b = make_bsdf(&s->mats[mi], it.ng, it.ns);
+    if (depth == 0) {
+      const int li = s->prim_light[s->tri_prim[h.id]];
+      if (li >= 0 && vdot(it.ng, it.wr) > 0)
+        px->direct = vadd(px->direct, vmul(beta, s->lights[li].radiance));
+    }
+    v3 sum_di = V3(0, 0, 0);
+    for (uint32_t li = 0; li < s->n_lights; ++li)
+      sum_di = vadd(sum_di, vmul(beta, mis_sample_light(s, &s->lights[li], &it, &b, smp, q)));
+    sum_di = vadd(sum_di, vmul(beta, mis_sample_bsdf(s, &it, &b, smp, q)));
+    px->direct = vadd(px->direct, vscale(sum_di, 1.f / 1));
+    /* HasDiffuseComponent: Lambert always, Disney when metallic < 1
+     * (disney.cpp:79) */
+    const int diffuse = b.m->type == ORC_MAT_DIFFUSE || b.m->metallic < 1.f;
+    if (diffuse || depth == max_depth - 1) {
+      if (!(isfinite(beta.x) && isfinite(beta.y) && isfinite(beta.z))) return;
+      px->valid = 1;
+      px->pos = it.pos;
+      px->beta = beta;
+      px->beta_traced = beta;
+      px->b = b;
+      px->wr = it.wr;
+      return;
+    }
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    bsdf_sample bs = bsdf_sample_all(&b, it.wr, sam);
+    if (!bs.valid) bs.f = V3(0, 0, 0), bs.pdf = 0.f;
+    if (is_black(bs.f) || bs.pdf < 0.f) return;
+    beta = vmul(beta, vdivs(vscale(bs.f, fabsf(vdot(it.ng, bs.dir))), bs.pdf));
+    ray = make_ray(eps_offset(&it, bs.dir), bs.dir, ORC_EPS, INFINITY);
+  }
+}
+
+/* VisiblePointSearcher (photon_mapping.h:70-151): 40960 entries. */
+#define SPPM_ENTRIES 40960u
+typedef struct sppm_grid {
+  box3 bound;
+  float side;
+  uint32_t *start; /* SPPM_ENTRIES + 1 */
+  uint32_t *list;
+} sppm_grid;
+
+static void sppm_to_grid(const sppm_grid *g, v3 p, int *c) {
+  const v3 hd = vsub(p, g->bound.mn);
+  c[0] = (int)(std_max(hd.x, 0.f) / g->side);
+  c[1] = (int)(std_max(hd.y, 0.f) / g->side);
+  c[2] = (int)(std_max(hd.z, 0.f) / g->side);
+}
+static uint64_t hash_combine(uint64_t a, uint64_t b) { /* utility/hash.h:31-36 */
+  return a ^ (b + 0x9e3779b97f4a7c15ull + (a << 6) + (a >> 2));
+}
+static uint32_t sppm_entry(int x, int y, int z) {
+  /* std::hash<int> is the value itself (sign-extended to size_t) */
+  const uint64_t low3 = (uint64_t)((x & 1) | ((y & 1) << 1) | (z & 2));
+  const uint64_t h = hash_combine(hash_combine((uint64_t)(int64_t)x, (uint64_t)(int64_t)y),
+                                  (uint64_t)(int64_t)z);
+  return (uint32_t)(((h << 3) % SPPM_ENTRIES) | low3);
+}
+
+static void sppm_vp_cells(const sppm_grid *g, const sppm_px *p, int *mn, int *mx) {
+  const v3 r = V3(p->radius, p->radius, p->radius);
+  sppm_to_grid(g, vsub(p->pos, r), mn);
+  sppm_to_grid(g, vadd(p->pos, r), mx);
+}
+
+static void sppm_grid_build(sppm_grid *g, const sppm_px *px, uint32_t n) {
+  memset(g->start, 0, sizeof(uint32_t) * (SPPM_ENTRIES + 1));
+  for (int pass = 0; pass < 2; ++pass) {
+    for (uint32_t i = 0; i < n; ++i) {
+      if (!px[i].valid) continue;
+      int mn[3], mx[3];
+      sppm_vp_cells(g, &px[i], mn, mx);
+      for (int z = mn[2]; z < mx[2]; ++z)
+        for (int y = mn[1]; y < mx[1]; ++y)
+          for (int x = mn[0]; x < mx[0]; ++x) {
+            const uint32_t e = sppm_entry(x, y, z);
+            if (pass == 0)
+              g->start[e + 1]++;
+            else
+              g->list[g->start[e]++] = i;
+          }
+    }
+    if (pass == 0) {
+      uint32_t total = 0;
+      for (uint32_t e = 1; e <= SPPM_ENTRIES; ++e) g->start[e] += g->start[e - 1];
+      total = g->start[SPPM_ENTRIES];
+      g->list = (uint32_t *)realloc(g->list, sizeof(uint32_t) * (total ? total : 1));
+    }
+  }
+  /* the fill pass advanced start[e] to the end of entry e: shift back */
+  for (uint32_t e = SPPM_ENTRIES; e > 0; --e) g->start[e] = g->start[e - 1];
+  g->start[0] = 0;
+}
+
+/* VisiblePointSearcher::AddPhoton (photon_mapping.h:103-121) */
+static int64_t sppm_add_photon(const sppm_grid *g, sppm_px *px, v3 pos, v3 phi, v3 wr) {
+  int c[3];
+  sppm_to_grid(g, pos, c);
+  const uint32_t e = sppm_entry(c[0], c[1], c[2]);
+  int64_t n = 0;
+  for (uint32_t k = g->start[e]; k < g->start[e + 1]; ++k) {
+    sppm_px *p = &px[g->list[k]];
+    const v3 d = vsub(p->pos, pos);
+    if (vdot(d, d) > p->radius * p->radius) continue;
+    const v3 dphi = vmul(phi, bsdf_eval_all(&p->b, wr, p->wr));
+    if (!(isfinite(dphi.x) && isfinite(dphi.y) && isfinite(dphi.z))) continue;
+    p->phi[0] += dphi.x;
+    p->phi[1] += dphi.y;
+    p->phi[2] += dphi.z;
+    p->m++;
+    ++n;
+  }
+  return n;
+}
+
+typedef struct light_emit {
+  v3 pos, dir, normal, radiance;
+  float pdf_pos, pdf_dir;
+} light_emit;
+
+/* Light::SampleEmit: AreaLight (area_light.cpp:60-73), DirectionalLight
+ * (directional_light.cpp:60-79), EnvLight (env_light.cpp:105-147). */
+static light_emit light_sample_emit(const light_rt *l, sampler *smp) {
+  light_emit r;
+  memset(&r, 0, sizeof(r));
+  if (l->type == ORC_LIGHT_AREA) {
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    v3 pos, ng;
+    float pdf_pos;
+    area_geom_sample(l, sam, &pos, &ng, &pdf_pos);
+    v2 s2;
+    s2.x = next1(smp);
+    s2.y = next1(smp);
+    float pdf_dir;
+    const v3 lwi = cosine_hemisphere(s2, &pdf_dir);
+    const frame f = make_frame(ng);
+    r.pos = pos;
+    r.dir = to_world(&f, lwi);
+    r.normal = ng;
+    r.radiance = l->radiance;
+    r.pdf_pos = pdf_pos;
+    r.pdf_dir = pdf_dir;
+  } else if (l->type == ORC_LIGHT_DIRECTIONAL) {
+    v2 c;
+    c.x = next1(smp);
+    c.y = next1(smp);
+    const v3 ref2light = to_world(&l->fr, uniform_cone(c, l->angle_cos));
+    const float pdf_dir = uniform_cone_pdf(l->angle_cos);
+    const v3 center = box_center(l->aabb);
+    const float radius = vlength(vsub(l->aabb.mx, center));
+    v2 d2;
+    d2.x = next1(smp);
+    d2.y = next1(smp);
+    const v2 disk = disk_concentric(d2);
+    const frame dc = make_frame(ref2light);
+    r.pos = vadd(center, vscale(vadd(vadd(vscale(dc.dx, disk.x), vscale(dc.dy, disk.y)), ref2light),
+                                radius));
+    r.dir = vneg(ref2light);
+    r.normal = vneg(vnormalize(ref2light));
+    r.radiance = l->radiance;
+    r.pdf_pos = 1.f / (ORC_PI * radius * radius);
+    r.pdf_dir = pdf_dir;
+  } else {
+    /* SampleContinuous(Next1D(), Next1D()): arguments drawn right to left */
+    const float second = next1(smp);
+    const float first = next1(smp);
+    float pdfs[2];
+    int iv;
+    const float v = distrib_sample_cont(&l->marg, second, &pdfs[1], &iv);
+    float u = distrib_sample_cont(&l->cond[iv], first, &pdfs[0], NULL);
+    float pdf = pdfs[0] * pdfs[1];
+    if (pdf == 0.f) return r;
+    u -= l->rot / 180 * ORC_PI * kInv2Pi;
+    if (u > 1.f) u -= 1.f;
+    if (u < 0.f) u += 1.f;
+    const float phi = u * kTwoPi;
+    const float theta = v * ORC_PI;
+    const float sin_t = sinf(theta);
+    pdf = sin_t != 0.f ? pdf / (2.f * ORC_PI * ORC_PI * sin_t) : 0.f;
+    v3 dir = V3(-sinf(theta) * cosf(phi), cosf(theta), -sinf(theta) * sinf(phi));
+    dir = vneg(dir);
+    const v3 center = box_center(l->aabb);
+    const float radius = vlength(vsub(l->aabb.mx, center));
+    v2 d2;
+    d2.x = next1(smp);
+    d2.y = next1(smp);
+    const v2 disk = disk_concentric(d2);
+    const frame dc = make_frame(vneg(dir));
+    r.pos = vadd(center, vscale(vsub(vadd(vscale(dc.dx, disk.x), vscale(dc.dy, disk.y)), dir),
+                                radius));
+    r.dir = dir;
+    r.normal = vnormalize(dir);
+    r.radiance = tex_sample(l, u, v);
+    r.pdf_pos = 1.f / (ORC_PI * radius * radius);
+    r.pdf_dir = pdf;
+  }
+  return r;
+}
+
+/* TracerPhoton (sppm.cpp:108-143); the light is picked by DefaultScene::
+ * SampleLight (default_scene.cpp:76-85) from a Distribution over equal
+ * weights (FP32 prefix sums, distribution.h:40-58). */
+static int64_t sppm_tracer_photon(const orc_scene *s, const orc_sppm_args *a,
+                                  const distrib *sel, float sel_pdf, const sppm_grid *g,
+                                  sppm_px *px, sampler *smp, uint64_t *q) {
+  if (s->n_lights == 0) return 0;
+  int idx;
+  (void)distrib_sample_cont(sel, next1(smp), NULL, &idx);
+  const light_rt *l = &s->lights[idx];
+  const light_emit em = light_sample_emit(l, smp);
+  if (is_black(em.radiance) || em.pdf_dir == 0.f || em.pdf_pos == 0.f) return 0;
+  v3 coef = vdivs(vscale(em.radiance, fabsf(vdot(em.normal, em.dir))),
+                  sel_pdf * em.pdf_pos * em.pdf_dir);
+  orc_ray ray = make_ray(em.pos, em.dir, ORC_EPS, INFINITY);
+  int64_t n = 0;
+  for (int depth = 1; depth <= a->photon_max_depth; ++depth) {
+    if (depth > a->photon_min_depth) {
+      if (next1(smp) > a->photon_cont_prob) return n;
+      coef = vdivs(coef, a->photon_cont_prob);
+    }
+    orc_hit h;
+    if (!scene_intersect(s, &ray, &h, q)) return n;
+    inct it = post_intersect(s, &ray, &h);
+    if (depth > 1) n += sppm_add_photon(g, px, it.pos, coef, it.wr);
+    uint32_t mi = s->prims[s->tri_prim[h.id]].material;
+    bsdf b = make_bsdf(&s->mats[mi], it.ng, it.ns);
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    bsdf_sample bs = bsdf_sample_all(&b, it.wr, sam);
+    if (!bs.valid) bs.f = V3(0, 0, 0), bs.pdf = 0.f;
+    if (is_black(bs.f) || bs.pdf < 0.f) return n;
+    coef = vmul(coef, vscale(vdivs(bs.f, bs.pdf), fabsf(vdot(it.ng, bs.dir))));
+    ray = make_ray(eps_offset(&it, bs.dir), bs.dir, ORC_EPS, INFINITY);
+  }
+  return n;
+}
+
+/* Test hook: the grid of visible points (pos, radius) with cell side `side`
+ * over a bound starting at bmin, and, per visible point, how many of the
+ * photon positions AddPhoton's lookup and distance test match to it. */
+void orc_sppm_grid_probe(const float *bmin, float side, uint32_t n_vp, const float *vp_pos,
+                         const float *vp_r, uint32_t n_ph, const float *ph_pos, uint32_t *counts) {
+  sppm_px *px = (sppm_px *)calloc(n_vp ? n_vp : 1, sizeof(sppm_px));
+  for (uint32_t i = 0; i < n_vp; ++i) {
+    px[i].valid = 1;
+    px[i].pos = vload(vp_pos + 3 * (size_t)i);
+    px[i].radius = vp_r[i];
+    counts[i] = 0;
+  }
+  sppm_grid g;
+  memset(&g, 0, sizeof(g));
+  g.bound.mn = vload(bmin);
+  g.side = side;
+  g.start = (uint32_t *)malloc(sizeof(uint32_t) * (SPPM_ENTRIES + 1));
+  sppm_grid_build(&g, px, n_vp);
+  for (uint32_t k = 0; k < n_ph; ++k) {
+    const v3 pos = vload(ph_pos + 3 * (size_t)k);
+    int c[3];
+    sppm_to_grid(&g, pos, c);
+    const uint32_t e = sppm_entry(c[0], c[1], c[2]);
+    for (uint32_t j = g.start[e]; j < g.start[e + 1]; ++j) {
+      const sppm_px *p = &px[g.list[j]];
+      const v3 d = vsub(p->pos, pos);
+      if (vdot(d, d) > p->radius * p->radius) continue;
+      counts[g.list[j]]++;
+    }
+  }
+  free(g.start);
+  free(g.list);
+  free(px);
+}
+
+/* UpdatePixel (sppm.cpp:145-167) */
+static void sppm_update(float alpha, sppm_px *p) {
+  if (p->m > 0) {
+    const float new_n = p->n + alpha * (float)p->m;
+    const float new_r = p->radius * sqrtf(new_n / (p->n + (float)p->m));
+    const v3 phi = V3(p->phi[0], p->phi[1], p->phi[2]);
+    p->tau = vdivs(vscale(vadd(p->tau, vmul(p->beta, phi)), new_r * new_r),
+                   p->radius * p->radius);
+    p->n = new_n;
+    p->radius = new_r;
+    p->m = 0;
+    p->phi[0] = p->phi[1] = p->phi[2] = 0.f;
+  }
+  p->beta = V3(0, 0, 0);
+}
+
+int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
+                        int32_t iterations, float *radiance, orc_sppm_pixel *state) {
+  const uint32_t W = (uint32_t)(int)cam->resolution[0], H = (uint32_t)(int)cam->resolution[1];
+  if (W % 64 || H % 64) return -1;
+  const uint32_t n = W * H;
+  if (iterations <= 0 || iterations > a->iteration_count) iterations = a->iteration_count;
+  sppm_px *px = (sppm_px *)calloc(n, sizeof(sppm_px));
+  float init_radius = a->init_radius;
+  if (init_radius < 0) init_radius = vlength(vsub(s->world.mx, s->world.mn)) / 1000.f;
+  sppm_grid g;
+  memset(&g, 0, sizeof(g));
+  g.bound.mn = vsub(s->world.mn, V3(init_radius, init_radius, init_radius));
+  g.bound.mx = vadd(s->world.mx, V3(init_radius, init_radius, init_radius));
+  g.start = (uint32_t *)malloc(sizeof(uint32_t) * (SPPM_ENTRIES + 1));
+  for (uint32_t i = 0; i < n; ++i) px[i].radius = init_radius;
+  /* light selector: pdf_table of ones normalised by its sum (default_scene.cpp:50-55) */
+  distrib sel;
+  float sel_pdf = 0.f;
+  const uint32_t nl = s->n_lights ? s->n_lights : 1;
+  {
+    float *tab = (float *)malloc(sizeof(float) * nl);
+    float sum = 0.f;
+    for (uint32_t i = 0; i < nl; ++i) tab[i] = 1.f;
+    for (uint32_t i = 0; i < nl; ++i) sum += tab[i];
+    const float ratio = 1.f / sum;
+    for (uint32_t i = 0; i < nl; ++i) tab[i] *= ratio;
+    sel_pdf = tab[0];
+    distrib_init(&sel, tab, nl, 0);
+    free(tab);
+  }
+  float max_radius = init_radius;
+  uint64_t q = 0;
+  int64_t deposits = 0;
+  for (int32_t it = 0; it < iterations; ++it) {
+    g.side = 1.05f * max_radius;
+    for (uint32_t y = 0; y < H; ++y)
+      for (uint32_t x = 0; x < W; ++x) {
+        sampler smp;
+        orc_pcg32_seed(&smp.rng, (uint64_t)(int64_t)(int)(x + y * W));
+        const float u = (float)x + next1(&smp);
+        const float v = (float)y + next1(&smp);
+        float lens[2];
+        lens[0] = next1(&smp);
+        lens[1] = next1(&smp);
+        orc_ray r;
+        orc_camera_ray(cam, u, v, lens, &r);
+        sppm_tracer_vp(s, a->forward_max_depth, r, &smp, &px[x + y * W], &q);
+      }
+    sppm_grid_build(&g, px, n);
+    for (int32_t i = 0; i < a->photons_per_iteration; ++i) {
+      sampler smp;
+      orc_pcg32_seed(&smp.rng, sample_seed(a->photon_seed_base, (uint32_t)i, (uint32_t)it));
+      deposits += sppm_tracer_photon(s, a, &sel, sel_pdf, &g, px, &smp, &q);
+    }
+    max_radius = 0.f;
+    for (uint32_t i = 0; i < n; ++i)
+      if (px[i].valid) max_radius = std_max(max_radius, px[i].radius);
+    if (max_radius == 0.f) max_radius = init_radius;
+    for (uint32_t i = 0; i < n; ++i)
+      if (px[i].valid) sppm_update(a->update_alpha, &px[i]);
+  }
+  /* ComputePixelRadiance (sppm.cpp:169-178) with the full photon count */
+  const uint64_t photon_n = (uint64_t)a->iteration_count * (uint64_t)a->photons_per_iteration;
+  for (uint32_t i = 0; i < n; ++i) {
+    const v3 direct = vdivs(px[i].direct, (float)a->iteration_count);
+    const float dem = (float)photon_n * ORC_PI * px[i].radius * px[i].radius;
+    const v3 L = vadd(direct, vdivs(px[i].tau, dem));
+    if (radiance) vstore(radiance + 3 * (size_t)i, L);
+    if (state) {
+      orc_sppm_pixel *o = &state[i];
+      vstore(o->pos, px[i].valid ? px[i].pos : V3(0, 0, 0));
+      vstore(o->beta, px[i].valid ? px[i].beta_traced : V3(0, 0, 0));
+      vstore(o->direct, px[i].direct);
+      vstore(o->tau, px[i].tau);
+      o->radius = px[i].radius;
+      o->n = px[i].n;
+      o->valid = (uint32_t)px[i].valid;
+      o->pad = 0;
+    }
+  }
+  distrib_free(&sel);
+  free(g.start);
+  free(g.list);
+  free(px);
+  return deposits;
+}
