@@ -63,7 +63,8 @@ struct SppmState {
   int *M;
   uint32_t *grid_count;              // kSppmEntries + 1
   uint32_t *grid_cursor;             // kSppmEntries
-  uint32_t *list;
+  uint32_t *list;                    // pixel of each registration
+  float4 *list_pr;                   // its visible point's position and radius
   uint32_t *maxr_bits;               // float bits of the largest radius (radii >= 0)
   unsigned long long *ctr;           // deposits, registrations, list overflow
 };
@@ -286,10 +287,8 @@ __device__ __forceinline__ float cell_side(const SppmState &S) {
 
 // AddVp (photon_mapping.h:84-101): count, then fill, the cells [min, max)
 template <bool FILL>
-__global__ void __launch_bounds__(256)
-k_sppm_grid(SppmParams P, SppmState S) {
-  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= P.n_pix || !S.valid[p]) return;
+__device__ __forceinline__ uint32_t sppm_register(const SppmParams &P, const SppmState &S,
+                                                  uint32_t p) {
   const float side = cell_side(S);
   const f3 c = ld3v(S.pos, p);
   const float r = S.radius[p];
@@ -301,7 +300,7 @@ k_sppm_grid(SppmParams P, SppmState S) {
   for (int k = 0; k < 3; ++k)
     if (mn[k] < 0 || mx[k] - mn[k] > 2) {
       if (!FILL) atomicAdd(&S.ctr[2], 1ull);
-      return;
+      return 0u;
     }
   uint32_t regs = 0;
   for (int z = mn[2]; z < mx[2]; ++z)
@@ -313,13 +312,25 @@ k_sppm_grid(SppmParams P, SppmState S) {
           ++regs;
         } else {
           const uint32_t at = S.grid_count[e] + atomicAdd(&S.grid_cursor[e], 1u);
-          if (at < P.list_cap)
+          if (at < P.list_cap) {
             S.list[at] = p;
-          else
+            S.list_pr[at] = make_float4(c.x, c.y, c.z, r);
+          } else
             atomicAdd(&S.ctr[2], 1ull);
         }
       }
-  if (!FILL && regs) atomicAdd(&S.ctr[1], (unsigned long long)regs);
+  return regs;
+}
+template <bool FILL>
+__global__ void __launch_bounds__(256)
+k_sppm_grid(SppmParams P, SppmState S) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  uint32_t regs = 0;
+  if (p < P.n_pix && S.valid[p]) regs = sppm_register<FILL>(P, S, p);
+  if (!FILL) {  // registrations: one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) regs += (uint32_t)__shfl_down((int)regs, off, 64);
+    if ((threadIdx.x & 63) == 0 && regs) atomicAdd(&S.ctr[1], (unsigned long long)regs);
+  }
 }
 
 // inclusive scan of grid_count[1..kSppmEntries] in one block (40 per thread)
@@ -431,10 +442,10 @@ __device__ uint32_t add_photon(const DevScene &sc, const SppmParams &P, const Sp
   const uint32_t b = S.grid_count[e], end = min(S.grid_count[e + 1], P.list_cap);
   uint32_t n = 0;
   for (uint32_t k = b; k < end; ++k) {
+    const float4 q = S.list_pr[k];
+    const f3 d = sub(mk3(q.x, q.y, q.z), pos);
+    if (dot(d, d) > q.w * q.w) continue;
     const uint32_t p = S.list[k];
-    const f3 d = sub(ld3v(S.pos, p), pos);
-    const float r = S.radius[p];
-    if (dot(d, d) > r * r) continue;
     const Bsdf bs = make_bsdf(sc.mats[S.mat[p]], ld3v(S.ng, p), ld3v(S.ns, p));
     const f3 dphi = mul(phi, bsdf_eval_all(bs, wr, ld3v(S.wr, p)));
     if (!finite3(dphi)) continue;
@@ -512,10 +523,16 @@ k_sppm_maxr_clear(SppmState S) {
 __global__ void __launch_bounds__(256)
 k_sppm_maxr(SppmParams P, SppmState S) {
   const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  __shared__ uint32_t wmax[4];
   uint32_t b = 0;
   if (p < P.n_pix && S.valid[p]) b = __float_as_uint(S.radius[p]);
   for (int off = 32; off > 0; off >>= 1) b = max(b, (uint32_t)__shfl_down((int)b, off, 64));
-  if ((threadIdx.x & 63) == 0 && b) atomicMax(S.maxr_bits, b);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // one atomic per block
+    const uint32_t m = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    if (m) atomicMax(S.maxr_bits, m);
+  }
 }
 
 // UpdatePixel (sppm.cpp:145-167); a zero largest radius falls back to the
@@ -605,6 +622,7 @@ static int ensure_sppm(ajg_ctx *ctx, SppmBuffers *b, uint32_t n, uint32_t cap) {
       (st = al((void **)&s.grid_count, 4 * (kSppmEntries + 1), "sppm grid")) ||
       (st = al((void **)&s.grid_cursor, 4 * kSppmEntries, "sppm grid")) ||
       (st = al((void **)&s.list, 4 * kSppmRegsPerVp * N, "sppm grid lists")) ||
+      (st = al((void **)&s.list_pr, 16 * kSppmRegsPerVp * N, "sppm grid lists")) ||
       (st = al((void **)&s.maxr_bits, 256, "sppm counters")) ||
       (st = al((void **)&s.ctr, 256, "sppm counters"))) {
     b->release();

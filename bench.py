@@ -97,8 +97,11 @@ def parse():
     ap.add_argument("--area-spp", type=int, default=64,
                     help="spp of the area-lit Cornell box render (0: skip)")
     ap.add_argument("--area-cpu-seconds", type=float, default=4.0)
+    ap.add_argument("--sppm-iters", type=int, default=100,
+                    help="SPPM iterations (x 100k photons) on the area-lit Cornell box (0: skip)")
+    ap.add_argument("--no-sppm-cpu", action="store_true")
     ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp", "guided",
-                                       "area"],
+                                       "area", "sppm"],
                     help="run one section alone and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -475,6 +478,65 @@ def area_bench(ctx, a):
     return out
 
 
+def sppm_bench(ctx, a):
+    """SPPM with the reference's default arguments (100 iterations x 100,000
+    photons, forward depth 8, photon depth 5..10, alpha 2/3) except the initial
+    radius, which takes the reference's scene-scaled option (init_radius < 0:
+    world-box diagonal / 1000; the default 0.1 is a tenth of this unit box),
+    on the area-lit Cornell box at 1024x1024. The oracle is timed on one
+    iteration of the same work (a visible point per pixel plus one
+    iteration's photons), which every iteration of the reference repeats."""
+    import dataclasses
+    import torch
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import GpuScene, SPPMRenderer
+
+    sc = scenes.cornell_area(("quad", "twosided", "mesh", "cube"), res=(1024, 1024))
+    spec = dataclasses.replace(scenes.SPPMSpec(), iteration_count=a.sppm_iters, photon_seed_base=1,
+                               init_radius=-1.0)
+    g = GpuScene(ctx, sc)
+    SPPMRenderer(g, dataclasses.replace(spec, iteration_count=1,
+                                        photons_per_iteration=1000)).render_device()
+    r = SPPMRenderer(g, spec, timing=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, _, st = r.render_device()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    w, h = sc.camera.res
+    out = {
+        "workload": "sppm: Cornell box, 4 area lights, 1024x1024, SPPMRendererArgs defaults "
+                    f"({spec.iteration_count} iterations x {spec.photons_per_iteration} photons, "
+                    "forward depth 8, photon depth 5..10, alpha 2/3) with init_radius -1 "
+                    "(world diagonal / 1000)",
+        "seconds": round(wall, 3),
+        "photons_per_s": round(st.photons / wall, 1),
+        "iterations_per_s": round(spec.iteration_count / wall, 2),
+        "deposits": int(st.deposits),
+        "visible_points": int(st.visible_points),
+        "ms_visible_points": round(st.ms_visible_points, 3),
+        "ms_photons_total": round(st.ms_photons, 3),
+        "ms_grid_total": round(st.ms_grid, 3),
+    }
+    g.close()
+    if not a.no_cpu_baseline and not a.no_sppm_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        o = orc.OracleScene(sc)
+        t = time.perf_counter()
+        o.render_sppm(dataclasses.replace(spec, iteration_count=1))
+        dt = time.perf_counter() - t
+        o.close()
+        est = dt * spec.iteration_count
+        out["cpu_baseline"] = {"value": round(1.0 / dt, 4), "unit": "iterations/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"1 of {spec.iteration_count} iterations ({w * h} visible "
+                                         f"points + {spec.photons_per_iteration} photons, "
+                                         f"{dt:.1f} s); whole render est. {est:.0f} s"}
+        out["gpu_over_cpu"] = round(est / wall, 1)
+    return out
+
+
 def film_bench(pt, film):
     """Film resolve (box, gaussian) and 8-bit sRGB encode of the heavy-mesh
     film: device time per call and algorithmic bytes (sums 28 B per traced
@@ -574,7 +636,7 @@ def main():
     if a.only:
         fn = {"batched": batched_bench, "instanced": instanced_bench,
               "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c),
-              "guided": guided_equal_time, "area": area_bench}[a.only]
+              "guided": guided_equal_time, "area": area_bench, "sppm": sppm_bench}[a.only]
         print(json.dumps({a.only: fn(ctx, a)}))
         return
     scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
@@ -651,6 +713,7 @@ def main():
     batched = batched_bench(ctx, a) if a.batch_jobs > 0 else None
     instanced = instanced_bench(ctx, a) if a.inst_spp > 0 else None
     area = area_bench(ctx, a) if a.area_spp > 0 else None
+    sppm = sppm_bench(ctx, a) if a.sppm_iters > 0 else None
     if instanced is not None and a.inst_guided_budget > 0:
         instanced["guided"] = guided_instanced(ctx, a)
 
@@ -695,6 +758,7 @@ def main():
         "batched": batched,
         "instanced": instanced,
         "area_lights": area,
+        "sppm": sppm,
         "mlp": mlp,
         "film": film_perf,
         "roofline": roof,
