@@ -116,6 +116,18 @@ SPPM_PIXEL_DTYPE = np.dtype([("pos", "<f4", 3), ("beta", "<f4", 3), ("direct", "
                              ("valid", "<u4"), ("pad", "<u4")])
 
 
+class VcmArgs(C.Structure):
+    _fields_ = [("iteration", C.c_int32), ("base_radius", C.c_float),
+                ("radius_alpha", C.c_float), ("use_vm", C.c_int32), ("use_vc", C.c_int32),
+                ("min_path_length", C.c_int32), ("max_path_length", C.c_int32),
+                ("cont_prob", C.c_float), ("seed_base", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class VcmStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("light_vertices", C.c_uint64), ("ms_total", C.c_float),
+                ("ms_light", C.c_float), ("ms_grid", C.c_float), ("ms_camera", C.c_float)]
+
+
 class GuideArgs(C.Structure):
     _fields_ = [("alpha", C.c_float), ("train", C.c_int32), ("batch", C.c_uint32),
                 ("lr", C.c_float), ("train_slots", C.c_uint32), ("max_steps", C.c_uint32),
@@ -195,6 +207,8 @@ _SIGS = {
                        C.POINTER(RenderStats), _P], C.c_int),
     "ajg_render_sppm": ([_P, C.POINTER(Camera), C.POINTER(SppmArgs), _P, _P,
                          C.POINTER(SppmStats), _P], C.c_int),
+    "ajg_render_vcm": ([_P, C.POINTER(Camera), C.POINTER(VcmArgs), _P, _P,
+                        C.POINTER(VcmStats), _P], C.c_int),
     "ajg_film_resolve": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P], C.c_int),
     "ajg_film_resolve_filter": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Filter), _P,
                                  _P, _P, _P], C.c_int),

@@ -355,6 +355,60 @@ class SPPMRenderer:
         return rgb.view(h, w, 3)
 
 
+class VCMRenderer:
+    """CreateVCMRenderer(VCMRendererArgs) (renderer.h:102-116) /
+    VCMRenderer::Render (vcm.cpp:135-147) on the device: per-pixel sums of
+    the camera sub-paths' colors over iterations plus the light-tracing
+    splats; `image` combines them as Film::WriteImage does."""
+
+    def __init__(self, gscene: GpuScene, spec=None, timing=False):
+        from .scenes import VCMSpec
+        self.g = gscene
+        sp = spec if spec is not None else VCMSpec()
+        self.spec = sp
+        self.args = capi.VcmArgs(sp.iteration, sp.base_radius, sp.radius_alpha, int(sp.use_vm),
+                                 int(sp.use_vc), sp.min_path_length, sp.max_path_length,
+                                 sp.cont_prob, sp.seed_base,
+                                 capi.RENDER_TIMING if timing else 0)
+
+    def render_device(self, cam_override=None, stream=None):
+        """-> (color_sum (H, W, 3), splat (H, W, 3)) device tensors, VcmStats."""
+        torch = _torch()
+        w, h = self.g.scene.camera.res
+        if cam_override is None:
+            cam = camera_setup(self.g.scene.camera)
+        else:
+            cam = capi.Camera()
+            C.memmove(C.byref(cam), C.byref(cam_override), C.sizeof(capi.Camera))
+        col = torch.empty(h * w * 3, dtype=torch.float32, device="cuda")
+        spl = torch.empty(h * w * 3, dtype=torch.float32, device="cuda")
+        stats = capi.VcmStats()
+        self.g.ctx.check(self.g.ctx.lib.ajg_render_vcm(
+            self.g.h, C.byref(cam), C.byref(self.args), _ptr(col), _ptr(spl), C.byref(stats),
+            stream), "ajg_render_vcm")
+        return col.view(h, w, 3), spl.view(h, w, 3), stats
+
+    def image(self, col_dev, spl_dev, ftype="box", radius=None, alpha=2.0):
+        """Film::WriteImage: the filtered camera-path colors (weight 1 per
+        iteration) plus the splats -> (H, W, 3) device tensor."""
+        torch = _torch()
+        h, w = col_dev.shape[:2]
+        t = capi.FILTER_BOX if ftype == "box" else capi.FILTER_GAUSSIAN
+        r = radius if radius is not None else ((0.5, 0.5) if t == capi.FILTER_BOX else (2.0, 2.0))
+        f = capi.Filter(t, (C.c_float * 2)(*r), alpha)
+        cnt = torch.full((h * w,), int(self.args.iteration), dtype=torch.int32, device="cuda")
+        rgb = torch.empty(h * w * 3, dtype=torch.float32, device="cuda")
+        self.g.ctx.check(self.g.ctx.lib.ajg_film_resolve_filter(
+            self.g.ctx.h, w, h, 1, C.byref(f), _ptr(col_dev.reshape(-1)), _ptr(cnt), _ptr(rgb),
+            None), "ajg_film_resolve_filter")
+        return rgb.view(h, w, 3) + spl_dev
+
+    # Renderer::Render (renderer.h:64-65)
+    def Render(self, cam_override=None):
+        col, spl, stats = self.render_device(cam_override)
+        return {"color_sum": col.cpu().numpy(), "splat": spl.cpu().numpy(), "stats": stats}
+
+
 class PathTracer:
     """CreatePTRenderer(PTRendererArgs) (renderer.h:37-45, 80) for TraceMIS on
     the device. Renders into per-traced-pixel film sums."""

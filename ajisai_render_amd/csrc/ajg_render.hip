@@ -1496,6 +1496,7 @@ static void scene_free(ajg_scene *s) {
   delete (RenderBuffers *)s->render_cache;
   delete (GuideBuffers *)s->guide_cache;
   sppm_free(s->sppm_cache);
+  vcm_free(s->vcm_cache);
   for (ajg_accel *b : s->blas) destroy_accel(b);
   (void)hipFree(s->insts);
   (void)hipFree(s->blases);

@@ -26,6 +26,7 @@ struct ajg_scene {
   void *render_cache = nullptr;  // RenderBuffers reused across renders
   void *guide_cache = nullptr;   // GuideBuffers reused across guided renders
   void *sppm_cache = nullptr;    // SppmBuffers reused across SPPM renders
+  void *vcm_cache = nullptr;     // VcmBuffers reused across VCM renders
   // instanced scenes: `accel` is the top-level tree over instance boxes
   std::vector<ajg_accel *> blas;   // one bottom-level tree per mesh
   ajg::DevInst *insts = nullptr;
@@ -133,5 +134,6 @@ __device__ __forceinline__ float lum_f(f3 c) {
 
 // The scene as kernels take it (by value).
 ajg::DevScene dev_scene(const ajg_scene *sc);
-// Frees a scene's SPPM buffers (sppm_cache).
+// Frees a scene's SPPM buffers (sppm_cache) / VCM buffers (vcm_cache).
 void sppm_free(void *cache);
+void vcm_free(void *cache);

@@ -633,7 +633,7 @@ struct LightSample {
 // EnvLight::Sample (env_light.cpp:68-103); u0 is SampleContinuous' first
 // argument (u), u1 its second (v).
 __device__ __forceinline__ LightSample env_sample(const DevEnv &e, f3 ref, float u0, float u1,
-                                                  float radius) {
+                                                  float radius, f3 *dir_out = nullptr) {
   LightSample r;
   r.pos = mk3(0, 0, 0);
   r.radiance = mk3(0, 0, 0);
@@ -656,6 +656,7 @@ __device__ __forceinline__ LightSample env_sample(const DevEnv &e, f3 ref, float
   const float sin_t = sinf(theta);
   pdf = sin_t != 0.f ? pdf / (2.f * AJG_PI * AJG_PI * sin_t) : 0.f;
   const f3 dir = mk3(-sinf(theta) * cosf(phi), cosf(theta), -sinf(theta) * sinf(phi));
+  if (dir_out) *dir_out = dir;
   r.pos = add(ref, scale(dir, 2.f * radius));
   r.radiance = tex_sample(e, u, v);
   r.pdf = pdf;

@@ -108,6 +108,21 @@ class SPPMSpec:
     photon_seed_base: int = 0
 
 
+@dataclasses.dataclass
+class VCMSpec:
+    """VCMRendererArgs (renderer.h:102-114) with the creator's defaults
+    (renderer_creators.cpp:111-118); seed_base seeds the path streams."""
+    iteration: int = 1
+    base_radius: float = 0.003
+    radius_alpha: float = 0.75
+    use_vm: bool = True
+    use_vc: bool = True
+    min_path_length: int = 0
+    max_path_length: int = 10
+    cont_prob: float = 0.9
+    seed_base: int = 0
+
+
 KEEP_MATERIAL = -1
 
 

@@ -559,6 +559,42 @@ int ajg_render_sppm(ajg_scene *scene, const ajg_camera *camera, const ajg_sppm_a
                     float *radiance, ajg_sppm_pixel *state, ajg_sppm_stats *stats,
                     void *stream);
 
+/* ------------------------------------------------------------------- VCM
+ * Replaces CreateVCMRenderer (renderer/renderer.h:102-116) /
+ * VCMRenderer::Render (renderer/vcm.cpp:135-147) over the GPU scene
+ * (one-level scenes). Fields and defaults of VCMRendererArgs
+ * (renderer_creators.cpp:111-118) plus seed_base: light sub-path i of
+ * iteration k draws from PCG32 seeded with mix(seed_base, i, 2k), the camera
+ * sub-path of pixel p from mix(seed_base, p, 2k+1) (the reference draws all
+ * paths from one sampler in sequence). One light sub-path per pixel per
+ * iteration, as the reference. color_sum (device float[W*H*3]) receives,
+ * per pixel, the sum over iterations of its camera sub-path's color (the
+ * film tile gets it with weight 1 per iteration: resolve it with
+ * ajg_film_resolve_filter, tile size 1, count = iteration); splat (device
+ * float[W*H*3]) receives the light-tracing splats (Film::AddSplat), summed
+ * over iterations; the reference's image is resolve + splat
+ * (Film::WriteImage). Synchronous. */
+typedef struct ajg_vcm_args {
+  int32_t iteration;        /* 1 */
+  float base_radius;        /* 0.003 (x the world box's smallest extent) */
+  float radius_alpha;       /* 0.75 */
+  int32_t use_vm, use_vc;   /* 1, 1 */
+  int32_t min_path_length;  /* 0 */
+  int32_t max_path_length;  /* 10 */
+  float cont_prob;          /* 0.9 */
+  uint32_t seed_base;
+  uint32_t flags;           /* AJG_RENDER_TIMING */
+} ajg_vcm_args;
+
+typedef struct ajg_vcm_stats {
+  uint64_t paths;            /* camera sub-paths (W*H x iteration) */
+  uint64_t light_vertices;   /* stored light vertices of the last iteration */
+  float ms_total, ms_light, ms_grid, ms_camera;  /* AJG_RENDER_TIMING */
+} ajg_vcm_stats;
+
+int ajg_render_vcm(ajg_scene *scene, const ajg_camera *camera, const ajg_vcm_args *args,
+                   float *color_sum, float *splat, ajg_vcm_stats *stats, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

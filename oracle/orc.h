@@ -330,6 +330,31 @@ int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_spp
 void orc_sppm_grid_probe(const float *bmin, float side, uint32_t n_vp, const float *vp_pos,
                          const float *vp_r, uint32_t n_ph, const float *ph_pos, uint32_t *counts);
 
+/* VCM (VCMRenderer, vcm.cpp:130-751). Fields and defaults of VCMRendererArgs
+ * (renderer.h:102-114, renderer_creators.cpp:111-118) plus seed_base: light
+ * path i of iteration k draws from PCG32 seeded with mix(seed_base, i, 2k),
+ * the camera path of pixel p from mix(seed_base, p, 2k+1). */
+typedef struct orc_vcm_args {
+  int32_t iteration;
+  float base_radius, radius_alpha;
+  int32_t use_vm, use_vc;
+  int32_t min_path_length, max_path_length;
+  float cont_prob;
+  uint32_t seed_base, pad;
+} orc_vcm_args;
+
+/* color_sum (w*h*3): per pixel, the sum over iterations of its camera path's
+ * color (the value the film's tile receives with weight 1 per iteration);
+ * splat (w*h*3): the light-tracing splats summed over iterations. Returns
+ * the number of light vertices of the last iteration. */
+int64_t orc_render_vcm(const orc_scene *s, const orc_camera *cam, const orc_vcm_args *a,
+                       float *color_sum, float *splat);
+/* Test hooks: ThinLensCamera::ToRaster (returns 0 when the point does not
+ * project into the film) and ImageAreaToSolidAngle. */
+int orc_vcm_to_raster(const orc_camera *cam, const float *u2, const float *pos, float *dir_out,
+                      float *raster_out);
+float orc_vcm_image_to_solid(const orc_camera *cam, const float *dir);
+
 #ifdef __cplusplus
 }
 #endif

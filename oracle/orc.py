@@ -69,6 +69,13 @@ class SppmArgs(C.Structure):
                 ("photon_seed_base", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class VcmArgs(C.Structure):
+    _fields_ = [("iteration", C.c_int32), ("base_radius", C.c_float),
+                ("radius_alpha", C.c_float), ("use_vm", C.c_int32), ("use_vc", C.c_int32),
+                ("min_path_length", C.c_int32), ("max_path_length", C.c_int32),
+                ("cont_prob", C.c_float), ("seed_base", C.c_uint32), ("pad", C.c_uint32)]
+
+
 SPPM_PIXEL_DTYPE = np.dtype([("pos", "<f4", 3), ("beta", "<f4", 3), ("direct", "<f4", 3),
                              ("tau", "<f4", 3), ("radius", "<f4"), ("n", "<f4"),
                              ("valid", "<u4"), ("pad", "<u4")])
@@ -137,6 +144,9 @@ def load():
         "orc_image_srgb8": ([C.c_uint32, C.c_uint32, _P, C.c_int, _P], None),
         "orc_render_sppm": ([_P, _P, _P, C.c_int32, _P, _P], C.c_int64),
         "orc_sppm_grid_probe": ([_P, C.c_float, C.c_uint32, _P, _P, C.c_uint32, _P, _P], None),
+        "orc_render_vcm": ([_P, _P, _P, _P, _P], C.c_int64),
+        "orc_vcm_to_raster": ([_P, _P, _P, _P, _P], C.c_int),
+        "orc_vcm_image_to_solid": ([_P, _P], C.c_float),
     }
     for name, (a, r) in sig.items():
         f = getattr(lib, name)
@@ -420,6 +430,19 @@ class OracleScene:
             raise ValueError("sppm: film width and height must be multiples of 64")
         return rad, st.reshape(h, w), int(n)
 
+    def render_vcm(self, spec, cam=None):
+        """VCM (the scene's VCMSpec): (color_sum (h, w, 3), splat (h, w, 3),
+        light vertices of the last iteration)."""
+        cam = cam or self.camera()
+        w, h = self.scene.camera.res
+        a = VcmArgs(spec.iteration, spec.base_radius, spec.radius_alpha, int(spec.use_vm),
+                    int(spec.use_vc), spec.min_path_length, spec.max_path_length, spec.cont_prob,
+                    spec.seed_base, 0)
+        col = np.zeros((h, w, 3), np.float32)
+        spl = np.zeros((h, w, 3), np.float32)
+        n = self.lib.orc_render_vcm(self.h, C.byref(cam), C.byref(a), _p(col), _p(spl))
+        return col, spl, int(n)
+
     def close(self):
         if self.h:
             self.lib.orc_scene_destroy(self.h)
@@ -443,6 +466,23 @@ def sppm_grid_probe(bmin, side, vp_pos, vp_r, ph_pos):
     lib.orc_sppm_grid_probe(_p(bmin), side, vp_pos.shape[0], _p(vp_pos), _p(vp_r),
                             ph_pos.shape[0], _p(ph_pos), _p(out))
     return out
+
+
+def vcm_to_raster(cam, u2, pos):
+    """ThinLensCamera::ToRaster: (ok, dir_to_camera, raster xy)."""
+    lib = load()
+    u2 = np.ascontiguousarray(u2, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    d = np.zeros(3, np.float32)
+    r = np.zeros(2, np.float32)
+    ok = lib.orc_vcm_to_raster(C.byref(cam), _p(u2), _p(pos), _p(d), _p(r))
+    return bool(ok), d, r
+
+
+def vcm_image_to_solid(cam, d):
+    lib = load()
+    d = np.ascontiguousarray(d, np.float32)
+    return float(lib.orc_vcm_image_to_solid(C.byref(cam), _p(d)))
 
 
 def film_resolve(w, h, ts, sums, counts):

@@ -2040,3 +2040,621 @@ int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_spp
   free(px);
   return deposits;
 }
+
+/* ---- VCM (VCMRenderer, vcm.cpp:130-751) ----------------------------------
+ * Restated as written, including: the reverse pdf of a scattering event is
+ * PdfAll(-new_dir, new_dir) (the state's direction is overwritten first,
+ * vcm.cpp:416-423); the camera path's first ray is offset twice (the
+ * primary ray's origin already carries the Ray constructor's offset and
+ * `Ray{origin, direction}` adds it again); a scattered path restarts from the
+ * hit point without EpsOffset; every vertex draws a BSDF sample (Next3D) that
+ * only its is_delta is read from; light tracing splats are summed over
+ * iterations while camera-path colors are averaged (Film::WriteImage).
+ * Streams: light path i of iteration k draws from PCG32 seeded with
+ * mix(seed_base, i, 2k), the camera path of pixel p from mix(seed_base, p,
+ * 2k+1); the reference draws every path from one sampler in sequence. */
+typedef struct vcm_state {
+  v3 origin, direction, throughput;
+  uint32_t path_length, is_finite_light, specular_path;
+  float dVCM, dVC, dVM;
+} vcm_state;
+
+typedef struct vcm_vertex {
+  v3 hit_point, in_dir, throughput;
+  uint32_t path_length;
+  v3 normal;
+  bsdf b;
+  float dVCM, dVC, dVM;
+} vcm_vertex;
+
+typedef struct vcm_ctx {
+  const orc_scene *s;
+  const orc_camera *cam;
+  const orc_vcm_args *a;
+  float w2c[16], c2r[16], w2r[16];
+  v3 cam_pos, cam_dir;
+  int W, H;
+  float light_count, mis_vm, mis_vc, vm_norm, radius, radius_sqr;
+  distrib sel;
+  float sel_pdf;
+  float *splat;
+  /* light vertices of the iteration (path order) */
+  vcm_vertex *lv;
+  uint32_t n_lv, cap_lv;
+  uint32_t *path_ends;
+  /* HashGrid (hashgrid.h:30-175) */
+  box3 gbox;
+  float cell_size, inv_cell;
+  uint32_t n_cells;
+  int *cell_ends;
+  int *indices;
+  uint64_t q;
+} vcm_ctx;
+
+static void camera_derived(const orc_camera *cam, float *w2c, float *c2r, float *w2r) {
+  double c2w[16], r2c[16], r2w[16], t[16];
+  for (int i = 0; i < 16; ++i) c2w[i] = cam->camera2world[i], r2c[i] = cam->raster2camera[i];
+  inv4d(c2w, t);
+  for (int i = 0; i < 16; ++i) w2c[i] = (float)t[i];
+  inv4d(r2c, t);
+  for (int i = 0; i < 16; ++i) c2r[i] = (float)t[i];
+  mul4d(c2w, r2c, r2w);
+  inv4d(r2w, t);
+  for (int i = 0; i < 16; ++i) w2r[i] = (float)t[i];
+}
+
+/* ThinLensCamera::ToRaster (thin_lens.cpp:85-119) */
+static int vcm_to_raster(const vcm_ctx *c, v2 u, v3 pos, v3 *dir_to_camera, v2 *p_raster) {
+  const orc_camera *cam = c->cam;
+  const v3 cam_coord = xform_point(c->w2c, pos);
+  v3 raster;
+  if (cam->lens_radius > 0 && cam->focal_distance > 0) {
+    v2 lens = disk_concentric(u);
+    lens.x = cam->lens_radius * lens.x;
+    lens.y = cam->lens_radius * lens.y;
+    const v3 origin = V3(lens.x, lens.y, 0.f);
+    const v3 direction = vnormalize(vsub(cam_coord, origin));
+    const float ft = cam->focal_distance / direction.z;
+    const v3 focus = vadd(origin, vscale(direction, ft));
+    raster = xform_point(c->c2r, focus);
+    if (!(raster.x < cam->resolution[0] && raster.x >= 0.f && raster.y < cam->resolution[1] &&
+          raster.y >= 0.f))
+      return 0;
+    *dir_to_camera = vsub(xform_point(cam->camera2world, origin), pos);
+  } else {
+    raster = xform_point(c->w2r, pos);
+    if (!(raster.x < cam->resolution[0] && raster.x >= 0.f && raster.y < cam->resolution[1] &&
+          raster.y >= 0.f))
+      return 0;
+    *dir_to_camera = vsub(c->cam_pos, pos);
+  }
+  p_raster->x = raster.x;
+  p_raster->y = raster.y;
+  if (vdot(c->cam_dir, vneg(*dir_to_camera)) <= 0.f) return 0;
+  return 1;
+}
+
+/* ImageAreaToSurfaceArea / ImageAreaToSolidAngle (thin_lens.cpp:121-147) */
+static float vcm_image_to_surface(const vcm_ctx *c, v3 dir_to_camera, float cos_to_camera,
+                                  float distance) {
+  const float cos_at_camera = vdot(c->cam_dir, vneg(vnormalize(dir_to_camera)));
+  const float d = c->cam->focal_distance / cos_at_camera;
+  const float image_to_solid = d * d / cos_at_camera;
+  return image_to_solid * fabsf(cos_to_camera) / (distance * distance);
+}
+static float vcm_image_to_solid(const vcm_ctx *c, v3 direction) {
+  const float cos_at_camera = vdot(c->cam_dir, vnormalize(direction));
+  const float d = c->cam->focal_distance / cos_at_camera;
+  return d * d / cos_at_camera;
+}
+
+static int vcm_pick_light(const vcm_ctx *c, sampler *smp) {
+  int idx;
+  (void)distrib_sample_cont(&c->sel, next1(smp), NULL, &idx);
+  return idx;
+}
+
+/* SampleScattering (vcm.cpp:409-451) */
+static int vcm_scatter(const vcm_ctx *c, vcm_state *st, sampler *smp, const bsdf *b,
+                       const inct *it) {
+  v3 sam;
+  sam.x = next1(smp);
+  sam.y = next1(smp);
+  sam.z = next1(smp);
+  bsdf_sample bs = bsdf_sample_all(b, vneg(st->direction), sam);
+  if (!bs.valid) bs.f = V3(0, 0, 0), bs.pdf = 0.f, bs.dir = V3(0, 0, 0);
+  if (is_black(bs.f)) return 0;
+  st->direction = bs.dir;
+  float dir_pdf = bs.pdf;
+  float rev_pdf = bsdf_pdf_all(b, vneg(st->direction), bs.dir);
+  if (next1(smp) > c->a->cont_prob) return 0;
+  dir_pdf *= c->a->cont_prob;
+  rev_pdf *= c->a->cont_prob;
+  const float cos_out = vdot(it->ng, bs.dir);
+  st->dVC = (cos_out / dir_pdf) * (st->dVC * rev_pdf + st->dVCM + c->mis_vm);
+  st->dVM = (cos_out / dir_pdf) * (st->dVM * rev_pdf + st->dVCM * c->mis_vc + 1.f);
+  st->dVCM = 1.f / dir_pdf;
+  st->specular_path &= 0;
+  st->origin = it->pos;
+  st->throughput = vmul(st->throughput, vscale(bs.f, cos_out / dir_pdf));
+  return 1;
+}
+
+/* ConnectToCamera (vcm.cpp:544-600) */
+static void vcm_connect_camera(vcm_ctx *c, const vcm_state *st, sampler *smp, const bsdf *b,
+                               const inct *it) {
+  v2 u;
+  u.x = next1(smp);
+  u.y = next1(smp);
+  v3 dir;
+  v2 img;
+  if (!vcm_to_raster(c, u, it->pos, &dir, &img)) return;
+  const float distance = vlength(dir);
+  dir = vnormalize(dir);
+  const v3 f = bsdf_eval_all(b, dir, it->wr);
+  if (is_black(f)) return;
+  float rev_pdf = bsdf_pdf_all(b, it->wr, dir);
+  rev_pdf *= c->a->cont_prob;
+  const float cos_to_cam = vdot(it->ng, dir);
+  const float i2s = vcm_image_to_surface(c, dir, cos_to_cam, distance);
+  const float w_light = (i2s / c->light_count) * (c->mis_vm + st->dVCM + st->dVC * rev_pdf);
+  const float mis = 1.f / (w_light + 1.f);
+  const float s2i = 1.f / i2s;
+  const v3 contrib = vdivs(vmul(vscale(st->throughput, mis), f), c->light_count * s2i);
+  if (is_black(contrib)) return;
+  orc_ray sr = make_ray(it->pos, dir, ORC_EPS, distance * 0.9f);
+  if (scene_occlude(c->s, &sr, &c->q)) return;
+  const int ix = (int)img.x, iy = (int)img.y;
+  float *px = c->splat + 3 * ((size_t)iy * c->W + ix);
+  px[0] += contrib.x;
+  px[1] += contrib.y;
+  px[2] += contrib.z;
+}
+
+/* Light::Sample as DirectIllumination reads it: position, normal, radiance,
+ * pdf, emit_pdf (area_light.cpp:37-58, env_light.cpp:68-103,
+ * directional_light.cpp:43-58). */
+typedef struct vcm_lsample {
+  v3 ref, pos, normal, radiance;
+  float pdf, emit_pdf;
+} vcm_lsample;
+static vcm_lsample vcm_light_sample(const vcm_ctx *c, const light_rt *l, v3 ref, sampler *smp) {
+  vcm_lsample r;
+  memset(&r, 0, sizeof(r));
+  if (l->type == ORC_LIGHT_AREA) {
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    v3 pos, ng;
+    float pdf_area;
+    area_geom_sample(l, sam, &pos, &ng, &pdf_area);
+    if (vdot(ng, vsub(ref, pos)) <= 0) return r;
+    const v3 i2r = vsub(ref, pos);
+    const float dist2 = vdot(i2r, i2r);
+    r.ref = ref;
+    r.pos = pos;
+    r.normal = ng;
+    r.radiance = l->radiance;
+    r.pdf = pdf_area * dist2 / fabsf(vdot(ng, i2r));
+    r.emit_pdf = pdf_area * vdot(ng, vnormalize(i2r)) / ORC_PI;
+  } else if (l->type == ORC_LIGHT_ENV) {
+    const float first = next1(smp);
+    const float second = next1(smp);
+    v3 dir;
+    light_sample ls = env_sample_uv(l, ref, second, first, &dir);
+    if (ls.pdf == 0.f && is_black(ls.radiance)) return r;
+    const float radius = vlength(vsub(l->aabb.mx, box_center(l->aabb)));
+    r.ref = ref;
+    r.pos = ls.pos;
+    r.normal = vneg(dir);
+    r.radiance = ls.radiance;
+    r.pdf = ls.pdf;
+    r.emit_pdf = ls.pdf / (ORC_PI * radius * radius);
+  } else {
+    v2 u;
+    u.x = next1(smp);
+    u.y = next1(smp);
+    light_sample ls = dir_sample(l, ref, u);
+    const float radius = vlength(vsub(l->aabb.mx, box_center(l->aabb)));
+    r.ref = ref;
+    r.pos = ls.pos;
+    r.radiance = ls.radiance;
+    r.pdf = ls.pdf;
+    r.emit_pdf = ls.pdf / (ORC_PI * radius * radius);
+  }
+  return r;
+}
+
+/* DirectIllumination (vcm.cpp:604-668) */
+static v3 vcm_direct(vcm_ctx *c, const vcm_state *st, const inct *it, const bsdf *b,
+                     sampler *smp) {
+  const v3 zero = V3(0, 0, 0);
+  const int idx = vcm_pick_light(c, smp);
+  const light_rt *l = &c->s->lights[idx];
+  const float pick = c->sel_pdf;
+  const vcm_lsample sr = vcm_light_sample(c, l, it->pos, smp);
+  if (is_black(sr.radiance)) return zero;
+  const v3 r2l = vnormalize(vsub(sr.pos, sr.ref));
+  const v3 bsdf_eval = bsdf_eval_all(b, r2l, it->wr);
+  if (is_black(bsdf_eval)) return zero;
+  float bsdf_pdf_w = bsdf_pdf_all(b, r2l, it->wr);
+  float bsdf_rev_pdf_w = bsdf_pdf_all(b, it->wr, r2l);
+  bsdf_pdf_w *= l->type == ORC_LIGHT_DIRECTIONAL ? 0.f : c->a->cont_prob;
+  bsdf_rev_pdf_w *= c->a->cont_prob;
+  const float w_light = bsdf_pdf_w / (pick * sr.pdf);
+  const float cos_to_light = vdot(it->ns, r2l);
+  const float cos_at_light = vdot(sr.normal, vneg(r2l));
+  const float w_camera = (sr.emit_pdf * cos_to_light / (sr.pdf * cos_at_light)) *
+                         (c->mis_vm + st->dVCM + st->dVC * bsdf_rev_pdf_w);
+  const float mis = 1.f / (w_light + 1.f + w_camera);
+  const v3 contrib = vscale(vmul(sr.radiance, bsdf_eval), mis * cos_to_light / (pick * sr.pdf));
+  if (is_black(contrib)) return zero;
+  orc_ray ray = make_ray(it->pos, r2l, ORC_EPS, vlength(vsub(sr.pos, sr.ref)));
+  if (scene_occlude(c->s, &ray, &c->q)) return zero;
+  return contrib;
+}
+
+/* ConnectVertices (vcm.cpp:474-542) */
+static v3 vcm_connect(vcm_ctx *c, const vcm_state *cs, const vcm_vertex *lv, const inct *it,
+                      const bsdf *b) {
+  const v3 zero = V3(0, 0, 0);
+  v3 direction = vsub(lv->hit_point, it->pos);
+  const float dist2 = vdot(direction, direction);
+  const float distance = vlength(direction);
+  direction = vnormalize(direction);
+  const v3 bsdf_eval = bsdf_eval_all(b, direction, it->wr);
+  if (is_black(bsdf_eval)) return zero;
+  float cam_dir_pdf = bsdf_pdf_all(b, direction, it->wr);
+  float cam_rev_pdf = bsdf_pdf_all(b, it->wr, direction);
+  cam_dir_pdf *= c->a->cont_prob;
+  cam_rev_pdf *= c->a->cont_prob;
+  const v3 light_eval = bsdf_eval_all(&lv->b, vneg(direction), lv->in_dir);
+  if (is_black(light_eval)) return zero;
+  float light_dir_pdf = bsdf_pdf_all(&lv->b, vneg(direction), lv->in_dir);
+  float light_rev_pdf = bsdf_pdf_all(&lv->b, lv->in_dir, vneg(direction));
+  light_dir_pdf *= c->a->cont_prob;
+  light_rev_pdf *= c->a->cont_prob;
+  const float g = vdot(lv->normal, vneg(direction)) * vdot(it->ns, direction) / dist2;
+  if (g < 0.f) return zero;
+  const float cam_dir_pdf_a = cam_dir_pdf * fabsf(vdot(lv->normal, vneg(direction))) / dist2;
+  const float light_dir_pdf_a = light_dir_pdf * fabsf(vdot(it->ns, direction)) / dist2;
+  const float w_light = cam_dir_pdf_a * (c->mis_vm + lv->dVCM + lv->dVC * light_rev_pdf);
+  const float w_camera = light_dir_pdf_a * (c->mis_vm + cs->dVCM + cs->dVC * cam_rev_pdf);
+  const float mis = 1.f / (w_light + 1.f + w_camera);
+  const v3 contrib = vmul(vscale(bsdf_eval, mis * g), light_eval);
+  if (is_black(contrib)) return zero;
+  orc_ray sr = make_ray(it->pos, direction, ORC_EPS, distance * 0.9f);
+  if (scene_occlude(c->s, &sr, &c->q)) return zero;
+  return contrib;
+}
+
+/* HashGrid::GetCellIndex (hashgrid.h:144-164) */
+static uint32_t vcm_cell(const vcm_ctx *c, int x, int y, int z) {
+  const uint32_t ux = (uint32_t)x, uy = (uint32_t)y, uz = (uint32_t)z;
+  return ((ux * 73856093u) ^ (uy * 19349663u) ^ (uz * 83492791u)) % c->n_cells;
+}
+static uint32_t vcm_cell_of(const vcm_ctx *c, v3 p) {
+  const v3 d = vsub(p, c->gbox.mn);
+  return vcm_cell(c, (int)floorf(c->inv_cell * d.x), (int)floorf(c->inv_cell * d.y),
+                  (int)floorf(c->inv_cell * d.z));
+}
+
+/* HashGrid::Build (hashgrid.h:34-70) */
+static void vcm_grid_build(vcm_ctx *c) {
+  c->cell_size = c->radius * 2.f;
+  c->inv_cell = 1.f / c->cell_size;
+  c->gbox = box_empty();
+  for (uint32_t i = 0; i < c->n_lv; ++i) {
+    c->gbox.mn = vmin(c->gbox.mn, c->lv[i].hit_point);
+    c->gbox.mx = vmax(c->gbox.mx, c->lv[i].hit_point);
+  }
+  c->indices = (int *)realloc(c->indices, sizeof(int) * (c->n_lv ? c->n_lv : 1));
+  memset(c->cell_ends, 0, sizeof(int) * c->n_cells);
+  for (uint32_t i = 0; i < c->n_lv; ++i) c->cell_ends[vcm_cell_of(c, c->lv[i].hit_point)]++;
+  int sum = 0;
+  for (uint32_t i = 0; i < c->n_cells; ++i) {
+    const int t = c->cell_ends[i];
+    c->cell_ends[i] = sum;
+    sum += t;
+  }
+  for (uint32_t i = 0; i < c->n_lv; ++i) {
+    const int at = c->cell_ends[vcm_cell_of(c, c->lv[i].hit_point)]++;
+    c->indices[at] = (int)i;
+  }
+}
+
+/* HashGrid::Process with RangeQuery::Process (hashgrid.h:72-136, vcm.cpp:88-126) */
+static v3 vcm_merge(const vcm_ctx *c, const vcm_state *cs, const inct *it, const bsdf *b) {
+  v3 contrib = V3(0, 0, 0);
+  const v3 q = it->pos;
+  const v3 dmin = vsub(q, c->gbox.mn), dmax = vsub(c->gbox.mx, q);
+  if (vminc(dmin) < 0.f || vminc(dmax) < 0.f) return contrib;
+  const v3 cp = vscale(dmin, c->inv_cell);
+  const v3 cf = V3(floorf(cp.x), floorf(cp.y), floorf(cp.z));
+  const int px = (int)cf.x, py = (int)cf.y, pz = (int)cf.z;
+  const v3 fr = vsub(cp, cf);
+  const int pxo = px + (fr.x < 0.5f ? -1 : +1);
+  const int pyo = py + (fr.y < 0.5f ? -1 : +1);
+  const int pzo = pz + (fr.z < 0.5f ? -1 : +1);
+  const int cells[8][3] = {{px, py, pz},   {px, py, pzo},   {px, pyo, pz},   {px, pyo, pzo},
+                           {pxo, py, pz},  {pxo, py, pzo},  {pxo, pyo, pz},  {pxo, pyo, pzo}};
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t ci = vcm_cell(c, cells[j][0], cells[j][1], cells[j][2]);
+    int lo = ci == 0 ? 0 : c->cell_ends[ci - 1], hi = c->cell_ends[ci];
+    for (; lo < hi; ++lo) {
+      const vcm_vertex *lv = &c->lv[c->indices[lo]];
+      const v3 d = vsub(q, lv->hit_point);
+      if (!(vdot(d, d) <= c->radius_sqr)) continue;
+      const uint32_t len = lv->path_length + cs->path_length;
+      if (len > (uint32_t)c->a->max_path_length || (int)len < c->a->min_path_length) continue;
+      const v3 light_dir = lv->in_dir;
+      const v3 bsdf_eval = bsdf_eval_all(b, light_dir, it->wr);
+      if (is_black(bsdf_eval)) continue;
+      float cam_dir_pdf = bsdf_pdf_all(b, light_dir, it->wr);
+      float cam_rev_pdf = bsdf_pdf_all(b, it->wr, light_dir);
+      if (cam_dir_pdf == 0.f || cam_rev_pdf == 0.f) continue;
+      cam_dir_pdf *= c->a->cont_prob;
+      cam_rev_pdf *= c->a->cont_prob;
+      const float w_light = lv->dVCM * c->mis_vc + lv->dVM * cam_dir_pdf;
+      const float w_camera = cs->dVCM * c->mis_vc + cs->dVM * cam_rev_pdf;
+      const float mis = 1.f / (w_light + 1.f + w_camera);
+      contrib = vadd(contrib, vmul(vscale(bsdf_eval, mis), lv->throughput));
+    }
+  }
+  return contrib;
+}
+
+/* Geometry::Pdf of an area light (for PdfBdpt, area_light.cpp:87-98) */
+static float vcm_area_pdf(const light_rt *l) {
+  float apdf = l->kind == ORC_GEOM_CUBE ? l->cube_pdf0 : 1.f / l->area;
+  if (l->two_sided) apdf = apdf * 0.5f;
+  return apdf;
+}
+
+static void vcm_light_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
+  const orc_scene *s = c->s;
+  const orc_vcm_args *a = c->a;
+  vcm_state st;
+  memset(&st, 0, sizeof(st));
+  /* GenerateLightSample (vcm.cpp:670-701) */
+  const int idx = vcm_pick_light(c, smp);
+  const light_rt *l = &s->lights[idx];
+  const light_emit em = light_sample_emit(l, smp);
+  st.throughput = vscale(em.radiance, fabsf(vdot(em.normal, em.dir)));
+  st.origin = em.pos;
+  st.direction = em.dir;
+  const float emission_pdf = em.pdf_pos * em.pdf_dir * c->sel_pdf;
+  st.throughput = vdivs(st.throughput, emission_pdf);
+  st.path_length = 1;
+  st.is_finite_light = l->type == ORC_LIGHT_AREA;
+  st.dVCM = 1.f / em.pdf_dir;
+  if (l->type != ORC_LIGHT_DIRECTIONAL) {
+    const float used_cos = st.is_finite_light ? fabsf(vdot(em.normal, em.dir)) : 1.f;
+    st.dVC = used_cos / emission_pdf;
+  } else {
+    st.dVC = 0.f;
+  }
+  st.dVM = st.dVC * c->mis_vc;
+  for (;; ++st.path_length) {
+    orc_ray ray = make_ray(st.origin, st.direction, ORC_EPS, INFINITY);
+    orc_hit h;
+    if (!scene_intersect(s, &ray, &h, &c->q)) break;
+    inct it = post_intersect(s, &ray, &h);
+    const bsdf b = make_bsdf(&s->mats[s->prims[s->tri_prim[h.id]].material], it.ng, it.ns);
+    (void)next1(smp), (void)next1(smp), (void)next1(smp); /* SampleAll(Next3D): is_delta only */
+    if (st.path_length > 1 || st.is_finite_light) st.dVCM *= h.t * h.t;
+    const float cos_in = fabsf(vdot(it.ns, vneg(vload(ray.d))));
+    st.dVCM /= fabsf(cos_in);
+    st.dVC /= fabsf(cos_in);
+    st.dVM /= fabsf(cos_in);
+    if (a->use_vc || a->use_vm) {
+      if (c->n_lv == c->cap_lv) {
+        c->cap_lv = c->cap_lv ? 2 * c->cap_lv : 1024;
+        c->lv = (vcm_vertex *)realloc(c->lv, sizeof(vcm_vertex) * c->cap_lv);
+      }
+      vcm_vertex *v = &c->lv[c->n_lv++];
+      v->hit_point = it.pos;
+      v->in_dir = it.wr;
+      v->throughput = st.throughput;
+      v->path_length = st.path_length;
+      v->b = b;
+      v->normal = it.ns;
+      v->dVCM = st.dVCM;
+      v->dVC = st.dVC;
+      v->dVM = st.dVM;
+    }
+    if (a->use_vc && (int)st.path_length + 1 >= a->min_path_length)
+      vcm_connect_camera(c, &st, smp, &b, &it);
+    if ((int)st.path_length + 2 > a->max_path_length) break;
+    if (!vcm_scatter(c, &st, smp, &b, &it)) break;
+  }
+  c->path_ends[path_idx] = c->n_lv;
+}
+
+static v3 vcm_camera_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
+  const orc_scene *s = c->s;
+  const orc_vcm_args *a = c->a;
+  vcm_state st;
+  memset(&st, 0, sizeof(st));
+  /* GenerateCameraSample (vcm.cpp:703-732) */
+  const int x = (int)path_idx % c->W, y = (int)path_idx / c->W;
+  const float u = (float)x + next1(smp);
+  const float v = (float)y + next1(smp);
+  float lens[2];
+  lens[0] = next1(smp);
+  lens[1] = next1(smp);
+  orc_ray primary;
+  orc_camera_ray(c->cam, u, v, lens, &primary);
+  const float camera_pdf = vcm_image_to_solid(c, vload(primary.d));
+  st.origin = vload(primary.o);
+  st.direction = vload(primary.d);
+  st.throughput = V3(1.f, 1.f, 1.f);
+  st.path_length = 1;
+  st.specular_path = 1;
+  st.dVCM = 1.f / camera_pdf;
+  v3 color = V3(0, 0, 0);
+  for (;; ++st.path_length) {
+    orc_ray ray = make_ray(st.origin, st.direction, ORC_EPS, INFINITY);
+    orc_hit h;
+    if (!scene_intersect(s, &ray, &h, &c->q)) {
+      if (s->env_index >= 0 && (int)st.path_length >= a->min_path_length)
+        color = vadd(color, vmul(st.throughput, env_radiance(&s->lights[s->env_index],
+                                                             vload(ray.d))));
+      break;
+    }
+    inct it = post_intersect(s, &ray, &h);
+    const bsdf b = make_bsdf(&s->mats[s->prims[s->tri_prim[h.id]].material], it.ng, it.ns);
+    (void)next1(smp), (void)next1(smp), (void)next1(smp); /* SampleAll(Next3D): is_delta only */
+    {
+      const float cos_in = fabsf(vdot(it.ns, vneg(vload(ray.d))));
+      st.dVCM *= h.t * h.t;
+      st.dVCM /= fabsf(cos_in);
+      st.dVC /= fabsf(cos_in);
+      st.dVM /= fabsf(cos_in);
+    }
+    const int li = s->prim_light[s->tri_prim[h.id]];
+    if (li >= 0) {
+      if ((int)st.path_length >= a->min_path_length) {
+        const light_rt *l = &s->lights[li];
+        const v3 radiance = vdot(it.ng, it.wr) > 0 ? l->radiance : V3(0, 0, 0);
+        if (!is_black(radiance)) {
+          if (st.path_length == 1) {
+            color = vadd(color, vmul(st.throughput, radiance));
+          } else if (a->use_vm && !a->use_vc) {
+            if (st.specular_path) color = vadd(color, vmul(st.throughput, radiance));
+          } else {
+            const float apdf = vcm_area_pdf(l);
+            float direct_pdf_a = apdf;
+            float emission_pdf_w = std_max(0.f, vdot(it.ng, it.wr)) / ORC_PI;
+            emission_pdf_w *= apdf;
+            direct_pdf_a *= a->cont_prob;
+            emission_pdf_w *= a->cont_prob;
+            const float w_camera = direct_pdf_a * st.dVCM + emission_pdf_w * st.dVC;
+            const float mis = 1.f / (1.f + w_camera);
+            color = vadd(color, vmul(vscale(st.throughput, mis), radiance));
+          }
+        }
+      }
+      break;
+    }
+    if ((int)st.path_length >= a->max_path_length) break;
+    if (a->use_vc && (int)st.path_length + 1 >= a->min_path_length)
+      color = vadd(color, vmul(st.throughput, vcm_direct(c, &st, &it, &b, smp)));
+    if (a->use_vc) {
+      const uint32_t lo = path_idx == 0 ? 0 : c->path_ends[path_idx - 1];
+      const uint32_t hi = c->path_ends[path_idx];
+      for (uint32_t i = lo; i < hi; ++i) {
+        const vcm_vertex *lv = &c->lv[i];
+        const int len = (int)(lv->path_length + 1 + st.path_length);
+        if (len < a->min_path_length || len > a->max_path_length) continue;
+        color = vadd(color, vmul(vmul(st.throughput, lv->throughput),
+                                 vcm_connect(c, &st, lv, &it, &b)));
+      }
+    }
+    if (a->use_vm) {
+      const v3 m = vcm_merge(c, &st, &it, &b);
+      color = vadd(color, vmul(vscale(st.throughput, c->vm_norm), m));
+    }
+    if (!vcm_scatter(c, &st, smp, &b, &it)) break;
+  }
+  return color;
+}
+
+int64_t orc_render_vcm(const orc_scene *s, const orc_camera *cam, const orc_vcm_args *a,
+                       float *color_sum, float *splat) {
+  vcm_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.s = s;
+  c.cam = cam;
+  c.a = a;
+  c.W = (int)cam->resolution[0];
+  c.H = (int)cam->resolution[1];
+  camera_derived(cam, c.w2c, c.c2r, c.w2r);
+  c.cam_pos = xform_point(cam->camera2world, V3(0.f, 0.f, 0.f));
+  c.cam_dir = vnormalize(xform_vector(cam->camera2world, V3(0.f, 0.f, -1.f)));
+  c.splat = splat;
+  const uint32_t n = (uint32_t)(c.W * c.H);
+  memset(color_sum, 0, sizeof(float) * 3 * n);
+  memset(splat, 0, sizeof(float) * 3 * n);
+  if (s->n_lights == 0) return 0;
+  {
+    const uint32_t nl = s->n_lights;
+    float *tab = (float *)malloc(sizeof(float) * nl);
+    float sum = 0.f;
+    for (uint32_t i = 0; i < nl; ++i) tab[i] = 1.f;
+    for (uint32_t i = 0; i < nl; ++i) sum += tab[i];
+    const float ratio = 1.f / sum;
+    for (uint32_t i = 0; i < nl; ++i) tab[i] *= ratio;
+    c.sel_pdf = tab[0];
+    distrib_init(&c.sel, tab, nl, 0);
+    free(tab);
+  }
+  c.path_ends = (uint32_t *)calloc(n, sizeof(uint32_t));
+  c.n_cells = n;
+  c.cell_ends = (int *)calloc(n, sizeof(int));
+  int64_t n_vertices = 0;
+  const v3 wsize = vsub(s->world.mx, s->world.mn);
+  for (int iter = 0; iter < a->iteration; ++iter) {
+    /* RenderIter (vcm.cpp:161-200) */
+    c.light_count = (float)(c.W * c.H);
+    float radius = a->base_radius * vminc(wsize);
+    radius /= powf((float)(iter + 1), 0.5f * (1.f - a->radius_alpha));
+    radius = std_max(radius, 1e-7f);
+    c.radius = radius;
+    c.radius_sqr = radius * radius;
+    c.vm_norm = 1.f / (c.radius_sqr * ORC_PI * c.light_count);
+    const float eta = (ORC_PI * c.radius_sqr) * c.light_count;
+    c.mis_vm = a->use_vm ? eta : 0.f;
+    c.mis_vc = a->use_vc ? 1.f / eta : 0.f;
+    c.n_lv = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      sampler smp;
+      orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, i, 2u * (uint32_t)iter));
+      vcm_light_path(&c, i, &smp);
+    }
+    n_vertices = c.n_lv;
+    if (a->use_vm) vcm_grid_build(&c);
+    for (uint32_t i = 0; i < n; ++i) {
+      sampler smp;
+      orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, i, 2u * (uint32_t)iter + 1u));
+      const v3 col = vcm_camera_path(&c, i, &smp);
+      color_sum[3 * (size_t)i] += col.x;
+      color_sum[3 * (size_t)i + 1] += col.y;
+      color_sum[3 * (size_t)i + 2] += col.z;
+    }
+  }
+  distrib_free(&c.sel);
+  free(c.path_ends);
+  free(c.cell_ends);
+  free(c.indices);
+  free(c.lv);
+  return n_vertices;
+}
+
+/* Test hooks: ThinLensCamera::ToRaster and ImageAreaToSolidAngle with the
+ * camera matrices VCM derives. */
+int orc_vcm_to_raster(const orc_camera *cam, const float *u2, const float *pos, float *dir_out,
+                      float *raster_out) {
+  vcm_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.cam = cam;
+  camera_derived(cam, c.w2c, c.c2r, c.w2r);
+  c.cam_pos = xform_point(cam->camera2world, V3(0.f, 0.f, 0.f));
+  c.cam_dir = vnormalize(xform_vector(cam->camera2world, V3(0.f, 0.f, -1.f)));
+  v2 u = {u2[0], u2[1]};
+  v3 d;
+  v2 r;
+  if (!vcm_to_raster(&c, u, vload(pos), &d, &r)) return 0;
+  vstore(dir_out, d);
+  raster_out[0] = r.x;
+  raster_out[1] = r.y;
+  return 1;
+}
+float orc_vcm_image_to_solid(const orc_camera *cam, const float *dir) {
+  vcm_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.cam = cam;
+  c.cam_dir = vnormalize(xform_vector(cam->camera2world, V3(0.f, 0.f, -1.f)));
+  return vcm_image_to_solid(&c, vload(dir));
+}
