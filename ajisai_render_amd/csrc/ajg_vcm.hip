@@ -370,10 +370,19 @@ k_vcm_bbox(VcmParams P, VcmState S) {
       mn[a] = min(mn[a], (uint32_t)__shfl_down((int)mn[a], off, 64));
       mx[a] = max(mx[a], (uint32_t)__shfl_down((int)mx[a], off, 64));
     }
-  if ((threadIdx.x & 63) == 0) {
+  __shared__ uint32_t wk[4][6];  // one atomic set per block
+  if ((threadIdx.x & 63) == 0)
     for (int a = 0; a < 3; ++a) {
-      if (mn[a] != 0xffffffffu) atomicMin(&S.bbox[a], mn[a]);
-      if (mx[a] != 0u) atomicMax(&S.bbox[3 + a], mx[a]);
+      wk[threadIdx.x >> 6][a] = mn[a];
+      wk[threadIdx.x >> 6][3 + a] = mx[a];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t m = min(min(wk[0][a], wk[1][a]), min(wk[2][a], wk[3][a]));
+      const uint32_t M = max(max(wk[0][3 + a], wk[1][3 + a]), max(wk[2][3 + a], wk[3][3 + a]));
+      if (m != 0xffffffffu) atomicMin(&S.bbox[a], m);
+      if (M != 0u) atomicMax(&S.bbox[3 + a], M);
     }
   }
 }

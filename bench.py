@@ -100,8 +100,11 @@ def parse():
     ap.add_argument("--sppm-iters", type=int, default=100,
                     help="SPPM iterations (x 100k photons) on the area-lit Cornell box (0: skip)")
     ap.add_argument("--no-sppm-cpu", action="store_true")
+    ap.add_argument("--vcm-iters", type=int, default=8,
+                    help="VCM iterations on the area-lit Cornell box at 1024x1024 (0: skip)")
+    ap.add_argument("--no-vcm-cpu", action="store_true")
     ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp", "guided",
-                                       "area", "sppm"],
+                                       "area", "sppm", "vcm"],
                     help="run one section alone and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -537,6 +540,57 @@ def sppm_bench(ctx, a):
     return out
 
 
+def vcm_bench(ctx, a):
+    """VCM with the reference's default arguments (radius 0.003 x the world
+    box's smallest extent, alpha 0.75, connections and merging, path length
+    0..10, continuation 0.9) on the area-lit Cornell box at 1024x1024, for
+    `--vcm-iters` iterations (one light and one camera sub-path per pixel
+    each); the oracle (one thread) is timed on one iteration of the same work."""
+    import dataclasses
+    import torch
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import GpuScene, VCMRenderer
+
+    sc = scenes.cornell_area(("quad", "twosided", "mesh", "cube"), res=(1024, 1024))
+    spec = dataclasses.replace(scenes.VCMSpec(), iteration=a.vcm_iters, seed_base=1)
+    g = GpuScene(ctx, sc)
+    VCMRenderer(g, dataclasses.replace(spec, iteration=1)).render_device()
+    r = VCMRenderer(g, spec, timing=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, _, st = r.render_device()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    w, h = sc.camera.res
+    paths = 2 * w * h * spec.iteration  # light + camera sub-paths
+    out = {
+        "workload": "vcm: Cornell box, 4 area lights, 1024x1024, VCMRendererArgs defaults "
+                    f"(radius 0.003, alpha 0.75, VC + VM, path length <= 10), "
+                    f"{spec.iteration} iterations",
+        "seconds": round(wall, 3),
+        "subpaths_per_s": round(paths / wall, 1),
+        "iterations_per_s": round(spec.iteration / wall, 2),
+        "light_vertices_last_iteration": int(st.light_vertices),
+        "ms_light": round(st.ms_light, 2), "ms_grid": round(st.ms_grid, 2),
+        "ms_camera": round(st.ms_camera, 2),
+    }
+    g.close()
+    if not a.no_cpu_baseline and not a.no_vcm_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        o = orc.OracleScene(sc)
+        t = time.perf_counter()
+        o.render_vcm(dataclasses.replace(spec, iteration=1))
+        dt = time.perf_counter() - t
+        o.close()
+        out["cpu_baseline"] = {"value": round(2 * w * h / dt, 1), "unit": "subpaths/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"1 of {spec.iteration} iterations ({2 * w * h} "
+                                         f"sub-paths, {dt:.1f} s)"}
+        out["gpu_over_cpu"] = round(out["subpaths_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def film_bench(pt, film):
     """Film resolve (box, gaussian) and 8-bit sRGB encode of the heavy-mesh
     film: device time per call and algorithmic bytes (sums 28 B per traced
@@ -636,7 +690,8 @@ def main():
     if a.only:
         fn = {"batched": batched_bench, "instanced": instanced_bench,
               "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c),
-              "guided": guided_equal_time, "area": area_bench, "sppm": sppm_bench}[a.only]
+              "guided": guided_equal_time, "area": area_bench, "sppm": sppm_bench,
+              "vcm": vcm_bench}[a.only]
         print(json.dumps({a.only: fn(ctx, a)}))
         return
     scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
@@ -714,6 +769,7 @@ def main():
     instanced = instanced_bench(ctx, a) if a.inst_spp > 0 else None
     area = area_bench(ctx, a) if a.area_spp > 0 else None
     sppm = sppm_bench(ctx, a) if a.sppm_iters > 0 else None
+    vcm = vcm_bench(ctx, a) if a.vcm_iters > 0 else None
     if instanced is not None and a.inst_guided_budget > 0:
         instanced["guided"] = guided_instanced(ctx, a)
 
@@ -759,6 +815,7 @@ def main():
         "instanced": instanced,
         "area_lights": area,
         "sppm": sppm,
+        "vcm": vcm,
         "mlp": mlp,
         "film": film_perf,
         "roofline": roof,
