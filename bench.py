@@ -671,6 +671,15 @@ def cpu_baseline(scene, target_s):
                       f"({n_paths} paths, {dt:.1f} s, {queries / dt / 1e6:.2f} Mrays/s)"}
 
 
+_T0 = time.time()
+
+
+def progress(msg):
+    """One progress line per finished section on stderr (the JSON result is
+    the only line on stdout)."""
+    print(f"[bench {time.time() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -762,20 +771,30 @@ def main():
                 "nodes_per_query": round(cs.nodes_fetched / max(1, closest_q), 2),
                 "tris_per_query": round(cs.tris_tested / max(1, closest_q), 2)}
 
+    progress("heavy-mesh path tracing timed")
     film_perf = film_bench(pt, film)
     mlp = mlp_bench(ctx)
+    progress("film and guide network timed")
     guided = guided_equal_time(ctx, a) if a.guided_budget > 0 else None
+    progress("guided equal-time comparison done")
     batched = batched_bench(ctx, a) if a.batch_jobs > 0 else None
+    progress("batched jobs done")
     instanced = instanced_bench(ctx, a) if a.inst_spp > 0 else None
+    progress("instanced scene done")
     area = area_bench(ctx, a) if a.area_spp > 0 else None
+    progress("area lights done")
     sppm = sppm_bench(ctx, a) if a.sppm_iters > 0 else None
+    progress("sppm done")
     vcm = vcm_bench(ctx, a) if a.vcm_iters > 0 else None
+    progress("vcm done")
     if instanced is not None and a.inst_guided_budget > 0:
         instanced["guided"] = guided_instanced(ctx, a)
+        progress("instanced guided comparison done")
 
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, a.cpu_seconds)
+        progress("cpu baseline done")
 
     value = paths / elapsed
     out = {
