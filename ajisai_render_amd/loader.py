@@ -8,9 +8,13 @@ the GPU path renders: the root keys "scene", "camera", "renderer"
 light, primitive_creators.cpp:65-72) included; "diffuse" and "disney"
 materials with "constant" textures; "directional" and "envmap" lights
 ("constant" or "hdr" textures, .pfm or Radiance .hdr files); the "thin_lens"
-camera with "box" or "gaussian" filters; the "pt" renderer; the "ploc"
-aggregate. Keys and defaults are the creators'. Anything else (spheres, media,
-other materials, textures, integrators) raises SceneError naming it.
+camera with "box" or "gaussian" filters; the "pt", "sppm" and "vcm" renderers
+(renderer_creators.cpp:27-135); the "ploc" aggregate, which also stands in for
+"native", "embree" and a missing aggregate key (the reference's default is
+the brute-force native aggregate, scene_creators.cpp:56-63; every aggregate
+returns the closest hit, so only exact ties and Embree's own triangle test can
+differ). Keys and defaults are the creators'. Anything else (spheres, media,
+other materials, textures) raises SceneError naming it.
 
 OBJ files are re-indexed as MeshGeoCreatorImpl does (geometry_creators.cpp:
 104-240): polygons fan-triangulated, one vertex per distinct (position,
@@ -47,6 +51,10 @@ class LoadedScene:
     scene: Scene
     filter: FilterSpec
     use_mis: bool = True
+    renderer: str = "pt"                  # "pt" | "sppm" | "vcm"
+    sppm: Optional[scenes.SPPMSpec] = None
+    vcm: Optional[scenes.VCMSpec] = None
+    aggregate: str = "ploc"               # the file's aggregate type (or "native")
 
 
 # ----------------------------------------------------------------- helpers
@@ -363,10 +371,10 @@ def load(source, base_dir: Optional[str] = None) -> LoadedScene:
         prims.append(dataclasses.replace(_geometry(p["geometry"], len(mats) - 1, base_dir),
                                          emission=emission))
     lights = [_light(l, base_dir) for l in (sc_node.get("lights", []) or [])]
-    agg = sc_node.get("aggregate") or {"type": "ploc"}
-    if agg.get("type") != "ploc":
-        raise SceneError(f"aggregate '{agg.get('type')}' is replaced by the GPU PLOC aggregate only")
-    radius = int(agg.get("search_radius", 14))
+    agg = sc_node.get("aggregate") or {"type": "native"}
+    if agg.get("type") not in ("ploc", "native", "embree"):
+        raise SceneError(f"aggregate '{agg.get('type')}' is not supported (ploc, native, embree)")
+    radius = int(agg.get("search_radius", 14)) if agg.get("type") == "ploc" else 14
     if cam_node.get("type") != "thin_lens":
         raise SceneError(f"camera type '{cam_node.get('type')}' is not supported (thin_lens)")
     res = _vec(cam_node.get("resolution"), 2, what="camera.resolution")
@@ -384,14 +392,39 @@ def load(source, base_dir: Optional[str] = None) -> LoadedScene:
                           float(fnode.get("alpha", 2.0)))
     else:
         raise SceneError(f"filter '{fnode.get('type')}' is not supported (box, gaussian)")
-    if r_node.get("type") != "pt":
-        raise SceneError(f"renderer '{r_node.get('type')}' is not supported (pt)")
-    use_mis = bool(r_node.get("use_mis", True))
-    if not use_mis:
-        raise SceneError("the GPU path tracer implements TraceMIS only (use_mis: true)")
-    pt = PTSpec(spp=int(r_node["spp"]), tile_size=int(r_node.get("tile_size", 16)),
-                min_bounces=int(r_node.get("min_bounces", 5)),
-                max_bounces=int(r_node.get("max_bounces", 10)),
-                cont_prob=float(r_node.get("cont_prob", 0.9)))
+    rtype = r_node.get("type")
+    sppm = vcm = None
+    use_mis = True
+    if rtype == "pt":
+        use_mis = bool(r_node.get("use_mis", True))
+        if not use_mis:
+            raise SceneError("the GPU path tracer implements TraceMIS only (use_mis: true)")
+        pt = PTSpec(spp=int(r_node["spp"]), tile_size=int(r_node.get("tile_size", 16)),
+                    min_bounces=int(r_node.get("min_bounces", 5)),
+                    max_bounces=int(r_node.get("max_bounces", 10)),
+                    cont_prob=float(r_node.get("cont_prob", 0.9)))
+    elif rtype == "sppm":  # SPPMCreatorImpl (renderer_creators.cpp:56-90)
+        pt = PTSpec()
+        sppm = scenes.SPPMSpec(
+            forward_max_depth=int(r_node.get("forward_max_depth", 8)),
+            init_radius=float(r_node.get("init_radius", 0.1)),
+            iteration_count=int(r_node.get("iteration_count", 100)),
+            photons_per_iteration=int(r_node.get("photons_per_iteration", 100000)),
+            photon_min_depth=int(r_node.get("photon_min_depth", 5)),
+            photon_max_depth=int(r_node.get("photon_max_depth", 10)),
+            photon_cont_prob=float(r_node.get("photon_cont_prob", 0.9)),
+            update_alpha=float(r_node.get("update_alpha", 2.0 / 3)))
+    elif rtype == "vcm":  # VCMCreatorImpl (renderer_creators.cpp:105-135)
+        pt = PTSpec()
+        vcm = scenes.VCMSpec(
+            iteration=int(r_node.get("iteration", 1)),
+            base_radius=float(r_node.get("radius", 0.003)),
+            radius_alpha=float(r_node.get("alpha", 0.75)),
+            use_vm=bool(r_node.get("use_vm", True)), use_vc=bool(r_node.get("use_vc", True)),
+            min_path_length=int(r_node.get("min_path_length", 0)),
+            max_path_length=int(r_node.get("max_path_length", 10)),
+            cont_prob=float(r_node.get("cont_prob", 0.9)))
+    else:
+        raise SceneError(f"renderer '{rtype}' is not supported (pt, sppm, vcm)")
     scene = Scene("yaml", prims, mats, lights, cam, pt, search_radius=radius)
-    return LoadedScene(scene, filt, use_mis)
+    return LoadedScene(scene, filt, use_mis, rtype, sppm, vcm, agg.get("type"))

@@ -125,3 +125,24 @@ def test_sppm_rejects_bad_inputs(gpu_ctx):
         SPPMRenderer(gi, _spec(iteration_count=1)).render_device()
     assert e.value.status == capi.AJG_ERR_INVALID
     gi.close()
+
+
+def test_sppm_scene_file(gpu_ctx, tmp_path):
+    """A scene file with an "sppm" renderer node renders like the oracle."""
+    import yaml
+    from ajisai_render_amd import loader
+    from test_loader import _cornell_yaml
+    cfg = yaml.safe_load(open(_cornell_yaml(str(tmp_path), lamp=True)))
+    cfg["camera"]["resolution"] = [64, 64]
+    cfg["renderer"] = {"type": "sppm", "iteration_count": 3, "photons_per_iteration": 10000}
+    ls = loader.load(cfg, base_dir=str(tmp_path))
+    g = GpuScene(gpu_ctx, ls.scene)
+    o = orc.OracleScene(ls.scene)
+    try:
+        cam = o.camera()
+        out = SPPMRenderer(g, ls.sppm).Render(cam_override=cam)
+        rad_o, _, _ = o.render_sppm(ls.sppm, cam=cam)
+    finally:
+        o.close()
+    assert _close(out["radiance"].reshape(-1, 3), rad_o.reshape(-1, 3), rel=1e-3).mean() >= 0.99
+    g.close()

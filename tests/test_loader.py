@@ -158,3 +158,29 @@ def test_emissive_primitive_is_an_area_light(tmp_path):
     assert lamp.emission == (9.0, 8.0, 6.0) and lamp.kind == scenes.GEOM_QUAD
     assert not lamp.two_sided
     assert all(max(p.emission) == 0 for p in sc.primitives[:-1])
+
+
+def test_sppm_vcm_renderers_and_aggregates(tmp_path):
+    """"sppm" and "vcm" renderer nodes with the creators' defaults and keys;
+    "native" / "embree" / no aggregate load onto the GPU PLOC tree."""
+    cfg = yaml.safe_load(open(_cornell_yaml(str(tmp_path), obj=False)))
+    c = yaml.safe_load(yaml.safe_dump(cfg))
+    c["renderer"] = {"type": "sppm", "photons_per_iteration": 5000}
+    ls = loader.load(c)
+    assert ls.renderer == "sppm" and ls.sppm == scenes.SPPMSpec(photons_per_iteration=5000)
+    c["renderer"] = {"type": "vcm", "radius": 0.01, "use_vm": False, "max_path_length": 6}
+    ls = loader.load(c)
+    assert ls.renderer == "vcm" and ls.vcm == scenes.VCMSpec(base_radius=0.01, use_vm=False,
+                                                               max_path_length=6)
+    for agg in ({"type": "native"}, {"type": "embree"}, None):
+        c2 = yaml.safe_load(yaml.safe_dump(cfg))
+        if agg is None:
+            del c2["scene"]["aggregate"]
+        else:
+            c2["scene"]["aggregate"] = agg
+        ls = loader.load(c2)
+        assert ls.aggregate == (agg["type"] if agg else "native")
+        assert ls.scene.search_radius == 14
+    c["renderer"] = {"type": "bdpt"}
+    with pytest.raises(loader.SceneError):
+        loader.load(c)
