@@ -158,10 +158,12 @@ struct Queues {
   DevRay *c_rays_in, *c_rays_out;
   float4 *c_hits_in;   // t, leaf slot (int bits), u, v
   const uint32_t *c_inst_in;  // instanced scenes: instance of each hit
-  uint32_t *c_count_in, *c_count_out;
+  uint32_t *c_count_out;   // kQStripes counter words of the striped queue
+  uint32_t c_len;          // entries per stripe
   DevRay *a_rays_in, *a_rays_out;
   uint8_t *a_occ_in;
-  uint32_t *a_count_in, *a_count_out;
+  uint32_t *a_count_out;
+  uint32_t a_len;
   uint32_t *live;
   uint32_t *work_counter;
   float *partial;      // total_work x 6 (sum3, sq3) or the film when n_chunks == 1
@@ -260,6 +262,71 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
   return want ? base + (uint32_t)__popcll(mask & lt) : kNone;
 }
 
+// Striped ray queues. Atomics on one counter word are served one at a time,
+// and every wave of a shade launch appends at the same few points of the
+// kernel, so a single counter per queue holds thousands of waves in line
+// (measured: one more append per wave on the closest-queue counter cost the
+// shade stage 0.85 ms per launch of 64K waves). Each queue is split into
+// kQStripes stripes: wave w appends to stripe w % kQStripes, whose entries
+// occupy [s R, s R + R) of the queue (R covers the stripe's worst case) and
+// whose counter word sits on a 256-byte line of its own. Consumers walk a
+// dense index v in [0, total) and map it through the stripes' prefix sums
+// to the entry; the entry index is what the shade stage records, and what
+// hits and occlusion results are stored at.
+static constexpr uint32_t kQStripes = 32;
+static constexpr uint32_t kQStride = 64;  // counter words between stripes
+static constexpr uint32_t kQWords = kQStripes * kQStride;
+
+// Entries one stripe holds: every wave of the stripe appending per_slot
+// entries for each of its 64 lanes.
+static inline uint32_t stripe_len(size_t slots, size_t per_slot) {
+  const size_t waves = (slots + 63) / 64;
+  return (uint32_t)((waves + kQStripes - 1) / kQStripes * 64 * per_slot);
+}
+
+struct QIn {
+  const uint32_t *ctr;  // kQStripes counter words, kQStride apart
+  uint32_t R;           // entries per stripe
+};
+
+__device__ __forceinline__ uint32_t q_append(bool want, uint32_t *ctr, uint32_t R, uint32_t stripe) {
+  const uint32_t i = wave_append(want, ctr + stripe * kQStride);
+  return i == kNone ? kNone : stripe * R + i;
+}
+
+// Prefix sums of a queue's stripe counts into s_pre[0 .. kQStripes]
+// (block-wide: every thread of the block calls it). Returns the total.
+__device__ __forceinline__ uint32_t q_prefix(const QIn &q, uint32_t *s_pre) {
+  if (threadIdx.x < 64) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t v = lane < kQStripes ? q.ctr[lane * kQStride] : 0u;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+    }
+    if (lane < kQStripes) s_pre[lane + 1] = v;
+    if (lane == 0) s_pre[0] = 0;
+  }
+  __syncthreads();
+  return s_pre[kQStripes];
+}
+
+// Entry of dense index v < total: the last stripe whose prefix is <= v.
+__device__ __forceinline__ uint32_t q_entry(const uint32_t *s_pre, uint32_t R, uint32_t v) {
+  uint32_t k = 0;
+  for (uint32_t step = kQStripes / 2; step > 0; step >>= 1)
+    if (s_pre[k + step] <= v) k += step;
+  return k * R + (v - s_pre[k]);
+}
+
+// Queue capacities (entries) for S slots: a closest-queue entry per slot for
+// the extension ray and one for the MIS-BSDF ray of scenes with area lights;
+// a shadow-queue entry per light and one for the MIS-BSDF ray.
+static inline size_t cq_capacity(size_t S) { return (size_t)kQStripes * stripe_len(S, kCqPerSlot); }
+static inline size_t aq_capacity(size_t S) {
+  return (size_t)kQStripes * stripe_len(S, kMaxLights + 1);
+}
+
 // BSDF value as the sampling routine reports it: LambertianReflection's
 // Sample_f multiplies in the shading-normal correction that its f() leaves
 // out (the plain path's continuation and MIS-BSDF ray use the sampled
@@ -337,21 +404,22 @@ static constexpr int kItersPerCheck = 4;
 
 template <int SPILL, bool COUNT>
 __global__ void __launch_bounds__(kTravThreads)
-k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count,
-             uint32_t *cursor, unsigned long long *counters) {
+k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, QIn qin, uint32_t *cursor,
+             unsigned long long *counters) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
-  const uint32_t n = *count;
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   uint32_t nn = 0, nt = 0;
   persistent_closest<SPILL, COUNT>(
       bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
-      [&](uint32_t i, DevBVH &) {
-        const DevRay ray = load_ray(rays, i);
+      [&](uint32_t v, DevBVH &) {
+        const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
         return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       },
-      [&](uint32_t i, const THit &h) {
-        hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+      [&](uint32_t v, const THit &h) {
+        hits[q_entry(s_pre, qin.R, v)] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
       },
       &nn, &nt);
   if (COUNT) {
@@ -362,19 +430,21 @@ k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, const uint32_t *count
 
 template <int SPILL, bool COUNT>
 __global__ void __launch_bounds__(kTravThreads)
-k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count, uint32_t *cursor,
+k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, QIn qin, uint32_t *cursor,
          unsigned long long *counters) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
-  const uint32_t n = *count;
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   uint32_t nn = 0, nt = 0;
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (kTravThreads / 64);
   (void)cursor;
   for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t i = c * 64 + lane;
-    if (i < n) {
+    const uint32_t v = c * 64 + lane;
+    if (v < n) {
+      const uint32_t i = q_entry(s_pre, qin.R, v);
       const DevRay ray = load_ray(rays, i);
       const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       occ[i] = trace_any<SPILL, COUNT>(bvh, r, s_node + threadIdx.x, s_entry + threadIdx.x, &nn,
@@ -391,22 +461,24 @@ k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, const uint32_t *count, ui
 template <int SPILL, int LDS>
 __global__ void __launch_bounds__(kTravThreads)
 k_pq_closest_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, float4 *hits,
-               const uint32_t *count, uint32_t *cursor) {
+               QIn qin, uint32_t *cursor) {
   __shared__ int s_node[LDS * kTravThreads];
   __shared__ float s_entry[LDS * kTravThreads];
-  const uint32_t n = *count;
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   const DevBVH first = jobs[0].sc.bvh;
   uint32_t nn = 0, nt = 0;
   persistent_closest<SPILL, false, true, LDS>(
       first, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
-      [&](uint32_t i, DevBVH &b) {
+      [&](uint32_t v, DevBVH &b) {
+        const uint32_t i = q_entry(s_pre, qin.R, v);
         b = jobs[job_of[i]].sc.bvh;
         const DevRay ray = load_ray(rays, i);
         return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       },
-      [&](uint32_t i, const THit &h) {
-        hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
+      [&](uint32_t v, const THit &h) {
+        hits[q_entry(s_pre, qin.R, v)] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
       },
       &nn, &nt);
 }
@@ -414,16 +486,18 @@ k_pq_closest_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays,
 template <int SPILL, int LDS>
 __global__ void __launch_bounds__(kTravThreads)
 k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uint8_t *occ,
-           const uint32_t *count) {
+           QIn qin) {
   __shared__ int s_node[LDS * kTravThreads];
   __shared__ float s_entry[LDS * kTravThreads];
-  const uint32_t n = *count;
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (kTravThreads / 64);
   for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t i = c * 64 + lane;
-    if (i < n) {
+    const uint32_t v = c * 64 + lane;
+    if (v < n) {
+      const uint32_t i = q_entry(s_pre, qin.R, v);
       const DevBVH b = jobs[job_of[i]].sc.bvh;
       const DevRay ray = load_ray(rays, i);
       const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
@@ -436,18 +510,20 @@ k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uin
 template <int SPILL>
 __global__ void __launch_bounds__(kTravThreads)
 k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
-              float4 *hits, uint32_t *hit_inst, const uint32_t *count, uint32_t *cursor) {
+              float4 *hits, uint32_t *hit_inst, QIn qin, uint32_t *cursor) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
-  const uint32_t n = *count;
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   persistent_closest_2l<SPILL, kLdsStack>(
       tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
-      [&](uint32_t i) {
-        const DevRay ray = load_ray(rays, i);
+      [&](uint32_t v) {
+        const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
         return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       },
-      [&](uint32_t i, const THit &h, uint32_t inst) {
+      [&](uint32_t v, const THit &h, uint32_t inst) {
+        const uint32_t i = q_entry(s_pre, qin.R, v);
         hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
         hit_inst[i] = inst;
       });
@@ -458,16 +534,18 @@ k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const De
 template <int SPILL>
 __global__ void __launch_bounds__(kTravThreads, 8)
 k_pq_any2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
-          uint8_t *occ, const uint32_t *count) {
+          uint8_t *occ, QIn qin) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
-  const uint32_t n = *count;
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   const int lane = threadIdx.x & 63;
   const uint32_t waves = gridDim.x * (kTravThreads / 64);
   for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t i = c * 64 + lane;
-    if (i < n) {
+    const uint32_t v = c * 64 + lane;
+    if (v < n) {
+      const uint32_t i = q_entry(s_pre, qin.R, v);
       const DevRay ray = load_ray(rays, i);
       const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       occ[i] = trace_any_2l<SPILL, kLdsStack>(tlas, insts, blases, r, s_node + threadIdx.x,
@@ -525,7 +603,7 @@ k_one_level_instance(const ajg_hit *hits, uint32_t n, uint32_t *inst) {
 }
 
 static void launch_pq2(int stack, const DevScene &ds, bool closest, const DevRay *rays, void *out,
-                       uint32_t *hit_inst, const uint32_t *count, uint32_t *cursor, uint32_t grid,
+                       uint32_t *hit_inst, QIn count, uint32_t *cursor, uint32_t grid,
                        hipStream_t s) {
   const dim3 g(grid), b(kTravThreads);
   if (closest) {
@@ -547,7 +625,7 @@ static void launch_pq2(int stack, const DevScene &ds, bool closest, const DevRay
 
 template <bool COUNT>
 static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *rays, void *out,
-                      const uint32_t *count, uint32_t *cursor, unsigned long long *ctr,
+                      QIn count, uint32_t *cursor, unsigned long long *ctr,
                       uint32_t grid, hipStream_t s) {
   const dim3 g(grid), b(kTravThreads);
   if (closest) {
@@ -569,7 +647,7 @@ static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *
 
 template <int SPILL>
 static void launch_pq_b(int lds, bool closest, const BatchJob *jobs, const uint16_t *job_of,
-                        const DevRay *rays, void *out, const uint32_t *count, uint32_t *cursor,
+                        const DevRay *rays, void *out, QIn count, uint32_t *cursor,
                         uint32_t grid, hipStream_t s) {
   const dim3 g(grid), b(kTravThreads);
 #define AJG_PQB(L)                                                                            \
@@ -626,13 +704,26 @@ struct StageTimer {
 
 // Start of a wavefront iteration: add the sizes of the queues traced in the
 // previous iteration to the totals, clear the queues the shade stage is
-// about to fill, the live counter and the traversal fetch cursors.
-__global__ void k_iter_begin(uint32_t *ctr, int cur, int nxt, unsigned long long *totals) {
-  if (threadIdx.x == 0) {
-    totals[0] += ctr[cur];
-    totals[1] += ctr[2 + cur];
-    ctr[nxt] = 0;
-    ctr[2 + nxt] = 0;
+// about to fill, the live flag and the traversal fetch cursors. qctr holds
+// the striped counters of closest queues 0 and 1, then shadow queues 0 and
+// 1, kQWords apart. One wave.
+__global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
+                             unsigned long long *totals) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t c = 0, a = 0;
+  if (lane < kQStripes) {
+    c = qctr[cur * kQWords + lane * kQStride];
+    a = qctr[(2 + cur) * kQWords + lane * kQStride];
+    qctr[nxt * kQWords + lane * kQStride] = 0;
+    qctr[(2 + nxt) * kQWords + lane * kQStride] = 0;
+  }
+  for (uint32_t o = 32; o > 0; o >>= 1) {
+    c += __shfl_down(c, o, 64);
+    a += __shfl_down(a, o, 64);
+  }
+  if (lane == 0) {
+    totals[0] += c;
+    totals[1] += a;
     ctr[4] = 0;
     ctr[6] = 0;
     ctr[7] = 0;
@@ -824,7 +915,7 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   }
   if (P.max_bounces <= 0) alive = false;
   // A path that dies before its first intersection contributes zero.
-  const uint32_t idx = wave_append(alive, q.c_count_out);
+  const uint32_t idx = q_append(alive, q.c_count_out, q.c_len, (slot >> 6) % kQStripes);
   if (alive) {
     store_ray(q.c_rays_out, idx, r);
     if (BATCH) B.c_job_out[idx] = (uint16_t)job;
@@ -847,6 +938,7 @@ template <bool GUIDED, bool BATCH>
 __global__ void __launch_bounds__(256)
 k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, BatchParams B) {
   const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t stripe = (slot >> 6) % kQStripes;  // queue stripe of this wave
   const bool valid = slot < (BATCH ? B.slots : P0.slots);
   const bool train = GUIDED && valid && G.train_stride && slot % G.train_stride == 0;
   const uint32_t hslot = train ? slot / G.train_stride : 0u;
@@ -1057,7 +1149,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
               }
             }
           }
-          const uint32_t si = wave_append(emit, q.a_count_out);
+          const uint32_t si = q_append(emit, q.a_count_out, q.a_len, stripe);
           if (emit) {
             store_ray(q.a_rays_out, si, sr);
             if (BATCH) B.a_job_out[si] = (uint16_t)j;
@@ -1065,40 +1157,61 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
             pf2 |= kPendLight0 << k;
           }
         }
-        // MIS-BSDF sample (MISSampleBsdf, helper.h:209-246)
-        {
-          bool emit = false;
-          DevRay mr;
-          const bool lit = sc.env_index >= 0 || sc.n_area > 0;
+        // Two independent draws of the BSDF sampler (of the guide mixture
+        // when guided): the MIS-BSDF sample (MISSampleBsdf, helper.h:209-246),
+        // then the continuation (pt.cpp:215-223). One loop over the two keeps
+        // a single inlined copy of the sampler in the kernel.
+        GSample g_mis, g_cont;
+#pragma unroll 1
+        for (int k = 0; k < 2; ++k) {
+          GSample g;
           if (GUIDED) {
-            const GSample g = guided_sample(bs, it.wr, gm, alpha, rng);
-            if (g.ok && lit) {
-              mr = make_ray(eps_offset(it, g.dir), g.dir, AJG_EPS, __int_as_float(0x7f800000));
-              st_f3(ps.pend_f, slot, g.f);
-              ps.pend_pdf[slot] = g.pdf;
-              ps.pend_pb[slot] = g.pb;
-              st_f3(ps.pend_ns, slot, it.ns);
-              emit = true;
-            }
+            g = guided_sample(bs, it.wr, gm, alpha, rng);
           } else {
             f3 sam;
             sam.x = pcg_next_float(rng);
             sam.y = pcg_next_float(rng);
             sam.z = pcg_next_float(rng);
-            BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
-            if (s.valid && !(is_black(s.f) || s.pdf == 0.f) && lit) {
-              const f3 dn = normalize(s.dir);
+            const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
+            g.dir = s.dir;
+            g.f = s.f;
+            g.pdf = s.pdf;
+            g.pb = s.pdf;
+            g.ok = s.valid;
+          }
+          if (k == 0)
+            g_mis = g;
+          else
+            g_cont = g;
+        }
+        // MIS-BSDF ray
+        {
+          bool emit = false;
+          DevRay mr;
+          const bool lit = sc.env_index >= 0 || sc.n_area > 0;
+          if (GUIDED) {
+            if (g_mis.ok && lit) {
+              mr = make_ray(eps_offset(it, g_mis.dir), g_mis.dir, AJG_EPS, __int_as_float(0x7f800000));
+              st_f3(ps.pend_f, slot, g_mis.f);
+              ps.pend_pdf[slot] = g_mis.pdf;
+              ps.pend_pb[slot] = g_mis.pb;
+              st_f3(ps.pend_ns, slot, it.ns);
+              emit = true;
+            }
+          } else {
+            if (g_mis.ok && !(is_black(g_mis.f) || g_mis.pdf == 0.f) && lit) {
+              const f3 dn = normalize(g_mis.dir);
               mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
-              st_f3(ps.pend_f, slot, s.f);
-              ps.pend_pdf[slot] = s.pdf;
+              st_f3(ps.pend_f, slot, g_mis.f);
+              ps.pend_pdf[slot] = g_mis.pdf;
               st_f3(ps.pend_ns, slot, it.ns);
               emit = true;
             }
           }
           // with area lights: a closest-hit query (the emitter it hits)
           const bool to_c = emit && sc.n_area > 0;
-          const uint32_t mc = wave_append(to_c, q.c_count_out);
-          const uint32_t ma = wave_append(emit && !to_c, q.a_count_out);
+          const uint32_t mc = q_append(to_c, q.c_count_out, q.c_len, stripe);
+          const uint32_t ma = q_append(emit && !to_c, q.a_count_out, q.a_len, stripe);
           if (to_c) {
             store_ray(q.c_rays_out, mc, mr);
             if (BATCH) B.c_job_out[mc] = (uint16_t)j;
@@ -1117,25 +1230,12 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
         DevRay nr;
         float cw = 0.f;
         {
-          f3 dir, fv;
-          float pdf;
-          if (GUIDED) {
-            const GSample g = guided_sample(bs, it.wr, gm, alpha, rng);
-            cont = g.ok;
-            dir = g.dir;
-            fv = g.f;
-            pdf = g.pdf;
-          } else {
-            f3 sam;
-            sam.x = pcg_next_float(rng);
-            sam.y = pcg_next_float(rng);
-            sam.z = pcg_next_float(rng);
-            const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
-            cont = s.valid && !(is_black(s.f) || s.pdf < 0.f);
-            if ((P.flags & AJG_RENDER_FINITE_PDF) && s.pdf == 0.f) cont = false;
-            dir = s.dir;
-            fv = s.f;
-            pdf = s.pdf;
+          const f3 dir = g_cont.dir, fv = g_cont.f;
+          const float pdf = g_cont.pdf;
+          cont = g_cont.ok;
+          if (!GUIDED) {
+            cont = cont && !(is_black(fv) || pdf < 0.f);
+            if ((P.flags & AJG_RENDER_FINITE_PDF) && pdf == 0.f) cont = false;
           }
           if (cont) {
             beta = mul(beta, divs(scale(fv, fabsf(dot(it.ng, dir))), pdf));
@@ -1152,7 +1252,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
             }
           }
         }
-        const uint32_t ni = wave_append(cont, q.c_count_out);
+        const uint32_t ni = q_append(cont, q.c_count_out, q.c_len, stripe);
         if (cont) {
           store_ray(q.c_rays_out, ni, nr);
           if (BATCH) B.c_job_out[ni] = (uint16_t)j;
@@ -1187,17 +1287,22 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
   next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live);
   if (valid) ps.rng[slot] = rng.state;
   const bool any_live = __ballot(valid && live) != 0ull;
-  if ((threadIdx.x & 63) == 0 && any_live) atomicAdd(q.live, 1u);
+  if ((threadIdx.x & 63) == 0 && any_live) *q.live = 1u;
 }
 
 // Guided rendering: position and unit shading normal of each closest-queue
-// hit (the guide network's input); misses get a zero record.
+// hit (the guide network's input), for every entry of the striped queue;
+// misses and unused entries get a zero record.
 __global__ void __launch_bounds__(256)
 k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t *hit_inst,
-             const uint32_t *count, float *pts) {
-  const uint32_t n = *count;
+             QIn qin, float *pts) {
+  __shared__ uint32_t s_pre[kQStripes + 1];
+  q_prefix(qin, s_pre);
+  const uint32_t n = kQStripes * qin.R;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const float4 h = hits[i];
+    const uint32_t k = i / qin.R;
+    const bool used = i - k * qin.R < s_pre[k + 1] - s_pre[k];
+    const float4 h = used ? hits[i] : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
     const int ls = __float_as_int(h.y);
     f3 p = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 1.f);
     if (ls >= 0) {
@@ -1363,6 +1468,7 @@ struct RenderBuffers {
   uint32_t *c_inst = nullptr;  // instance of each closest hit (instanced scenes)
   uint8_t *a_occ = nullptr;
   uint32_t *counters = nullptr;
+  uint32_t *qctr = nullptr;  // striped queue counters: closest 0, 1, shadow 0, 1
   float *partial = nullptr;
   uint32_t *partial_n = nullptr;
   // shadow rays are traced on a second stream, concurrently with the
@@ -1418,8 +1524,8 @@ static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint
     return r;
   };
   int st = 0;
-  if ((st = al((void **)&gb->pts, 24 * kCqPerSlot * S, "guide hit points")) ||
-      (st = al((void **)&gb->mix, 2 * (size_t)kGMix * kCqPerSlot * S, "guide mixtures")) ||
+  if ((st = al((void **)&gb->pts, 24 * cq_capacity(S), "guide hit points")) ||
+      (st = al((void **)&gb->mix, 2 * (size_t)kGMix * cq_capacity(S), "guide mixtures")) ||
       (st = al((void **)&gb->pend_pos, 12 * (size_t)S, "guide pending positions")) ||
       (st = al((void **)&gb->pend_tw, 4 * (size_t)kMaxLights * S, "guide NEE weights")) ||
       (st = al((void **)&gb->hist, 4 * (size_t)kGHist * T * D, "guide vertex history")) ||
@@ -1444,7 +1550,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
     return r;
   };
   PathState &ps = rb->ps;
-  const size_t cq_cap = kCqPerSlot * S, aq_cap = (size_t)(kMaxLights + 1) * S;
+  const size_t cq_cap = cq_capacity(S), aq_cap = aq_capacity(S);
   int st = 0;
   if ((st = al((void **)&ps.work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.sample, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.sample_end, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bounce, 4 * (size_t)S, "state")) ||
@@ -1471,6 +1577,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&rb->a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
       (st = al((void **)&rb->a_occ, aq_cap, "shadow results")) ||
       (st = al((void **)&rb->counters, 4 * kCounterWords, "counters")) ||
+      (st = al((void **)&rb->qctr, 4 * 4 * kQWords, "queue counters")) ||
       (st = al((void **)&rb->partial, sizeof(float) * 6 * total, "partial sums")) ||
       (st = al((void **)&rb->partial_n, 4 * total, "partial counts"))) {
     rb->release();
@@ -2239,7 +2346,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   }
   Queues q;
   memset(&q, 0, sizeof(q));
-  const size_t cq_cap = kCqPerSlot * S, aq_cap = (size_t)(kMaxLights + 1) * S;
+  const size_t cq_cap = cq_capacity(S), aq_cap = aq_capacity(S);
   DevRay **c_rays = rb->c_rays, **a_rays = rb->a_rays;
   float4 *c_hits = rb->c_hits;
   uint8_t *a_occ = rb->a_occ;
@@ -2271,6 +2378,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   AJG_RCHK(hipMemsetAsync(partial, 0, sizeof(float) * 6 * total, s));
   AJG_RCHK(hipMemsetAsync(partial_n, 0, 4 * total, s));
   AJG_RCHK(hipMemsetAsync(counters, 0, 4 * kCounterWords, s));
+  AJG_RCHK(hipMemsetAsync(rb->qctr, 0, 4 * 4 * kQWords, s));
   AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
   AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
   unsigned long long rec_before = 0;
@@ -2315,6 +2423,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   // an untrained guide is not sampled (alpha 0 until the first step)
   if (guided && batch && guide_steps(gd) == 0) G.alpha = 0.f;
   const uint32_t shade_grid = cdiv(S, 256);
+  // entries per queue stripe: the extension ray (and, with area lights, the
+  // MIS-BSDF ray) per slot; one shadow ray per light and the MIS-BSDF ray
+  const uint32_t c_len = stripe_len(S, ds.n_area ? 2 : 1);
+  const uint32_t a_len = stripe_len(S, ds.n_lights + 1);
   const uint32_t trav_grid = (uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU;
   // counter words: 0-1 closest queue sizes, 2-3 shadow queue sizes, 4 live
   // waves, 5 work items handed out, 6-7 traversal fetch cursors, 8-15 u64
@@ -2326,13 +2438,16 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       const int nxt = cur ^ 1;
       q.c_rays_in = c_rays[cur];
       q.c_rays_out = c_rays[nxt];
-      q.c_count_in = counters + cur;
-      q.c_count_out = counters + nxt;
+      q.c_count_out = rb->qctr + nxt * kQWords;
+      q.c_len = c_len;
       q.a_rays_in = a_rays[cur];
       q.a_rays_out = a_rays[nxt];
-      q.a_count_in = counters + 2 + cur;
-      q.a_count_out = counters + 2 + nxt;
-      hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
+      q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
+      q.a_len = a_len;
+      const QIn cq_in = {rb->qctr + nxt * kQWords, c_len};
+      const QIn aq_in = {rb->qctr + (2 + nxt) * kQWords, a_len};
+      hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, rb->qctr, cur, nxt,
+                         totals);
       timer.begin(2, s);
       if (guided)
         hipLaunchKernelGGL((k_shade<true, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G,
@@ -2345,26 +2460,26 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       AJG_RCHK(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       timer.begin(0, s);
       if (inst)
-        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, counters + nxt, counters + 6,
+        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, counters + 6,
                    trav_grid, s);
       else if (counting)
-        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, counters + 6,
+        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, counters + 6,
                         trav_ctr, trav_grid, s);
       else
-        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, counters + nxt, counters + 6,
+        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, counters + 6,
                          trav_ctr, trav_grid, s);
       timer.end(0, s);
       {
         hipStream_t sa = rb->aux;
         timer.begin(1, sa);
         if (inst)
-          launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, counters + 2 + nxt, nullptr,
+          launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, aq_in, nullptr,
                      trav_grid, sa);
         else if (counting)
-          launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
+          launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
                           counters + 7, trav_ctr + 2, trav_grid, sa);
         else
-          launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, counters + 2 + nxt,
+          launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
                            counters + 7, trav_ctr + 2, trav_grid, sa);
         timer.end(1, sa);
         AJG_RCHK(hipEventRecord(rb->ev_join, sa));
@@ -2372,11 +2487,12 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       if (guided) {
         // guide mixtures of the new closest hits, read by the next shade
         timer.begin(3, s);
-        hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(S, 256), 4096)), dim3(256), 0, s,
-                           ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
-                           (const uint32_t *)rb->c_inst, (const uint32_t *)(counters + nxt), gb->pts);
-        const uint32_t n_cq = ds.n_area ? (uint32_t)(kCqPerSlot * S) : S;
-        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, n_cq, counters + nxt, 6, nullptr,
+        // every entry of the striped queue (unused ones get a zero record)
+        const uint32_t n_cq = kQStripes * c_len;
+        hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(n_cq, 256), 4096)), dim3(256),
+                           0, s, ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
+                           (const uint32_t *)rb->c_inst, cq_in, gb->pts);
+        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, n_cq, nullptr, 6, nullptr,
                                           gb->mix, nullptr, s))
           return done(st);
         timer.end(3, s);
@@ -2513,10 +2629,10 @@ static int ensure_batch_buffers(ajg_ctx *ctx, BatchBuffers *bb, uint32_t S, uint
     if (r == AJG_OK) bb->bufs.push_back(*p);
     return r;
   };
-  const size_t aq = (size_t)(kMaxLights + 1) * S;
+  const size_t aq = aq_capacity(S);
   int st = 0;
-  if ((st = al((void **)&bb->c_job[0], 2 * kCqPerSlot * S, "batch ray jobs")) ||
-      (st = al((void **)&bb->c_job[1], 2 * kCqPerSlot * S, "batch ray jobs")) ||
+  if ((st = al((void **)&bb->c_job[0], 2 * cq_capacity(S), "batch ray jobs")) ||
+      (st = al((void **)&bb->c_job[1], 2 * cq_capacity(S), "batch ray jobs")) ||
       (st = al((void **)&bb->a_job[0], 2 * aq, "batch ray jobs")) ||
       (st = al((void **)&bb->a_job[1], 2 * aq, "batch ray jobs")) ||
       (st = al((void **)&bb->jobs, sizeof(BatchJob) * (size_t)n_jobs, "batch job table")) ||
@@ -2657,8 +2773,9 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   AJG_BCHK(hipMemsetAsync(rb->partial, 0, sizeof(float) * 6 * total, s));
   AJG_BCHK(hipMemsetAsync(rb->partial_n, 0, 4 * total, s));
   AJG_BCHK(hipMemsetAsync(rb->counters, 0, 4 * kCounterWords, s));
-  AJG_BCHK(hipMemsetAsync(rb->a_occ, 0, (size_t)(kMaxLights + 1) * S, s));
-  AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * kCqPerSlot * S, s));
+  AJG_BCHK(hipMemsetAsync(rb->a_occ, 0, aq_capacity(S), s));
+  AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * cq_capacity(S), s));
+  AJG_BCHK(hipMemsetAsync(rb->qctr, 0, 4 * 4 * kQWords, s));
   for (uint32_t j = 0; j < n_jobs; ++j) {
     const ajg_film &f = jobs[j].film;
     const size_t np = hj[j].P.n_pix;
@@ -2693,6 +2810,10 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   q.live = counters + 4;
   q.work_counter = counters + 5;
   const uint32_t shade_grid = cdiv(S, 256);
+  // queue stripes sized for the most any job can append
+  const uint32_t c_len = stripe_len(S, kCqPerSlot), a_len = stripe_len(S, kMaxLights + 1);
+  q.c_len = c_len;
+  q.a_len = a_len;
   const uint32_t trav_grid = (uint32_t)std::max(1, cu) * (uint32_t)ba.trav_blocks_per_cu;
   const int K = ba.iters_per_check;
   // One block of K iterations starting and ending on queue generation 0;
@@ -2708,31 +2829,32 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       const int nxt = cur ^ 1;
       q.c_rays_in = rb->c_rays[cur];
       q.c_rays_out = rb->c_rays[nxt];
-      q.c_count_in = counters + cur;
-      q.c_count_out = counters + nxt;
+      q.c_count_out = rb->qctr + nxt * kQWords;
       q.a_rays_in = rb->a_rays[cur];
       q.a_rays_out = rb->a_rays[nxt];
-      q.a_count_in = counters + 2 + cur;
-      q.a_count_out = counters + 2 + nxt;
+      q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
+      const QIn cq_in = {rb->qctr + nxt * kQWords, c_len};
+      const QIn aq_in = {rb->qctr + (2 + nxt) * kQWords, a_len};
       B.c_job_out = bb->c_job[nxt];
       B.a_job_out = bb->a_job[nxt];
       B.c_job_in = bb->c_job[cur];
       B.a_job_in = bb->a_job[cur];
-      hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, cur, nxt, totals);
+      hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, rb->qctr, cur, nxt,
+                         totals);
       hipLaunchKernelGGL((k_shade<false, true>), dim3(shade_grid), dim3(256), 0, s, sc0, P0, ps, q, G,
                          B);
       q_ok(hipEventRecord(rb->ev_fork, s));
       q_ok(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       if (spill == 48) {
         launch_pq_b<48>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
-                        counters + nxt, counters + 6, trav_grid, s);
+                        cq_in, counters + 6, trav_grid, s);
         launch_pq_b<48>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
-                        counters + 2 + nxt, counters + 7, trav_grid, rb->aux);
+                        aq_in, counters + 7, trav_grid, rb->aux);
       } else {
         launch_pq_b<1024>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
-                          counters + nxt, counters + 6, trav_grid, s);
+                          cq_in, counters + 6, trav_grid, s);
         launch_pq_b<1024>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
-                          counters + 2 + nxt, counters + 7, trav_grid, rb->aux);
+                          aq_in, counters + 7, trav_grid, rb->aux);
       }
       q_ok(hipEventRecord(rb->ev_join, rb->aux));
       q_ok(hipStreamWaitEvent(s, rb->ev_join, 0));

@@ -458,6 +458,8 @@ __device__ __forceinline__ BsdfSample disney_sample(const Bsdf &b, f3 wo, f3 sam
   f3 lwi = mk3(0, 0, 0);
   bool sample_coat = false;
   float rs0 = sam.x, rs1 = sam.y, rs2 = sam.z;
+  const float rough = clamp_r(m.roughness, 0.02f, 1.0f);
+  float alpha = rough * rough;
   if (m.clearcoat > 0.f) {
     const float cw = m.clearcoat / (m.clearcoat + luminance(bsdf_albedo(b)));
     const float fc = fresnel_schlick_coat(fabsf(lwo.z));
@@ -465,16 +467,18 @@ __device__ __forceinline__ BsdfSample disney_sample(const Bsdf &b, f3 wo, f3 sam
     if (sam.y < pc) {
       sample_coat = true;
       rs1 /= pc;
-      const float coat_rough = std_lerp(0.005f, 0.1f, m.clearcoat_gloss);
-      const f3 lwh = ggx_sample_vn(lwo, rs0, rs1, coat_rough);
-      lwi = reflect(neg(lwo), lwh);
+      alpha = std_lerp(0.005f, 0.1f, m.clearcoat_gloss);
     } else {
       rs1 = (sam.y - pc) / (1.f - pc);
     }
   }
-  if (!sample_coat) {
-    const float rough = clamp_r(m.roughness, 0.02f, 1.0f);
-    const f3 lwh = ggx_sample_vn(lwo, rs0, rs1, rough * rough);
+  // The coat and the base lobe draw their half vector with the same sampler
+  // and the same random numbers, only alpha differs: one call serves both
+  // (one inlined copy, and no lane waits through the other lobe's draw).
+  const f3 lwh = ggx_sample_vn(lwo, rs0, rs1, alpha);
+  if (sample_coat) {
+    lwi = reflect(neg(lwo), lwh);
+  } else {
     const float nr = std_lerp(0.0f, 0.08f, m.specular);
     const float odh = dot(lwo, lwh);
     const float prob_spec = fresnel_schlick(m, odh, nr);
