@@ -859,7 +859,13 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     if (sample >= ps.sample_end[slot]) work = kNone;
   }
   const bool fetch = need && work == kNone;
-  const uint32_t w = wave_append(fetch, q.work_counter);
+  // Once the counter has passed the last item no wave takes another, and
+  // every slot that finishes near the end of a render would otherwise add
+  // one more atomic on this word. (A plain read: the counter only grows, so
+  // a stale value only costs the atomic.)
+  const uint32_t limit = BATCH ? B.total_work : P0.total_work;
+  const bool open = __ballot(fetch) != 0ull && *q.work_counter < limit;
+  const uint32_t w = wave_append(fetch && open, q.work_counter);
   if (fetch) {
     if (w < (BATCH ? B.total_work : P0.total_work)) {
       uint32_t local = w;
@@ -934,8 +940,12 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   }
 }
 
+// Plain and batched shading run three waves per SIMD (at most 168
+// registers): measured faster than the compiler's own budget (two waves)
+// and than four (128 registers, with spills). The guided variant keeps the
+// compiler's budget (at three waves it would spill more than twice as much).
 template <bool GUIDED, bool BATCH>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED ? 1 : 3)))
 k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, BatchParams B) {
   const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
   const uint32_t stripe = (slot >> 6) % kQStripes;  // queue stripe of this wave
