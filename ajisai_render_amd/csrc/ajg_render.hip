@@ -276,6 +276,10 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
 static constexpr uint32_t kQStripes = 32;
 static constexpr uint32_t kQStride = 64;  // counter words between stripes
 static constexpr uint32_t kQWords = kQStripes * kQStride;
+// Fetch-cursor stripes of the closest-hit queue kernels (lane refill); the
+// cursor words follow the four queue counter blocks, kCursorStride apart.
+static constexpr uint32_t kCursorStripes = 16;
+static constexpr uint32_t kQctrWords = 4 * kQWords + kCursorStripes * kCursorStride;
 
 // Entries one stripe holds: every wave of the stripe appending per_slot
 // entries for each of its 64 lanes.
@@ -412,7 +416,7 @@ k_pq_closest(DevBVH bvh, const DevRay *rays, float4 *hits, QIn qin, uint32_t *cu
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   uint32_t nn = 0, nt = 0;
-  persistent_closest<SPILL, COUNT>(
+  persistent_closest<SPILL, COUNT, false, kLdsStack, kCursorStripes>(
       bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
       [&](uint32_t v, DevBVH &) {
         const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
@@ -469,7 +473,7 @@ k_pq_closest_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays,
   if (n == 0) return;
   const DevBVH first = jobs[0].sc.bvh;
   uint32_t nn = 0, nt = 0;
-  persistent_closest<SPILL, false, true, LDS>(
+  persistent_closest<SPILL, false, true, LDS, kCursorStripes>(
       first, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
       [&](uint32_t v, DevBVH &b) {
         const uint32_t i = q_entry(s_pre, qin.R, v);
@@ -516,7 +520,7 @@ k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const De
   __shared__ uint32_t s_pre[kQStripes + 1];
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
-  persistent_closest_2l<SPILL, kLdsStack>(
+  persistent_closest_2l<SPILL, kLdsStack, kCursorStripes>(
       tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
       [&](uint32_t v) {
         const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
@@ -706,7 +710,7 @@ struct StageTimer {
 // previous iteration to the totals, clear the queues the shade stage is
 // about to fill, the live flag and the traversal fetch cursors. qctr holds
 // the striped counters of closest queues 0 and 1, then shadow queues 0 and
-// 1, kQWords apart. One wave.
+// 1, kQWords apart, then the closest-hit fetch-cursor stripes. One wave.
 __global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
                              unsigned long long *totals) {
   const uint32_t lane = threadIdx.x;
@@ -717,6 +721,7 @@ __global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
     qctr[nxt * kQWords + lane * kQStride] = 0;
     qctr[(2 + nxt) * kQWords + lane * kQStride] = 0;
   }
+  if (lane < kCursorStripes) qctr[4 * kQWords + lane * kCursorStride] = 0;
   for (uint32_t o = 32; o > 0; o >>= 1) {
     c += __shfl_down(c, o, 64);
     a += __shfl_down(a, o, 64);
@@ -1587,7 +1592,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&rb->a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
       (st = al((void **)&rb->a_occ, aq_cap, "shadow results")) ||
       (st = al((void **)&rb->counters, 4 * kCounterWords, "counters")) ||
-      (st = al((void **)&rb->qctr, 4 * 4 * kQWords, "queue counters")) ||
+      (st = al((void **)&rb->qctr, 4 * kQctrWords, "queue counters")) ||
       (st = al((void **)&rb->partial, sizeof(float) * 6 * total, "partial sums")) ||
       (st = al((void **)&rb->partial_n, 4 * total, "partial counts"))) {
     rb->release();
@@ -2388,7 +2393,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   AJG_RCHK(hipMemsetAsync(partial, 0, sizeof(float) * 6 * total, s));
   AJG_RCHK(hipMemsetAsync(partial_n, 0, 4 * total, s));
   AJG_RCHK(hipMemsetAsync(counters, 0, 4 * kCounterWords, s));
-  AJG_RCHK(hipMemsetAsync(rb->qctr, 0, 4 * 4 * kQWords, s));
+  AJG_RCHK(hipMemsetAsync(rb->qctr, 0, 4 * kQctrWords, s));
   AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
   AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
   unsigned long long rec_before = 0;
@@ -2470,13 +2475,13 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       AJG_RCHK(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       timer.begin(0, s);
       if (inst)
-        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, counters + 6,
+        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, rb->qctr + 4 * kQWords,
                    trav_grid, s);
       else if (counting)
-        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, counters + 6,
+        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, rb->qctr + 4 * kQWords,
                         trav_ctr, trav_grid, s);
       else
-        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, counters + 6,
+        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, rb->qctr + 4 * kQWords,
                          trav_ctr, trav_grid, s);
       timer.end(0, s);
       {
@@ -2785,7 +2790,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   AJG_BCHK(hipMemsetAsync(rb->counters, 0, 4 * kCounterWords, s));
   AJG_BCHK(hipMemsetAsync(rb->a_occ, 0, aq_capacity(S), s));
   AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * cq_capacity(S), s));
-  AJG_BCHK(hipMemsetAsync(rb->qctr, 0, 4 * 4 * kQWords, s));
+  AJG_BCHK(hipMemsetAsync(rb->qctr, 0, 4 * kQctrWords, s));
   for (uint32_t j = 0; j < n_jobs; ++j) {
     const ajg_film &f = jobs[j].film;
     const size_t np = hj[j].P.n_pix;
@@ -2857,12 +2862,12 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       q_ok(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       if (spill == 48) {
         launch_pq_b<48>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
-                        cq_in, counters + 6, trav_grid, s);
+                        cq_in, rb->qctr + 4 * kQWords, trav_grid, s);
         launch_pq_b<48>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
                         aq_in, counters + 7, trav_grid, rb->aux);
       } else {
         launch_pq_b<1024>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
-                          cq_in, counters + 6, trav_grid, s);
+                          cq_in, rb->qctr + 4 * kQWords, trav_grid, s);
         launch_pq_b<1024>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
                           aq_in, counters + 7, trav_grid, rb->aux);
       }

@@ -321,18 +321,27 @@ static constexpr int kRefillMin = 48;
 
 // Wave-uniform refill: idle lanes take consecutive batch indices. Returns the
 // lane's new index (kNone if none) and sets *drained when the batch is used up.
+// With STRIPES > 1 the cursor is striped: wave w takes from cursor word
+// w % STRIPES (kCursorStride words apart), which hands out the 64-ray chunks
+// c = s (mod STRIPES) in order (cursor value j: chunk s + STRIPES (j / 64),
+// ray j % 64). Atomics on one word are served one at a time, and where rays
+// are short every wave comes back for more within a few steps.
+static constexpr uint32_t kCursorStride = 64;
+template <uint32_t STRIPES = 1>
 __device__ __forceinline__ uint32_t refill(bool idle, uint32_t n, uint32_t *cursor, bool *drained) {
   const uint64_t mask = __ballot(idle);
   const int lane = threadIdx.x & 63;
   const int leader = __ffsll((unsigned long long)mask) - 1;
   const uint32_t cnt = (uint32_t)__popcll(mask);
+  const uint32_t s = STRIPES == 1 ? 0u : ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % STRIPES;
+  auto at = [&](uint32_t j) { return STRIPES == 1 ? j : (s + STRIPES * (j >> 6)) * 64 + (j & 63); };
   uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(cursor, cnt);
+  if (lane == leader) base = atomicAdd(cursor + s * kCursorStride, cnt);
   base = __shfl(base, leader, 64);
-  if (base + cnt >= n) *drained = true;
+  if (at(base + cnt) >= n) *drained = true;
   if (!idle) return kNone;
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const uint32_t idx = base + (uint32_t)__popcll(mask & lt);
+  const uint32_t idx = at(base + (uint32_t)__popcll(mask & lt));
   return idx < n ? idx : kNone;
 }
 
@@ -341,8 +350,8 @@ __device__ __forceinline__ uint32_t refill(bool idle, uint32_t n, uint32_t *curs
 // in (batched renders); store(i, hit) writes its result. The per-lane state
 // is kept in plain locals (registers); only the stack spill array is
 // private memory.
-template <int SPILL, bool COUNT, bool PER_RAY_BVH = false, int LDS = kLdsStack, class Load,
-          class Store>
+template <int SPILL, bool COUNT, bool PER_RAY_BVH = false, int LDS = kLdsStack,
+          uint32_t STRIPES = 1, class Load, class Store>
 __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, uint32_t n,
                                                    uint32_t *cursor, int *lds_node,
                                                    float *lds_entry, Load load, Store store,
@@ -362,7 +371,7 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
   for (;;) {
     const bool idle = ray == kNone;
     if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
-      const uint32_t idx = refill(idle, n, cursor, &drained);
+      const uint32_t idx = refill<STRIPES>(idle, n, cursor, &drained);
       if (idx != kNone) {
         r = load(idx, bvh_lane);
         h.t = __int_as_float(0x7f800000);
@@ -471,7 +480,7 @@ __device__ __forceinline__ uint32_t tlas_leaf_instance(const DevBVH &tlas, int s
 // Closest hit over a two-level scene; store(i, hit, instance). The world
 // ray is kept as origin and direction only (registers); its reciprocal
 // direction is recomputed (same operations) when a mesh's tree is left.
-template <int SPILL, int LDS, class Load, class Store>
+template <int SPILL, int LDS, uint32_t STRIPES = 1, class Load, class Store>
 __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const DevInst *insts,
                                                       const DevBlas *blases, uint32_t n,
                                                       uint32_t *cursor, int *lds_node,
@@ -493,7 +502,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
   for (;;) {
     const bool idle = ray == kNone;
     if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
-      const uint32_t idx = refill(idle, n, cursor, &drained);
+      const uint32_t idx = refill<STRIPES>(idle, n, cursor, &drained);
       if (idx != kNone) {
         r = load(idx);
         wo = r.o;
