@@ -154,12 +154,20 @@ struct PathState {
   uint32_t *job, *pend_job;
 };
 
+// A striped ray queue as its consumers see it (the striping is described
+// with q_append below).
+struct QIn {
+  const uint32_t *ctr;  // kQStripes counter words, kQStride apart
+  uint32_t R;           // entries per stripe
+};
+
 struct Queues {
   DevRay *c_rays_in, *c_rays_out;
   float4 *c_hits_in;   // t, leaf slot (int bits), u, v
   const uint32_t *c_inst_in;  // instanced scenes: instance of each hit
   uint32_t *c_count_out;   // kQStripes counter words of the striped queue
   uint32_t c_len;          // entries per stripe
+  QIn c_in;                // the traced closest queue's stripes (guided: mixture index)
   DevRay *a_rays_in, *a_rays_out;
   uint8_t *a_occ_in;
   uint32_t *a_count_out;
@@ -279,7 +287,7 @@ static constexpr uint32_t kQWords = kQStripes * kQStride;
 // Fetch-cursor stripes of the closest-hit queue kernels (lane refill); the
 // cursor words follow the four queue counter blocks, kCursorStride apart.
 static constexpr uint32_t kCursorStripes = 16;
-static constexpr uint32_t kQctrWords = 4 * kQWords + kCursorStripes * kCursorStride;
+static constexpr uint32_t kQctrWords = 4 * kQWords + 2 * kCursorStripes * kCursorStride;
 
 // Entries one stripe holds: every wave of the stripe appending per_slot
 // entries for each of its 64 lanes.
@@ -287,11 +295,6 @@ static inline uint32_t stripe_len(size_t slots, size_t per_slot) {
   const size_t waves = (slots + 63) / 64;
   return (uint32_t)((waves + kQStripes - 1) / kQStripes * 64 * per_slot);
 }
-
-struct QIn {
-  const uint32_t *ctr;  // kQStripes counter words, kQStride apart
-  uint32_t R;           // entries per stripe
-};
 
 __device__ __forceinline__ uint32_t q_append(bool want, uint32_t *ctr, uint32_t R, uint32_t stripe) {
   const uint32_t i = wave_append(want, ctr + stripe * kQStride);
@@ -442,19 +445,13 @@ k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, QIn qin, uint32_t *cursor
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   uint32_t nn = 0, nt = 0;
-  const int lane = threadIdx.x & 63;
-  const uint32_t waves = gridDim.x * (kTravThreads / 64);
-  (void)cursor;
-  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t v = c * 64 + lane;
-    if (v < n) {
-      const uint32_t i = q_entry(s_pre, qin.R, v);
-      const DevRay ray = load_ray(rays, i);
-      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-      occ[i] = trace_any<SPILL, COUNT>(bvh, r, s_node + threadIdx.x, s_entry + threadIdx.x, &nn,
-                                       &nt) ? 1 : 0;
-    }
-  }
+  persistent_any<SPILL, COUNT, kLdsStack, kCursorStripes>(
+      bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t v) {
+        const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
+        return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      },
+      [&](uint32_t v, bool hit) { occ[q_entry(s_pre, qin.R, v)] = hit ? 1 : 0; }, &nn, &nt);
   if (COUNT) {
     wave_count_add(nn, counters);
     wave_count_add(nt, counters + 1);
@@ -721,7 +718,7 @@ __global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
     qctr[nxt * kQWords + lane * kQStride] = 0;
     qctr[(2 + nxt) * kQWords + lane * kQStride] = 0;
   }
-  if (lane < kCursorStripes) qctr[4 * kQWords + lane * kCursorStride] = 0;
+  if (lane < 2 * kCursorStripes) qctr[4 * kQWords + lane * kCursorStride] = 0;
   for (uint32_t o = 32; o > 0; o >>= 1) {
     c += __shfl_down(c, o, 64);
     a += __shfl_down(a, o, 64);
@@ -952,6 +949,9 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
 template <bool GUIDED, bool BATCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED ? 1 : 3)))
 k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, BatchParams B) {
+  // guided: the guide mixtures are stored by dense index of the closest queue
+  __shared__ uint32_t s_pre[GUIDED ? kQStripes + 1 : 1];
+  if (GUIDED) q_prefix(q.c_in, s_pre);
   const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
   const uint32_t stripe = (slot >> 6) % kQStripes;  // queue stripe of this wave
   const bool valid = slot < (BATCH ? B.slots : P0.slots);
@@ -1088,7 +1088,11 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
       } else {
         const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w, sc.inst ? q.c_inst_in[ei] : 0u);
         const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
-        const _Float16 *gm = GUIDED ? G.mix + (size_t)ei * kGMix : nullptr;
+        const _Float16 *gm = nullptr;
+        if (GUIDED) {
+          const uint32_t k = ei / q.c_in.R;
+          gm = G.mix + (size_t)(s_pre[k] + (ei - k * q.c_in.R)) * kGMix;
+        }
         // The guide is mixed in only where the BSDF's pdf is the density of
         // its own sampler (Lambert). The Disney sampler's selection between
         // its specular and diffuse branches reuses the half-vector's random
@@ -1306,18 +1310,17 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
 }
 
 // Guided rendering: position and unit shading normal of each closest-queue
-// hit (the guide network's input), for every entry of the striped queue;
-// misses and unused entries get a zero record.
+// hit (the guide network's input), by dense queue index; misses get a zero
+// record. Writes the queue's total to *total (the guide's batch size).
 __global__ void __launch_bounds__(256)
 k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t *hit_inst,
-             QIn qin, float *pts) {
+             QIn qin, float *pts, uint32_t *total) {
   __shared__ uint32_t s_pre[kQStripes + 1];
-  q_prefix(qin, s_pre);
-  const uint32_t n = kQStripes * qin.R;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t k = i / qin.R;
-    const bool used = i - k * qin.R < s_pre[k + 1] - s_pre[k];
-    const float4 h = used ? hits[i] : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+  const uint32_t n = q_prefix(qin, s_pre);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = n;
+  for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < n; v += gridDim.x * 256) {
+    const uint32_t i = q_entry(s_pre, qin.R, v);
+    const float4 h = hits[i];
     const int ls = __float_as_int(h.y);
     f3 p = mk3(0.f, 0.f, 0.f), nn = mk3(0.f, 0.f, 1.f);
     if (ls >= 0) {
@@ -1326,7 +1329,7 @@ k_hit_points(DevScene sc, const DevRay *rays, const float4 *hits, const uint32_t
       p = it.pos;
       nn = normalize(it.ns);
     }
-    float *o = pts + (size_t)i * 6;
+    float *o = pts + (size_t)v * 6;
     st3(o, p);
     st3(o + 3, nn);
   }
@@ -2455,6 +2458,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       q.c_rays_out = c_rays[nxt];
       q.c_count_out = rb->qctr + nxt * kQWords;
       q.c_len = c_len;
+      q.c_in = {rb->qctr + cur * kQWords, c_len};
       q.a_rays_in = a_rays[cur];
       q.a_rays_out = a_rays[nxt];
       q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
@@ -2492,22 +2496,22 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
                      trav_grid, sa);
         else if (counting)
           launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
-                          counters + 7, trav_ctr + 2, trav_grid, sa);
+                          rb->qctr + 4 * kQWords + kCursorStripes * kCursorStride, trav_ctr + 2, trav_grid, sa);
         else
           launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
-                           counters + 7, trav_ctr + 2, trav_grid, sa);
+                           rb->qctr + 4 * kQWords + kCursorStripes * kCursorStride, trav_ctr + 2, trav_grid, sa);
         timer.end(1, sa);
         AJG_RCHK(hipEventRecord(rb->ev_join, sa));
       }
       if (guided) {
         // guide mixtures of the new closest hits, read by the next shade
         timer.begin(3, s);
-        // every entry of the striped queue (unused ones get a zero record)
+        // dense queue order; the queue's total goes to counter word 6
         const uint32_t n_cq = kQStripes * c_len;
         hipLaunchKernelGGL(k_hit_points, dim3(std::min<uint32_t>(cdiv(n_cq, 256), 4096)), dim3(256),
                            0, s, ds, (const DevRay *)c_rays[nxt], (const float4 *)c_hits,
-                           (const uint32_t *)rb->c_inst, cq_in, gb->pts);
-        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, n_cq, nullptr, 6, nullptr,
+                           (const uint32_t *)rb->c_inst, cq_in, gb->pts, counters + 6);
+        if (int st = guide_forward_launch(gd, gb->pts, gb->pts + 3, n_cq, counters + 6, 6, nullptr,
                                           gb->mix, nullptr, s))
           return done(st);
         timer.end(3, s);

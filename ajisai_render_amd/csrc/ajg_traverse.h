@@ -438,6 +438,77 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
   }
 }
 
+// Any-hit traversal over rays [0, n) with the same lane refill: per step a
+// lane tests its leaf and, when that finds nothing and the next node is
+// interior, visits it, in trace_any's order. store(i, occluded).
+template <int SPILL, bool COUNT, int LDS = kLdsStack, uint32_t STRIPES = 1, class Load,
+          class Store>
+__device__ __forceinline__ void persistent_any(const DevBVH &bvh, uint32_t n, uint32_t *cursor,
+                                               int *lds_node, float *lds_entry, Load load,
+                                               Store store, uint32_t *nn, uint32_t *nt) {
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0;
+  TRay r;
+  int node = 0;
+  uint32_t ray = kNone;
+  bool drained = false;
+  for (;;) {
+    const bool idle = ray == kNone;
+    if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
+      const uint32_t idx = refill<STRIPES>(idle, n, cursor, &drained);
+      if (idx != kNone) {
+        r = load(idx);
+        d = 0;
+        float e;
+        if (slab_static(r, bvh.root_mn[0], bvh.root_mn[1], bvh.root_mn[2], bvh.root_mx[0],
+                        bvh.root_mx[1], bvh.root_mx[2], &e) &&
+            e < r.tmax) {
+          node = bvh.root_ref;
+          ray = idx;
+        } else {
+          store(idx, false);
+        }
+      }
+      continue;
+    }
+    if (__ballot(ray != kNone) == 0ull) {
+      if (drained) break;
+      continue;
+    }
+    if (ray == kNone) continue;
+    bool next = true, hit = false;
+    if (node < 0) {
+      if (COUNT) ++*nt;
+      f3 a, b, c;
+      load_leaf(bvh.leaf_tris, ~node, &a, &b, &c);
+      float t, u, v;
+      hit = mt_test(a, b, c, r, r.tmax, &t, &u, &v);
+      if (!hit) {
+        next = d > 0;
+        if (next) {
+          float en;
+          stack_pop<LDS>(st, d, &node, &en);
+        }
+      }
+    }
+    if (!hit && next && node >= 0) {
+      if (COUNT) ++*nn;
+      next = any_step<LDS>(bvh.wide, node, r, r.tmax, st, d);
+      if (!next && d > 0) {
+        float en;
+        stack_pop<LDS>(st, d, &node, &en);
+        next = true;
+      }
+    }
+    if (hit || !next) {
+      store(ray, hit);
+      ray = kNone;
+    }
+  }
+}
+
 // ---- Two-level traversal (instancing; no counterpart in the reference).
 // The top-level tree's leaves are instances; an instance holds the
 // world->object affine map of its mesh and the index of the mesh's own
