@@ -445,9 +445,9 @@ k_pq_any(DevBVH bvh, const DevRay *rays, uint8_t *occ, QIn qin, uint32_t *cursor
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
   uint32_t nn = 0, nt = 0;
-  persistent_any<SPILL, COUNT, kLdsStack, kCursorStripes>(
+  persistent_any<SPILL, COUNT, false, kLdsStack, kCursorStripes>(
       bvh, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
-      [&](uint32_t v) {
+      [&](uint32_t v, DevBVH &) {
         const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
         return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       },
@@ -487,24 +487,23 @@ k_pq_closest_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays,
 template <int SPILL, int LDS>
 __global__ void __launch_bounds__(kTravThreads)
 k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uint8_t *occ,
-           QIn qin) {
+           QIn qin, uint32_t *cursor) {
   __shared__ int s_node[LDS * kTravThreads];
   __shared__ float s_entry[LDS * kTravThreads];
   __shared__ uint32_t s_pre[kQStripes + 1];
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
-  const int lane = threadIdx.x & 63;
-  const uint32_t waves = gridDim.x * (kTravThreads / 64);
-  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t v = c * 64 + lane;
-    if (v < n) {
-      const uint32_t i = q_entry(s_pre, qin.R, v);
-      const DevBVH b = jobs[job_of[i]].sc.bvh;
-      const DevRay ray = load_ray(rays, i);
-      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-      occ[i] = trace_any<SPILL, false, LDS>(b, r, s_node + threadIdx.x, s_entry + threadIdx.x) ? 1 : 0;
-    }
-  }
+  const DevBVH first = jobs[0].sc.bvh;
+  uint32_t nn = 0, nt = 0;
+  persistent_any<SPILL, false, true, LDS, kCursorStripes>(
+      first, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t v, DevBVH &b) {
+        const uint32_t i = q_entry(s_pre, qin.R, v);
+        b = jobs[job_of[i]].sc.bvh;
+        const DevRay ray = load_ray(rays, i);
+        return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      },
+      [&](uint32_t v, bool hit) { occ[q_entry(s_pre, qin.R, v)] = hit ? 1 : 0; }, &nn, &nt);
 }
 
 // Instanced scenes: two-level traversal (top-level tree over instances).
@@ -656,7 +655,8 @@ static void launch_pq_b(int lds, bool closest, const BatchJob *jobs, const uint1
     hipLaunchKernelGGL((k_pq_closest_b<SPILL, L>), g, b, 0, s, jobs, job_of, rays, (float4 *)out, \
                        count, cursor);                                                        \
   else                                                                                        \
-    hipLaunchKernelGGL((k_pq_any_b<SPILL, L>), g, b, 0, s, jobs, job_of, rays, (uint8_t *)out, count);
+    hipLaunchKernelGGL((k_pq_any_b<SPILL, L>), g, b, 0, s, jobs, job_of, rays, (uint8_t *)out, count, \
+                       cursor);
   if (lds == 4) {
     AJG_PQB(4)
   } else if (lds == 16) {
@@ -727,8 +727,6 @@ __global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
     totals[0] += c;
     totals[1] += a;
     ctr[4] = 0;
-    ctr[6] = 0;
-    ctr[7] = 0;
   }
 }
 
@@ -2368,7 +2366,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   DevRay **c_rays = rb->c_rays, **a_rays = rb->a_rays;
   float4 *c_hits = rb->c_hits;
   uint8_t *a_occ = rb->a_occ;
-  uint32_t *counters = rb->counters;  // c0 c1 a0 a1 live work | traversal u64 x4 | totals
+  uint32_t *counters = rb->counters;  // live, work, guide total | traversal u64 x4 | totals
   float *partial = rb->partial;
   uint32_t *partial_n = rb->partial_n;
   hipEvent_t ev0, ev1;
@@ -2445,10 +2443,13 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   // MIS-BSDF ray) per slot; one shadow ray per light and the MIS-BSDF ray
   const uint32_t c_len = stripe_len(S, ds.n_area ? 2 : 1);
   const uint32_t a_len = stripe_len(S, ds.n_lights + 1);
+  // striped fetch cursors of the closest-hit and shadow-ray traversal
+  uint32_t *const c_cursor = rb->qctr + 4 * kQWords;
+  uint32_t *const a_cursor = c_cursor + kCursorStripes * kCursorStride;
   const uint32_t trav_grid = (uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU;
-  // counter words: 0-1 closest queue sizes, 2-3 shadow queue sizes, 4 live
-  // waves, 5 work items handed out, 6-7 traversal fetch cursors, 8-15 u64
-  // traversal counts, 16-19 u64 query totals (closest, shadow).
+  // counter words: 4 live flag, 5 work items handed out, 6 closest-queue
+  // total for the guide, 8-15 u64 traversal counts, 16-19 u64 query totals
+  // (closest, shadow); the queue counters and fetch cursors are in rb->qctr.
   unsigned long long *totals = (unsigned long long *)(counters + 16);
   for (;;) {
     for (int k = 0; k < kItersPerCheck; ++k) {
@@ -2479,13 +2480,13 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       AJG_RCHK(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       timer.begin(0, s);
       if (inst)
-        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, rb->qctr + 4 * kQWords,
+        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, c_cursor,
                    trav_grid, s);
       else if (counting)
-        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, rb->qctr + 4 * kQWords,
+        launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, c_cursor,
                         trav_ctr, trav_grid, s);
       else
-        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, rb->qctr + 4 * kQWords,
+        launch_pq<false>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, c_cursor,
                          trav_ctr, trav_grid, s);
       timer.end(0, s);
       {
@@ -2496,10 +2497,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
                      trav_grid, sa);
         else if (counting)
           launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
-                          rb->qctr + 4 * kQWords + kCursorStripes * kCursorStride, trav_ctr + 2, trav_grid, sa);
+                          a_cursor, trav_ctr + 2, trav_grid, sa);
         else
           launch_pq<false>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
-                           rb->qctr + 4 * kQWords + kCursorStripes * kCursorStride, trav_ctr + 2, trav_grid, sa);
+                           a_cursor, trav_ctr + 2, trav_grid, sa);
         timer.end(1, sa);
         AJG_RCHK(hipEventRecord(rb->ev_join, sa));
       }
@@ -2831,6 +2832,8 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   const uint32_t shade_grid = cdiv(S, 256);
   // queue stripes sized for the most any job can append
   const uint32_t c_len = stripe_len(S, kCqPerSlot), a_len = stripe_len(S, kMaxLights + 1);
+  uint32_t *const c_cursor = rb->qctr + 4 * kQWords;
+  uint32_t *const a_cursor = c_cursor + kCursorStripes * kCursorStride;
   q.c_len = c_len;
   q.a_len = a_len;
   const uint32_t trav_grid = (uint32_t)std::max(1, cu) * (uint32_t)ba.trav_blocks_per_cu;
@@ -2866,14 +2869,14 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       q_ok(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       if (spill == 48) {
         launch_pq_b<48>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
-                        cq_in, rb->qctr + 4 * kQWords, trav_grid, s);
+                        cq_in, c_cursor, trav_grid, s);
         launch_pq_b<48>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
-                        aq_in, counters + 7, trav_grid, rb->aux);
+                        aq_in, a_cursor, trav_grid, rb->aux);
       } else {
         launch_pq_b<1024>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
-                          cq_in, rb->qctr + 4 * kQWords, trav_grid, s);
+                          cq_in, c_cursor, trav_grid, s);
         launch_pq_b<1024>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
-                          aq_in, counters + 7, trav_grid, rb->aux);
+                          aq_in, a_cursor, trav_grid, rb->aux);
       }
       q_ok(hipEventRecord(rb->ev_join, rb->aux));
       q_ok(hipStreamWaitEvent(s, rb->ev_join, 0));

@@ -440,12 +440,16 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
 
 // Any-hit traversal over rays [0, n) with the same lane refill: per step a
 // lane tests its leaf and, when that finds nothing and the next node is
-// interior, visits it, in trace_any's order. store(i, occluded).
-template <int SPILL, bool COUNT, int LDS = kLdsStack, uint32_t STRIPES = 1, class Load,
-          class Store>
-__device__ __forceinline__ void persistent_any(const DevBVH &bvh, uint32_t n, uint32_t *cursor,
-                                               int *lds_node, float *lds_entry, Load load,
-                                               Store store, uint32_t *nn, uint32_t *nt) {
+// interior, visits it, in trace_any's order. load(i, bvh) as for
+// persistent_closest; store(i, occluded).
+template <int SPILL, bool COUNT, bool PER_RAY_BVH = false, int LDS = kLdsStack,
+          uint32_t STRIPES = 1, class Load, class Store>
+__device__ __forceinline__ void persistent_any(const DevBVH &bvh_uniform, uint32_t n,
+                                               uint32_t *cursor, int *lds_node, float *lds_entry,
+                                               Load load, Store store, uint32_t *nn,
+                                               uint32_t *nt) {
+  DevBVH bvh_lane = bvh_uniform;
+  const DevBVH &bvh = PER_RAY_BVH ? bvh_lane : bvh_uniform;
   int sn[SPILL];
   float se[SPILL];
   const StackRef st = {lds_node, lds_entry, sn, se};
@@ -459,7 +463,7 @@ __device__ __forceinline__ void persistent_any(const DevBVH &bvh, uint32_t n, ui
     if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
       const uint32_t idx = refill<STRIPES>(idle, n, cursor, &drained);
       if (idx != kNone) {
-        r = load(idx);
+        r = load(idx, bvh_lane);
         d = 0;
         float e;
         if (slab_static(r, bvh.root_mn[0], bvh.root_mn[1], bvh.root_mn[2], bvh.root_mx[0],
