@@ -10,7 +10,7 @@ closest-hit MIS-BSDF ray and one shadow ray per bounce. Inputs (scene, BVH,
 env tables) are resident in HBM before the timed region.
 
 Random numbers: by default one counter-based PCG32 stream per (pixel,
-sample) (seed_mode 1), so the 64 samples of a pixel run in parallel and 4M
+sample) (seed_mode 1), so the 64 samples of a pixel run in parallel and 8M
 paths are in flight; the reference's per-pixel sequential stream (seed_mode
 0: a pixel's samples one after another, 1M paths in flight at 1024x1024) is
 timed as well (paths_per_s_seed_mode0). Both trace the same paths per pixel
@@ -72,7 +72,7 @@ def parse():
     ap.add_argument("--seed-mode", type=int, default=1,
                     help="1: one PCG32 stream per (pixel, sample), samples of a pixel in parallel; "
                          "0: the reference's per-pixel sequential stream (one path per pixel in flight)")
-    ap.add_argument("--slots", type=int, default=1 << 22, help="paths in flight (seed_mode 1)")
+    ap.add_argument("--slots", type=int, default=1 << 23, help="paths in flight (seed_mode 1)")
     ap.add_argument("--no-seed0", action="store_true",
                     help="skip the per-pixel-stream (seed_mode 0) comparison render")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
