@@ -247,6 +247,10 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first "
                            "(the HIP extension is required; there is no CPU fallback)")
+    # torch ships its own HIP runtime under the same soname as the system one
+    # this library links; whichever is loaded first serves both, and torch
+    # finds no GPU on a runtime other than its own. Load torch's first.
+    import torch  # noqa: F401
     lib = C.CDLL(path)
     for name, (args, res) in _SIGS.items():
         fn = getattr(lib, name, None)
