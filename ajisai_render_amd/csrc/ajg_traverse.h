@@ -315,8 +315,9 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
 // wave takes that many new rays from the batch with one atomic on a shared
 // cursor, so lanes stay busy whatever the spread of path lengths within a
 // wave. Each ray's traversal (order, culling, arithmetic) is the same as
-// trace_closest, so results are unchanged. (Any-hit rays are short and
-// uniform enough that a plain 64-ray chunk per wave is faster.)
+// trace_closest, so results are unchanged. The queue kernels' any-hit
+// traversal (persistent_any below) refills the same way; the instanced
+// any-hit kernel keeps a plain 64-ray chunk per wave.
 static constexpr int kRefillMin = 48;
 
 // Wave-uniform refill: idle lanes take consecutive batch indices. Returns the
