@@ -249,8 +249,12 @@ def load(path: str = LIB_PATH):
                            "(the HIP extension is required; there is no CPU fallback)")
     # torch ships its own HIP runtime under the same soname as the system one
     # this library links; whichever is loaded first serves both, and torch
-    # finds no GPU on a runtime other than its own. Load torch's first.
-    import torch  # noqa: F401
+    # finds no GPU on a runtime other than its own. Load torch's first where
+    # torch is installed; without it the library takes the system runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (args, res) in _SIGS.items():
         fn = getattr(lib, name, None)

@@ -132,6 +132,8 @@ static constexpr uint32_t kPendMis = 8u;
 static constexpr uint32_t kPendHist = 16u;    // guided: the vertex has a history entry
 static constexpr uint32_t kPendLight0 = 32u;  // + k
 static constexpr uint32_t kPendMisClosest = 1u << 12;  // the MIS-BSDF ray is in the closest queue
+// guided: the env light's MIS weights of this vertex use the mixture density
+static constexpr uint32_t kPendMisMix = 1u << 13;
 
 struct PathState {
   uint32_t *work;       // current work item (kNone = none)
@@ -408,6 +410,12 @@ __device__ __forceinline__ void wave_count_add(uint32_t v, unsigned long long *d
 // 6 blocks per CU: with lane refill from striped cursors measured 1-2% faster
 // than 4, 8 or 12 on the heavy mesh (the two queue kernels run concurrently).
 static constexpr int kTravBlocksPerCU = 6;
+// Every fetch-cursor stripe needs a wave that drains it: wave w of a
+// persistent traversal launch takes from stripe w mod kCursorStripes, so a
+// grid of fewer than kCursorStripes waves would leave rays untraced.
+static constexpr uint32_t kMinTravGrid = (kCursorStripes * 64 + kTravThreads - 1) / kTravThreads;
+static_assert(kMinTravGrid * (kTravThreads / 64) >= kCursorStripes, "traversal grid covers every stripe");
+static inline uint32_t trav_grid_size(uint32_t blocks) { return std::max(blocks, kMinTravGrid); }
 static constexpr int kItersPerCheck = 4;
 
 template <int SPILL, bool COUNT>
@@ -1029,7 +1037,13 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
                                fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
             const float pdf = ps.pend_pdf[slot];
             const float lpdf = env_pdf(e, mr.d);
-            const float wb = power_heuristic(pdf, lpdf);
+            // guided: the mixture density is the MIS partner only where the
+            // light-side and BSDF-side integrands are the same function
+            // (flag set in the shade that queued this ray); elsewhere the
+            // plain path's weight functions, still dividing by the density
+            // the direction was drawn from
+            const float wb = power_heuristic(
+                (!GUIDED || (pf & kPendMisMix)) ? pdf : ps.pend_pb[slot], lpdf);
             mis = scale(divs(f, pdf), wb);
             if (train) emit_record(G, tpos, tnrm, mr.d, lum_f(le) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
           }
@@ -1102,6 +1116,27 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
         // value the render converges to. Disney vertices therefore sample
         // the BSDF alone (alpha 0), exactly as the plain path does.
         const float alpha = (GUIDED && bs.m.type != AJG_MAT_DISNEY) ? G.alpha : 0.f;
+        // The env light's two estimates measure different integrands where
+        // the shading normal differs from the geometric one: the light side
+        // takes Eval and |wi . normalize(ns)| (helper.h:120-131), the BSDF
+        // side the sample's f, which carries |ns . wi| / |ng . wi|
+        // (aggregate_bsdf.cpp:82-89), times |ns . wi| (helper.h:228-231).
+        // The plain render converges to the sum of the two weighted by the
+        // BSDF-pdf power heuristic, so guided weights may differ from plain
+        // only where the two integrands coincide: ns equal to ng. Their
+        // ratio is |ns|^2 |ns^ . wi| / |ng . wi|, within 1 + 2e-5 + theta tan(beta)
+        // of 1 for ns at angle theta from ng; the test below admits
+        // theta <= 1e-5 and |ns|^2 within 2e-5 of 1 (quads and cubes, whose
+        // ns is ng, and flat-shaded meshes up to rounding): there the two
+        // integrands differ by at most 2e-5 + 1e-5 tan(beta) of their value
+        // at angle beta from ng, so the weights' choice moves the converged
+        // value by no more than that.
+        bool mix_w = false;
+        if (GUIDED && alpha > 0.f) {
+          const float n2 = dot(it.ns, it.ns);
+          const f3 cx = cross(it.ns, it.ng);
+          mix_w = dot(cx, cx) <= 1e-10f * n2 && fabsf(n2 - 1.f) <= 2e-5f;
+        }
         // emitted radiance at the first hit (pt.cpp:188-194)
         if (b == 0 && it.light && dot(it.ng, it.wr) > 0)
           L = add(L, mul(beta, sc.lights[it.light - 1].radiance));
@@ -1150,9 +1185,10 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
                                    fabsf(dot(to_light, l.type == kLightArea ? it.ns : nrm)));
                 float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
                 // guided: the env light's MIS partner is the mixture density
-                // (a delta light is never reached by directional sampling, so
-                // its weight keeps the BSDF pdf, as in the plain path)
-                if (GUIDED && l.type == AJG_LIGHT_ENV && alpha > 0.f)
+                // where ns equals ng (see mix_w); a delta light is never
+                // reached by directional sampling, so its weight keeps the
+                // BSDF pdf, as in the plain path
+                if (mix_w && l.type == AJG_LIGHT_ENV)
                   bpdf = alpha * gmix_pdf(gm, to_light) + (1.f - alpha) * bpdf;
                 const float wl = power_heuristic(lsmp.pdf, bpdf);
                 const f3 c = mul(beta, scale(divs(f, lsmp.pdf), wl));
@@ -1241,6 +1277,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
             ps.pend_mis[slot] = ma;
             pf2 |= kPendMis;
           }
+          if (mix_w) pf2 |= kPendMisMix;
         }
         // continuation (pt.cpp:215-223) and Russian roulette of the next
         // bounce (pt.cpp:107-110)
@@ -2447,7 +2484,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   // striped fetch cursors of the closest-hit and shadow-ray traversal
   uint32_t *const c_cursor = rb->qctr + 4 * kQWords;
   uint32_t *const a_cursor = c_cursor + kCursorStripes * kCursorStride;
-  const uint32_t trav_grid = (uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU;
+  const uint32_t trav_grid = trav_grid_size((uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU);
   // counter words: 4 live flag, 5 work items handed out, 6 closest-queue
   // total for the guide, 8-15 u64 traversal counts, 16-19 u64 query totals
   // (closest, shadow); the queue counters and fetch cursors are in rb->qctr.
@@ -2837,7 +2874,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   uint32_t *const a_cursor = c_cursor + kCursorStripes * kCursorStride;
   q.c_len = c_len;
   q.a_len = a_len;
-  const uint32_t trav_grid = (uint32_t)std::max(1, cu) * (uint32_t)ba.trav_blocks_per_cu;
+  const uint32_t trav_grid = trav_grid_size((uint32_t)std::max(1, cu) * (uint32_t)ba.trav_blocks_per_cu);
   const int K = ba.iters_per_check;
   // One block of K iterations starting and ending on queue generation 0;
   // shadow rays are traced on the aux stream concurrently with the closest

@@ -489,13 +489,18 @@ def flat_mesh(pos4, faces, material, two_sided=True) -> Primitive:
 
 
 def occluded_room(seed: int = 20261206, res=(512, 512), spp=64, env_size=(1024, 512),
-                  furniture=((6, 0.55), (6, 0.5), (5, 0.35), (5, 0.3))) -> Scene:
+                  furniture=((6, 0.55), (6, 0.5), (5, 0.35), (5, 0.3)), smooth=False,
+                  sun=None) -> Scene:
     """Closed 4 x 3 x 4 room (x, y, z in [0,4] x [0,3] x [0,4]) whose x = 4
     wall is pierced by a 0.2 x 1.0 slit (z in [1.9, 2.1], y in [1.2, 2.2]);
     lit only by the heavy-mesh scene's HDR env map through the slit. Furniture:
     flat-shaded displaced icospheres (20*4^s triangles each; 204,800 for the
     default set) on box pedestals, Lambert or Disney (clearcoat 0) from the
-    seed. 512 x 512, max 16 bounces, Russian roulette after bounce 3 (p 0.9)."""
+    seed. 512 x 512, max 16 bounces, Russian roulette after bounce 3 (p 0.9).
+    smooth=True gives the icospheres their interpolated vertex normals
+    (shading normal != geometric normal); sun=(radiance, toward-light
+    direction) adds a directional light, e.g. one shining through the slit
+    onto the floor."""
     rng = np.random.default_rng(seed)
     wall = Material(MAT_DIFFUSE, (0.75, 0.75, 0.75))
     floor = Material(MAT_DIFFUSE, (0.62, 0.56, 0.5))
@@ -524,11 +529,19 @@ def occluded_room(seed: int = 20261206, res=(512, 512), spp=64, env_size=(1024, 
         mats.append(m)
         ped = 0.25 + 0.35 * rng.random()
         prims.append(cube(translate(cx, ped / 2, cz) @ scale3(0.7, ped, 0.7), 0))
-        pos4, _, f = displaced_icosphere(sub, seed + 10 + k, amplitude=0.15, radius=rad,
-                                         center=(cx, ped + rad, cz))
-        prims.append(flat_mesh(pos4, f, len(mats) - 1, two_sided=True))
+        pos4, nrm4, f = displaced_icosphere(sub, seed + 10 + k, amplitude=0.15, radius=rad,
+                                            center=(cx, ped + rad, cz))
+        if smooth:
+            prims.append(mesh(pos4, nrm4, f, len(mats) - 1, two_sided=True))
+        else:
+            prims.append(flat_mesh(pos4, f, len(mats) - 1, two_sided=True))
     env = hdr_env(env_size[0], env_size[1], 20261204 + 2)
     lights = [Light(LIGHT_ENV, env=env, rot=0.0, scaler=1.0)]
+    if sun is not None:
+        rad_, d = sun
+        d = np.asarray(d, np.float64)
+        lights.append(Light(LIGHT_DIRECTIONAL, dir=tuple(float(x) for x in d / np.linalg.norm(d)),
+                            radiance=tuple(float(x) for x in rad_)))
     cam = CameraSpec(pos=(0.5, 1.5, 0.5), look_at=(3.3, 0.8, 3.0), up=(0.0, 1.0, 0.0),
                      fov=75.0, res=res)
     return Scene("occluded_room", prims, mats, lights, cam,

@@ -94,6 +94,8 @@ def parse():
     ap.add_argument("--inst-guided-budget", type=float, default=5.0,
                     help="equal-time budget (s) of guided vs plain on the instanced scene (0: skip)")
     ap.add_argument("--inst-guided-res", type=int, default=480)
+    ap.add_argument("--inst-guided-ref-spp", type=int, default=4096,
+                    help="spp of the plain-PT reference of the instanced guided comparison")
     ap.add_argument("--area-spp", type=int, default=64,
                     help="spp of the area-lit Cornell box render (0: skip)")
     ap.add_argument("--area-cpu-seconds", type=float, default=4.0)
@@ -172,15 +174,17 @@ def _img(out):
     return out["sum"].astype(np.float64) / np.maximum(n, 1)[:, None]
 
 
+RELMSE_EPS = 1e-2
+
+
 def relmse(img, ref):
-    """relMSE over the pixels finite in both images (the reference's
-    estimator can overflow to inf/NaN on rare heavy-tailed Disney paths;
-    such pixels are counted separately by nonfinite())."""
+    """relMSE = mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2),
+    over the pixels finite in both images (the reference's estimator can
+    overflow to inf/NaN on rare heavy-tailed Disney paths; such pixels are
+    counted separately by nonfinite())."""
     import numpy as np
     ok = np.isfinite(img).all(axis=1) & np.isfinite(ref).all(axis=1)
-    eps = 1e-2 * float(np.mean(ref[ok])) ** 2
-    return float(np.mean(np.mean((img[ok] - ref[ok]) ** 2, axis=1) /
-                         (np.mean(ref[ok], axis=1) ** 2 + eps)))
+    return float(np.mean((img[ok] - ref[ok]) ** 2 / (ref[ok] ** 2 + RELMSE_EPS)))
 
 
 def nonfinite(img):
@@ -212,12 +216,13 @@ def guided_instanced(ctx, a):
     sc = scenes.instanced_field(res=(w, h), spp=16)
     label = (f"instanced_field: 2000 instances of 8 meshes ({sc.n_tri} instanced triangles), "
              f"HDR env 2048x1024 + directional, {w}x{h}, max 24 bounces")
-    out = {"disney": guided_compare(ctx, sc, a.inst_guided_budget, "guided", a.guided_ref_spp, label)}
+    out = {"disney": guided_compare(ctx, sc, a.inst_guided_budget, "plain", a.inst_guided_ref_spp,
+                                    label)}
     lam = dataclasses.replace(sc, materials=[
         scenes.Material(scenes.MAT_DIFFUSE, tuple(float(c) for c in m.base_color))
         for m in sc.materials])
-    out["lambert_variant"] = guided_compare(ctx, lam, a.inst_guided_budget, "guided",
-                                            a.guided_ref_spp, label + ", all materials Lambertian")
+    out["lambert_variant"] = guided_compare(ctx, lam, a.inst_guided_budget, "plain",
+                                            a.inst_guided_ref_spp, label + ", all materials Lambertian")
     out["note"] = ("disney: relMSE of both runs is set by a few samples of weight >= 1e5 from "
                    "the Disney sampler (pdf not its sampling density); the ratio is not a "
                    "measure of the guide. lambert_variant: finite-variance comparison.")
@@ -251,7 +256,8 @@ def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
         acc, t_ref = None, 0.0
         chunk = min(1024, ref_spp)
         for k in range(max(1, ref_spp // chunk)):
-            o, dt = timed(lambda: PathTracer(g, spp=chunk, seed_mode=1, seed_base=901 + k).Render())
+            o, dt = timed(lambda: PathTracer(g, spp=chunk, seed_mode=1, seed_base=901 + k,
+                                             flags=capi.RENDER_FINITE_PDF).Render())
             t_ref += dt
             acc = o if acc is None else {"sum": acc["sum"] + o["sum"], "sum_sq": acc["sum_sq"] + o["sum_sq"],
                                          "count": acc["count"] + o["count"]}
@@ -262,8 +268,7 @@ def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
     with np.errstate(invalid="ignore", over="ignore"):
         var_mean = np.maximum(ref["sum_sq"].astype(np.float64) / n[:, None] - R * R, 0) / n[:, None]
     ok = np.isfinite(var_mean).all(axis=1) & np.isfinite(R).all(axis=1)
-    eps = 1e-2 * float(np.mean(R[ok])) ** 2
-    ref_self = float(np.mean(np.mean(var_mean[ok], axis=1) / (np.mean(R[ok], axis=1) ** 2 + eps)))
+    ref_self = float(np.mean(var_mean[ok] / (R[ok] ** 2 + RELMSE_EPS)))
     # plain: calibrate, then render for the budget. Plain paths end on a
     # zero-pdf continuation sample as guided ones do (AJG_RENDER_FINITE_PDF),
     # so the two estimators differ only where the guide is mixed in.

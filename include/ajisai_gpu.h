@@ -465,7 +465,7 @@ typedef struct ajg_batch_args {
   int32_t max_slots;          /* path slots in flight (0: 1 << 20) */
   int32_t samples_per_item;   /* seed_mode 1 jobs: samples per work item
                                  (0: as ajg_render_pt chooses for max_slots) */
-  int32_t trav_blocks_per_cu; /* persistent traversal grid (0: 8) */
+  int32_t trav_blocks_per_cu; /* persistent traversal grid (0: 6; at least 16 waves in all) */
   int32_t lds_stack;          /* LDS ring entries per lane: 4, 8 or 16 (0: 8) */
   int32_t iters_per_check;    /* iterations per host check, even (0: 4) */
   int32_t use_graph;          /* 1: replay the iterations from a hipGraph */

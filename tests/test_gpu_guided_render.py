@@ -96,6 +96,42 @@ def test_guided_room_matches_plain(gpu_ctx):
     g.close()
 
 
+SLIT_SUN = ((20.0, 19.0, 17.0), (1.0, 0.6, 0.1))   # shines through the slit onto the floor
+
+
+def _lambert_furniture(sc):
+    for i in range(2, len(sc.materials)):
+        sc.materials[i] = Material(MAT_DIFFUSE, (0.6, 0.6, 0.6))
+    return sc
+
+
+def test_guided_smooth_room_with_sun_matches_plain(gpu_ctx):
+    """Smooth-shaded furniture (interpolated vertex normals) and a
+    directional light through the slit, all Lambert: the env light's
+    light-side and BSDF-side estimates measure different integrands at the
+    smooth vertices, so a guided render converges to the plain render's
+    value only if it keeps the plain MIS weight functions there. Image means
+    within 5 combined standard errors, per pixel |z| > 4 in < 1%."""
+    from ajisai_render_amd.core import PathTracer, guide_defaults
+    print("case: smooth room + sun 64x64, plain 2048 spp vs guided 2048 spp", flush=True)
+    sc = _lambert_furniture(scenes.occluded_room(res=(64, 64), spp=64, env_size=(256, 128),
+                                                 smooth=True, sun=SLIT_SUN))
+    g, gd = _objs(gpu_ctx, sc)
+    op = PathTracer(g, spp=2048, seed_mode=1, seed_base=31).Render()
+    og = PathTracer(g, spp=2048, seed_mode=1, seed_base=32).Render(
+        guide=gd, guide_args=guide_defaults(warmup_passes=4, warmup_spp=16, batch=1 << 14))
+    assert og["guide_stats"].train_steps > 0
+    mp, sp, ip, vp = film_stats(op)
+    mg, sg, ig, vg = film_stats(og)
+    z = np.abs(ip - ig) / np.sqrt(vp + vg + 1e-30)
+    print(f"means {mp:.5f} {mg:.5f} se {np.hypot(sp, sg):.2e}; |z|>4 fraction {np.mean(z > 4):.4f}",
+          flush=True)
+    assert abs(mp - mg) < 5 * np.hypot(sp, sg), (mp, mg, sp, sg)
+    assert np.mean(z > 4) < 0.01, np.mean(z > 4)
+    gd.close()
+    g.close()
+
+
 def test_records_collect_across_renders(gpu_ctx):
     """Training records stay in the guide until a batch is full; a guide
     that never reached a batch is not sampled (alpha forced to 0)."""
@@ -113,26 +149,36 @@ def test_records_collect_across_renders(gpu_ctx):
 
 
 def _relmse(a, r):
-    eps = 1e-2 * float(np.mean(r)) ** 2
-    return float(np.mean(np.mean((a - r) ** 2, axis=1) / (np.mean(r, axis=1) ** 2 + eps)))
+    """Mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2)."""
+    return float(np.mean((a - r) ** 2 / (r ** 2 + 1e-2)))
 
 
-def test_guiding_gain_equal_time_occluded_room(gpu_ctx):
+@pytest.mark.parametrize("variant", ["flat", "smooth_sun"])
+def test_guiding_gain_equal_time_occluded_room(gpu_ctx, variant):
     """What the trained network gains: on the occluded room (light enters
     only through a 0.2 x 1 slit), guided rendering (warm-up training
-    included in its time) reaches at least 1.5x lower relMSE than plain path
-    tracing at equal time (relMSE x time compared, the 1/time law of Monte
-    Carlo error). Reference: 4096-spp guided render (unbiased, see the
-    tests above). At 128 x 128 a 2^18 batch would take most of the budget to
-    fill, so this test trains on batches of 2^16 records. (At equal sample
-    counts guided relMSE is about 2.3x lower; a guided sample also pays the
-    network's inference at every bounce, about a third more time than a
-    plain one, which leaves an equal-time gain of about 1.7-1.9x here.)"""
+    included in its time) reaches lower relMSE than plain path tracing at
+    equal time (relMSE x time compared, the 1/time law of Monte Carlo
+    error): at least 1.5x with flat-shaded furniture lit by the environment
+    alone (measured 2.0x), at least 1.3x with smooth-shaded furniture plus a
+    directional light through the slit (measured 1.5x: the direct sun is
+    light-sampled alone, and smooth vertices keep the plain MIS weights). Reference: a 16384-spp plain render. relMSE = mean over pixels and
+    channels of (I - R)^2 / (R^2 + 1e-2). At 128 x 128 a 2^18 batch would
+    take most of the budget to fill, so this test trains on batches of 2^16
+    records. Plain paths end on a zero-pdf continuation sample as guided
+    ones do (the reference would carry an infinite throughput on and give a
+    NaN pixel)."""
     import torch
+    from ajisai_render_amd import _capi as capi
     from ajisai_render_amd.core import PathGuide, PathTracer, guide_defaults
-    print("case: room 128x128, equal time ~3 s", flush=True)
-    sc = scenes.occluded_room(res=(128, 128), spp=16)
-    g, gref = _objs(gpu_ctx, sc, seed=99)
+    print(f"case: room 128x128 ({variant}), equal time ~3 s", flush=True)
+    if variant == "flat":
+        sc = scenes.occluded_room(res=(128, 128), spp=16)
+    else:
+        sc = scenes.occluded_room(res=(128, 128), spp=16, smooth=True, sun=SLIT_SUN)
+    from ajisai_render_amd.core import GpuScene
+    g = GpuScene(gpu_ctx, sc)
+    fin = capi.RENDER_FINITE_PDF
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -141,21 +187,23 @@ def test_guiding_gain_equal_time_occluded_room(gpu_ctx):
         torch.cuda.synchronize()
         return o, time.time() - t0
 
+    acc = None
+    for k in range(8):
+        o = PathTracer(g, spp=2048, seed_mode=1, seed_base=900 + k, flags=fin).Render()
+        acc = o if acc is None else {key: acc[key] + o[key] for key in ("sum", "sum_sq", "count")}
+    R = film_stats(acc)[2]
     ga = guide_defaults(warmup_passes=8, warmup_spp=32, batch=1 << 16, train=False)
-    ref, _ = timed(lambda: PathTracer(g, spp=4096, seed_mode=1, seed_base=901).Render(
-        guide=gref, guide_args=ga))
-    R = film_stats(ref)[2]
-    _, t16 = timed(lambda: PathTracer(g, spp=16, seed_mode=1, seed_base=2).Render())
+    _, t16 = timed(lambda: PathTracer(g, spp=16, seed_mode=1, seed_base=2, flags=fin).Render())
+    _, t16 = timed(lambda: PathTracer(g, spp=16, seed_mode=1, seed_base=2, flags=fin).Render())
     sp = max(16, int(3.0 / (t16 / 16)))
-    op, tp = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=3).Render())
+    op, tp = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=3, flags=fin).Render())
     gd = PathGuide(gpu_ctx, sc.scene_box(), seed=5)
     og, tg = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=4).Render(
         guide=gd, guide_args=ga))
     rp, rg = _relmse(film_stats(op)[2], R), _relmse(film_stats(og)[2], R)
     ratio = (rp * tp) / (rg * tg)
-    print(f"plain {sp} spp {tp:.2f}s relMSE {rp:.4f}; guided {sp} spp {tg:.2f}s relMSE {rg:.4f};"
+    print(f"plain {sp} spp {tp:.2f}s relMSE {rp:.5f}; guided {sp} spp {tg:.2f}s relMSE {rg:.5f};"
           f" equal-time ratio {ratio:.2f}", flush=True)
-    assert ratio >= 1.5, ratio
+    assert ratio >= (1.5 if variant == "flat" else 1.3), ratio
     gd.close()
-    gref.close()
     g.close()

@@ -120,20 +120,30 @@ def test_instanced_render_matches_world_copy(gpu_ctx):
 
 
 def test_instanced_guided_render_runs_unbiased(gpu_ctx):
-    """Guided and plain renders of an instanced scene agree within 5 standard
-    errors (a mild env map: with a 1e4 sun patch the per-pixel variance
-    estimates are too heavy-tailed for a 5-sigma bound at 256 spp)."""
+    """Guided and plain renders of an instanced scene under the full
+    lighting of the large instanced configuration (HDR env with a 1e4 sun
+    patch plus a directional light) agree: image means within 5 combined
+    standard errors, and per pixel |z| > 4 in under 1% of the pixels. The
+    meshes carry interpolated vertex normals (shading normal != geometric
+    normal), where the env light's two estimates measure different
+    integrands; the guided render keeps the plain render's MIS weight
+    functions there, so it converges to the same image."""
     from ajisai_render_amd.core import GpuScene, PathGuide, PathTracer, guide_defaults
-    sc = _lit(small_instanced(seed=5, n_inst=40), sun_gain=30.0)
+    print("case: instanced guided vs plain 64x48, 1024 spp, sun 1e4 + directional", flush=True)
+    sc = _lit(small_instanced(seed=5, n_inst=40))
     g = GpuScene(gpu_ctx, sc)
     gd = PathGuide(gpu_ctx, sc.scene_box(), seed=3)
-    op = PathTracer(g, spp=256, seed_mode=1, seed_base=7, max_bounces=6).Render()
-    og = PathTracer(g, spp=256, seed_mode=1, seed_base=8, max_bounces=6).Render(
+    op = PathTracer(g, spp=1024, seed_mode=1, seed_base=7, max_bounces=6).Render()
+    og = PathTracer(g, spp=1024, seed_mode=1, seed_base=8, max_bounces=6).Render(
         guide=gd, guide_args=guide_defaults(warmup_passes=2, warmup_spp=16, batch=1 << 12))
     assert og["guide_stats"].train_steps > 0
     (m1, v1), (m2, v2) = _stats(op), _stats(og)
     se = np.sqrt(v1.sum() + v2.sum()) / m1.size
-    assert abs(m1.mean() - m2.mean()) < 5 * se + 1e-6
+    assert abs(m1.mean() - m2.mean()) < 5 * se + 1e-6, (m1.mean(), m2.mean(), se)
+    z = np.abs(m1 - m2) / np.sqrt(v1 + v2 + 1e-30)
+    print(f"means {m1.mean():.5f} {m2.mean():.5f} se {se:.2e}; |z|>4 fraction {np.mean(z > 4):.4f}",
+          flush=True)
+    assert np.mean(z > 4) < 0.01, np.mean(z > 4)
     gd.close()
     g.close()
 
