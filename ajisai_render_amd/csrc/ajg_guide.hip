@@ -66,8 +66,6 @@ struct ajg_guide {
 namespace ajg {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
-typedef short s16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32)
@@ -81,12 +79,13 @@ __device__ __forceinline__ float bf2f(uint16_t h) {
 __device__ __forceinline__ f32x4 mfma16(s16x4 a, s16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
-// 16x16x32: lane l supplies k-slots 8(l/16) .. +7 of row/column l%16.
+// Two k-steps of 16, accumulated in order. (One v_mfma_f32_16x16x32_bf16
+// over both steps gave results that changed from run to run in the
+// training kernel as this toolchain schedules it: identical steps on
+// identical inputs differed by up to 12% of the largest gradient entry.
+// With two 16x16x16 MFMAs a step is bitwise reproducible.)
 __device__ __forceinline__ f32x4 mfma32(s16x4 a0, s16x4 a1, s16x4 b0, s16x4 b1, f32x4 c) {
-  const s16x8 a = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-  const s16x8 b = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  return mfma16(a1, b1, mfma16(a0, b0, c));
 }
 // four consecutive BF16 values; every caller's address is 8-byte aligned
 __device__ __forceinline__ s16x4 ld4(const uint16_t *p) {
@@ -214,10 +213,7 @@ __device__ __forceinline__ s16x4 encode_frag(int j, int g, f3 ph, f3 nn, bool li
 
 // D[m][n] = sum_k A[m][k] B[k][n] over KT 16-wide k-steps; A = weight rows
 // from LDS (row stride LD), B = fragments in registers; MT row tiles, NT
-// column tiles. Pairs of k-steps run as one 16x16x32 MFMA whose k-slots
-// 8g .. 8g+7 are the features 32s + 4g .. +3 and 32s + 16 + 4g .. +3 (the
-// same permutation on both operands leaves the sum unchanged); an odd last
-// k-step runs as 16x16x16.
+// column tiles; one 16x16x16 MFMA per k-step (pairs through mfma32).
 template <int KT, int MT, int NT, int LD>
 __device__ __forceinline__ void net_layer(const uint16_t *Wl, const s16x4 (&bf)[KT][NT],
                                           f32x4 (&acc)[MT][NT]) {
@@ -731,10 +727,23 @@ int guide_forward_launch(ajg_guide *gd, const float *pos, const float *nrm, uint
   return e == hipSuccess ? AJG_OK : hip_fail(gd->ctx, e, "k_guide_fwd");
 }
 
+// Per-step work buffers of a training step: the feature-major activations
+// and their gradients (rows x npad BF16 each, columns = records padded to
+// 16), the weight-gradient partials of every block and the forward blocks'
+// weight and loss sums. Each step writes every element it later reads, so
+// none needs clearing between steps.
+struct TrainBufs {
+  uint16_t *x0t, *x1t, *x2t, *x3t, *dy4t, *dx3t, *dx2t, *dx1t;
+  float *partial, *scal;
+};
+
+// Feature rows of the eight activation arrays, in TrainBufs order.
+static constexpr int kActRows[8] = {kGIn, kGHid, kGHid, kGHid, kGOut, kGHid, kGHid, kGHid};
+static constexpr size_t kActRowsTotal = kGIn + 6 * kGHid + kGOut;  // 544
+
 // One training step enqueued on s (no host synchronisation).
-int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr, hipStream_t s) {
-  const uint32_t npad = (n + 15) / 16 * 16;
-  if (int st = ensure_train(gd, npad)) return st;
+static int train_launch(ajg_guide *gd, const TrainBufs &tb, const float *samples, uint32_t n,
+                        uint32_t npad, float lr, hipStream_t s) {
   FwdArgs a = fwd_args(gd);
   a.samples = samples;
   a.pos = samples;
@@ -742,31 +751,31 @@ int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr
   a.stride = 10;
   a.n = n;
   a.npad = npad;
-  a.x0t = gd->x0t;
-  a.x1t = gd->x1t;
-  a.x2t = gd->x2t;
-  a.x3t = gd->x3t;
-  a.dy4t = gd->dy4t;
-  a.dx3t = gd->dx3t;
-  a.dx2t = gd->dx2t;
-  a.dx1t = gd->dx1t;
-  a.scal = gd->scal;
+  a.x0t = tb.x0t;
+  a.x1t = tb.x1t;
+  a.x2t = tb.x2t;
+  a.x3t = tb.x3t;
+  a.dy4t = tb.dy4t;
+  a.dx3t = tb.dx3t;
+  a.dx2t = tb.dx2t;
+  a.dx1t = tb.dx1t;
+  a.scal = tb.scal;
   constexpr int W = NetCfg<1>::kWaves;
   const uint32_t tiles = npad / 16;
   const uint32_t fblocks = std::min<uint32_t>(kFwdMaxBlocks, (tiles + W - 1) / W);
   hipLaunchKernelGGL(k_guide_fwd<1>, dim3(fblocks), dim3(64 * W), 0, s, a);
   WgArgs wa;
-  wa.x0t = gd->x0t;
-  wa.x1t = gd->x1t;
-  wa.x2t = gd->x2t;
-  wa.x3t = gd->x3t;
-  wa.dy4t = gd->dy4t;
-  wa.dx3t = gd->dx3t;
-  wa.dx2t = gd->dx2t;
-  wa.dx1t = gd->dx1t;
+  wa.x0t = tb.x0t;
+  wa.x1t = tb.x1t;
+  wa.x2t = tb.x2t;
+  wa.x3t = tb.x3t;
+  wa.dy4t = tb.dy4t;
+  wa.dx3t = tb.dx3t;
+  wa.dx2t = tb.dx2t;
+  wa.dx1t = tb.dx1t;
   wa.npad = npad;
   wa.chunk = (tiles + kWgBlocks - 1) / kWgBlocks * 16;
-  wa.partial = gd->partial;
+  wa.partial = tb.partial;
   hipLaunchKernelGGL(k_guide_wgrad, dim3(kWgBlocks, 4), dim3(64 * kWgWaves), 0, s, wa);
   float b1c = 1.f, b2c = 1.f;
   if (lr > 0.f) {
@@ -775,10 +784,19 @@ int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr
     b2c = 1.f - std::pow(0.999f, (float)gd->step);
   }
   hipLaunchKernelGGL(k_guide_update, dim3((kGParams + 255) / 256), dim3(256), 0, s,
-                     (const float *)gd->partial, kWgBlocks, (const float *)gd->scal, (int)fblocks,
+                     (const float *)tb.partial, kWgBlocks, (const float *)tb.scal, (int)fblocks,
                      gd->grad, gd->w, gd->m, gd->v, gd->wt, gd->wi, lr, b1c, b2c, gd->lossbuf);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AJG_OK : hip_fail(gd->ctx, e, "guide train step");
+}
+
+// The guide's own buffers (activations grown on demand: may allocate).
+int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr, hipStream_t s) {
+  const uint32_t npad = (n + 15) / 16 * 16;
+  if (int st = ensure_train(gd, npad)) return st;
+  const TrainBufs tb = {gd->x0t, gd->x1t, gd->x2t, gd->x3t, gd->dy4t, gd->dx3t, gd->dx2t, gd->dx1t,
+                        gd->partial, gd->scal};
+  return train_launch(gd, tb, samples, n, npad, lr, s);
 }
 
 int guide_records(ajg_guide *gd, uint32_t cap, float **rec, uint32_t **ctr, uint32_t *cap_out) {

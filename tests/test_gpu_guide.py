@@ -138,3 +138,17 @@ def test_training_learns_a_position_dependent_lobe(gpu_ctx):
     q_mean = G.mixture_pdf(raw.cpu().numpy(), mu)
     assert np.median(q_mean) > 2.0  # uniform density is 1/(4 pi) = 0.08
     g.close()
+
+
+@pytest.mark.parametrize("n", [4096, 32768])
+def test_training_step_is_bitwise_reproducible(gpu_ctx, n):
+    """Identical steps on identical inputs give bit-identical gradients
+    (fixed-order reductions throughout; the network's MFMAs are 16x16x16)."""
+    print(f"case: guide gradient reproducibility n={n}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    g = _guide(gpu_ctx, rand_params(rng).astype(np.float32))
+    smp = _dev(rand_samples(rng, n))
+    grads = [g.train_step(smp, lr=0.0, want_grad=True)[1] for _ in range(4)]
+    for k in range(1, 4):
+        assert np.array_equal(grads[0], grads[k]), np.abs(grads[0] - grads[k]).max()
+    g.close()
