@@ -645,6 +645,16 @@ class PathGuide:
             grad.ctypes.data if grad is not None else None, stream), "train step")
         return float(loss.value), grad
 
+    NOALLOC_MAX_SAMPLES = 32768
+
+    def train_step_noalloc(self, samples_dev, lr=1e-3, stream=None):
+        """The same step through ajg_guide_train_step_noalloc: no device
+        allocation, work buffers in the library's static scratch, at most
+        NOALLOC_MAX_SAMPLES samples; enqueued only (no loss returned)."""
+        n = int(samples_dev.shape[0])
+        self.ctx.check(self.ctx.lib.ajg_guide_train_step_noalloc(
+            self.h, _ptr(samples_dev), n, float(lr), stream), "train step (no allocation)")
+
     def close(self):
         if self.h:
             self.ctx.lib.ajg_guide_destroy(self.h)

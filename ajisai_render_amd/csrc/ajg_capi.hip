@@ -136,6 +136,11 @@ int ajg_ctx_create(int device, ajg_ctx **out) {
   if (e != hipSuccess) return AJG_ERR_HIP;
   ajg_ctx *c = new ajg_ctx();
   c->device = device;
+  // the allocation-free training step's scratch address on this device
+  if (guide_scratch_setup(c) != AJG_OK) {
+    delete c;
+    return AJG_ERR_HIP;
+  }
   *out = c;
   return AJG_OK;
 }

@@ -28,6 +28,7 @@
 // All reductions run in a fixed order, so a step is deterministic.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ajg_guide.h"
@@ -799,6 +800,70 @@ int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr
   return train_launch(gd, tb, samples, n, npad, lr, s);
 }
 
+// ---- allocation-free training step
+// Its work buffers live in `scratch`, one copy per device for the process's
+// lifetime, sized for kNoAllocMaxRecords records: gradient partials
+// (9,560,064 B), forward block sums (8,192 B), activations (1,088 B per
+// record). Host code never passes `scratch` itself (in host code the name is
+// a stand-in, not a device address): guide_scratch_setup records each
+// device's address of it from hipGetSymbolAddress.
+static constexpr uint32_t kNoAllocMaxRecords = 1u << 15;
+static constexpr size_t kScratchPartialBytes = 4ull * kWgBlocks * kGParams;
+static constexpr size_t kScratchScalBytes = 8ull * kFwdMaxBlocks;
+static constexpr size_t kScratchBytes =
+    kScratchPartialBytes + kScratchScalBytes + 2ull * kActRowsTotal * kNoAllocMaxRecords;
+static_assert(kScratchPartialBytes % 256 == 0 && kScratchScalBytes % 256 == 0,
+              "scratch sub-buffers start on 256-byte boundaries");
+static_assert(kScratchBytes == 45219840ull, "scratch size as documented");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxScratchDevices];
+
+// Called by ajg_ctx_create with the context's device current.
+int guide_scratch_setup(ajg_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxScratchDevices)
+    return fail(ctx, AJG_ERR_INVALID, "device index above the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[ctx->device]) return AJG_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return hip_fail(ctx, e, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[ctx->device] = (unsigned char *)p;
+  return AJG_OK;
+}
+
+// Carves the step's buffers for npad records out of the device's scratch.
+static TrainBufs scratch_bufs(unsigned char *base, uint32_t npad) {
+  TrainBufs tb;
+  tb.partial = (float *)base;
+  tb.scal = (float *)(base + kScratchPartialBytes);
+  unsigned char *act = base + kScratchPartialBytes + kScratchScalBytes;
+  uint16_t **dst[8] = {&tb.x0t, &tb.x1t, &tb.x2t, &tb.x3t, &tb.dy4t, &tb.dx3t, &tb.dx2t, &tb.dx1t};
+  for (int k = 0; k < 8; ++k) {
+    *dst[k] = (uint16_t *)act;
+    act += 2ull * kActRows[k] * npad;  // a multiple of 256 bytes (npad % 16 == 0)
+  }
+  return tb;
+}
+
+int guide_train_launch_noalloc(ajg_guide *gd, const float *samples, uint32_t n, float lr,
+                               hipStream_t s) {
+  if (n > kNoAllocMaxRecords)
+    return fail(gd->ctx, AJG_ERR_INVALID,
+                "guide train step (no allocation): at most 32768 records per step");
+  const int dev = gd->ctx->device;
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev >= 0 && dev < kMaxScratchDevices) base = g_scratch_ptr[dev];
+  }
+  if (!base) return fail(gd->ctx, AJG_ERR_INTERNAL, "guide scratch not set up on this device");
+  const uint32_t npad = (n + 15) / 16 * 16;
+  return train_launch(gd, scratch_bufs(base, npad), samples, n, npad, lr, s);
+}
+
 int guide_records(ajg_guide *gd, uint32_t cap, float **rec, uint32_t **ctr, uint32_t *cap_out) {
   if (gd->rec_cap < cap) {
     (void)hipFree(gd->rec);
@@ -921,6 +986,13 @@ int ajg_guide_forward(ajg_guide *gd, const float *pos, const float *nrm, uint32_
   if (int st = injected(gd->ctx)) return st;
   return guide_forward_launch(gd, pos, nrm, n, nullptr, 3, mixture, nullptr, raw,
                               (hipStream_t)stream);
+}
+
+int ajg_guide_train_step_noalloc(ajg_guide *gd, const float *samples, uint32_t n, float lr,
+                                 void *stream) {
+  if (!gd || !samples || n == 0 || !(lr >= 0.f)) return AJG_ERR_INVALID;
+  if (int st = injected(gd->ctx)) return st;
+  return guide_train_launch_noalloc(gd, samples, n, lr, (hipStream_t)stream);
 }
 
 int ajg_guide_train_step(ajg_guide *gd, const float *samples, uint32_t n, float lr,

@@ -69,6 +69,9 @@ int guide_forward_launch(ajg_guide *gd, const float *pos, const float *nrm, uint
                          const uint32_t *n_dev, uint32_t stride, float *mix, _Float16 *mixh,
                          float *raw, hipStream_t s);
 int guide_train_launch(ajg_guide *gd, const float *samples, uint32_t n, float lr, hipStream_t s);
+// Records the device address of the allocation-free training step's scratch
+// for ctx's device (called once per context, with that device current).
+int guide_scratch_setup(ajg_ctx *ctx);
 // The guide's training-record pool (grown to at least cap rows of 10 floats;
 // growing empties it); ctr[0] = rows held, ctr[2..3] = u64 running total.
 int guide_records(ajg_guide *gd, uint32_t cap, float **rec, uint32_t **ctr, uint32_t *cap_out);

@@ -394,6 +394,18 @@ int ajg_guide_forward(ajg_guide *guide, const float *pos, const float *nrm,
 int ajg_guide_train_step(ajg_guide *guide, const float *samples, uint32_t n,
                          float lr, float *loss_out, float *grad_out,
                          void *stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same
+ * training step as ajg_guide_train_step, with bit-identical results, that
+ * allocates nothing. Its work buffers (activations, gradient partials) live
+ * in a static per-device scratch of 45,219,840 bytes that each GPU the
+ * library runs on holds until the process ends; ajg_ctx_create records its
+ * address. At most 32768 samples per step (AJG_ERR_INVALID above). Only
+ * enqueues on `stream` (no loss/gradient copies, no synchronisation; the
+ * updated parameters stay in the guide). Calls that overlap on one GPU
+ * share the scratch and race: issue them on one stream, one at a time. */
+int ajg_guide_train_step_noalloc(ajg_guide *guide, const float *samples,
+                                 uint32_t n, float lr, void *stream);
 
 /* Guided rendering options. */
 typedef struct ajg_guide_args {
