@@ -439,25 +439,114 @@ k_build_fat(const PlocNode *fin, const uint32_t *fat_idx, uint32_t nn, FatNode *
   if (r1 >= 0) parent_of_fat[r1] = me; else parent_of_leaf[~r1] = me;
 }
 
+// Quantisation grid of one axis of a node: step 2^e and origin o (a
+// multiple of 2^e) such that [lo, hi] lies within o + [0, 253] 2^e and every
+// plane o + q 2^e, q = 0..255, is an exact float: 2^e is at least the ulp of
+// the largest magnitude on the axis (e >= exponent - 23), so |o + q 2^e| /
+// 2^e stays below 2^24 + 256 on a grid of 2^e. e in [-126, 127].
+__device__ void qaxis(float lo, float hi, float *origin, int *e_out) {
+  const float M = fmaxf(fabsf(lo), fabsf(hi));
+  int e = -126;
+  const int em = (int)((__float_as_uint(M) >> 23) & 0xffu) - 127;
+  if (M > 0.f) e = max(e, em - 23);
+  const double ext = (double)hi - (double)lo;
+  while (e < 127 && ldexp(253.0, e) < ext) ++e;
+  const double step = ldexp(1.0, e);
+  *origin = (float)(floor((double)lo / step) * step);
+  *e_out = e;
+}
+
+// Outward rounding of one plane: the largest q with o + q s <= v (lower
+// planes) or the smallest with o + q s >= v (upper planes), checked in the
+// float decode itself.
+__device__ uint32_t q_down(float v, float o, float s) {
+  const double t = floor(((double)v - (double)o) / (double)s);
+  int q = (int)fmin(fmax(t, 0.0), 255.0);
+  while (q > 0 && fmaf((float)q, s, o) > v) --q;
+  return (uint32_t)q;
+}
+__device__ uint32_t q_up(float v, float o, float s) {
+  const double t = ceil(((double)v - (double)o) / (double)s);
+  int q = (int)fmin(fmax(t, 0.0), 255.0);
+  while (q < 255 && fmaf((float)q, s, o) < v) ++q;
+  return (uint32_t)q;
+}
+
+// The six plane bytes of box (mn, mx) in the node's grid; *bad is raised
+// if the decoded box does not contain it (non-finite coordinates).
+__device__ void q_box(const float *mn, const float *mx, const float *o, const float *st, uint8_t *out,
+                      uint32_t *bad) {
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t lo = q_down(mn[a], o[a], st[a]), hi = q_up(mx[a], o[a], st[a]);
+    out[a] = (uint8_t)lo;
+    out[3 + a] = (uint8_t)hi;
+    if (!(fmaf((float)lo, st[a], o[a]) <= mn[a] && fmaf((float)hi, st[a], o[a]) >= mx[a])) atomicAdd(bad, 1u);
+  }
+}
+
+// FatNode child boxes: c0 = (a.xyz, (a.w, b.x, b.y)), c1 = ((b.z, b.w, c.x), c.yzw).
+__device__ void fat_boxes(const FatNode &f, float (*mn)[3], float (*mx)[3]) {
+  const float v[12] = {f.a.x, f.a.y, f.a.z, f.a.w, f.b.x, f.b.y, f.b.z, f.b.w, f.c.x, f.c.y, f.c.z, f.c.w};
+  for (int k = 0; k < 2; ++k)
+    for (int a = 0; a < 3; ++a) {
+      mn[k][a] = v[6 * k + a];
+      mx[k][a] = v[6 * k + 3 + a];
+    }
+}
+
 __global__ void __launch_bounds__(256)
-k_build_wide(const FatNode *fat, uint32_t n_int, WideNode *wide) {
+k_build_wide(const FatNode *fat, uint32_t n_int, WideNode *wide, uint32_t *bad) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_int) return;
   const FatNode f = fat[i];
-  WideNode w;
-  w.a = f.a;
-  w.b = f.b;
-  w.c = f.c;
-  w.r = make_int4(f.d.x, f.d.y, -1, -1);
-  w.la = w.lb = w.lc = w.pad = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (f.d.x >= 0) {
-    const FatNode l = fat[f.d.x];
-    w.la = l.a;
-    w.lb = l.b;
-    w.lc = l.c;
-    w.r.z = l.d.x;
-    w.r.w = l.d.y;
+  float cmn[2][3], cmx[2][3];
+  fat_boxes(f, cmn, cmx);
+  // slots: the grandchildren in left-first order (a leaf child stands for itself)
+  float mn[4][3], mx[4][3];
+  int ref[4] = {kNoRef, kNoRef, kNoRef, kNoRef};
+  int ns = 0;
+  const int child[2] = {f.d.x, f.d.y};
+  for (int c = 0; c < 2; ++c) {
+    if (child[c] >= 0) {
+      const FatNode g = fat[child[c]];
+      float gmn[2][3], gmx[2][3];
+      fat_boxes(g, gmn, gmx);
+      const int gr[2] = {g.d.x, g.d.y};
+      for (int k = 0; k < 2; ++k, ++ns) {
+        for (int a = 0; a < 3; ++a) {
+          mn[ns][a] = gmn[k][a];
+          mx[ns][a] = gmx[k][a];
+        }
+        ref[ns] = gr[k];
+      }
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        mn[ns][a] = cmn[c][a];
+        mx[ns][a] = cmx[c][a];
+      }
+      ref[ns++] = child[c];
+    }
   }
+  // grid over the node's box (every slot box lies inside it)
+  float o[3], st[3];
+  uint32_t ebits = 0;
+  for (int a = 0; a < 3; ++a) {
+    int e;
+    qaxis(fmin_r(cmn[0][a], cmn[1][a]), fmax_r(cmx[0][a], cmx[1][a]), &o[a], &e);
+    st[a] = ldexpf(1.f, e);
+    ebits |= (uint32_t)(e + 127) << (8 * a);
+  }
+  uint8_t q[32] = {0};
+  for (int k = 0; k < ns; ++k) q_box(mn[k], mx[k], o, st, q + 6 * k, bad);
+  uint32_t wd[8];
+  for (int k = 0; k < 8; ++k)
+    wd[k] = (uint32_t)q[4 * k] | (uint32_t)q[4 * k + 1] << 8 | (uint32_t)q[4 * k + 2] << 16 |
+            (uint32_t)q[4 * k + 3] << 24;
+  WideNode w;
+  w.o = make_float4(o[0], o[1], o[2], __uint_as_float(ebits));
+  w.qa = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+  w.qb = make_uint4(wd[4], wd[5], 0u, 0u);
+  w.r = make_int4(ref[0], ref[1], ref[2], ref[3]);
   wide[i] = w;
 }
 
@@ -655,15 +744,19 @@ int build_accel(ajg_ctx *ctx, const float *tri, uint32_t n, const float *scene_b
     scan_u32(flag, fat_idx, nn, false, scan_tmp, s);
     hipLaunchKernelGGL(k_build_fat, dim3(gn), dim3(256), 0, s, a->final_nodes, fat_idx, nn, a->fat, pof, pol);
     hipLaunchKernelGGL(k_set_parent, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, a->fat, pof, n - 1);
+    uint32_t *ddepth;  // [0] tree depth, [1] boxes the quantised grid failed to contain
+    if ((st = tmp.alloc(&ddepth, 2, "depth"))) return bail(st);
+    AJG_HIP(ctx, hipMemsetAsync(ddepth, 0, 8, s));
     hipLaunchKernelGGL(k_build_wide, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, (const FatNode *)a->fat,
-                       n - 1, a->wide);
-    uint32_t *ddepth;
-    if ((st = tmp.alloc(&ddepth, 1, "depth"))) return bail(st);
-    AJG_HIP(ctx, hipMemsetAsync(ddepth, 0, 4, s));
+                       n - 1, a->wide, ddepth + 1);
     hipLaunchKernelGGL(k_depth, dim3(cdiv(n, 256)), dim3(256), 0, s, pof, pol, n, n - 1, ddepth);
-    hipError_t e = hipMemcpyAsync(&a->depth, ddepth, 4, hipMemcpyDeviceToHost, s);
+    uint32_t hd[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(hd, ddepth, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(hip_fail(ctx, e, "fat layout"));
+    a->depth = hd[0];
+    if (hd[1] != 0)
+      return bail(fail(ctx, AJG_ERR_INVALID, "accel build: non-finite triangle coordinates"));
     // traversal reads the wide nodes only
     (void)hipFree(a->fat);
     a->fat = nullptr;

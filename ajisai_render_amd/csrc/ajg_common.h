@@ -137,18 +137,38 @@ struct FatNode {
   int4 d;    // child0, child1, parent (fat index, -1 at root), unused
 };
 
-// Traversal node (128 bytes, one per interior node, same index as the fat
-// node): the node's two child boxes and refs and, when its left child is
-// interior, that child's two child boxes and refs. A traversal that enters
-// the left child tests its children from the same fetch, so a left descent
-// costs one dependent load per two tree levels; the tests, their order and
-// t_max at each test are those of the binary traversal.
+// Traversal node (64 bytes, one per interior node, same index as the fat
+// node): the node's grandchildren in left-first order — for each child, its
+// two children if it is interior, else the child itself — so 2 to 4 slots
+// and one fetch per two tree levels. Skipping the children's own box tests
+// keeps every result: a grandchild's box lies inside its parent's, so its
+// test fails wherever the parent's would (and the traversal only ever
+// visits more nodes than the reference, never fewer or in another order).
+//
+// The slot boxes are quantised: per axis a grid of step 2^e anchored at
+// o (a multiple of 2^e), planes o + q 2^e with q in 0..255, each box rounded
+// outward (lower planes down, upper planes up) and checked at build time, so
+// a decoded box (fmaf(q, 2^e, o), the same operation in the builder and the
+// traversal) contains the exact PLOC box. A slab test of a containing box
+// passes wherever the exact one does; leaves are tested against their exact
+// boxes (the triangle's AABB) at visit time, which keeps hits, t, u, v
+// bit-identical to the reference.
+static constexpr int kNoRef = (int)0x80000000;  // empty slot
 struct WideNode {
-  float4 a, b, c;     // child boxes (FatNode layout)
-  int4 r;             // left, right, left-left, left-right (fat index, or ~leaf slot)
-  float4 la, lb, lc;  // child boxes of the left child (when r.x >= 0)
-  float4 pad;
+  float4 o;  // grid origin xyz; w: the bits (e_x + 127) | (e_y + 127) << 8 | (e_z + 127) << 16
+  uint4 qa;  // plane bytes 0..15 (slot k: bytes 6k .. 6k+5 = lo xyz, hi xyz)
+  uint4 qb;  // plane bytes 16..23 in .x, .y
+  int4 r;    // slot refs (fat index, ~leaf slot, or kNoRef), left-first order
 };
+static_assert(sizeof(WideNode) == 64, "64-byte traversal node");
+
+// Plane k (0..23) of a node's quantised boxes: byte k of qa.xyzw, qb.xy.
+__device__ __forceinline__ float qplane(uint32_t word, int byte, float step, float origin) {
+  return fmaf((float)((word >> (8 * byte)) & 0xffu), step, origin);
+}
+__device__ __forceinline__ float qstep(float4 o, int axis) {
+  return __uint_as_float(((__float_as_uint(o.w) >> (8 * axis)) & 0xffu) << 23);
+}
 
 static constexpr uint32_t kNone = 0xFFFFFFFFu;
 

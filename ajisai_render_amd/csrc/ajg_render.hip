@@ -616,18 +616,18 @@ static void launch_pq2(int stack, const DevScene &ds, bool closest, const DevRay
                        hipStream_t s) {
   const dim3 g(grid), b(kTravThreads);
   if (closest) {
-    if (stack == 48)
-      hipLaunchKernelGGL((k_pq_closest2<48>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+    if (stack == kSpillS)
+      hipLaunchKernelGGL((k_pq_closest2<kSpillS>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
                          (float4 *)out, hit_inst, count, cursor);
     else
-      hipLaunchKernelGGL((k_pq_closest2<1024>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+      hipLaunchKernelGGL((k_pq_closest2<kSpillL>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
                          (float4 *)out, hit_inst, count, cursor);
   } else {
-    if (stack == 48)
-      hipLaunchKernelGGL((k_pq_any2<48>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+    if (stack == kSpillS)
+      hipLaunchKernelGGL((k_pq_any2<kSpillS>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
                          (uint8_t *)out, count);
     else
-      hipLaunchKernelGGL((k_pq_any2<1024>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+      hipLaunchKernelGGL((k_pq_any2<kSpillL>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
                          (uint8_t *)out, count);
   }
 }
@@ -638,18 +638,18 @@ static void launch_pq(int stack, const DevBVH &bvh, bool closest, const DevRay *
                       uint32_t grid, hipStream_t s) {
   const dim3 g(grid), b(kTravThreads);
   if (closest) {
-    if (stack == 48)
-      hipLaunchKernelGGL((k_pq_closest<48, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count,
+    if (stack == kSpillS)
+      hipLaunchKernelGGL((k_pq_closest<kSpillS, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count,
                          cursor, ctr);
     else
-      hipLaunchKernelGGL((k_pq_closest<1024, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count,
+      hipLaunchKernelGGL((k_pq_closest<kSpillL, COUNT>), g, b, 0, s, bvh, rays, (float4 *)out, count,
                          cursor, ctr);
   } else {
-    if (stack == 48)
-      hipLaunchKernelGGL((k_pq_any<48, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
+    if (stack == kSpillS)
+      hipLaunchKernelGGL((k_pq_any<kSpillS, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
                          cursor, ctr);
     else
-      hipLaunchKernelGGL((k_pq_any<1024, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
+      hipLaunchKernelGGL((k_pq_any<kSpillL, COUNT>), g, b, 0, s, bvh, rays, (uint8_t *)out, count,
                          cursor, ctr);
   }
 }
@@ -2192,16 +2192,16 @@ int ajg_scene_trace_closest(ajg_scene *scene, const ajg_ray *rays, uint32_t n, a
                          (const ajg_hit *)hits, n, instance);
   } else {
     const int spill = spill_for_depth(scene->depth2);
-    if (!spill) return fail(scene->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+    if (!spill) return fail(scene->ctx, AJG_ERR_INTERNAL, "trace: tree too deep for the traversal stack");
     const DevScene ds = dev_scene(scene);
     const uint32_t grid = std::min<uint32_t>(cdiv(n, kTravThreads),
                                              (uint32_t)scene->cu_count * kTravBlocksPerCU);
     hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(64), 0, s, scene->trace_ctr);
-    if (spill == 48)
-      hipLaunchKernelGGL(k_scene_closest2<48>, dim3(grid), dim3(kTravThreads), 0, s, ds.bvh, ds.inst,
+    if (spill == kSpillS)
+      hipLaunchKernelGGL(k_scene_closest2<kSpillS>, dim3(grid), dim3(kTravThreads), 0, s, ds.bvh, ds.inst,
                          ds.blas, rays, n, hits, instance, scene->trace_ctr);
     else
-      hipLaunchKernelGGL(k_scene_closest2<1024>, dim3(grid), dim3(kTravThreads), 0, s, ds.bvh, ds.inst,
+      hipLaunchKernelGGL(k_scene_closest2<kSpillL>, dim3(grid), dim3(kTravThreads), 0, s, ds.bvh, ds.inst,
                          ds.blas, rays, n, hits, instance, scene->trace_ctr);
   }
   hipError_t e = hipGetLastError();
@@ -2216,13 +2216,13 @@ int ajg_scene_trace_any(ajg_scene *scene, const ajg_ray *rays, uint32_t n, uint8
   hipStream_t s = (hipStream_t)stream;
   if (!scene->insts) return launch_trace_any(scene->accel, rays, n, occluded, s);
   const int spill = spill_for_depth(scene->depth2);
-  if (!spill) return fail(scene->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+  if (!spill) return fail(scene->ctx, AJG_ERR_INTERNAL, "trace: tree too deep for the traversal stack");
   const DevScene ds = dev_scene(scene);
-  if (spill == 48)
-    hipLaunchKernelGGL(k_scene_any2<48>, dim3(cdiv(n, kTravThreads)), dim3(kTravThreads), 0, s, ds.bvh,
+  if (spill == kSpillS)
+    hipLaunchKernelGGL(k_scene_any2<kSpillS>, dim3(cdiv(n, kTravThreads)), dim3(kTravThreads), 0, s, ds.bvh,
                        ds.inst, ds.blas, rays, n, occluded);
   else
-    hipLaunchKernelGGL(k_scene_any2<1024>, dim3(cdiv(n, kTravThreads)), dim3(kTravThreads), 0, s, ds.bvh,
+    hipLaunchKernelGGL(k_scene_any2<kSpillL>, dim3(cdiv(n, kTravThreads)), dim3(kTravThreads), 0, s, ds.bvh,
                        ds.inst, ds.blas, rays, n, occluded);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AJG_OK : hip_fail(scene->ctx, e, "scene trace any");
@@ -2461,7 +2461,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   q.per_sample = film->per_sample;
   const bool inst = ds.inst != nullptr;
   const int stack = spill_for_depth(inst ? sc->depth2 : a->depth);
-  if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree depth above 1024"));
+  if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree too deep for the traversal stack"));
 
   const bool counting = (args->flags & AJG_RENDER_COUNT) != 0;
   if (inst && counting)
@@ -2773,7 +2773,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   begin[n_jobs] = (uint32_t)total;
   const uint32_t S = (uint32_t)std::min<uint64_t>(target, total);
   const int spill = spill_for_depth(max_depth, ba.lds_stack);
-  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "batch: tree depth above 1024");
+  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "batch: tree too deep for the traversal stack");
 
   BatchBuffers *bb = (BatchBuffers *)ctx->batch_cache;
   if (!bb) {
@@ -2905,15 +2905,15 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
                          B);
       q_ok(hipEventRecord(rb->ev_fork, s));
       q_ok(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
-      if (spill == 48) {
-        launch_pq_b<48>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
+      if (spill == kSpillS) {
+        launch_pq_b<kSpillS>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
                         cq_in, c_cursor, trav_grid, s);
-        launch_pq_b<48>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
+        launch_pq_b<kSpillS>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
                         aq_in, a_cursor, trav_grid, rb->aux);
       } else {
-        launch_pq_b<1024>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
+        launch_pq_b<kSpillL>(ba.lds_stack, true, bb->jobs, bb->c_job[nxt], rb->c_rays[nxt], rb->c_hits,
                           cq_in, c_cursor, trav_grid, s);
-        launch_pq_b<1024>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
+        launch_pq_b<kSpillL>(ba.lds_stack, false, bb->jobs, bb->a_job[nxt], rb->a_rays[nxt], rb->a_occ,
                           aq_in, a_cursor, trav_grid, rb->aux);
       }
       q_ok(hipEventRecord(rb->ev_join, rb->aux));

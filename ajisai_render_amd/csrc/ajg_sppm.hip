@@ -663,7 +663,7 @@ extern "C" int ajg_render_sppm(ajg_scene *sc, const ajg_camera *camera, const aj
   hipStream_t s = (hipStream_t)stream;
   const ajg_accel *acc = sc->accel;
   const int spill = spill_for_depth(acc->depth);
-  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "sppm: tree depth above 1024");
+  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "sppm: tree too deep for the traversal stack");
 
   SppmParams P;
   memset(&P, 0, sizeof(P));
@@ -735,10 +735,10 @@ extern "C" int ajg_render_sppm(ajg_scene *sc, const ajg_camera *camera, const aj
   const dim3 g256(cdiv(P.n_pix, 256)), gtrav(cdiv(P.n_pix, kTravThreads));
   (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_sppm_init, g256, dim3(256), 0, s, P, S);
-  if (spill == 48)
-    hipLaunchKernelGGL(k_sppm_vp<48>, gtrav, dim3(kTravThreads), 0, s, ds, P, S);
+  if (spill == kSpillS)
+    hipLaunchKernelGGL(k_sppm_vp<kSpillS>, gtrav, dim3(kTravThreads), 0, s, ds, P, S);
   else
-    hipLaunchKernelGGL(k_sppm_vp<1024>, gtrav, dim3(kTravThreads), 0, s, ds, P, S);
+    hipLaunchKernelGGL(k_sppm_vp<kSpillL>, gtrav, dim3(kTravThreads), 0, s, ds, P, S);
   (void)hipEventRecord(ev[1], s);
   const dim3 gph(cdiv(std::max(P.photons, 1u), kTravThreads));
   for (int it = 0; it < a->iteration_count; ++it) {
@@ -752,10 +752,10 @@ extern "C" int ajg_render_sppm(ajg_scene *sc, const ajg_camera *camera, const aj
     hipLaunchKernelGGL(k_sppm_grid<true>, g256, dim3(256), 0, s, P, S);
     if (timing) (void)hipEventRecord(ev[3], s);
     if (P.photons && P.n_lights) {
-      if (spill == 48)
-        hipLaunchKernelGGL(k_sppm_photon<48>, gph, dim3(kTravThreads), 0, s, ds, P, S, (uint32_t)it);
+      if (spill == kSpillS)
+        hipLaunchKernelGGL(k_sppm_photon<kSpillS>, gph, dim3(kTravThreads), 0, s, ds, P, S, (uint32_t)it);
       else
-        hipLaunchKernelGGL(k_sppm_photon<1024>, gph, dim3(kTravThreads), 0, s, ds, P, S, (uint32_t)it);
+        hipLaunchKernelGGL(k_sppm_photon<kSpillL>, gph, dim3(kTravThreads), 0, s, ds, P, S, (uint32_t)it);
     }
     if (timing) (void)hipEventRecord(ev[4], s);
     hipLaunchKernelGGL(k_sppm_maxr_clear, dim3(1), dim3(64), 0, s, S);

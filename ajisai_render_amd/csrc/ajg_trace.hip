@@ -100,14 +100,14 @@ int launch_trace_closest(ajg_accel *a, const ajg_ray *rays, uint32_t n, ajg_hit 
   const DevBVH b = dev_bvh(a);
   const uint32_t g = persistent_grid(a, n);
   hipLaunchKernelGGL(k_cursor_clear, dim3(1), dim3(64), 0, s, a->cursor);
-  if (spill_for_depth(a->depth) == 48)
-    hipLaunchKernelGGL(k_trace_closest<48>, dim3(g), dim3(256), 0, s, b, rays, n, hits,
+  if (spill_for_depth(a->depth) == kSpillS)
+    hipLaunchKernelGGL(k_trace_closest<kSpillS>, dim3(g), dim3(256), 0, s, b, rays, n, hits,
                        (const uint32_t *)a->perm, a->cursor);
-  else if (spill_for_depth(a->depth) == 1024)
-    hipLaunchKernelGGL(k_trace_closest<1024>, dim3(g), dim3(256), 0, s, b, rays, n, hits,
+  else if (spill_for_depth(a->depth) == kSpillL)
+    hipLaunchKernelGGL(k_trace_closest<kSpillL>, dim3(g), dim3(256), 0, s, b, rays, n, hits,
                        (const uint32_t *)a->perm, a->cursor);
   else
-    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree too deep for the traversal stack");
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AJG_OK : hip_fail(a->ctx, e, "k_trace_closest");
 }
@@ -116,12 +116,12 @@ int launch_trace_counts(ajg_accel *a, const ajg_ray *rays, uint32_t n, uint32_t 
                         uint32_t *tris, hipStream_t s) {
   if (n == 0) return AJG_OK;
   const DevBVH b = dev_bvh(a);
-  if (spill_for_depth(a->depth) == 48)
-    hipLaunchKernelGGL(k_trace_counts<48>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
-  else if (spill_for_depth(a->depth) == 1024)
-    hipLaunchKernelGGL(k_trace_counts<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
+  if (spill_for_depth(a->depth) == kSpillS)
+    hipLaunchKernelGGL(k_trace_counts<kSpillS>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
+  else if (spill_for_depth(a->depth) == kSpillL)
+    hipLaunchKernelGGL(k_trace_counts<kSpillL>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, nodes, tris);
   else
-    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree too deep for the traversal stack");
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AJG_OK : hip_fail(a->ctx, e, "k_trace_counts");
 }
@@ -130,12 +130,12 @@ int launch_trace_any(ajg_accel *a, const ajg_ray *rays, uint32_t n, uint8_t *occ
                      hipStream_t s) {
   if (n == 0) return AJG_OK;
   const DevBVH b = dev_bvh(a);
-  if (spill_for_depth(a->depth) == 48)
-    hipLaunchKernelGGL(k_trace_any<48>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
-  else if (spill_for_depth(a->depth) == 1024)
-    hipLaunchKernelGGL(k_trace_any<1024>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
+  if (spill_for_depth(a->depth) == kSpillS)
+    hipLaunchKernelGGL(k_trace_any<kSpillS>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
+  else if (spill_for_depth(a->depth) == kSpillL)
+    hipLaunchKernelGGL(k_trace_any<kSpillL>, dim3(grid(n)), dim3(256), 0, s, b, rays, n, occ);
   else
-    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree depth above 1024");
+    return fail(a->ctx, AJG_ERR_INTERNAL, "trace: tree too deep for the traversal stack");
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? AJG_OK : hip_fail(a->ctx, e, "k_trace_any");
 }

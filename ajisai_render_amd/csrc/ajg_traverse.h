@@ -11,9 +11,15 @@
 // with its entry distance, and compares that entry with the t_max current at
 // pop time — exactly when the recursive reference would test it. A fetched
 // WideNode also carries its left child's child boxes, so entering an interior
-// left child performs that child's tests without another fetch (same tests,
-// same order, same t_max). Leaf boxes are exact (the PLOC leaf AABB), so ids,
-// t, u, v are bit-identical to the reference, ties included.
+// left child performs that child's tests without another fetch (same order,
+// same t_max). The node's boxes are quantised outward (ajg_common.h), so an
+// interior test passes wherever the reference's passes and the traversal
+// visits a superset of its nodes; a leaf is tested against its exact box
+// (the triangle's AABB, as the PLOC leaf node holds it) at visit time, with
+// the t_max current then, as the reference tests it. A leaf the reference
+// never reaches lies inside an ancestor whose exact test failed, so its own
+// exact test (an interval inside that ancestor's) fails as well: ids, t, u,
+// v are bit-identical to the reference, ties included.
 // Any hit restates OccludeAux (:345-365); its result does not depend on the
 // visit order because t_max never changes.
 #pragma once
@@ -109,13 +115,21 @@ __device__ __forceinline__ void load_leaf(const float4 *lt, int slot, f3 *a, f3 
   *c = mk3(p2.x, p2.y, p2.z);
 }
 
+// The exact leaf box test (IntersectP on the triangle's AABB,
+// ploc_aggregate.cpp:154-162, 168-182) with the given t_max.
+__device__ __forceinline__ bool leaf_box_test(const TRay &r, f3 a, f3 b, f3 c, float tmax) {
+  const box3 bx = tri_box(a, b, c);
+  float e;
+  return slab_static(r, bx.mn.x, bx.mn.y, bx.mn.z, bx.mx.x, bx.mx.y, bx.mx.z, &e) && e < tmax;
+}
+
 // The leaf box test at visit time + MT, updating the closest hit.
 __device__ __forceinline__ void visit_leaf_closest(const DevBVH &bvh, const TRay &r,
                                                    int slot, THit *h, float *tmax) {
   f3 a, b, c;
   load_leaf(bvh.leaf_tris, slot, &a, &b, &c);
   float t, u, v;
-  if (mt_test(a, b, c, r, *tmax, &t, &u, &v)) {
+  if (leaf_box_test(r, a, b, c, *tmax) && mt_test(a, b, c, r, *tmax, &t, &u, &v)) {
     *tmax = t;
     h->t = t;
     h->u = u;
@@ -164,33 +178,58 @@ __device__ __forceinline__ void stack_pop(const StackRef &s, int &d, int *n, flo
   }
 }
 
-// Closest-hit visit of interior node `node` from one fetch: the child tests
-// of the node (right child pushed with its entry distance, left child entered
-// if its entry is below t_max) and, when the entered left child is interior,
-// that child's tests too, with the same t_max, exactly as the binary
-// traversal does at its next fetch. Returns false when the stack must be
-// popped; otherwise `node` is the next node (interior or ~leaf).
+// Static slab part of quantised slot k (0..3) of a node: plane bytes
+// 6k .. 6k+5 (lo xyz, hi xyz) of qa, qb.
+struct QBoxes {
+  float4 o;
+  float sx, sy, sz;
+  uint32_t w[6];
+};
+__device__ __forceinline__ bool qslab(const TRay &r, const QBoxes &q, int k, float *entry) {
+  const int p = 6 * k;
+  const float mnx = qplane(q.w[(p + 0) >> 2], (p + 0) & 3, q.sx, q.o.x);
+  const float mny = qplane(q.w[(p + 1) >> 2], (p + 1) & 3, q.sy, q.o.y);
+  const float mnz = qplane(q.w[(p + 2) >> 2], (p + 2) & 3, q.sz, q.o.z);
+  const float mxx = qplane(q.w[(p + 3) >> 2], (p + 3) & 3, q.sx, q.o.x);
+  const float mxy = qplane(q.w[(p + 4) >> 2], (p + 4) & 3, q.sy, q.o.y);
+  const float mxz = qplane(q.w[(p + 5) >> 2], (p + 5) & 3, q.sz, q.o.z);
+  return slab_static(r, mnx, mny, mnz, mxx, mxy, mxz, entry);
+}
+__device__ __forceinline__ QBoxes qload(const WideNode *w) {
+  QBoxes q;
+  q.o = w->o;
+  q.sx = qstep(q.o, 0);
+  q.sy = qstep(q.o, 1);
+  q.sz = qstep(q.o, 2);
+  const uint4 a = w->qa;
+  const uint4 b = w->qb;
+  q.w[0] = a.x;
+  q.w[1] = a.y;
+  q.w[2] = a.z;
+  q.w[3] = a.w;
+  q.w[4] = b.x;
+  q.w[5] = b.y;
+  return q;
+}
+
+// Closest-hit visit of interior node `node` from one fetch: slots 3, 2, 1
+// (when present and their static slab part passes) are pushed with their
+// entry distances, to be compared with t_max when popped — when the
+// recursive reference would test them; slot 0 is entered if its entry is
+// below t_max. Returns false when the stack must be popped; otherwise `node`
+// is the next node (interior or ~leaf).
 template <int LDS>
 __device__ __forceinline__ bool closest_step(const WideNode *wide, int &node, const TRay &r,
                                              float tmax, const StackRef &st, int &d) {
   const WideNode *w = wide + node;
-  const float4 a = w->a, b = w->b, c = w->c;
+  const QBoxes q = qload(w);
   const int4 ref = w->r;
-  const float4 la = w->la, lb = w->lb, lc = w->lc;
-  float e0, e1;
-  const bool s0 = slab_static(r, a.x, a.y, a.z, a.w, b.x, b.y, &e0);
-  const bool s1 = slab_static(r, b.z, b.w, c.x, c.y, c.z, c.w, &e1);
-  if (s1) stack_push<LDS>(st, d, ref.y, e1);
-  if (!(s0 && e0 < tmax)) return false;
-  if (ref.x < 0) {
+  float e;
+  if (ref.w != kNoRef && qslab(r, q, 3, &e)) stack_push<LDS>(st, d, ref.w, e);
+  if (ref.z != kNoRef && qslab(r, q, 2, &e)) stack_push<LDS>(st, d, ref.z, e);
+  if (qslab(r, q, 1, &e)) stack_push<LDS>(st, d, ref.y, e);
+  if (qslab(r, q, 0, &e) && e < tmax) {
     node = ref.x;
-    return true;
-  }
-  const bool t0 = slab_static(r, la.x, la.y, la.z, la.w, lb.x, lb.y, &e0);
-  const bool t1 = slab_static(r, lb.z, lb.w, lc.x, lc.y, lc.z, lc.w, &e1);
-  if (t1) stack_push<LDS>(st, d, ref.w, e1);
-  if (t0 && e0 < tmax) {
-    node = ref.z;
     return true;
   }
   return false;
@@ -202,33 +241,20 @@ template <int LDS>
 __device__ __forceinline__ bool any_step(const WideNode *wide, int &node, const TRay &r,
                                          float tmax, const StackRef &st, int &d) {
   const WideNode *w = wide + node;
-  const float4 a = w->a, b = w->b, c = w->c;
+  const QBoxes q = qload(w);
   const int4 ref = w->r;
-  float e0, e1;
-  const bool s0 = slab_static(r, a.x, a.y, a.z, a.w, b.x, b.y, &e0) && e0 < tmax;
-  const bool s1 = slab_static(r, b.z, b.w, c.x, c.y, c.z, c.w, &e1) && e1 < tmax;
-  if (!s0) {
-    node = ref.y;
-    return s1;
-  }
-  if (s1) stack_push<LDS>(st, d, ref.y, e1);
-  if (ref.x < 0) {
+  float e;
+  if (ref.w != kNoRef && qslab(r, q, 3, &e) && e < tmax) stack_push<LDS>(st, d, ref.w, e);
+  if (ref.z != kNoRef && qslab(r, q, 2, &e) && e < tmax) stack_push<LDS>(st, d, ref.z, e);
+  if (qslab(r, q, 1, &e) && e < tmax) stack_push<LDS>(st, d, ref.y, e);
+  if (qslab(r, q, 0, &e) && e < tmax) {
     node = ref.x;
     return true;
   }
-  const float4 la = w->la, lb = w->lb, lc = w->lc;
-  const bool t0 = slab_static(r, la.x, la.y, la.z, la.w, lb.x, lb.y, &e0) && e0 < tmax;
-  const bool t1 = slab_static(r, lb.z, lb.w, lc.x, lc.y, lc.z, lc.w, &e1) && e1 < tmax;
-  if (!t0) {
-    node = ref.w;
-    return t1;
-  }
-  if (t1) stack_push<LDS>(st, d, ref.w, e1);
-  node = ref.z;
-  return true;
+  return false;
 }
 
-// Stack bound: entries <= tree depth - 1, so SPILL >= depth - kLdsStack.
+// Stack bound: spill_for_depth.
 template <int SPILL, bool COUNT>
 __device__ __forceinline__ THit trace_closest(const DevBVH &bvh, const TRay &r, int *lds_node,
                                               float *lds_entry, uint32_t *n_nodes,
@@ -295,7 +321,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
       f3 a, b, c;
       load_leaf(bvh.leaf_tris, ~node, &a, &b, &c);
       float t, u, v;
-      if (mt_test(a, b, c, r, r.tmax, &t, &u, &v)) return true;
+      if (leaf_box_test(r, a, b, c, r.tmax) && mt_test(a, b, c, r, r.tmax, &t, &u, &v)) return true;
     } else {
       if (COUNT) ++*n_nodes;
       if (any_step<LDS>(bvh.wide, node, r, r.tmax, st, d)) continue;
@@ -489,7 +515,7 @@ __device__ __forceinline__ void persistent_any(const DevBVH &bvh_uniform, uint32
       f3 a, b, c;
       load_leaf(bvh.leaf_tris, ~node, &a, &b, &c);
       float t, u, v;
-      hit = mt_test(a, b, c, r, r.tmax, &t, &u, &v);
+      hit = leaf_box_test(r, a, b, c, r.tmax) && mt_test(a, b, c, r, r.tmax, &t, &u, &v);
       if (!hit) {
         next = d > 0;
         if (next) {
@@ -613,12 +639,15 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
     bool next = false;
     if (node < 0) {
       if (base < 0) {  // an instance: enter its mesh's tree in object space
+        f3 ia, ib, ic;  // the instance box as its degenerate leaf triangle
+        load_leaf(tlas.leaf_tris, ~node, &ia, &ib, &ic);
         cur_inst = tlas_leaf_instance(tlas, ~node);
         const DevInst *I = insts + cur_inst;
         const DevBlas *B = blases + I->info.x;
         const TRay ro = make_tray(xform_point(I->w2o, wo), xform_vec(I->w2o, wd), r.tmin, tmax);
         float e;
-        if (slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
+        if (leaf_box_test(r, ia, ib, ic, tmax) &&
+            slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
                         B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
             e < tmax) {
           r = ro;
@@ -632,7 +661,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         f3 a, b, c;
         load_leaf(leaf, ~node, &a, &b, &c);
         float t, u, v;
-        if (mt_test(a, b, c, r, tmax, &t, &u, &v)) {
+        if (leaf_box_test(r, a, b, c, tmax) && mt_test(a, b, c, r, tmax, &t, &u, &v)) {
           tmax = t;
           h.t = t;
           h.u = u;
@@ -692,11 +721,14 @@ __device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *
     bool next = false;
     if (node < 0) {
       if (base < 0) {
+        f3 ia, ib, ic;  // the instance box as its degenerate leaf triangle
+        load_leaf(tlas.leaf_tris, ~node, &ia, &ib, &ic);
         const DevInst *I = insts + tlas_leaf_instance(tlas, ~node);
         const DevBlas *B = blases + I->info.x;
         const TRay ro = make_tray(xform_point(I->w2o, rw.o), xform_vec(I->w2o, rw.d), rw.tmin,
                                   rw.tmax);
-        if (slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
+        if (leaf_box_test(rw, ia, ib, ic, rw.tmax) &&
+            slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
                         B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
             e < rw.tmax) {
           r = ro;
@@ -710,7 +742,8 @@ __device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *
         f3 a, b, c;
         load_leaf(leaf, ~node, &a, &b, &c);
         float t, u, v;
-        if (mt_test(a, b, c, r, rw.tmax, &t, &u, &v)) return true;
+        if (leaf_box_test(r, a, b, c, rw.tmax) && mt_test(a, b, c, r, rw.tmax, &t, &u, &v))
+          return true;
       }
     } else {
       next = any_step<LDS>(wn, node, r, rw.tmax, st, d);
@@ -729,11 +762,16 @@ __device__ __forceinline__ bool trace_any_2l(const DevBVH &tlas, const DevInst *
   }
 }
 
-// Spill capacity for a tree of the given depth with an LDS ring of `lds`
-// entries per lane (0: unsupported depth).
+// Spill capacity for a tree of the given depth (for two-level scenes, the
+// sum of both levels' depths) with an LDS ring of `lds` entries per lane
+// (0: unsupported depth). A fetch pushes at most three entries and covers
+// two tree levels, so a path below a node of depth D holds at most
+// 3 ceil(D / 2) entries; +3 covers a second level's rounding.
+static constexpr int kSpillS = 64, kSpillL = 1024;
 inline int spill_for_depth(uint32_t depth, int lds = kLdsStack) {
-  if (depth <= (uint32_t)lds + 48) return 48;
-  if (depth <= 1024) return 1024;
+  const uint32_t entries = 3 * ((depth + 1) / 2) + 3;
+  if (entries <= (uint32_t)lds + kSpillS) return kSpillS;
+  if (entries <= (uint32_t)lds + kSpillL) return kSpillL;
   return 0;
 }
 

@@ -852,7 +852,7 @@ extern "C" int ajg_render_vcm(ajg_scene *sc, const ajg_camera *camera, const ajg
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, AJG_ERR_HIP, "hipSetDevice");
   hipStream_t s = (hipStream_t)stream;
   const int spill = spill_for_depth(sc->accel->depth);
-  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "vcm: tree depth above 1024");
+  if (!spill) return fail(ctx, AJG_ERR_INTERNAL, "vcm: tree too deep for the traversal stack");
 
   VcmParams P;
   memset(&P, 0, sizeof(P));
@@ -959,10 +959,10 @@ extern "C" int ajg_render_vcm(ajg_scene *sc, const ajg_camera *camera, const ajg
       return done(fail(ctx, AJG_ERR_HIP, "vcm clear"));
     hipLaunchKernelGGL(k_vcm_clear, g256, dim3(256), 0, s, P, S);
     if (timing) (void)hipEventRecord(ev[1], s);
-    if (spill == 48)
-      hipLaunchKernelGGL(k_vcm_light<48>, gt, dim3(kTravThreads), 0, s, ds, P, S);
+    if (spill == kSpillS)
+      hipLaunchKernelGGL(k_vcm_light<kSpillS>, gt, dim3(kTravThreads), 0, s, ds, P, S);
     else
-      hipLaunchKernelGGL(k_vcm_light<1024>, gt, dim3(kTravThreads), 0, s, ds, P, S);
+      hipLaunchKernelGGL(k_vcm_light<kSpillL>, gt, dim3(kTravThreads), 0, s, ds, P, S);
     if (timing) (void)hipEventRecord(ev[2], s);
     if (P.use_vm) {
       hipLaunchKernelGGL(k_vcm_bbox, g256, dim3(256), 0, s, P, S);
@@ -971,10 +971,10 @@ extern "C" int ajg_render_vcm(ajg_scene *sc, const ajg_camera *camera, const ajg
       hipLaunchKernelGGL(k_vcm_grid<true>, g256, dim3(256), 0, s, P, S);
     }
     if (timing) (void)hipEventRecord(ev[3], s);
-    if (spill == 48)
-      hipLaunchKernelGGL(k_vcm_camera<48>, gt, dim3(kTravThreads), 0, s, ds, P, S);
+    if (spill == kSpillS)
+      hipLaunchKernelGGL(k_vcm_camera<kSpillS>, gt, dim3(kTravThreads), 0, s, ds, P, S);
     else
-      hipLaunchKernelGGL(k_vcm_camera<1024>, gt, dim3(kTravThreads), 0, s, ds, P, S);
+      hipLaunchKernelGGL(k_vcm_camera<kSpillL>, gt, dim3(kTravThreads), 0, s, ds, P, S);
     if (iter == a->iteration - 1)
       hipLaunchKernelGGL(k_vcm_count_vertices, g256, dim3(256), 0, s, P, S);
     if (timing) {

@@ -55,7 +55,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 112         # traversal node fetch: two child boxes, four refs, the left child's two child boxes
+NODE_BYTES = 64          # traversal node fetch: grid origin and steps, four quantised slot boxes, four refs
 LEAF_TRI_BYTES = 48      # leaf triangle: 3 x float4
 RAY_BYTES = 32           # o, d, tmin, tmax
 HIT_BYTES = 16           # t, leaf, u, v
