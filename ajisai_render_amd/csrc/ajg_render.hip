@@ -176,6 +176,7 @@ struct Queues {
   uint32_t a_len;
   uint32_t *live;
   uint32_t *work_counter;
+  uint32_t first_static;  // a slot's first work item is its own index (next_sample)
   float *partial;      // total_work x 6 (sum3, sq3) or the film when n_chunks == 1
   uint32_t *partial_n;
   float *per_sample;   // optional
@@ -271,6 +272,56 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
   const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   return want ? base + (uint32_t)__popcll(mask & lt) : kNone;
 }
+
+// Per-render clears in one launch: each entry sets n 32-bit words (or, for
+// a byte array, n bytes) to v. Separate hipMemsetAsync calls cost a launch
+// each (about 10 us apart), and a fill kernel may be captured in a graph.
+struct FillEntry {
+  void *p;
+  uint64_t n;
+  uint32_t v;
+  uint32_t bytes;  // 1: p is a byte array
+};
+static constexpr int kFillEntries = 24;
+struct FillTable {
+  FillEntry e[kFillEntries];
+  int count;
+};
+__global__ void __launch_bounds__(256) k_fill(FillTable t) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = 0; k < t.count; ++k) {
+    const FillEntry f = t.e[k];
+    if (f.bytes) {
+      uint8_t *p = (uint8_t *)f.p;
+      for (uint64_t i = i0; i < f.n; i += stride) p[i] = (uint8_t)f.v;
+    } else {
+      uint32_t *p = (uint32_t *)f.p;
+      for (uint64_t i = i0; i < f.n; i += stride) p[i] = f.v;
+    }
+  }
+}
+struct Fills {
+  FillTable t;
+  uint64_t max_n = 0;
+  Fills() { t.count = 0; }
+  void words(void *p, uint64_t n, uint32_t v = 0u) { add(p, n, v, 0u); }
+  void bytes(void *p, uint64_t n, uint32_t v = 0u) { add(p, n, v, 1u); }
+  void add(void *p, uint64_t n, uint32_t v, uint32_t b) {
+    if (!p || n == 0) return;
+    t.e[t.count++] = FillEntry{p, n, v, b};
+    max_n = std::max(max_n, n);
+  }
+  bool full() const { return t.count + 4 > kFillEntries; }
+  hipError_t launch(hipStream_t s) {
+    if (t.count == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(8192, (max_n + 255) / 256);
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, s, t);
+    t.count = 0;
+    max_n = 0;
+    return hipGetLastError();
+  }
+};
 
 // Striped ray queues. Atomics on one counter word are served one at a time,
 // and every wave of a shade launch appends at the same few points of the
@@ -868,13 +919,19 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     if (sample >= ps.sample_end[slot]) work = kNone;
   }
   const bool fetch = need && work == kNone;
-  // Once the counter has passed the last item no wave takes another, and
-  // every slot that finishes near the end of a render would otherwise add
-  // one more atomic on this word. (A plain read: the counter only grows, so
-  // a stale value only costs the atomic.)
+  // A slot's first work item is its own index: every slot fetches in the
+  // render's first shade (its sample_end is still 0), and one atomic per
+  // wave on the single work counter would hold S/64 waves in line; the
+  // counter starts at min(S, total) instead. Later fetches take the next
+  // items from the counter. Once it has passed the last item no wave takes
+  // another, and every slot that finishes near the end of a render would
+  // otherwise add one more atomic on this word. (A plain read: the counter
+  // only grows, so a stale value only costs the atomic.)
+  const bool first = q.first_static && fetch && ps.sample_end[slot] == 0u;
   const uint32_t limit = BATCH ? B.total_work : P0.total_work;
-  const bool open = __ballot(fetch) != 0ull && *q.work_counter < limit;
-  const uint32_t w = wave_append(fetch && open, q.work_counter);
+  const bool open = __ballot(fetch && !first) != 0ull && *q.work_counter < limit;
+  uint32_t w = wave_append(fetch && !first && open, q.work_counter);
+  if (first) w = slot;
   if (fetch) {
     if (w < (BATCH ? B.total_work : P0.total_work)) {
       uint32_t local = w;
@@ -1215,12 +1272,18 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
         // when guided): the MIS-BSDF sample (MISSampleBsdf, helper.h:209-246),
         // then the continuation (pt.cpp:215-223). One loop over the two keeps
         // a single inlined copy of the sampler in the kernel.
+        // Where ns differs from ng the MIS-BSDF estimate carries the factor
+        // |ns . wi| / |ng . wi| (aggregate_bsdf.cpp:82-89), unbounded for
+        // directions grazing ng; the plain sampler draws those rarely, a
+        // guide may not, so there the MIS-BSDF direction comes from the BSDF
+        // alone (alpha 0, with the plain weights) and only the continuation,
+        // whose weight stays bounded, is guided.
         GSample g_mis, g_cont;
 #pragma unroll 1
         for (int k = 0; k < 2; ++k) {
           GSample g;
           if (GUIDED) {
-            g = guided_sample(bs, it.wr, gm, alpha, rng);
+            g = guided_sample(bs, it.wr, gm, (k == 0 && !mix_w) ? 0.f : alpha, rng);
           } else {
             f3 sam;
             sam.x = pcg_next_float(rng);
@@ -2420,24 +2483,40 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     hipError_t e_ = (call);                                    \
     if (e_ != hipSuccess) return done(hip_fail(ctx, e_, #call)); \
   } while (0)
-  AJG_RCHK(hipMemsetAsync(ps.work, 0xFF, 4 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.sample, 0, 4 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.sample_end, 0, 4 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.ext_idx, 0xFF, 4 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.bank, 0, 4 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.rng, 0, 8 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.pend_flags, 0, 4 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.acc, 0, 4 * 24 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(ps.acc_n, 0, 8 * (size_t)S, s));
-  AJG_RCHK(hipMemsetAsync(partial, 0, sizeof(float) * 6 * total, s));
-  AJG_RCHK(hipMemsetAsync(partial_n, 0, 4 * total, s));
-  AJG_RCHK(hipMemsetAsync(counters, 0, 4 * kCounterWords, s));
-  AJG_RCHK(hipMemsetAsync(rb->qctr, 0, 4 * kQctrWords, s));
-  AJG_RCHK(hipMemsetAsync(a_occ, 0, aq_cap, s));
-  AJG_RCHK(hipMemsetAsync(c_hits, 0, sizeof(float4) * cq_cap, s));
+  // Slot state, per-work-item partial sums, counters, queues and the film
+  // start cleared; the work counter starts at min(S, total): a slot's first
+  // work item is its own index (next_sample), with no atomic.
+  Fills fl;
+  fl.words(ps.work, S, kNone);
+  fl.words(ps.sample, S);
+  fl.words(ps.sample_end, S);
+  fl.words(ps.ext_idx, S, kNone);
+  fl.words(ps.bank, S);
+  fl.words(ps.rng, 2ull * S);
+  fl.words(ps.pend_flags, S);
+  fl.words(ps.acc, 24ull * S);
+  fl.words(ps.acc_n, 2ull * S);
+  fl.words(partial, 6ull * total);
+  fl.words(partial_n, total);
+  fl.words(counters, kCounterWords);
+  fl.words(rb->qctr, kQctrWords);
+  fl.bytes(a_occ, aq_cap);
+  fl.words(c_hits, 4ull * cq_cap);
+  const size_t n_pix_sz = P.n_pix;
+  fl.words(film->sum, 3ull * n_pix_sz);
+  fl.words(film->sum_sq, 3ull * n_pix_sz);
+  fl.words(film->count, n_pix_sz);
+  if (film->per_sample) fl.words(film->per_sample, 3ull * n_pix_sz * P.spp);
+  if (guided) fl.words(gb->hist_n, gb->train_slots);
+  AJG_RCHK(fl.launch(s));
+  // guided renders keep the fetch order through the counter for every item
+  // (their training slots are every train_stride-th slot, and a slot's
+  // first item equal to its index gave the guide a markedly worse fit)
+  const bool first_static = !guided;
+  if (first_static) fl.words(counters + 5, 1, (uint32_t)std::min<uint64_t>(S, total));
+  AJG_RCHK(fl.launch(s));
   unsigned long long rec_before = 0;
   if (guided) {
-    AJG_RCHK(hipMemsetAsync(gb->hist_n, 0, 4 * (size_t)gb->train_slots, s));
     uint32_t rc[4] = {0, 0, 0, 0};
     AJG_RCHK(hipMemcpyAsync(rc, G.rec_count, 16, hipMemcpyDeviceToHost, s));
     AJG_RCHK(hipStreamSynchronize(s));
@@ -2445,17 +2524,13 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     memcpy(&rt, rc + 2, 8);
     rec_before = rt + rc[0];
   }
-  const size_t n_pix_sz = P.n_pix;
-  AJG_RCHK(hipMemsetAsync(film->sum, 0, 12 * n_pix_sz, s));
-  AJG_RCHK(hipMemsetAsync(film->sum_sq, 0, 12 * n_pix_sz, s));
-  AJG_RCHK(hipMemsetAsync(film->count, 0, 4 * n_pix_sz, s));
-  if (film->per_sample) AJG_RCHK(hipMemsetAsync(film->per_sample, 0, 12 * n_pix_sz * P.spp, s));
 
   q.c_hits_in = c_hits;
   q.c_inst_in = rb->c_inst;
   q.a_occ_in = a_occ;
   q.live = counters + 4;
   q.work_counter = counters + 5;
+  q.first_static = first_static ? 1u : 0u;
   q.partial = partial;
   q.partial_n = partial_n;
   q.per_sample = film->per_sample;
@@ -2816,31 +2891,40 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   AJG_BCHK(hipMemcpyAsync(bb->jobs, hj.data(), sizeof(BatchJob) * n_jobs, hipMemcpyHostToDevice, s));
   AJG_BCHK(hipMemcpyAsync(bb->job_begin, begin.data(), 4 * ((size_t)n_jobs + 1), hipMemcpyHostToDevice, s));
   AJG_BCHK(hipMemcpyAsync(bb->remaining, work_of.data(), 4 * (size_t)n_jobs, hipMemcpyHostToDevice, s));
-  AJG_BCHK(hipMemsetAsync(bb->clocks, 0, 8 * ((size_t)n_jobs + 1), s));
-  AJG_BCHK(hipMemsetAsync(ps.work, 0xFF, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.sample, 0, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.sample_end, 0, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.ext_idx, 0xFF, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.bank, 0, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.rng, 0, 8 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.pend_flags, 0, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.acc, 0, 4 * 24 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.acc_n, 0, 8 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.job, 0, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(ps.pend_job, 0, 4 * (size_t)S, s));
-  AJG_BCHK(hipMemsetAsync(rb->partial, 0, sizeof(float) * 6 * total, s));
-  AJG_BCHK(hipMemsetAsync(rb->partial_n, 0, 4 * total, s));
-  AJG_BCHK(hipMemsetAsync(rb->counters, 0, 4 * kCounterWords, s));
-  AJG_BCHK(hipMemsetAsync(rb->a_occ, 0, aq_capacity(S), s));
-  AJG_BCHK(hipMemsetAsync(rb->c_hits, 0, sizeof(float4) * cq_capacity(S), s));
-  AJG_BCHK(hipMemsetAsync(rb->qctr, 0, 4 * kQctrWords, s));
-  for (uint32_t j = 0; j < n_jobs; ++j) {
-    const ajg_film &f = jobs[j].film;
-    const size_t np = hj[j].P.n_pix;
-    AJG_BCHK(hipMemsetAsync(f.sum, 0, 12 * np, s));
-    AJG_BCHK(hipMemsetAsync(f.sum_sq, 0, 12 * np, s));
-    AJG_BCHK(hipMemsetAsync(f.count, 0, 4 * np, s));
-    if (f.per_sample) AJG_BCHK(hipMemsetAsync(f.per_sample, 0, 12 * np * hj[j].P.spp, s));
+  // clears as in a single render (one launch per table), the work counter
+  // starting at min(S, total)
+  {
+    Fills fl;
+    fl.words(bb->clocks, 2ull * ((size_t)n_jobs + 1));
+    fl.words(ps.work, S, kNone);
+    fl.words(ps.sample, S);
+    fl.words(ps.sample_end, S);
+    fl.words(ps.ext_idx, S, kNone);
+    fl.words(ps.bank, S);
+    fl.words(ps.rng, 2ull * S);
+    fl.words(ps.pend_flags, S);
+    fl.words(ps.acc, 24ull * S);
+    fl.words(ps.acc_n, 2ull * S);
+    fl.words(ps.job, S);
+    fl.words(ps.pend_job, S);
+    fl.words(rb->partial, 6ull * total);
+    fl.words(rb->partial_n, total);
+    fl.words(rb->counters, kCounterWords);
+    fl.bytes(rb->a_occ, aq_capacity(S));
+    fl.words(rb->c_hits, 4ull * cq_capacity(S));
+    fl.words(rb->qctr, kQctrWords);
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+      if (fl.full()) AJG_BCHK(fl.launch(s));
+      const ajg_film &f = jobs[j].film;
+      const size_t np = hj[j].P.n_pix;
+      fl.words(f.sum, 3ull * np);
+      fl.words(f.sum_sq, 3ull * np);
+      fl.words(f.count, np);
+      if (f.per_sample) fl.words(f.per_sample, 3ull * np * hj[j].P.spp);
+    }
+    AJG_BCHK(fl.launch(s));
+    fl.words(rb->counters + 5, 1, (uint32_t)std::min<uint64_t>(S, total));
+    AJG_BCHK(fl.launch(s));
   }
 
   uint32_t *counters = rb->counters;  // word layout as in a single render
@@ -2867,6 +2951,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   q.a_occ_in = rb->a_occ;
   q.live = counters + 4;
   q.work_counter = counters + 5;
+  q.first_static = 1u;
   const uint32_t shade_grid = cdiv(S, 256);
   // queue stripes sized for the most any job can append
   const uint32_t c_len = stripe_len(S, kCqPerSlot), a_len = stripe_len(S, kMaxLights + 1);

@@ -62,6 +62,39 @@ HIT_BYTES = 16           # t, leaf, u, v
 OCC_BYTES = 1
 
 
+def shade_vertex_bytes(n_lights):
+    """Algorithmic bytes of the shade stage per shaded vertex (a closest-hit
+    result turned into the next rays), one environment or directional light
+    per entry of n_lights, no area lights: the records k_shade reads and
+    writes for a live slot.
+    Reads: the previous bounce's pending record (flags 4, per light shadow
+    index 4 + occlusion byte 1 + contribution 12, MIS index 4, MIS ray 32,
+    f 12, pdf 4, shading normal 12, throughput 12, radiance 12, work / sample
+    / bank 12), the extension ray 32 and hit 16, slot state (index 4,
+    bounce 4, radiance 12, throughput 12, work / sample / end 12, rng 8),
+    the triangle's shade record 16 and vertices 48, three vertex normals 48,
+    the material 48.
+    Writes: the new pending record (radiance 12, throughput 12, work /
+    sample / bank 12, flags 4, per light contribution 12 + shadow index 4 +
+    shadow ray 32, MIS ray 32 + f / pdf / normal 28 + index 4), the
+    continuation ray 32, slot index 4, bounce 4, throughput 12, rng 8."""
+    reads = 4 + 17 * n_lights + 4 + 32 + 12 + 4 + 12 + 12 + 12 + 12 + \
+        32 + 16 + 4 + 4 + 12 + 12 + 12 + 8 + 16 + 48 + 48 + 48
+    writes = 12 + 12 + 12 + 4 + 48 * n_lights + 32 + 28 + 4 + 32 + 4 + 4 + 12 + 8
+    return reads + writes
+
+
+def stage_roofline(name, alg_bytes, ms, launches, extra=None):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    out = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "ms_per_step": round(ms, 3), "launches_per_step": launches,
+           "avg_launch_ms": round(ms / max(1, launches), 4), "bytes_per_step": int(alg_bytes)}
+    if extra:
+        out.update(extra)
+    return out
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,12 +109,15 @@ def parse():
     ap.add_argument("--no-seed0", action="store_true",
                     help="skip the per-pixel-stream (seed_mode 0) comparison render")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cornell", dest="cornell", action="store_false",
+                    help="skip the Cornell-box (256x256, 16 spp) section")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--guided-budget", type=float, default=10.0,
                     help="equal-time budget (s) of the guided vs plain comparison; 0 skips it")
     ap.add_argument("--guided-res", type=int, default=512)
-    ap.add_argument("--guided-ref", choices=["guided", "plain"], default="guided")
+    ap.add_argument("--guided-ref", choices=["committed", "guided", "plain"], default="committed",
+                    help="committed: the 65,536-spp plain reference in tests/golden (512 x 512 only)")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--batch-jobs", type=int, default=64,
                     help="jobs of the batched uneven workload (0: skip)")
@@ -106,7 +142,7 @@ def parse():
                     help="VCM iterations on the area-lit Cornell box at 1024x1024 (0: skip)")
     ap.add_argument("--no-vcm-cpu", action="store_true")
     ap.add_argument("--only", choices=["batched", "instanced", "instanced_guided", "mlp", "guided",
-                                       "area", "sppm", "vcm"],
+                                       "area", "sppm", "vcm", "cornell"],
                     help="run one section alone and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (calibrates the "
@@ -192,13 +228,20 @@ def nonfinite(img):
     return int((~np.isfinite(img).all(axis=1)).sum())
 
 
+ROOM_REFERENCE = os.path.join(ROOT, "tests", "golden", "occluded_room_512_plain_64k.npz")
+
+
 def guided_equal_time(ctx, a):
-    """Equal-time relMSE of guided vs plain path tracing on the occluded room."""
+    """Equal-time relMSE of guided vs plain path tracing on the occluded room,
+    against the committed 65,536-spp plain-PT reference by default."""
     from ajisai_render_amd import scenes
     sc = scenes.occluded_room(res=(a.guided_res, a.guided_res), spp=16)
     label = (f"occluded_room: 4x3x4 room, 0.2x1.0 slit, HDR env 1024x512 through the slit, "
              f"{sc.n_tri} triangles, {a.guided_res}x{a.guided_res}, max 16 bounces")
-    return guided_compare(ctx, sc, a.guided_budget, a.guided_ref, a.guided_ref_spp, label)
+    kind = a.guided_ref
+    if kind == "committed" and (a.guided_res != 512 or not os.path.exists(ROOM_REFERENCE)):
+        kind = "plain"
+    return guided_compare(ctx, sc, a.guided_budget, kind, a.guided_ref_spp, label)
 
 
 def guided_instanced(ctx, a):
@@ -247,7 +290,13 @@ def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
         return o, time.perf_counter() - t0
 
     # reference
-    if ref_kind == "guided":
+    if ref_kind == "committed":
+        import numpy as np
+        z = np.load(ROOM_REFERENCE)
+        R = z["mean"].astype(np.float64).reshape(-1, 3)
+        VR = z["var_mean"].astype(np.float64).reshape(-1, 3)
+        ref, t_ref, ref_spp = None, float(z["seconds"]), int(z["spp"])
+    elif ref_kind == "guided":
         gr = PathGuide(ctx, box, seed=99)
         ref, t_ref = timed(lambda: PathTracer(g, spp=ref_spp, seed_mode=1, seed_base=901)
                            .Render(guide=gr, guide_args=ga))
@@ -263,10 +312,12 @@ def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
                                          "count": acc["count"] + o["count"]}
             print(f"reference chunk {k}: {dt:.1f} s", file=sys.stderr, flush=True)
         ref = acc
-    R = _img(ref)
-    n = ref["count"].astype(np.float64)
-    with np.errstate(invalid="ignore", over="ignore"):
-        var_mean = np.maximum(ref["sum_sq"].astype(np.float64) / n[:, None] - R * R, 0) / n[:, None]
+    if ref is not None:
+        R = _img(ref)
+        n = ref["count"].astype(np.float64)
+        with np.errstate(invalid="ignore", over="ignore"):
+            VR = np.maximum(ref["sum_sq"].astype(np.float64) / n[:, None] - R * R, 0) / n[:, None]
+    var_mean = VR
     ok = np.isfinite(var_mean).all(axis=1) & np.isfinite(R).all(axis=1)
     ref_self = float(np.mean(var_mean[ok] / (R[ok] ** 2 + RELMSE_EPS)))
     # plain: calibrate, then render for the budget. Plain paths end on a
@@ -277,6 +328,7 @@ def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
     sp = max(1, int(budget / (t32 / 32)))
     op, tp = timed(lambda: PathTracer(g, spp=sp, seed_mode=1, seed_base=3, flags=fin).Render())
     rp = relmse(_img(op), R)
+    rp_c = rp - ref_self
     # guided: calibrate warm-up time and per-spp time with a throw-away guide
     gd = PathGuide(ctx, box, seed=5)
     oc, tc = timed(lambda: PathTracer(g, spp=32, seed_mode=1, seed_base=4).Render(guide=gd, guide_args=ga))
@@ -289,27 +341,36 @@ def guided_compare(ctx, sc, budget, ref_kind, ref_spp, label):
     gst = og["guide_stats"]
     gd.close()
     rg = relmse(_img(og), R)
+    rg_c = rg - ref_self
     g.close()
     return {
         "workload": label,
         "budget_s": budget,
         "reference": {"kind": ref_kind, "spp": ref_spp, "seconds": round(t_ref, 2),
+                      "file": os.path.relpath(ROOM_REFERENCE, ROOT) if ref_kind == "committed" else None,
                       "mean": float(np.mean(R[ok])), "relmse_self_estimate": round(ref_self, 5),
                       "nonfinite_pixels": nonfinite(R)},
         "plain": {"spp": sp, "seconds": round(tp, 3), "relmse": round(rp, 5),
+                  "relmse_less_reference_variance": round(rp_c, 5),
                   "flags": "AJG_RENDER_FINITE_PDF",
                   "nonfinite_pixels": nonfinite(_img(op))},
         "guided": {"spp": sg, "seconds": round(tg, 3), "relmse": round(rg, 5),
+                   "relmse_less_reference_variance": round(rg_c, 5),
                    "nonfinite_pixels": nonfinite(_img(og)),
                    "warmup_seconds": round(gst.ms_warmup * 1e-3, 3),
                    "warmup": "8 passes x 16 spp, 64 Adam steps per 2^18-record batch, lr 1e-3",
                    "train_steps": gst.train_steps, "records": gst.records,
                    "last_loss": round(gst.last_loss, 4), "alpha": 0.5},
         "equal_time_relmse_ratio": round((rp * tp) / (rg * tg), 3),
+        "equal_time_relmse_ratio_less_reference_variance":
+            round((rp_c * tp) / (rg_c * tg), 3) if rg_c > 0 else None,
+        "relmse": "mean over pixels and channels of (I - R)^2 / (R^2 + 1e-2); the "
+                  "'less_reference_variance' figures subtract the reference's own share "
+                  "mean(Var(R) / (R^2 + 1e-2)), since E(I - R)^2 = E(I - mu)^2 + Var(R)",
     }
 
 
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_pq_closest_heavy_mesh_1024_64spp.txt")
+PROFILE_TRAFFIC = "profiles/traversal_shade_heavy_mesh_1024_64spp.txt"
 
 
 def batched_bench(ctx, a):
@@ -596,6 +657,81 @@ def vcm_bench(ctx, a):
     return out
 
 
+def cornell_bench(ctx, a):
+    """The Cornell box of the reference's CPU-runnable configuration (5 quads
+    + two rotated boxes = 34 triangles, Lambert walls, directional light of
+    radiance 5 plus a constant 0.1 environment, pinhole camera), 256 x 256,
+    16 spp, max 8 bounces, Russian roulette after bounce 3: the GPU render
+    with the reference's per-pixel sequential streams (seed_mode 0) and with
+    per-sample streams (seed_mode 1), each the mean of 20 timed renders, and
+    the oracle's full render of the same image on the host (median of 3)."""
+    import numpy as np
+    import torch
+    from ajisai_render_amd import _capi as capi
+    from ajisai_render_amd import scenes
+    from ajisai_render_amd.core import GpuScene, PathTracer
+
+    sc = scenes.cornell()
+    w, h = sc.camera.res
+    paths = w * h * sc.pt.spp
+    g = GpuScene(ctx, sc)
+    out = {"workload": "cornell: 34 triangles, Lambert, directional + constant env, "
+                       f"{w}x{h}, {sc.pt.spp} spp, max {sc.pt.max_bounces} bounces",
+           "paths": paths}
+    for mode in (0, 1):
+        pt = PathTracer(g, seed_mode=mode)
+        film, _ = pt.render_device()
+        for _ in range(3):
+            pt.render_device(film=film)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        reps = 20
+        q = 0
+        for _ in range(reps):
+            _, st = pt.render_device(film=film)
+            q += st.closest_queries + st.any_queries
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        out[f"seed_mode{mode}"] = {"ms": round(dt * 1e3, 3), "paths_per_s": round(paths / dt, 1),
+                                   "mrays_per_s": round(q / reps / dt / 1e6, 2),
+                                   "slots": int(st.slots)}
+    g.close()
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        threads = cpu_threads()
+        o = orc.OracleScene(sc)
+        cam, args = o.camera(), o.args()
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            res = o.render(cam=cam, args=args, threads=threads)
+            times.append(time.perf_counter() - t)
+        o.close()
+        dt = float(np.median(times))
+        out["cpu_baseline"] = {"value": round(paths / dt, 1), "unit": "paths/s", "cores": threads,
+                               "kind": "port", "host": host_cpu(),
+                               "mrays_per_s": round(res["queries"] / dt / 1e6, 3),
+                               "sample": f"the full image ({paths} paths), median of 3 renders "
+                                         f"({', '.join(f'{x:.2f}' for x in times)} s)"}
+        out["gpu_over_cpu_paths_seed_mode0"] = round(out["seed_mode0"]["paths_per_s"] /
+                                                     out["cpu_baseline"]["value"], 1)
+        out["gpu_over_cpu_paths_seed_mode1"] = round(out["seed_mode1"]["paths_per_s"] /
+                                                     out["cpu_baseline"]["value"], 1)
+    return out
+
+
+def ploc_cpu_baseline(tri_host, box):
+    """The oracle's PLOC build (one thread) of the same triangle soup."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    t = time.perf_counter()
+    orc.ploc_build(tri_host, box)
+    dt = time.perf_counter() - t
+    return {"value": round(tri_host.shape[0] / dt / 1e6, 3), "unit": "Mtris/s", "cores": 1,
+            "kind": "port", "sample": f"one build of the {tri_host.shape[0]} triangles ({dt:.2f} s)"}
+
+
 def film_bench(pt, film):
     """Film resolve (box, gaussian) and 8-bit sRGB encode of the heavy-mesh
     film: device time per call and algorithmic bytes (sums 28 B per traced
@@ -629,18 +765,47 @@ def film_bench(pt, film):
     return out
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the closest-hit traversal kernel as measured by
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a
-    plain copy); None when that summary is absent."""
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured by the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the heavy-mesh render
+    (calibrated against a plain copy); None when the summary is absent."""
+    key = f"traffic_bytes_per_launch {kernel}:"
     try:
-        with open(PROFILE_TRAFFIC) as fh:
+        with open(os.path.join(ROOT, PROFILE_TRAFFIC)) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch k_pq_closest:"):
+                if line.startswith(key):
                     return int(line.split(":")[1])
     except OSError:
         return None
     return None
+
+
+def host_cpu():
+    """The host's CPU model, logical CPU count and the CPUs this process may
+    run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "allowed": allowed}
+
+
+# The CPU baselines use every CPU this process may run on, up to the host
+# share of one GPU on a shared box (16 threads).
+CPU_SHARE = 16
+
+
+def cpu_threads():
+    return max(1, min(CPU_SHARE, host_cpu()["allowed"]))
 
 
 def cpu_baseline(scene, target_s):
@@ -649,7 +814,7 @@ def cpu_baseline(scene, target_s):
     row order, so any prefix samples every part of the image alike)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     o = orc.OracleScene(scene)
     cam = o.camera()
     args = o.args()
@@ -672,6 +837,8 @@ def cpu_baseline(scene, target_s):
     n_paths = rows * wp * args.spp
     o.close()
     return {"value": n_paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
+            "host": host_cpu(),
+            "mrays_per_s": round(queries / dt / 1e6, 3),
             "sample": f"{rows} image rows (bit-reversed order) x {wp} px x {args.spp} spp "
                       f"({n_paths} paths, {dt:.1f} s, {queries / dt / 1e6:.2f} Mrays/s)"}
 
@@ -705,13 +872,14 @@ def main():
         fn = {"batched": batched_bench, "instanced": instanced_bench,
               "instanced_guided": guided_instanced, "mlp": lambda c, _: mlp_bench(c),
               "guided": guided_equal_time, "area": area_bench, "sppm": sppm_bench,
-              "vcm": vcm_bench}[a.only]
+              "vcm": vcm_bench, "cornell": cornell_bench}[a.only]
         print(json.dumps({a.only: fn(ctx, a)}))
         return
     scene = scenes.heavy_mesh(res=(a.res, a.res), spp=a.spp)
 
     # --- PLOC build throughput from a device-resident soup
-    tri = torch.from_numpy(scene.soup()).cuda()
+    tri_host = scene.soup()
+    tri = torch.from_numpy(tri_host).cuda()
     box = scene.scene_box()
     build_ms = []
     for i in range(3):
@@ -757,24 +925,45 @@ def main():
         pps0 = round(w * h * a.spp / (time.perf_counter() - t), 1)
         slots0 = st0.slots
 
-    roof = None
+    roof, stages = None, None
     if not a.no_count:
         cpt = PathTracer(gs, seed_mode=a.seed_mode, max_slots=a.slots, flags=capi.RENDER_COUNT)
         _, cs = cpt.render_device(film=film)
         assert cs.closest_queries == closest_q, "counting pass diverged from the timed pass"
-        alg_bytes = (closest_q * (RAY_BYTES + HIT_BYTES) + cs.nodes_fetched * NODE_BYTES +
+        any_q = stats[0].any_queries
+        ms_any = sum(s.ms_trace_any for s in stats) / a.steps
+        ms_shade = sum(s.ms_shade for s in stats) / a.steps
+        b_closest = (closest_q * (RAY_BYTES + HIT_BYTES) + cs.nodes_fetched * NODE_BYTES +
                      cs.tris_tested * LEAF_TRI_BYTES)
-        achieved = alg_bytes / (ms_closest * 1e-3) / 1e9
-        traffic = profiled_traffic()
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/k_pq_closest_heavy_mesh_1024_64spp.txt" if traffic else None,
-                "kernel": "k_pq_closest (closest-hit traversal)",
-                "launches_per_step": launches_closest,
-                "avg_launch_ms": round(ms_closest / max(1, launches_closest), 4),
-                "bytes_per_step": int(alg_bytes),
-                "nodes_per_query": round(cs.nodes_fetched / max(1, closest_q), 2),
-                "tris_per_query": round(cs.tris_tested / max(1, closest_q), 2)}
+        b_any = (any_q * (RAY_BYTES + OCC_BYTES) + cs.nodes_fetched_any * NODE_BYTES +
+                 cs.tris_tested_any * LEAF_TRI_BYTES)
+        b_shade = closest_q * shade_vertex_bytes(1)
+        stages = {
+            "trace_closest": stage_roofline(
+                "k_pq_closest", b_closest, ms_closest, launches_closest,
+                {"nodes_per_query": round(cs.nodes_fetched / max(1, closest_q), 3),
+                 "tris_per_query": round(cs.tris_tested / max(1, closest_q), 3),
+                 "traffic": profiled_traffic("k_pq_closest")}),
+            "trace_any": stage_roofline(
+                "k_pq_any", b_any, ms_any, stats[0].launches_any,
+                {"nodes_per_query": round(cs.nodes_fetched_any / max(1, any_q), 3),
+                 "tris_per_query": round(cs.tris_tested_any / max(1, any_q), 3),
+                 "traffic": profiled_traffic("k_pq_any")}),
+            "shade": stage_roofline("k_shade", b_shade, ms_shade, stats[0].launches_shade,
+                                    {"bytes_per_vertex": shade_vertex_bytes(1),
+                                     "traffic": profiled_traffic("k_shade")}),
+        }
+        # the dominant kernel: the stage with the most device time per step
+        dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
+        d = stages[dom]
+        roof = {"bound": "hbm", "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["frac"], "traffic": d["traffic"],
+                "traffic_source": PROFILE_TRAFFIC if d["traffic"] else None,
+                "kernel": d["kernel"], "launches_per_step": d["launches_per_step"],
+                "avg_launch_ms": d["avg_launch_ms"], "bytes_per_step": d["bytes_per_step"],
+                "byte_model": ("per query: 32 B ray + 16 B hit (any-hit: 1 B) + 64 B per node "
+                               "fetch + 48 B per leaf triangle; counts from an untimed "
+                               "AJG_RENDER_COUNT pass of the same render")}
 
     progress("heavy-mesh path tracing timed")
     film_perf = film_bench(pt, film)
@@ -796,10 +985,13 @@ def main():
         instanced["guided"] = guided_instanced(ctx, a)
         progress("instanced guided comparison done")
 
-    cpu = None
+    cornell = cornell_bench(ctx, a) if a.cornell else None
+    progress("cornell box done")
+    cpu, ploc_cpu = None, None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, a.cpu_seconds)
-        progress("cpu baseline done")
+        ploc_cpu = ploc_cpu_baseline(tri_host, box)
+        progress("cpu baselines done")
 
     value = paths / elapsed
     out = {
@@ -827,9 +1019,12 @@ def main():
         "build_mtris_per_s": round(n_tri / (min(build_ms) * 1e-3) / 1e6, 2),
         "build_ms": round(min(build_ms), 3),
         "build_iterations": build_iters,
+        "build_cpu_baseline": ploc_cpu,
+        "cornell": cornell,
         "stage_ms_per_step": {"trace_closest": round(ms_closest, 3),
                               "trace_any": round(sum(s.ms_trace_any for s in stats) / a.steps, 3),
                               "shade": round(sum(s.ms_shade for s in stats) / a.steps, 3)},
+        "stage_roofline": stages,
         "wavefront_iterations": stats[0].iterations,
         # relMSE(guided) / relMSE(plain) at equal time (time-normalised)
         "relmse_guided_vs_plain": (round(1.0 / guided["equal_time_relmse_ratio"], 4)

@@ -82,14 +82,37 @@ def secondary_rays(prim_rays, hits, n, seed, shadow_dir=None, shadow_tmax=None):
     return make_rays(o - 1e-4 * prim_rays["d"][ok][idx], d, tmax=np.float32(shadow_tmax))
 
 
+def near_threshold_rays(tri, n, rng):
+    """Rays through triangle centroids at grazing angles chosen so that
+    Moller-Trumbore's determinant |e0 . (d x e1)| = |d . (e0 x e1)| lies
+    around the reference's 1e-6 rejection threshold (0.5x .. 2x of it, both
+    signs): the det window of ploc_aggregate.cpp:194."""
+    t = tri[rng.integers(0, tri.shape[0], n)].astype(np.float64)
+    e0, e1 = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+    nrm = np.cross(e0, e1)
+    nl = np.linalg.norm(nrm, axis=1)
+    keep = nl > 1e-12
+    t, e0, nrm, nl = t[keep], e0[keep], nrm[keep], nl[keep]
+    nh = nrm / nl[:, None]
+    tang = e0 / np.linalg.norm(e0, axis=1, keepdims=True)
+    target = rng.uniform(0.5e-6, 2e-6, t.shape[0]) * rng.choice([-1.0, 1.0], t.shape[0])
+    d = tang + (target / nl)[:, None] * nh
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    c = t.mean(axis=1)
+    size = np.linalg.norm(e0, axis=1)
+    src = c - d * size[:, None]
+    return make_rays(src.astype(np.float32), d.astype(np.float32), offset=False)
+
+
 def adversarial_rays(tri, box, n, seed):
     """B4: axis-aligned directions with +-0 components, rays aimed at shared
-    vertices and edge midpoints, origins on box planes."""
+    vertices and edge midpoints, origins on box planes, and grazing rays
+    whose MT determinant lies around the 1e-6 threshold."""
     rng = np.random.default_rng(seed)
     tri = tri.reshape(-1, 3, 3)
     mn, mx = np.asarray(box[:3], np.float32), np.asarray(box[3:], np.float32)
     out = []
-    q = n // 4
+    q = n // 5
     # 1) axis-aligned with signed zeros
     o = rng.uniform(mn, mx, (q, 3)).astype(np.float32)
     axis = rng.integers(0, 3, q)
@@ -112,8 +135,10 @@ def adversarial_rays(tri, box, n, seed):
     target = (t[np.arange(q), k] + t[np.arange(q), (k + 1) % 3]) * np.float32(0.5)
     src = target + rng.normal(size=(q, 3)).astype(np.float32)
     out.append(make_rays(src, target - src, offset=False))
-    # 4) origins on the planes of the world box, random directions
-    m = n - 3 * q
+    # 4) MT determinant near its rejection threshold
+    out.append(near_threshold_rays(tri, q, rng))
+    # 5) origins on the planes of the world box, random directions
+    m = n - sum(len(x) for x in out)
     o = rng.uniform(mn, mx, (m, 3)).astype(np.float32)
     ax = rng.integers(0, 3, m)
     side = rng.integers(0, 2, m)

@@ -138,3 +138,21 @@ def test_shadow_rays_any_hit_agree_with_closest():
     h = orc.trace_closest(nodes, perm, tri, rays)
     occ = orc.trace_any(nodes, perm, tri, rays)
     assert np.array_equal(occ.astype(bool), h["id"] != 0xFFFFFFFF)
+
+
+def test_adversarial_rays_reach_the_determinant_window():
+    """B4's grazing class: Moller-Trumbore's determinant of the aimed
+    triangle, computed in float32 as the reference does, lies within
+    0.4e-6 .. 2.2e-6 for most rays, on both sides of the 1e-6 threshold."""
+    rng = np.random.default_rng(3)
+    sc = scenes.cornell()
+    tri = sc.soup().reshape(-1, 3, 3)
+    idx = rng.integers(0, tri.shape[0], 4000)
+    r = raygen.near_threshold_rays(tri, 4000, np.random.default_rng(3))
+    t = tri[idx].astype(np.float32)
+    e0, e1 = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+    h = np.cross(r["d"], e1).astype(np.float32)
+    a = np.abs(np.sum(e0 * h, axis=1, dtype=np.float32))
+    inside = (a > 0.4e-6) & (a < 2.2e-6)
+    assert inside.mean() > 0.8, inside.mean()
+    assert (a < 1e-6).mean() > 0.2 and (a > 1e-6).mean() > 0.2
