@@ -230,6 +230,7 @@ _SIGS = {
     "ajg_guide_forward": ([_P, _P, _P, C.c_uint32, _P, _P, _P], C.c_int),
     "ajg_guide_train_step": ([_P, _P, C.c_uint32, C.c_float, _P, _P, _P], C.c_int),
     "ajg_guide_train_step_noalloc": ([_P, _P, C.c_uint32, C.c_float, _P], C.c_int),
+    "ajg_ctx_set_cancel": ([_P, C.c_int], C.c_int),
 }
 
 _lib = None

@@ -54,6 +54,11 @@ class Context:
     def inject_failure(self, countdown: int, status: int):
         self.check(self.lib.ajg_ctx_inject_failure(self.h, countdown, status))
 
+    def set_cancel(self, cancel: bool = True):
+        """Raise (or clear) the context's cancel flag: renders in progress on
+        this context stop at their next host checkpoint with AJG_ERR_CANCELLED."""
+        self.check(self.lib.ajg_ctx_set_cancel(self.h, 1 if cancel else 0))
+
     def close(self):
         if self.h:
             self.lib.ajg_ctx_destroy(self.h)
@@ -415,15 +420,17 @@ class PathTracer:
 
     def __init__(self, gscene: GpuScene, spp=None, tile_size=None, min_bounces=None,
                  max_bounces=None, cont_prob=None, seed_mode=0, seed_base=0, env_cdf_fp64=1,
-                 max_slots=0, flags=0):
+                 max_slots=0, flags=0, use_mis=None):
         pt = gscene.scene.pt
         self.g = gscene
+        mis = getattr(pt, "use_mis", True) if use_mis is None else use_mis
         self.args = capi.PTArgs(pt.spp if spp is None else spp,
                                 pt.tile_size if tile_size is None else tile_size,
                                 pt.min_bounces if min_bounces is None else min_bounces,
                                 pt.max_bounces if max_bounces is None else max_bounces,
-                                pt.cont_prob if cont_prob is None else cont_prob, 1, seed_mode,
-                                seed_base, env_cdf_fp64, max_slots, flags)
+                                pt.cont_prob if cont_prob is None else cont_prob,
+                                1 if mis else 0, seed_mode, seed_base, env_cdf_fp64, max_slots,
+                                flags)
 
     def traced_dims(self):
         w, h = self.g.scene.camera.res

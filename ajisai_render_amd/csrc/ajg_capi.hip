@@ -155,6 +155,12 @@ const char *ajg_last_error(const ajg_ctx *ctx) {
   return ctx ? ctx->last_error.c_str() : "null context";
 }
 
+int ajg_ctx_set_cancel(ajg_ctx *ctx, int cancel) {
+  if (!ctx) return AJG_ERR_INVALID;
+  ctx->cancel.store(cancel ? 1 : 0, std::memory_order_relaxed);
+  return AJG_OK;
+}
+
 int ajg_ctx_inject_failure(ajg_ctx *ctx, int countdown, int status) {
   if (!ctx) return AJG_ERR_INVALID;
   ctx->inject_countdown = countdown;

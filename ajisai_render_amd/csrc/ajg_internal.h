@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -13,6 +14,7 @@
 
 struct ajg_ctx {
   int device = 0;
+  std::atomic<int> cancel{0};              // raised by ajg_ctx_set_cancel, polled by renders
   std::string last_error;
   int inject_countdown = 0;
   int inject_status = 0;
@@ -38,6 +40,10 @@ struct ajg_accel {
 };
 
 namespace ajg {
+
+// A render polls this at each host checkpoint (every few wavefront
+// iterations, every SPPM/VCM iteration) and stops with AJG_ERR_CANCELLED.
+inline bool cancelled(const ajg_ctx *ctx) { return ctx->cancel.load(std::memory_order_relaxed) != 0; }
 
 // Error helpers: record a message on the context and return a status.
 int fail(ajg_ctx *ctx, int status, const std::string &msg);

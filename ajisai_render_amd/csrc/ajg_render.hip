@@ -110,6 +110,7 @@ struct RenderParams {
   uint32_t slots;
   uint32_t tile, tiles_x;  // work-list pixel order: T x T tiles (0: scanline)
   uint32_t flags;          // AJG_RENDER_* bits
+  uint32_t use_mis;        // 1: TraceMIS (pt.cpp:98-227), 0: Trace (pt.cpp:229-311)
 };
 
 // Pixel of the i-th entry of a work list's pixel order: scanline, or T x T
@@ -1150,7 +1151,11 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
       const uint32_t work = ps.work[slot], sample = ps.sample[slot];
       const bool last = sample + 1 >= ps.sample_end[slot];
       if (ls < 0) {
-        if (b == 0 && sc.env_index >= 0) L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
+        // TraceMIS adds the environment at the first bounce only (pt.cpp:
+        // 114-121; later escapes are its MIS-BSDF term); Trace adds it at
+        // every bounce, unweighted by the throughput (pt.cpp:245-249)
+        if ((b == 0 || !P.use_mis) && sc.env_index >= 0)
+          L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
         if (train) emit_history(G, hslot, L);
         accumulate(ps, BATCH ? acc_job(B, j) : acc_single(q, P), slot, ps.bank[slot], work, sample,
                    L, last);
@@ -1194,8 +1199,9 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
           const f3 cx = cross(it.ns, it.ng);
           mix_w = dot(cx, cx) <= 1e-10f * n2 && fabsf(n2 - 1.f) <= 2e-5f;
         }
-        // emitted radiance at the first hit (pt.cpp:188-194)
-        if (b == 0 && it.light && dot(it.ng, it.wr) > 0)
+        // emitted radiance at the first hit (pt.cpp:188-194); Trace adds
+        // it at every hit (pt.cpp:289-293)
+        if ((b == 0 || !P.use_mis) && it.light && dot(it.ng, it.wr) > 0)
           L = add(L, mul(beta, sc.lights[it.light - 1].radiance));
         uint32_t pf2 = kPendActive;
         st_f3(ps.pend_L, slot, L);
@@ -1208,8 +1214,10 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
           st_f3(G.pend_pos, slot, it.pos);
           st_f3(ps.pend_ns, slot, it.ns);
         }
-        // NEE for each light (MISSampleLight, helper.h:100-133)
-        for (uint32_t k = 0; k < sc.n_lights; ++k) {
+        // NEE for each light (MISSampleLight, helper.h:100-133); Trace
+        // samples no light (its direct-light loop is empty, pt.cpp:296-301)
+        const uint32_t n_nee = P.use_mis ? sc.n_lights : 0u;
+        for (uint32_t k = 0; k < n_nee; ++k) {
           const DevLight &l = sc.lights[k];
           LightSample lsmp;
           if (l.type == AJG_LIGHT_ENV) {
@@ -1279,8 +1287,9 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
         // alone (alpha 0, with the plain weights) and only the continuation,
         // whose weight stays bounded, is guided.
         GSample g_mis, g_cont;
+        g_mis.ok = false;
 #pragma unroll 1
-        for (int k = 0; k < 2; ++k) {
+        for (int k = P.use_mis ? 0 : 1; k < 2; ++k) {
           GSample g;
           if (GUIDED) {
             g = guided_sample(bs, it.wr, gm, (k == 0 && !mix_w) ? 0.f : alpha, rng);
@@ -2327,8 +2336,8 @@ int ajg_scene_env_tables(ajg_scene *s, uint32_t li, float *cond_cdf, float *cond
 // (0: enough chunks to fill `target` slots).
 static int setup_params(ajg_ctx *ctx, const ajg_camera *camera, const ajg_pt_args *args,
                         uint64_t target, uint32_t samples_per_item, RenderParams *out) {
-  if (args->spp <= 0 || args->tile_size <= 0 || args->max_bounces < 0 || !args->use_mis)
-    return fail(ctx, AJG_ERR_INVALID, "render: need spp > 0, tile_size > 0, max_bounces >= 0, use_mis");
+  if (args->spp <= 0 || args->tile_size <= 0 || args->max_bounces < 0)
+    return fail(ctx, AJG_ERR_INVALID, "render: need spp > 0, tile_size > 0, max_bounces >= 0");
   const float rx = camera->resolution[0], ry = camera->resolution[1];
   if (!(rx >= 1.f) || !(ry >= 1.f) || rx > 65536.f || ry > 65536.f)
     return fail(ctx, AJG_ERR_INVALID, "render: bad resolution");
@@ -2351,6 +2360,7 @@ static int setup_params(ajg_ctx *ctx, const ajg_camera *camera, const ajg_pt_arg
   P.seed_mode = args->seed_mode;
   P.seed_base = args->seed_base;
   P.flags = args->flags;
+  P.use_mis = args->use_mis ? 1u : 0u;
   if (P.seed_mode == 0) {
     P.chunk = P.spp;
     P.n_chunks = 1;
@@ -2403,6 +2413,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   if (guided) {
     if (!ga || !(ga->alpha >= 0.f && ga->alpha <= 1.f) || (ga->train && (ga->batch == 0 || !(ga->lr > 0.f))))
       return fail(ctx, AJG_ERR_INVALID, "guided render: need 0 <= alpha <= 1, batch > 0 and lr > 0");
+    if (!args->use_mis)
+      return fail(ctx, AJG_ERR_INVALID, "guided render: the guide mixes into TraceMIS (use_mis 1)");
   }
   hipStream_t s = (hipStream_t)stream;
   const ajg_accel *a = sc->accel;
@@ -2638,6 +2650,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     AJG_RCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     if (guided && batch) AJG_RCHK(hipMemcpyAsync(&held, G.rec_count, 4, hipMemcpyDeviceToHost, s));
     AJG_RCHK(hipStreamSynchronize(s));
+    if (cancelled(ctx)) return done(fail(ctx, AJG_ERR_CANCELLED, "render cancelled"));
     const uint32_t live = hc[4];
     if (guided && batch) {
       const uint32_t avail = std::min(held, G.rec_cap);
@@ -3029,6 +3042,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
     iters += (uint32_t)K;
     AJG_BCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     AJG_BCHK(hipStreamSynchronize(s));
+    if (cancelled(ctx)) return done(fail(ctx, AJG_ERR_CANCELLED, "batch cancelled"));
     if (hc[4] == 0) break;
     if (iters > 400000) return done(fail(ctx, AJG_ERR_INTERNAL, "batch: iteration limit"));
   }

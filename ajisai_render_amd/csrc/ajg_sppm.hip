@@ -742,6 +742,10 @@ extern "C" int ajg_render_sppm(ajg_scene *sc, const ajg_camera *camera, const aj
   (void)hipEventRecord(ev[1], s);
   const dim3 gph(cdiv(std::max(P.photons, 1u), kTravThreads));
   for (int it = 0; it < a->iteration_count; ++it) {
+    if (cancelled(ctx)) {  // polled as each iteration is enqueued
+      (void)hipStreamSynchronize(s);
+      return done(fail(ctx, AJG_ERR_CANCELLED, "sppm cancelled"));
+    }
     hipLaunchKernelGGL(k_sppm_direct, g256, dim3(256), 0, s, P, S);
     if (timing) (void)hipEventRecord(ev[2], s);
     if (hipMemsetAsync(S.grid_count, 0, 4 * (kSppmEntries + 1), s) != hipSuccess ||

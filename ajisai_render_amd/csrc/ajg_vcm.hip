@@ -943,6 +943,10 @@ extern "C" int ajg_render_vcm(ajg_scene *sc, const ajg_camera *camera, const ajg
   const dim3 gt(vcm_cdiv(P.n, kTravThreads)), g256(vcm_cdiv(P.n + 1, 256));
   const f3 wsize = sub(wmx, wmn);
   for (int iter = 0; iter < a->iteration && P.n_lights; ++iter) {
+    if (cancelled(ctx)) {  // polled as each iteration is enqueued
+      (void)hipStreamSynchronize(s);
+      return done(fail(ctx, AJG_ERR_CANCELLED, "vcm cancelled"));
+    }
     // RenderIter (vcm.cpp:161-200)
     P.iter = (uint32_t)iter;
     P.light_count = (float)(W * H);

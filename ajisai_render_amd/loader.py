@@ -397,12 +397,10 @@ def load(source, base_dir: Optional[str] = None) -> LoadedScene:
     use_mis = True
     if rtype == "pt":
         use_mis = bool(r_node.get("use_mis", True))
-        if not use_mis:
-            raise SceneError("the GPU path tracer implements TraceMIS only (use_mis: true)")
         pt = PTSpec(spp=int(r_node["spp"]), tile_size=int(r_node.get("tile_size", 16)),
                     min_bounces=int(r_node.get("min_bounces", 5)),
                     max_bounces=int(r_node.get("max_bounces", 10)),
-                    cont_prob=float(r_node.get("cont_prob", 0.9)))
+                    cont_prob=float(r_node.get("cont_prob", 0.9)), use_mis=use_mis)
     elif rtype == "sppm":  # SPPMCreatorImpl (renderer_creators.cpp:56-90)
         pt = PTSpec()
         sppm = scenes.SPPMSpec(

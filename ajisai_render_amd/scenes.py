@@ -91,6 +91,7 @@ class PTSpec:
     min_bounces: int = 5
     max_bounces: int = 10
     cont_prob: float = 0.9
+    use_mis: bool = True   # False: PathTracing::Trace (BSDF sampling only)
 
 
 @dataclasses.dataclass

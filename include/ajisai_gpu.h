@@ -29,7 +29,7 @@ enum ajg_status {
   AJG_ERR_HIP = 1,       /* a HIP runtime call failed */
   AJG_ERR_OOM = 2,       /* device allocation failed */
   AJG_ERR_INVALID = 3,   /* bad argument, unsupported type, empty scene */
-  AJG_ERR_CANCELLED = 4, /* the host cancel flag was raised */
+  AJG_ERR_CANCELLED = 4, /* the context's cancel flag was raised (ajg_ctx_set_cancel) */
   AJG_ERR_INTERNAL = 5   /* an internal limit or invariant was hit */
 };
 
@@ -48,6 +48,14 @@ const char *ajg_last_error(const ajg_ctx *ctx);
 /* Fault injection for host-side failure tests: the next `countdown`-th
  * entry point call on this context returns `status` (0 disables). */
 int ajg_ctx_inject_failure(ajg_ctx *ctx, int countdown, int status);
+/* Cancellation (no counterpart in the reference, whose Render runs to the
+ * end): cancel = 1 raises the context's flag, 0 clears it. Renders on the
+ * context (ajg_render_pt, _guided, _batch, _sppm, _vcm) poll it at their
+ * host checkpoints (every few wavefront iterations; each SPPM/VCM
+ * iteration as it is enqueued), wait for the work in flight and return
+ * AJG_ERR_CANCELLED, leaving the film partial. The flag stays raised until
+ * cleared. May be called from any thread. */
+int ajg_ctx_set_cancel(ajg_ctx *ctx, int cancel);
 
 /* ------------------------------------------------------------ accelerator */
 
@@ -204,7 +212,7 @@ typedef struct ajg_camera {
 typedef struct ajg_pt_args {
   int32_t spp, tile_size, min_bounces, max_bounces;
   float cont_prob;
-  int32_t use_mis;      /* must be 1: TraceMIS (pt.cpp:98) */
+  int32_t use_mis;      /* 1: TraceMIS (pt.cpp:98-227); 0: Trace (pt.cpp:229-311), not guided */
   int32_t seed_mode;    /* 0: per-pixel stream x + y*W, samples in sequence
                            (pt.cpp:56-64); 1: one stream per sample */
   uint32_t seed_base;   /* seed_mode 1 only */

@@ -385,10 +385,11 @@ class OracleScene:
                                   cs.res[1], cs.lens_radius, cs.focal_distance)
         return cam
 
-    def args(self, spp=None, seed_mode=0, seed_base=0, env_cdf_fp64=1):
+    def args(self, spp=None, seed_mode=0, seed_base=0, env_cdf_fp64=1, use_mis=True):
         pt = self.scene.pt
         return Args(pt.spp if spp is None else spp, pt.tile_size, pt.min_bounces,
-                    pt.max_bounces, pt.cont_prob, 1, seed_mode, seed_base, env_cdf_fp64, 0)
+                    pt.max_bounces, pt.cont_prob, 1 if use_mis else 0, seed_mode, seed_base,
+                    env_cdf_fp64, 0)
 
     def traced_dims(self, args=None):
         ts = self.scene.pt.tile_size

@@ -1435,6 +1435,43 @@ static v3 trace_mis(const orc_scene *s, const orc_pt_args *a, orc_ray ray,
   return L;
 }
 
+/* PathTracing::Trace (pt.cpp:229-311), selected when use_mis is false
+ * (pt.cpp:85-87): BSDF sampling only. An escaped ray adds the environment's
+ * radiance unweighted by the path throughput, at every bounce (pt.cpp:
+ * 245-249, as the reference writes it); every hit on an emitter's front adds
+ * throughput x emission (pt.cpp:289-293); the direct-light loop is empty and
+ * draws nothing (pt.cpp:296-301). */
+static v3 trace_plain(const orc_scene *s, const orc_pt_args *a, orc_ray ray,
+                      sampler *smp, uint64_t *q) {
+  v3 L = V3(0, 0, 0), beta = V3(1.f, 1.f, 1.f);
+  for (int bounce = 0; bounce < a->max_bounces; bounce++) {
+    if (bounce > a->min_bounces) {
+      if (next1(smp) > a->cont_prob) return L;
+      beta = vdivs(beta, a->cont_prob);
+    }
+    orc_hit h;
+    if (!scene_intersect(s, &ray, &h, q)) {
+      if (s->env_index >= 0) L = vadd(L, env_radiance(&s->lights[s->env_index], vload(ray.d)));
+      return L;
+    }
+    inct it = post_intersect(s, &ray, &h);
+    const int li = s->prim_light[s->tri_prim[h.id]];
+    if (li >= 0 && vdot(it.ng, it.wr) > 0) L = vadd(L, vmul(beta, s->lights[li].radiance));
+    uint32_t mi = s->prims[s->tri_prim[h.id]].material;
+    bsdf b = make_bsdf(&s->mats[mi], it.ng, it.ns);
+    v3 sam;
+    sam.x = next1(smp);
+    sam.y = next1(smp);
+    sam.z = next1(smp);
+    bsdf_sample bs = bsdf_sample_all(&b, it.wr, sam);
+    if (!bs.valid) bs.f = V3(0, 0, 0), bs.pdf = 0.f;
+    if (is_black(bs.f) || bs.pdf < 0.f) return L;
+    beta = vmul(beta, vdivs(vscale(bs.f, fabsf(vdot(it.ng, bs.dir))), bs.pdf));
+    ray = make_ray(eps_offset(&it, bs.dir), bs.dir, ORC_EPS, INFINITY);
+  }
+  return L;
+}
+
 static void render_pixel(const orc_scene *s, const orc_camera *cam,
                          const orc_pt_args *a, uint32_t x, uint32_t y,
                          uint32_t W, uint32_t Wp, double *sum, double *sum_sq,
@@ -1452,7 +1489,7 @@ static void render_pixel(const orc_scene *s, const orc_camera *cam,
     lens[1] = next1(&smp);
     orc_ray r;
     orc_camera_ray(cam, u, v, lens, &r);
-    v3 L = trace_mis(s, a, r, &smp, q);
+    v3 L = a->use_mis ? trace_mis(s, a, r, &smp, q) : trace_plain(s, a, r, &smp, q);
     if (per_sample) vstore(per_sample + 3 * ((size_t)p * a->spp + si), L);
     acc[0] += L.x;
     acc[1] += L.y;

@@ -149,3 +149,93 @@ def test_yaml_scene_file_renders_like_the_oracle(gpu_ctx, tmp_path):
     want = orc.film_resolve_filter(w, h, sc.pt.tile_size, s, n, 1, ls.filter.radius, ls.filter.alpha)
     assert np.allclose(rgb.cpu().numpy(), want, rtol=2e-5, atol=1e-6)
     gs.close()
+
+
+@pytest.mark.parametrize("seed_mode", [0, 1])
+def test_trace_without_mis_per_sample_parity(gpu_ctx, seed_mode):
+    """use_mis: false (PathTracing::Trace, pt.cpp:229-311): BSDF sampling
+    only, emission at every hit, escaped environment unweighted. The
+    area-lit Cornell box (emitters hit by BSDF-sampled rays) and the
+    directional-lit one (the delta light is never hit), per sample against
+    the oracle."""
+    print(f"case: Trace (no MIS) seed_mode {seed_mode}", flush=True)
+    for sc in (scenes.cornell_area(("quad", "twosided", "cube"), res=(48, 40), spp=4, env=0.2),
+               scenes.cornell(res=(48, 40), spp=4)):
+        gs = GpuScene(gpu_ctx, sc)
+        o = orc.OracleScene(sc)
+        cam = o.camera()
+        g = PathTracer(gs, spp=4, seed_mode=seed_mode, seed_base=9, use_mis=False).Render(
+            cam_override=cam, per_sample=True)
+        r = o.render(cam=cam, args=o.args(spp=4, seed_mode=seed_mode, seed_base=9, use_mis=False),
+                     per_sample=True)
+        compare_samples(g, r)
+        assert np.abs(r["per_sample"]).sum() > 0
+        # and it is not the MIS estimator
+        m = o.render(cam=cam, args=o.args(spp=4, seed_mode=seed_mode, seed_base=9), per_sample=True)
+        assert not np.allclose(m["per_sample"], r["per_sample"])
+        o.close()
+        gs.close()
+
+
+def test_trace_without_mis_rejected_for_guided(gpu_ctx):
+    from ajisai_render_amd import _capi as capi
+    from ajisai_render_amd.core import PathGuide, guide_defaults
+    sc = scenes.cornell(res=(16, 16), spp=1)
+    gs = GpuScene(gpu_ctx, sc)
+    gd = PathGuide(gpu_ctx, sc.scene_box(), seed=1)
+    with pytest.raises(capi.AjgError):
+        PathTracer(gs, use_mis=False).Render(guide=gd, guide_args=guide_defaults())
+    gd.close()
+    gs.close()
+
+
+def test_cancel_flag_stops_renders(gpu_ctx):
+    """A raised cancel flag stops a render at its first host checkpoint
+    with AJG_ERR_CANCELLED (plain, batched and SPPM), from another thread
+    too; once cleared, renders run again."""
+    try:
+        _cancel_checks(gpu_ctx)
+    finally:
+        gpu_ctx.set_cancel(False)
+
+
+def _cancel_checks(gpu_ctx):
+    import threading
+    import time
+    from ajisai_render_amd import _capi as capi
+    from ajisai_render_amd.core import BatchRenderer, SPPMRenderer
+    sc = scenes.cornell(res=(64, 64), spp=4)
+    gs = GpuScene(gpu_ctx, sc)
+    gpu_ctx.set_cancel(True)
+    for fn in (lambda: PathTracer(gs).Render(),
+               lambda: BatchRenderer(gpu_ctx, [PathTracer(gs)]).render_device(),
+               lambda: SPPMRenderer(gs, scenes.SPPMSpec(iteration_count=2,
+                                                        photons_per_iteration=1000)).render_device()):
+        with pytest.raises(capi.AjgError) as e:
+            fn()
+        assert e.value.status == capi.AJG_ERR_CANCELLED
+    gpu_ctx.set_cancel(False)
+    assert PathTracer(gs).Render()["stats"].paths == 64 * 64 * 4
+    # from another thread, during a long render (seed_mode 0: one path per pixel in flight)
+    big = GpuScene(gpu_ctx, scenes.cornell(res=(256, 256), spp=4096))
+    status = {}
+
+    def run():
+        try:
+            PathTracer(big, seed_mode=0).Render()
+            status["st"] = capi.AJG_OK
+        except capi.AjgError as ex:
+            status["st"] = ex.status
+
+    th = threading.Thread(target=run)
+    t0 = time.time()
+    th.start()
+    time.sleep(0.3)
+    gpu_ctx.set_cancel(True)
+    th.join(60)
+    gpu_ctx.set_cancel(False)
+    print(f"cancelled after {time.time() - t0:.2f} s", flush=True)
+    assert status.get("st") == capi.AJG_ERR_CANCELLED
+    assert PathTracer(gs).Render()["stats"].paths == 64 * 64 * 4
+    big.close()
+    gs.close()

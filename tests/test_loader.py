@@ -144,10 +144,12 @@ def test_unsupported_inputs_are_named(tmp_path):
         node[path[-2]] = path[-1]
         with pytest.raises(loader.SceneError):
             loader.load(c)
+
+    # use_mis: false selects PathTracing::Trace (renderer_creators.cpp:38, pt.cpp:85-87)
     c = yaml.safe_load(yaml.safe_dump(cfg))
     c["renderer"]["use_mis"] = False
-    with pytest.raises(loader.SceneError):
-        loader.load(c)
+    assert loader.load(c).scene.pt.use_mis is False
+    assert loader.load(cfg).scene.pt.use_mis is True
 
 
 def test_emissive_primitive_is_an_area_light(tmp_path):

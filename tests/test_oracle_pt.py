@@ -282,3 +282,47 @@ def test_render_reference_seeding_is_deterministic_and_per_pixel():
     r3 = o.render(pix_begin=40, pix_end=60, per_sample=True)
     assert np.array_equal(r3["per_sample"][80:120], r1["per_sample"][80:120])
     assert r1["queries"] > 0
+
+
+def _plane_scene(albedo, env=1.0, spp=64, min_bounces=3, max_bounces=8, emissive=False):
+    from ajisai_render_amd.scenes import (CameraSpec, LIGHT_ENV, Light, MAT_DIFFUSE, Material,
+                                          PTSpec, Scene, quad)
+    prim = quad((-50, 0, -50), (-50, 0, 50), (50, 0, 50), (50, 0, -50), 0, two_sided=False)
+    if emissive:
+        import dataclasses
+        prim = dataclasses.replace(prim, emission=(3.0, 2.0, 1.0))
+    cam = CameraSpec(pos=(0, 2, 0.01), look_at=(0, 0, 0), up=(0, 0, 1), fov=40, res=(16, 16))
+    lights = [Light(LIGHT_ENV, env=scenes.constant_env(env))] if env else []
+    return Scene("plane", [prim], [Material(MAT_DIFFUSE, (albedo,) * 3)], lights, cam,
+                 PTSpec(spp=spp, tile_size=16, min_bounces=min_bounces, max_bounces=max_bounces,
+                        cont_prob=0.9, use_mis=False))
+
+
+def test_trace_without_mis_adds_escaped_environment_unweighted():
+    """PathTracing::Trace (pt.cpp:229-311) adds the environment's radiance
+    to the pixel unweighted by the path throughput (pt.cpp:245-249). A
+    Lambert plane (albedo 0.5) under a constant unit environment: every
+    camera ray hits the plane, its BSDF-sampled continuation escapes at
+    bounce 1 (before any Russian roulette, min_bounces 3), so every sample
+    is exactly 1.0, where TraceMIS gives albedo x environment = 0.5 on
+    average. A throughput-weighted environment would give 0.5 here."""
+    sc = _plane_scene(0.5)
+    o = orc.OracleScene(sc)
+    r = o.render(args=o.args(use_mis=False), per_sample=True)
+    ps = r["per_sample"]
+    # a cosine sample with pdf < 1e-3 is rejected by the Lambert sampler
+    # (diffuse_component.cpp:38-40): probability cos^2 < 1e-5 per sample
+    assert np.mean(np.all(ps == 1.0, axis=1)) > 0.999
+    m = o.render(args=o.args(use_mis=True), per_sample=True)["per_sample"]
+    assert abs(m.mean() - 0.5) < 0.05
+
+
+def test_trace_without_mis_adds_emission_at_every_hit():
+    """An emissive black plane (emission (3, 2, 1), albedo 0) seen by the
+    camera, no environment: Trace adds throughput x emission at the hit
+    (pt.cpp:289-293) and the black BSDF ends the path (pt.cpp:303): every
+    sample is exactly the emission."""
+    sc = _plane_scene(0.0, env=0.0, emissive=True)
+    o = orc.OracleScene(sc)
+    ps = o.render(args=o.args(use_mis=False), per_sample=True)["per_sample"]
+    assert np.all(ps == np.float32([3.0, 2.0, 1.0]))
