@@ -218,6 +218,9 @@ void orc_env_radiance(const orc_scene *s, uint32_t li, const float *dir3,
                       float *rgb);
 
 /* BSDF entry points on an explicit shading point (ng, ns, wo). */
+void orc_bsdf_sample_batch(const orc_material *m, const float *ng, const float *ns,
+                           const float *wo, uint32_t n, const float *sam3, float *wi,
+                           int *valid);
 void orc_bsdf_sample(const orc_material *m, const float *ng, const float *ns,
                      const float *wo, const float *sam3, float *wi,
                      float *f, float *pdf, int *valid);

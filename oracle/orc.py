@@ -127,6 +127,7 @@ def load():
         "orc_env_pdf": ([_P, C.c_uint32, _P], C.c_float),
         "orc_env_radiance": ([_P, C.c_uint32, _P, _P], None),
         "orc_bsdf_sample": ([_P, _P, _P, _P, _P, _P, _P, _P, _P], None),
+        "orc_bsdf_sample_batch": ([_P, _P, _P, _P, C.c_uint32, _P, _P, _P], None),
         "orc_bsdf_eval": ([_P, _P, _P, _P, _P, _P], None),
         "orc_bsdf_pdf": ([_P, _P, _P, _P, _P], C.c_float),
         "orc_square_to_disk": ([_P, _P], None),
@@ -539,6 +540,20 @@ def bsdf_sample(mat, ng, ns, wo, sam):
     args = [np.ascontiguousarray(x, np.float32) for x in (ng, ns, wo, sam)]
     lib.orc_bsdf_sample(C.byref(m), *[_p(a) for a in args], _p(wi), _p(f), _p(pdf), _p(valid))
     return wi, f, float(pdf[0]), bool(valid[0])
+
+
+def bsdf_sample_batch(mat, ng, ns, wo, sam):
+    """Directions (n x 3) and validity (n) of bsdf_sample at one wo over the
+    rows of sam (n x 3)."""
+    lib = load()
+    m = _mat(mat)
+    sam = np.ascontiguousarray(sam, np.float32)
+    n = sam.shape[0]
+    wi = np.zeros((n, 3), np.float32)
+    valid = np.zeros(n, np.int32)
+    args = [np.ascontiguousarray(x, np.float32) for x in (ng, ns, wo)]
+    lib.orc_bsdf_sample_batch(C.byref(m), *[_p(a) for a in args], n, _p(sam), _p(wi), _p(valid))
+    return wi, valid.astype(bool)
 
 
 def _mat(m):

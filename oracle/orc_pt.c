@@ -1021,6 +1021,17 @@ void orc_bsdf_sample(const orc_material *m, const float *ng, const float *ns,
   *pdf = r.pdf;
   *valid = r.valid;
 }
+/* n samples of one BSDF at one wo (sam3 and wi are n x 3; valid is n). */
+void orc_bsdf_sample_batch(const orc_material *m, const float *ng, const float *ns,
+                           const float *wo, uint32_t n, const float *sam3, float *wi,
+                           int *valid) {
+  bsdf b = make_bsdf(m, vload(ng), vload(ns));
+  for (uint32_t i = 0; i < n; ++i) {
+    bsdf_sample r = bsdf_sample_all(&b, vload(wo), vload(sam3 + 3 * (size_t)i));
+    vstore(wi + 3 * (size_t)i, r.dir);
+    valid[i] = r.valid;
+  }
+}
 void orc_bsdf_eval(const orc_material *m, const float *ng, const float *ns,
                    const float *wi, const float *wo, float *f) {
   bsdf b = make_bsdf(m, vload(ng), vload(ns));
