@@ -327,6 +327,18 @@ typedef struct orc_sppm_pixel {
  * multiple of 64. */
 int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
                         int32_t iterations, float *radiance, orc_sppm_pixel *state);
+/* Test hook: n draws of Light::SampleEmit (area_light.cpp:60-73,
+ * directional_light.cpp:60-79, env_light.cpp:105-147) from light li (the
+ * emissive primitives' lights first, then the scene's), draw i from its own
+ * seeded stream; pos/dir/normal/radiance are n x 3. */
+void orc_light_emit_batch(const orc_scene *s, uint32_t li, uint32_t seed, uint32_t n,
+                          float *pos, float *dir, float *normal, float *radiance,
+                          float *pdf_pos, float *pdf_dir);
+/* Test hook: UpdatePixel (sppm.cpp:145-167) on one pixel over `iters`
+ * iterations of m[k] photons with flux phi (iters x 3) at throughput beta;
+ * out (iters x 5) holds radius, N, tau after each. */
+void orc_sppm_update_probe(float alpha, float r0, const float *beta, uint32_t iters,
+                           const uint32_t *m, const float *phi, float *out);
 /* Test hook: the visible-point grid alone (VisiblePointSearcher AddVp and
  * AddPhoton's lookup and distance test, no BSDF): per visible point, the
  * number of the photon positions matched to it. */

@@ -145,6 +145,9 @@ def load():
         "orc_image_srgb8": ([C.c_uint32, C.c_uint32, _P, C.c_int, _P], None),
         "orc_render_sppm": ([_P, _P, _P, C.c_int32, _P, _P], C.c_int64),
         "orc_sppm_grid_probe": ([_P, C.c_float, C.c_uint32, _P, _P, C.c_uint32, _P, _P], None),
+        "orc_light_emit_batch": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P,
+                                  _P], None),
+        "orc_sppm_update_probe": ([C.c_float, C.c_float, _P, C.c_uint32, _P, _P, _P], None),
         "orc_render_vcm": ([_P, _P, _P, _P, _P], C.c_int64),
         "orc_vcm_to_raster": ([_P, _P, _P, _P, _P], C.c_int),
         "orc_vcm_image_to_solid": ([_P, _P], C.c_float),
@@ -367,6 +370,17 @@ class OracleScene:
         self.lib.orc_env_sample(self.h, li, u0, u1, _p(d), _p(pdf), _p(rgb))
         return d, float(pdf[0]), rgb
 
+    def light_emit(self, li, n, seed=0):
+        """n draws of Light::SampleEmit from light li (emissive primitives'
+        lights first, then the scene's): dict of pos, dir, normal, radiance
+        (n x 3), pdf_pos, pdf_dir (n)."""
+        out = {k: np.zeros((n, 3), np.float32) for k in ("pos", "dir", "normal", "radiance")}
+        out["pdf_pos"] = np.zeros(n, np.float32)
+        out["pdf_dir"] = np.zeros(n, np.float32)
+        self.lib.orc_light_emit_batch(self.h, li, seed, n, *[_p(out[k]) for k in (
+            "pos", "dir", "normal", "radiance", "pdf_pos", "pdf_dir")])
+        return out
+
     def env_pdf(self, li, d):
         d = np.ascontiguousarray(d, np.float32)
         return float(self.lib.orc_env_pdf(self.h, li, _p(d)))
@@ -554,6 +568,17 @@ def bsdf_sample_batch(mat, ng, ns, wo, sam):
     args = [np.ascontiguousarray(x, np.float32) for x in (ng, ns, wo)]
     lib.orc_bsdf_sample_batch(C.byref(m), *[_p(a) for a in args], n, _p(sam), _p(wi), _p(valid))
     return wi, valid.astype(bool)
+
+
+def sppm_update_probe(alpha, r0, beta, m, phi):
+    """UpdatePixel over len(m) iterations: (radius, N, tau (3)) after each."""
+    lib = load()
+    m = np.ascontiguousarray(m, np.uint32)
+    phi = np.ascontiguousarray(phi, np.float32).reshape(-1, 3)
+    out = np.zeros((len(m), 5), np.float32)
+    lib.orc_sppm_update_probe(float(alpha), float(r0), _p(np.ascontiguousarray(beta, np.float32)),
+                              len(m), _p(m), _p(phi), _p(out))
+    return out[:, 0], out[:, 1], out[:, 2:]
 
 
 def _mat(m):

@@ -1911,6 +1911,24 @@ static light_emit light_sample_emit(const light_rt *l, sampler *smp) {
   return r;
 }
 
+/* Test hook: n draws of Light::SampleEmit from light li, draw i from PCG32
+ * seeded with sample_seed(seed, i, 0). */
+void orc_light_emit_batch(const orc_scene *s, uint32_t li, uint32_t seed, uint32_t n,
+                          float *pos, float *dir, float *normal, float *radiance,
+                          float *pdf_pos, float *pdf_dir) {
+  for (uint32_t i = 0; i < n; ++i) {
+    sampler smp;
+    orc_pcg32_seed(&smp.rng, sample_seed(seed, i, 0));
+    const light_emit em = light_sample_emit(&s->lights[li], &smp);
+    vstore(pos + 3 * (size_t)i, em.pos);
+    vstore(dir + 3 * (size_t)i, em.dir);
+    vstore(normal + 3 * (size_t)i, em.normal);
+    vstore(radiance + 3 * (size_t)i, em.radiance);
+    pdf_pos[i] = em.pdf_pos;
+    pdf_dir[i] = em.pdf_dir;
+  }
+}
+
 /* TracerPhoton (sppm.cpp:108-143); the light is picked by DefaultScene::
  * SampleLight (default_scene.cpp:76-85) from a Distribution over equal
  * weights (FP32 prefix sums, distribution.h:40-58). */
@@ -2000,6 +2018,30 @@ static void sppm_update(float alpha, sppm_px *p) {
     p->phi[0] = p->phi[1] = p->phi[2] = 0.f;
   }
   p->beta = V3(0, 0, 0);
+}
+
+/* Test hook: one pixel through `iters` UpdatePixel calls, iteration k
+ * bringing m[k] photons of flux phi[3k..3k+2] at throughput beta (3);
+ * writes the radius, N and tau (3) after each call (out: iters x 5). */
+void orc_sppm_update_probe(float alpha, float r0, const float *beta, uint32_t iters,
+                           const uint32_t *m, const float *phi, float *out) {
+  sppm_px p;
+  memset(&p, 0, sizeof(p));
+  p.valid = 1;
+  p.radius = r0;
+  for (uint32_t k = 0; k < iters; ++k) {
+    p.beta = vload(beta);
+    p.m = m[k];
+    p.phi[0] = phi[3 * k];
+    p.phi[1] = phi[3 * k + 1];
+    p.phi[2] = phi[3 * k + 2];
+    sppm_update(alpha, &p);
+    out[5 * k] = p.radius;
+    out[5 * k + 1] = p.n;
+    out[5 * k + 2] = p.tau.x;
+    out[5 * k + 3] = p.tau.y;
+    out[5 * k + 4] = p.tau.z;
+  }
 }
 
 int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
