@@ -355,8 +355,27 @@ typedef struct orc_vcm_args {
   int32_t use_vm, use_vc;
   int32_t min_path_length, max_path_length;
   float cont_prob;
-  uint32_t seed_base, pad;
+  uint32_t seed_base;
+  uint32_t variant; /* 0: the reference as written; else ORC_VCM_* bits (test-only) */
 } orc_vcm_args;
+
+/* Test-only estimator variants that each replace one formula of the
+ * reference's VCM by its consistent form (Georgiev et al. 2012, "Light
+ * transport simulation with vertex connection and merging", and its
+ * published SmallVCM code), so that tests can show the rest converges:
+ *   PIXEL_UNITS  the camera's solid-angle pdf and ConnectToCamera's
+ *                image-to-surface factor per unit pixel area instead of
+ *                per unit image-plane area at the focal distance, and the
+ *                camera's dVCM = light path count / camera pdf
+ *                (vcm.cpp:556-600, 713-729 use image-plane units and
+ *                dVCM = 1 / camera pdf);
+ *   REVERSE_PDF  the reverse pdf of a scattering event is the BSDF's pdf of
+ *                the incoming direction given the new one (vcm.cpp:416-423
+ *                evaluates PdfAll(-new_dir, new_dir));
+ *   LT_ONLY      light tracing alone: every light vertex connects to the
+ *                camera with weight 1, nothing is stored, and camera paths
+ *                only collect what their first ray hits or escapes to. */
+enum { ORC_VCM_PIXEL_UNITS = 1, ORC_VCM_REVERSE_PDF = 2, ORC_VCM_LT_ONLY = 4 };
 
 /* color_sum (w*h*3): per pixel, the sum over iterations of its camera path's
  * color (the value the film's tile receives with weight 1 per iteration);

@@ -69,11 +69,14 @@ class SppmArgs(C.Structure):
                 ("photon_seed_base", C.c_uint32), ("pad", C.c_uint32)]
 
 
+VCM_PIXEL_UNITS, VCM_REVERSE_PDF, VCM_LT_ONLY = 1, 2, 4
+
+
 class VcmArgs(C.Structure):
     _fields_ = [("iteration", C.c_int32), ("base_radius", C.c_float),
                 ("radius_alpha", C.c_float), ("use_vm", C.c_int32), ("use_vc", C.c_int32),
                 ("min_path_length", C.c_int32), ("max_path_length", C.c_int32),
-                ("cont_prob", C.c_float), ("seed_base", C.c_uint32), ("pad", C.c_uint32)]
+                ("cont_prob", C.c_float), ("seed_base", C.c_uint32), ("variant", C.c_uint32)]
 
 
 SPPM_PIXEL_DTYPE = np.dtype([("pos", "<f4", 3), ("beta", "<f4", 3), ("direct", "<f4", 3),
@@ -446,14 +449,15 @@ class OracleScene:
             raise ValueError("sppm: film width and height must be multiples of 64")
         return rad, st.reshape(h, w), int(n)
 
-    def render_vcm(self, spec, cam=None):
+    def render_vcm(self, spec, cam=None, variant=0):
         """VCM (the scene's VCMSpec): (color_sum (h, w, 3), splat (h, w, 3),
-        light vertices of the last iteration)."""
+        light vertices of the last iteration). variant: VCM_* bits of the
+        test-only estimator variants (0: the reference as written)."""
         cam = cam or self.camera()
         w, h = self.scene.camera.res
         a = VcmArgs(spec.iteration, spec.base_radius, spec.radius_alpha, int(spec.use_vm),
                     int(spec.use_vc), spec.min_path_length, spec.max_path_length, spec.cont_prob,
-                    spec.seed_base, 0)
+                    spec.seed_base, variant)
         col = np.zeros((h, w, 3), np.float32)
         spl = np.zeros((h, w, 3), np.float32)
         n = self.lib.orc_render_vcm(self.h, C.byref(cam), C.byref(a), _p(col), _p(spl))

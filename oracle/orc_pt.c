@@ -2165,6 +2165,7 @@ typedef struct vcm_ctx {
   v3 cam_pos, cam_dir;
   int W, H;
   float light_count, mis_vm, mis_vc, vm_norm, radius, radius_sqr;
+  float pix_k; /* pixels per unit image-plane area at the focal distance */
   distrib sel;
   float sel_pdf;
   float *splat;
@@ -2256,7 +2257,8 @@ static int vcm_scatter(const vcm_ctx *c, vcm_state *st, sampler *smp, const bsdf
   if (is_black(bs.f)) return 0;
   st->direction = bs.dir;
   float dir_pdf = bs.pdf;
-  float rev_pdf = bsdf_pdf_all(b, vneg(st->direction), bs.dir);
+  float rev_pdf = (c->a->variant & ORC_VCM_REVERSE_PDF) ? bsdf_pdf_all(b, it->wr, bs.dir)
+                                                          : bsdf_pdf_all(b, vneg(st->direction), bs.dir);
   if (next1(smp) > c->a->cont_prob) return 0;
   dir_pdf *= c->a->cont_prob;
   rev_pdf *= c->a->cont_prob;
@@ -2286,9 +2288,10 @@ static void vcm_connect_camera(vcm_ctx *c, const vcm_state *st, sampler *smp, co
   float rev_pdf = bsdf_pdf_all(b, it->wr, dir);
   rev_pdf *= c->a->cont_prob;
   const float cos_to_cam = vdot(it->ng, dir);
-  const float i2s = vcm_image_to_surface(c, dir, cos_to_cam, distance);
+  float i2s = vcm_image_to_surface(c, dir, cos_to_cam, distance);
+  if (c->a->variant & ORC_VCM_PIXEL_UNITS) i2s *= c->pix_k;
   const float w_light = (i2s / c->light_count) * (c->mis_vm + st->dVCM + st->dVC * rev_pdf);
-  const float mis = 1.f / (w_light + 1.f);
+  const float mis = (c->a->variant & ORC_VCM_LT_ONLY) ? 1.f : 1.f / (w_light + 1.f);
   const float s2i = 1.f / i2s;
   const v3 contrib = vdivs(vmul(vscale(st->throughput, mis), f), c->light_count * s2i);
   if (is_black(contrib)) return;
@@ -2538,7 +2541,7 @@ static void vcm_light_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
     st.dVCM /= fabsf(cos_in);
     st.dVC /= fabsf(cos_in);
     st.dVM /= fabsf(cos_in);
-    if (a->use_vc || a->use_vm) {
+    if ((a->use_vc || a->use_vm) && !(a->variant & ORC_VCM_LT_ONLY)) {
       if (c->n_lv == c->cap_lv) {
         c->cap_lv = c->cap_lv ? 2 * c->cap_lv : 1024;
         c->lv = (vcm_vertex *)realloc(c->lv, sizeof(vcm_vertex) * c->cap_lv);
@@ -2554,7 +2557,8 @@ static void vcm_light_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
       v->dVC = st.dVC;
       v->dVM = st.dVM;
     }
-    if (a->use_vc && (int)st.path_length + 1 >= a->min_path_length)
+    if ((a->use_vc || (a->variant & ORC_VCM_LT_ONLY)) &&
+        (int)st.path_length + 1 >= a->min_path_length)
       vcm_connect_camera(c, &st, smp, &b, &it);
     if ((int)st.path_length + 2 > a->max_path_length) break;
     if (!vcm_scatter(c, &st, smp, &b, &it)) break;
@@ -2582,7 +2586,8 @@ static v3 vcm_camera_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
   st.throughput = V3(1.f, 1.f, 1.f);
   st.path_length = 1;
   st.specular_path = 1;
-  st.dVCM = 1.f / camera_pdf;
+  st.dVCM = (a->variant & ORC_VCM_PIXEL_UNITS) ? c->light_count / (camera_pdf * c->pix_k)
+                                               : 1.f / camera_pdf;
   v3 color = V3(0, 0, 0);
   for (;; ++st.path_length) {
     orc_ray ray = make_ray(st.origin, st.direction, ORC_EPS, INFINITY);
@@ -2628,6 +2633,7 @@ static v3 vcm_camera_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
       }
       break;
     }
+    if (a->variant & ORC_VCM_LT_ONLY) break;
     if ((int)st.path_length >= a->max_path_length) break;
     if (a->use_vc && (int)st.path_length + 1 >= a->min_path_length)
       color = vadd(color, vmul(st.throughput, vcm_direct(c, &st, &it, &b, smp)));
@@ -2684,6 +2690,16 @@ int64_t orc_render_vcm(const orc_scene *s, const orc_camera *cam, const orc_vcm_
   c.n_cells = n;
   c.cell_ends = (int *)calloc(n, sizeof(int));
   int64_t n_vertices = 0;
+  {
+    /* one pixel's area on the plane at the focal distance */
+    const v3 q0 = xform_point(cam->raster2camera, V3(0.f, 0.f, 0.f));
+    const v3 q1 = xform_point(cam->raster2camera, V3(1.f, 0.f, 0.f));
+    const v3 q2 = xform_point(cam->raster2camera, V3(0.f, 1.f, 0.f));
+    const float fd = cam->focal_distance;
+    const v3 p0 = vscale(q0, fd / fabsf(q0.z)), p1 = vscale(q1, fd / fabsf(q1.z)),
+             p2 = vscale(q2, fd / fabsf(q2.z));
+    c.pix_k = 1.f / vlength(vcross(vsub(p1, p0), vsub(p2, p0)));
+  }
   const v3 wsize = vsub(s->world.mx, s->world.mn);
   for (int iter = 0; iter < a->iteration; ++iter) {
     /* RenderIter (vcm.cpp:161-200) */

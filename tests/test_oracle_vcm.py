@@ -14,8 +14,15 @@ fixes it:
   only collects emission at its first hit: each pixel's sum is exactly 0, the
   lamp's radiance, or the environment's, times the iteration count; nothing
   is splatted.
+- Convergence (the second half of this file): with the reference's reverse
+  pdf and camera units swapped for their consistent forms, light tracing,
+  vertex connection, vertex merging and full VCM each converge to the
+  pinned path tracer; as written they do not, by the amounts the tests
+  print, and each swap leaves the images bit-identical where it cannot
+  matter.
 """
 import dataclasses
+import math
 
 import numpy as np
 import pytest
@@ -115,3 +122,151 @@ def test_vcm_render_deterministic_finite():
     assert n1 == n2 > 0 and np.array_equal(c1, c2) and np.array_equal(s1, s2)
     assert np.isfinite(c1).all() and np.isfinite(s1).all()
     assert s1.sum() > 0 and not np.array_equal(c1, c3)
+
+
+# ---- convergence: the reference's estimator with two formulas replaced ----
+#
+# As written, the reference's VCM is not a consistent estimator: its reverse
+# pdf of a scattering event is PdfAll(-new_dir, new_dir) (vcm.cpp:416-423),
+# so the MIS weights of paths with two or more scatterings on one side do
+# not sum to one, and its camera pdf and ConnectToCamera factor are per unit
+# image-plane area at the focal distance (thin_lens.cpp:121-147) where the
+# light-tracing splat and the camera's dVCM need them per pixel with the
+# light path count (vcm.cpp:556-600, 713-729). The oracle restates all of
+# it as written (what the GPU matches) and, for these tests only, can swap
+# each of the two formulas for its consistent form (orc.h, ORC_VCM_*). With
+# both swapped, light tracing alone, vertex connection alone, vertex merging
+# alone and full VCM each converge to the oracle's path tracer (pinned in
+# test_oracle_pt.py), which fixes every other formula on their paths (the
+# dVCM/dVC/dVM recursions, vcm.cpp:300-313; ConnectVertices, :474-542;
+# ConnectToCamera, :544-600; DirectIllumination, :602-668; the merging
+# weights and normalisation, :88-126, :170-185); the as-written estimators
+# miss the same bounds, and each swap leaves results bit-identical where its
+# formula cannot matter. The scene: the Cornell box lit by a narrow cone
+# light (half-angle 0.05; a zero-angle light has an infinite cone pdf and
+# emits nothing, directional_light.cpp:60-79) and no environment (escaped
+# camera rays add the environment unweighted, vcm.cpp:332-339, which
+# double-counts it against its connections), and the splat sum divided by
+# the iteration count (Film::WriteImage adds the splat sum to the averaged
+# colors).
+
+MAX_LEN = 64
+
+
+def _cone_cornell():
+    sc = scenes.cornell(res=(64, 64), spp=256)
+    sc.lights = [dataclasses.replace(sc.lights[0], angle=0.05, radiance=(500.0, 400.0, 300.0))]
+    sc.pt = dataclasses.replace(sc.pt, max_bounces=MAX_LEN - 1, min_bounces=3, cont_prob=0.9)
+    return sc
+
+
+def _pt_image_mean(o, spp, use_mis=True):
+    r = o.render(args=o.args(spp=spp, use_mis=use_mis), threads=8)
+    wp, hp = o.traced_dims()
+    cnt = r["count"].astype(np.float64)[:, None]
+    mean = (r["sum"] / cnt).reshape(hp, wp, 3)[:64, :64]
+    var = (r["sum_sq"] / cnt - (r["sum"] / cnt) ** 2) / cnt
+    se = np.sqrt(var.reshape(hp, wp, 3)[:64, :64].sum(axis=(0, 1))) / 4096
+    return mean.reshape(-1, 3).mean(0), se
+
+
+def _vcm_image_mean(o, vc, vm, variant, iters, seeds, max_len=MAX_LEN, radius=0.01):
+    ms = []
+    for seed in range(seeds):
+        col, spl, _ = o.render_vcm(scenes.VCMSpec(iteration=iters, use_vc=vc, use_vm=vm,
+                                                  max_path_length=max_len, seed_base=seed,
+                                                  base_radius=radius), variant=variant)
+        ms.append(((col.astype(np.float64) + spl) / iters).reshape(-1, 3).mean(0))
+    ms = np.array(ms)
+    return ms.mean(0), ms.std(0, ddof=1) / math.sqrt(seeds)
+
+
+@pytest.fixture(scope="module")
+def cone_cornell():
+    o = orc.OracleScene(_cone_cornell())
+    try:
+        yield o, _pt_image_mean(o, 1024)
+    finally:
+        o.close()
+
+
+FIX = orc.VCM_PIXEL_UNITS | orc.VCM_REVERSE_PDF
+
+
+@pytest.mark.parametrize("name,vc,vm,variant,iters,seeds,bias,as_written", [
+    ("light tracing", False, False, orc.VCM_LT_ONLY | orc.VCM_PIXEL_UNITS, 16, 4, 0.004, None),
+    ("vertex connection", True, False, FIX, 16, 4, 0.004, 0),
+    ("vertex connection, units only", True, False, orc.VCM_PIXEL_UNITS, 16, 4, None, None),
+    ("vertex merging", False, True, orc.VCM_REVERSE_PDF, 32, 6, 0.02, 0),
+    ("vcm", True, True, FIX, 16, 4, 0.02, 0),
+])
+def test_vcm_estimators_converge_to_path_tracer(cone_cornell, name, vc, vm, variant, iters,
+                                                seeds, bias, as_written):
+    """Image means (per channel) against the path tracer's, within four
+    standard errors plus an allowance for the merging kernel's bias (2%, at
+    a base radius of 0.01) or the connections' float error (0.4%). The
+    as-written estimator (and the connection estimator with only the units
+    swapped) misses the bound on the sum of the channels."""
+    o, (pt, pt_se) = cone_cornell
+    est, se = _vcm_image_mean(o, vc, vm, variant, iters, seeds)
+    tol = 4 * np.sqrt(se ** 2 + pt_se ** 2) + (bias if bias is not None else 0.004) * pt
+    print(f"{name}: {est} vs {pt} (tol {tol})")
+    if bias is None:   # the single swap is not enough
+        assert abs(est.sum() - pt.sum()) > tol.sum(), (est, pt, tol)
+        return
+    assert np.all(np.abs(est - pt) < tol), (est, pt, tol)
+    if as_written is not None:
+        raw, raw_se = _vcm_image_mean(o, vc, vm, as_written, iters, seeds)
+        tol_raw = 4 * (raw_se + pt_se) + bias * pt
+        print(f"{name} as written: {raw}")
+        assert abs(raw.sum() - pt.sum()) > tol_raw.sum(), (raw, pt, tol_raw)
+
+
+def test_vcm_swaps_touch_only_their_formula(cone_cornell):
+    """The reverse-pdf swap cannot matter for paths of at most three
+    segments (no side scatters twice before a connection or merge): merging
+    and connection images are bit-identical at max length 3. The units swap
+    only enters connections to the camera and weights through dVCM times
+    the connection factor, which merging alone multiplies by zero: merging
+    images are bit-identical at any length."""
+    o, _ = cone_cornell
+
+    def img(vc, vm, variant, max_len):
+        col, spl, _ = o.render_vcm(scenes.VCMSpec(iteration=2, use_vc=vc, use_vm=vm,
+                                                  max_path_length=max_len, seed_base=3),
+                                   variant=variant)
+        return col, spl
+
+    for vc, vm in ((True, False), (False, True)):
+        a, b = img(vc, vm, 0, 3), img(vc, vm, orc.VCM_REVERSE_PDF, 3)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (vc, vm)
+    a, b = img(False, True, 0, MAX_LEN), img(False, True, orc.VCM_PIXEL_UNITS, MAX_LEN)
+    assert np.array_equal(a[0], b[0])
+    # and the swaps do change the images where they matter
+    a, b = img(True, False, 0, MAX_LEN), img(True, False, FIX, MAX_LEN)
+    assert not np.array_equal(a[0], b[0]) and not np.array_equal(a[1], b[1])
+
+
+def test_vertex_merging_converges_on_an_area_light():
+    """Merging alone reads no Light::Sample (whose area-light pdf carries the
+    reference's unnormalised-vector factor, pinned in
+    test_oracle_area_light.py), so on the Cornell box lit by its ceiling
+    lamp (black, so that camera paths, which stop at emitters, and the path
+    tracer, which continues, see the same light) it converges to the plain
+    path tracer (BSDF sampling only, unbiased for area lights) with the
+    reverse-pdf swap, and misses it as written."""
+    sc = scenes.cornell_area(("quad",), res=(64, 64), spp=256)
+    sc.materials[-1] = scenes.Material(0, (0.0, 0.0, 0.0))
+    sc.pt = dataclasses.replace(sc.pt, max_bounces=MAX_LEN, min_bounces=3, cont_prob=0.9)
+    o = orc.OracleScene(sc)
+    try:
+        pt, pt_se = _pt_image_mean(o, 2048, use_mis=False)
+        est, se = _vcm_image_mean(o, False, True, orc.VCM_REVERSE_PDF, 16, 6)
+        raw, raw_se = _vcm_image_mean(o, False, True, 0, 16, 6)
+    finally:
+        o.close()
+    tol = 4 * np.sqrt(se ** 2 + pt_se ** 2) + 0.02 * pt
+    print(f"vm {est} raw {raw} pt {pt} tol {tol}")
+    assert np.all(np.abs(est - pt) < tol), (est, pt, tol)
+    tol_raw = 4 * (raw_se + pt_se) + 0.02 * pt
+    assert raw.sum() - pt.sum() > tol_raw.sum(), (raw, pt, tol_raw)
