@@ -552,9 +552,10 @@ def sppm_bench(ctx, a):
     photons, forward depth 8, photon depth 5..10, alpha 2/3) except the initial
     radius, which takes the reference's scene-scaled option (init_radius < 0:
     world-box diagonal / 1000; the default 0.1 is a tenth of this unit box),
-    on the area-lit Cornell box at 1024x1024. The oracle is timed on one
-    iteration of the same work (a visible point per pixel plus one
-    iteration's photons), which every iteration of the reference repeats."""
+    on the area-lit Cornell box at 1024x1024. The oracle is timed on the
+    host's cores on one iteration of the same work (a visible point per
+    pixel plus one iteration's photons), which every iteration of the
+    reference repeats."""
     import dataclasses
     import torch
     from ajisai_render_amd import scenes
@@ -592,13 +593,14 @@ def sppm_bench(ctx, a):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
         o = orc.OracleScene(sc)
+        threads = cpu_threads()
         t = time.perf_counter()
-        o.render_sppm(dataclasses.replace(spec, iteration_count=1))
+        o.render_sppm(dataclasses.replace(spec, iteration_count=1), threads=threads)
         dt = time.perf_counter() - t
         o.close()
         est = dt * spec.iteration_count
-        out["cpu_baseline"] = {"value": round(1.0 / dt, 4), "unit": "iterations/s", "cores": 1,
-                               "kind": "port",
+        out["cpu_baseline"] = {"value": round(1.0 / dt, 4), "unit": "iterations/s",
+                               "cores": threads, "kind": "port", "host": host_cpu(),
                                "sample": f"1 of {spec.iteration_count} iterations ({w * h} visible "
                                          f"points + {spec.photons_per_iteration} photons, "
                                          f"{dt:.1f} s); whole render est. {est:.0f} s"}
@@ -611,7 +613,8 @@ def vcm_bench(ctx, a):
     box's smallest extent, alpha 0.75, connections and merging, path length
     0..10, continuation 0.9) on the area-lit Cornell box at 1024x1024, for
     `--vcm-iters` iterations (one light and one camera sub-path per pixel
-    each); the oracle (one thread) is timed on one iteration of the same work."""
+    each); the oracle is timed on the host's cores on one iteration of the
+    same work."""
     import dataclasses
     import torch
     from ajisai_render_amd import scenes
@@ -645,12 +648,13 @@ def vcm_bench(ctx, a):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
         o = orc.OracleScene(sc)
+        threads = cpu_threads()
         t = time.perf_counter()
-        o.render_vcm(dataclasses.replace(spec, iteration=1))
+        o.render_vcm(dataclasses.replace(spec, iteration=1), threads=threads)
         dt = time.perf_counter() - t
         o.close()
-        out["cpu_baseline"] = {"value": round(2 * w * h / dt, 1), "unit": "subpaths/s", "cores": 1,
-                               "kind": "port",
+        out["cpu_baseline"] = {"value": round(2 * w * h / dt, 1), "unit": "subpaths/s",
+                               "cores": threads, "kind": "port", "host": host_cpu(),
                                "sample": f"1 of {spec.iteration} iterations ({2 * w * h} "
                                          f"sub-paths, {dt:.1f} s)"}
         out["gpu_over_cpu"] = round(out["subpaths_per_s"] / out["cpu_baseline"]["value"], 1)

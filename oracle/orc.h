@@ -327,6 +327,11 @@ typedef struct orc_sppm_pixel {
  * multiple of 64. */
 int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
                         int32_t iterations, float *radiance, orc_sppm_pixel *state);
+/* The same on n_threads threads (visible points and photons split across
+ * them; with more than one, photon flux sums depend on arrival order). */
+int64_t orc_render_sppm_mt(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
+                           int32_t iterations, float *radiance, orc_sppm_pixel *state,
+                           int n_threads);
 /* Test hook: n draws of Light::SampleEmit (area_light.cpp:60-73,
  * directional_light.cpp:60-79, env_light.cpp:105-147) from light li (the
  * emissive primitives' lights first, then the scene's), draw i from its own
@@ -383,6 +388,10 @@ enum { ORC_VCM_PIXEL_UNITS = 1, ORC_VCM_REVERSE_PDF = 2, ORC_VCM_LT_ONLY = 4 };
  * the number of light vertices of the last iteration. */
 int64_t orc_render_vcm(const orc_scene *s, const orc_camera *cam, const orc_vcm_args *a,
                        float *color_sum, float *splat);
+/* The same on n_threads threads (sub-paths split across them; with more
+ * than one, splat sums depend on arrival order). */
+int64_t orc_render_vcm_mt(const orc_scene *s, const orc_camera *cam, const orc_vcm_args *a,
+                          float *color_sum, float *splat, int n_threads);
 /* Test hooks: ThinLensCamera::ToRaster (returns 0 when the point does not
  * project into the film) and ImageAreaToSolidAngle. */
 int orc_vcm_to_raster(const orc_camera *cam, const float *u2, const float *pos, float *dir_out,

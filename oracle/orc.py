@@ -147,6 +147,8 @@ def load():
                                      C.c_float, C.c_float, _P, _P, _P], None),
         "orc_image_srgb8": ([C.c_uint32, C.c_uint32, _P, C.c_int, _P], None),
         "orc_render_sppm": ([_P, _P, _P, C.c_int32, _P, _P], C.c_int64),
+        "orc_render_sppm_mt": ([_P, _P, _P, C.c_int32, _P, _P, C.c_int], C.c_int64),
+        "orc_render_vcm_mt": ([_P, _P, _P, _P, _P, C.c_int], C.c_int64),
         "orc_sppm_grid_probe": ([_P, C.c_float, C.c_uint32, _P, _P, C.c_uint32, _P, _P], None),
         "orc_light_emit_batch": ([_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P,
                                   _P], None),
@@ -433,7 +435,7 @@ class OracleScene:
                                     _p(s), _p(s2), _p(cnt), _p(ps) if ps is not None else None)
         return dict(sum=s, sum_sq=s2, count=cnt, per_sample=ps, queries=int(q))
 
-    def render_sppm(self, spec, iterations=0, cam=None):
+    def render_sppm(self, spec, iterations=0, cam=None, threads=1):
         """SPPM (the scene's SPPMSpec): per-pixel radiance (h, w, 3), the
         per-pixel state (SPPM_PIXEL_DTYPE) and the photon deposit count."""
         cam = cam or self.camera()
@@ -443,13 +445,17 @@ class OracleScene:
                      spec.photon_cont_prob, spec.update_alpha, spec.photon_seed_base, 0)
         rad = np.zeros((h, w, 3), np.float32)
         st = np.zeros(w * h, SPPM_PIXEL_DTYPE)
-        n = self.lib.orc_render_sppm(self.h, C.byref(cam), C.byref(a), iterations, _p(rad),
-                                     st.ctypes.data)
+        if threads > 1:
+            n = self.lib.orc_render_sppm_mt(self.h, C.byref(cam), C.byref(a), iterations, _p(rad),
+                                            st.ctypes.data, threads)
+        else:
+            n = self.lib.orc_render_sppm(self.h, C.byref(cam), C.byref(a), iterations, _p(rad),
+                                         st.ctypes.data)
         if n < 0:
             raise ValueError("sppm: film width and height must be multiples of 64")
         return rad, st.reshape(h, w), int(n)
 
-    def render_vcm(self, spec, cam=None, variant=0):
+    def render_vcm(self, spec, cam=None, variant=0, threads=1):
         """VCM (the scene's VCMSpec): (color_sum (h, w, 3), splat (h, w, 3),
         light vertices of the last iteration). variant: VCM_* bits of the
         test-only estimator variants (0: the reference as written)."""
@@ -460,7 +466,11 @@ class OracleScene:
                     spec.seed_base, variant)
         col = np.zeros((h, w, 3), np.float32)
         spl = np.zeros((h, w, 3), np.float32)
-        n = self.lib.orc_render_vcm(self.h, C.byref(cam), C.byref(a), _p(col), _p(spl))
+        if threads > 1:
+            n = self.lib.orc_render_vcm_mt(self.h, C.byref(cam), C.byref(a), _p(col), _p(spl),
+                                           threads)
+        else:
+            n = self.lib.orc_render_vcm(self.h, C.byref(cam), C.byref(a), _p(col), _p(spl))
         return col, spl, int(n)
 
     def close(self):

@@ -1804,7 +1804,18 @@ static void sppm_grid_build(sppm_grid *g, const sppm_px *px, uint32_t n) {
 }
 
 /* VisiblePointSearcher::AddPhoton (photon_mapping.h:103-121) */
-static int64_t sppm_add_photon(const sppm_grid *g, sppm_px *px, v3 pos, v3 phi, v3 wr) {
+static void atomic_addf(float *p, float v) {
+  float old, nw;
+  __atomic_load(p, &old, __ATOMIC_RELAXED);
+  do {
+    nw = old + v;
+  } while (!__atomic_compare_exchange(p, &old, &nw, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+}
+
+/* shared: other threads deposit into the same pixels (atomic adds; the sums
+ * then depend on the order photons arrive in) */
+static int64_t sppm_add_photon(const sppm_grid *g, sppm_px *px, v3 pos, v3 phi, v3 wr,
+                               int shared) {
   int c[3];
   sppm_to_grid(g, pos, c);
   const uint32_t e = sppm_entry(c[0], c[1], c[2]);
@@ -1815,10 +1826,17 @@ static int64_t sppm_add_photon(const sppm_grid *g, sppm_px *px, v3 pos, v3 phi, 
     if (vdot(d, d) > p->radius * p->radius) continue;
     const v3 dphi = vmul(phi, bsdf_eval_all(&p->b, wr, p->wr));
     if (!(isfinite(dphi.x) && isfinite(dphi.y) && isfinite(dphi.z))) continue;
-    p->phi[0] += dphi.x;
-    p->phi[1] += dphi.y;
-    p->phi[2] += dphi.z;
-    p->m++;
+    if (shared) {
+      atomic_addf(&p->phi[0], dphi.x);
+      atomic_addf(&p->phi[1], dphi.y);
+      atomic_addf(&p->phi[2], dphi.z);
+      __atomic_fetch_add(&p->m, 1, __ATOMIC_RELAXED);
+    } else {
+      p->phi[0] += dphi.x;
+      p->phi[1] += dphi.y;
+      p->phi[2] += dphi.z;
+      p->m++;
+    }
     ++n;
   }
   return n;
@@ -1934,7 +1952,7 @@ void orc_light_emit_batch(const orc_scene *s, uint32_t li, uint32_t seed, uint32
  * weights (FP32 prefix sums, distribution.h:40-58). */
 static int64_t sppm_tracer_photon(const orc_scene *s, const orc_sppm_args *a,
                                   const distrib *sel, float sel_pdf, const sppm_grid *g,
-                                  sppm_px *px, sampler *smp, uint64_t *q) {
+                                  sppm_px *px, sampler *smp, uint64_t *q, int shared) {
   if (s->n_lights == 0) return 0;
   int idx;
   (void)distrib_sample_cont(sel, next1(smp), NULL, &idx);
@@ -1953,7 +1971,7 @@ static int64_t sppm_tracer_photon(const orc_scene *s, const orc_sppm_args *a,
     orc_hit h;
     if (!scene_intersect(s, &ray, &h, q)) return n;
     inct it = post_intersect(s, &ray, &h);
-    if (depth > 1) n += sppm_add_photon(g, px, it.pos, coef, it.wr);
+    if (depth > 1) n += sppm_add_photon(g, px, it.pos, coef, it.wr, shared);
     uint32_t mi = s->prims[s->tri_prim[h.id]].material;
     bsdf b = make_bsdf(&s->mats[mi], it.ng, it.ns);
     v3 sam;
@@ -2044,8 +2062,83 @@ void orc_sppm_update_probe(float alpha, float r0, const float *beta, uint32_t it
   }
 }
 
+/* One SPPM phase over items [0, total) split across threads: the visible
+ * points of every pixel, or one iteration's photons. */
+typedef struct sppm_job {
+  const orc_scene *s;
+  const orc_camera *cam;
+  const orc_sppm_args *a;
+  const distrib *sel;
+  float sel_pdf;
+  const sppm_grid *g;
+  sppm_px *px;
+  uint32_t W, it, total, phase, shared;
+  uint32_t next;    /* guarded by the atomic builtins */
+  int64_t deposits; /* ditto */
+  uint64_t q;       /* ditto */
+} sppm_job;
+
+static void sppm_vp_pixel(const sppm_job *j, uint32_t i, uint64_t *q) {
+  const uint32_t x = i % j->W, y = i / j->W;
+  sampler smp;
+  orc_pcg32_seed(&smp.rng, (uint64_t)(int64_t)(int)(x + y * j->W));
+  const float u = (float)x + next1(&smp);
+  const float v = (float)y + next1(&smp);
+  float lens[2];
+  lens[0] = next1(&smp);
+  lens[1] = next1(&smp);
+  orc_ray r;
+  orc_camera_ray(j->cam, u, v, lens, &r);
+  sppm_tracer_vp(j->s, j->a->forward_max_depth, r, &smp, &j->px[i], q);
+}
+
+static void *sppm_worker(void *arg) {
+  sppm_job *j = (sppm_job *)arg;
+  uint64_t q = 0;
+  int64_t dep = 0;
+  for (;;) {
+    const uint32_t i0 = __atomic_fetch_add(&j->next, 256u, __ATOMIC_RELAXED);
+    if (i0 >= j->total) break;
+    const uint32_t i1 = i0 + 256 < j->total ? i0 + 256 : j->total;
+    for (uint32_t i = i0; i < i1; ++i) {
+      if (j->phase == 0) {
+        sppm_vp_pixel(j, i, &q);
+      } else {
+        sampler smp;
+        orc_pcg32_seed(&smp.rng, sample_seed(j->a->photon_seed_base, i, j->it));
+        dep += sppm_tracer_photon(j->s, j->a, j->sel, j->sel_pdf, j->g, j->px, &smp, &q,
+                                  (int)j->shared);
+      }
+    }
+  }
+  __atomic_fetch_add(&j->q, q, __ATOMIC_RELAXED);
+  __atomic_fetch_add(&j->deposits, dep, __ATOMIC_RELAXED);
+  return NULL;
+}
+
+static int64_t sppm_phase(sppm_job *j, int n_threads) {
+  j->next = 0;
+  j->deposits = 0;
+  j->shared = n_threads > 1;
+  if (n_threads <= 1) {
+    sppm_worker(j);
+    return j->deposits;
+  }
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+  for (int i = 0; i < n_threads; ++i) pthread_create(&th[i], NULL, sppm_worker, j);
+  for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
+  free(th);
+  return j->deposits;
+}
+
 int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
                         int32_t iterations, float *radiance, orc_sppm_pixel *state) {
+  return orc_render_sppm_mt(s, cam, a, iterations, radiance, state, 1);
+}
+
+int64_t orc_render_sppm_mt(const orc_scene *s, const orc_camera *cam, const orc_sppm_args *a,
+                           int32_t iterations, float *radiance, orc_sppm_pixel *state,
+                           int n_threads) {
   const uint32_t W = (uint32_t)(int)cam->resolution[0], H = (uint32_t)(int)cam->resolution[1];
   if (W % 64 || H % 64) return -1;
   const uint32_t n = W * H;
@@ -2075,29 +2168,27 @@ int64_t orc_render_sppm(const orc_scene *s, const orc_camera *cam, const orc_spp
     free(tab);
   }
   float max_radius = init_radius;
-  uint64_t q = 0;
   int64_t deposits = 0;
+  sppm_job job;
+  memset(&job, 0, sizeof(job));
+  job.s = s;
+  job.cam = cam;
+  job.a = a;
+  job.sel = &sel;
+  job.sel_pdf = sel_pdf;
+  job.g = &g;
+  job.px = px;
+  job.W = W;
   for (int32_t it = 0; it < iterations; ++it) {
     g.side = 1.05f * max_radius;
-    for (uint32_t y = 0; y < H; ++y)
-      for (uint32_t x = 0; x < W; ++x) {
-        sampler smp;
-        orc_pcg32_seed(&smp.rng, (uint64_t)(int64_t)(int)(x + y * W));
-        const float u = (float)x + next1(&smp);
-        const float v = (float)y + next1(&smp);
-        float lens[2];
-        lens[0] = next1(&smp);
-        lens[1] = next1(&smp);
-        orc_ray r;
-        orc_camera_ray(cam, u, v, lens, &r);
-        sppm_tracer_vp(s, a->forward_max_depth, r, &smp, &px[x + y * W], &q);
-      }
+    job.phase = 0;
+    job.total = n;
+    (void)sppm_phase(&job, n_threads);
     sppm_grid_build(&g, px, n);
-    for (int32_t i = 0; i < a->photons_per_iteration; ++i) {
-      sampler smp;
-      orc_pcg32_seed(&smp.rng, sample_seed(a->photon_seed_base, (uint32_t)i, (uint32_t)it));
-      deposits += sppm_tracer_photon(s, a, &sel, sel_pdf, &g, px, &smp, &q);
-    }
+    job.phase = 1;
+    job.it = (uint32_t)it;
+    job.total = (uint32_t)(a->photons_per_iteration > 0 ? a->photons_per_iteration : 0);
+    deposits += sppm_phase(&job, n_threads);
     max_radius = 0.f;
     for (uint32_t i = 0; i < n; ++i)
       if (px[i].valid) max_radius = std_max(max_radius, px[i].radius);
@@ -2169,6 +2260,8 @@ typedef struct vcm_ctx {
   distrib sel;
   float sel_pdf;
   float *splat;
+  int shared_splat;    /* other threads splat too: atomic adds */
+  float *camera_color; /* the camera phase's color sums (threaded phase) */
   /* light vertices of the iteration (path order) */
   vcm_vertex *lv;
   uint32_t n_lv, cap_lv;
@@ -2299,9 +2392,15 @@ static void vcm_connect_camera(vcm_ctx *c, const vcm_state *st, sampler *smp, co
   if (scene_occlude(c->s, &sr, &c->q)) return;
   const int ix = (int)img.x, iy = (int)img.y;
   float *px = c->splat + 3 * ((size_t)iy * c->W + ix);
-  px[0] += contrib.x;
-  px[1] += contrib.y;
-  px[2] += contrib.z;
+  if (c->shared_splat) {
+    atomic_addf(&px[0], contrib.x);
+    atomic_addf(&px[1], contrib.y);
+    atomic_addf(&px[2], contrib.z);
+  } else {
+    px[0] += contrib.x;
+    px[1] += contrib.y;
+    px[2] += contrib.z;
+  }
 }
 
 /* Light::Sample as DirectIllumination reads it: position, normal, radiance,
@@ -2657,8 +2756,110 @@ static v3 vcm_camera_path(vcm_ctx *c, uint32_t path_idx, sampler *smp) {
   return color;
 }
 
+/* The light or camera sub-paths of one iteration on n_threads threads. Light
+ * paths: thread t traces the contiguous range t of paths into its own
+ * vertex buffer, and the buffers are joined in thread order (path order).
+ * Camera paths write only their own pixel. */
+typedef struct vcm_job {
+  vcm_ctx *c;
+  vcm_ctx tc;
+  uint32_t lo, hi, iter, camera;
+  uint32_t *next; /* camera phase cursor, guarded by the atomic builtins */
+} vcm_job;
+
+static void *vcm_worker(void *arg) {
+  vcm_job *j = (vcm_job *)arg;
+  vcm_ctx *c = &j->tc;
+  const uint32_t n = (uint32_t)(c->W * c->H);
+  if (!j->camera) {
+    for (uint32_t i = j->lo; i < j->hi; ++i) {
+      sampler smp;
+      orc_pcg32_seed(&smp.rng, sample_seed(c->a->seed_base, i, 2u * j->iter));
+      vcm_light_path(c, i, &smp);
+    }
+    return NULL;
+  }
+  float *color_sum = (float *)j->c->camera_color;
+  for (;;) {
+    const uint32_t i0 = __atomic_fetch_add(j->next, 64u, __ATOMIC_RELAXED);
+    if (i0 >= n) break;
+    const uint32_t i1 = i0 + 64 < n ? i0 + 64 : n;
+    for (uint32_t i = i0; i < i1; ++i) {
+      sampler smp;
+      orc_pcg32_seed(&smp.rng, sample_seed(c->a->seed_base, i, 2u * j->iter + 1u));
+      const v3 col = vcm_camera_path(c, i, &smp);
+      color_sum[3 * (size_t)i] += col.x;
+      color_sum[3 * (size_t)i + 1] += col.y;
+      color_sum[3 * (size_t)i + 2] += col.z;
+    }
+  }
+  return NULL;
+}
+
+static void vcm_light_phase_mt(vcm_ctx *c, uint32_t iter, int n_threads) {
+  const uint32_t n = (uint32_t)(c->W * c->H);
+  vcm_job *jobs = (vcm_job *)calloc((size_t)n_threads, sizeof(vcm_job));
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+  for (int t = 0; t < n_threads; ++t) {
+    jobs[t].c = c;
+    jobs[t].tc = *c;
+    jobs[t].tc.lv = NULL;
+    jobs[t].tc.n_lv = jobs[t].tc.cap_lv = 0;
+    jobs[t].tc.q = 0;
+    jobs[t].tc.shared_splat = 1;
+    jobs[t].lo = (uint32_t)((uint64_t)n * (uint64_t)t / (uint64_t)n_threads);
+    jobs[t].hi = (uint32_t)((uint64_t)n * (uint64_t)(t + 1) / (uint64_t)n_threads);
+    jobs[t].iter = iter;
+    pthread_create(&th[t], NULL, vcm_worker, &jobs[t]);
+  }
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+  uint32_t total = 0;
+  for (int t = 0; t < n_threads; ++t) total += jobs[t].tc.n_lv;
+  if (total > c->cap_lv) {
+    c->cap_lv = total;
+    c->lv = (vcm_vertex *)realloc(c->lv, sizeof(vcm_vertex) * (total ? total : 1));
+  }
+  uint32_t base = 0;
+  for (int t = 0; t < n_threads; ++t) {
+    vcm_ctx *tc = &jobs[t].tc;
+    if (tc->n_lv) memcpy(c->lv + base, tc->lv, sizeof(vcm_vertex) * tc->n_lv);
+    for (uint32_t i = jobs[t].lo; i < jobs[t].hi; ++i) c->path_ends[i] += base;
+    base += tc->n_lv;
+    c->q += tc->q;
+    free(tc->lv);
+  }
+  c->n_lv = total;
+  free(th);
+  free(jobs);
+}
+
+static void vcm_camera_phase_mt(vcm_ctx *c, uint32_t iter, float *color_sum, int n_threads) {
+  vcm_job *jobs = (vcm_job *)calloc((size_t)n_threads, sizeof(vcm_job));
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+  uint32_t next = 0;
+  c->camera_color = color_sum;
+  for (int t = 0; t < n_threads; ++t) {
+    jobs[t].c = c;
+    jobs[t].tc = *c;
+    jobs[t].tc.q = 0;
+    jobs[t].iter = iter;
+    jobs[t].camera = 1;
+    jobs[t].next = &next;
+    pthread_create(&th[t], NULL, vcm_worker, &jobs[t]);
+  }
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+  for (int t = 0; t < n_threads; ++t) c->q += jobs[t].tc.q;
+  free(th);
+  free(jobs);
+}
+
 int64_t orc_render_vcm(const orc_scene *s, const orc_camera *cam, const orc_vcm_args *a,
                        float *color_sum, float *splat) {
+  return orc_render_vcm_mt(s, cam, a, color_sum, splat, 1);
+}
+
+int64_t orc_render_vcm_mt(const orc_scene *s, const orc_camera *cam, const orc_vcm_args *a,
+                          float *color_sum, float *splat, int n_threads) {
   vcm_ctx c;
   memset(&c, 0, sizeof(c));
   c.s = s;
@@ -2714,20 +2915,28 @@ int64_t orc_render_vcm(const orc_scene *s, const orc_camera *cam, const orc_vcm_
     c.mis_vm = a->use_vm ? eta : 0.f;
     c.mis_vc = a->use_vc ? 1.f / eta : 0.f;
     c.n_lv = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-      sampler smp;
-      orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, i, 2u * (uint32_t)iter));
-      vcm_light_path(&c, i, &smp);
+    if (n_threads > 1) {
+      vcm_light_phase_mt(&c, (uint32_t)iter, n_threads);
+    } else {
+      for (uint32_t i = 0; i < n; ++i) {
+        sampler smp;
+        orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, i, 2u * (uint32_t)iter));
+        vcm_light_path(&c, i, &smp);
+      }
     }
     n_vertices = c.n_lv;
     if (a->use_vm) vcm_grid_build(&c);
-    for (uint32_t i = 0; i < n; ++i) {
-      sampler smp;
-      orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, i, 2u * (uint32_t)iter + 1u));
-      const v3 col = vcm_camera_path(&c, i, &smp);
-      color_sum[3 * (size_t)i] += col.x;
-      color_sum[3 * (size_t)i + 1] += col.y;
-      color_sum[3 * (size_t)i + 2] += col.z;
+    if (n_threads > 1) {
+      vcm_camera_phase_mt(&c, (uint32_t)iter, color_sum, n_threads);
+    } else {
+      for (uint32_t i = 0; i < n; ++i) {
+        sampler smp;
+        orc_pcg32_seed(&smp.rng, sample_seed(a->seed_base, i, 2u * (uint32_t)iter + 1u));
+        const v3 col = vcm_camera_path(&c, i, &smp);
+        color_sum[3 * (size_t)i] += col.x;
+        color_sum[3 * (size_t)i + 1] += col.y;
+        color_sum[3 * (size_t)i + 2] += col.z;
+      }
     }
   }
   distrib_free(&c.sel);

@@ -284,3 +284,21 @@ def test_sppm_photon_estimate_converges_to_one_bounce_indirect():
     m, se = got.mean(), got.std(ddof=1) / 2.0
     print(f"photon part {got} mean {m:.5g} want {want.mean():.5g} se {se:.3g}")
     assert abs(m - want.mean()) < 4 * se + 0.01 * want.mean(), (m, want.mean(), se)
+
+
+def test_threaded_sppm_matches_one_thread():
+    """The threaded oracle (the CPU baseline) does the same work: the same
+    deposits and, up to the order photon fluxes are summed in, the same
+    radiance."""
+    sc = scenes.cornell_area(("quad", "twosided", "mesh", "cube"), res=(64, 64))
+    spec = _spec(iteration_count=2, photons_per_iteration=30000, photon_seed_base=1,
+                 init_radius=-1.0)
+    o = orc.OracleScene(sc)
+    try:
+        r1, s1, n1 = o.render_sppm(spec)
+        r8, s8, n8 = o.render_sppm(spec, threads=8)
+    finally:
+        o.close()
+    assert n1 == n8 > 0
+    assert np.array_equal(s1["valid"], s8["valid"]) and np.array_equal(s1["n"], s8["n"])
+    assert np.allclose(r1, r8, rtol=1e-4, atol=1e-6)

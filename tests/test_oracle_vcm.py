@@ -270,3 +270,19 @@ def test_vertex_merging_converges_on_an_area_light():
     assert np.all(np.abs(est - pt) < tol), (est, pt, tol)
     tol_raw = 4 * (raw_se + pt_se) + 0.02 * pt
     assert raw.sum() - pt.sum() > tol_raw.sum(), (raw, pt, tol_raw)
+
+
+def test_threaded_vcm_matches_one_thread():
+    """The threaded oracle (the CPU baseline) joins the light vertices in
+    path order, so camera paths see the same vertices and give the same
+    colors bit for bit; splats agree up to the order they are summed in."""
+    sc = scenes.cornell_area(("quad", "twosided", "mesh", "cube"), res=(64, 64))
+    spec = dataclasses.replace(scenes.VCMSpec(), iteration=2, seed_base=1)
+    o = orc.OracleScene(sc)
+    try:
+        c1, p1, v1 = o.render_vcm(spec)
+        c8, p8, v8 = o.render_vcm(spec, threads=8)
+    finally:
+        o.close()
+    assert v1 == v8 > 0 and np.array_equal(c1, c8)
+    assert np.allclose(p1, p8, rtol=1e-4, atol=1e-9) and p1.sum() > 0
