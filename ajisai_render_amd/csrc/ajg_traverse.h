@@ -580,8 +580,9 @@ __device__ __forceinline__ uint32_t tlas_leaf_instance(const DevBVH &tlas, int s
 }
 
 // Closest hit over a two-level scene; store(i, hit, instance). The world
-// ray is kept as origin and direction only (registers); its reciprocal
-// direction is recomputed (same operations) when a mesh's tree is left.
+// ray is not kept while a mesh's tree is traversed: leaving the tree loads
+// it again (load(i), the same operations as at the start), which keeps six
+// registers per lane out of the live state.
 template <int SPILL, int LDS, uint32_t STRIPES = 1, class Load, class Store>
 __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const DevInst *insts,
                                                       const DevBlas *blases, uint32_t n,
@@ -591,8 +592,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
   float se[SPILL];
   const StackRef st = {lds_node, lds_entry, sn, se};
   int d = 0;
-  TRay r;        // ray of the current level
-  f3 wo, wd;     // world ray
+  TRay r;  // ray of the current level
   THit h;
   uint32_t h_inst = kNone, cur_inst = 0;
   float tmax = 0.f;
@@ -607,8 +607,6 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
       const uint32_t idx = refill<STRIPES>(idle, n, cursor, &drained);
       if (idx != kNone) {
         r = load(idx);
-        wo = r.o;
-        wd = r.d;
         h.t = __int_as_float(0x7f800000);
         h.u = 0.f;
         h.v = 0.f;
@@ -644,7 +642,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         cur_inst = tlas_leaf_instance(tlas, ~node);
         const DevInst *I = insts + cur_inst;
         const DevBlas *B = blases + I->info.x;
-        const TRay ro = make_tray(xform_point(I->w2o, wo), xform_vec(I->w2o, wd), r.tmin, tmax);
+        const TRay ro = make_tray(xform_point(I->w2o, r.o), xform_vec(I->w2o, r.d), r.tmin, tmax);
         float e;
         if (leaf_box_test(r, ia, ib, ic, tmax) &&
             slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
@@ -677,7 +675,8 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
       for (;;) {
         if (d == base) {  // the mesh's tree is done: back to the top level
           base = -1;
-          r = make_tray(wo, wd, r.tmin, tmax);
+          r = load(ray);
+          r.tmax = tmax;
           wn = tlas.wide;
           leaf = tlas.leaf_tris;
         }
