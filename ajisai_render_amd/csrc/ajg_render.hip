@@ -568,8 +568,9 @@ k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uin
 }
 
 // Instanced scenes: two-level traversal (top-level tree over instances).
+// (6 waves per SIMD: the register budget that measured fastest; 8 spills)
 template <int SPILL>
-__global__ void __launch_bounds__(kTravThreads)
+__global__ void __launch_bounds__(kTravThreads, 6)
 k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
               float4 *hits, uint32_t *hit_inst, QIn qin, uint32_t *cursor) {
   __shared__ int s_node[kLdsStack * kTravThreads];

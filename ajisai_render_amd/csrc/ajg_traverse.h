@@ -74,10 +74,11 @@ __device__ __forceinline__ bool slab_static(const TRay &r, float mnx, float mny,
 }
 
 // Triangle::Intersect / Occlude (ploc_aggregate.cpp:184-242) without the
-// t < best part: returns true with t, u, v when tmin < t < tmax.
+// t < best part: returns true with t, u, v when tmin < t < tmax (t <= tmax
+// with `incl`, which only two-level traversal asks for, see below).
 __device__ __forceinline__ bool mt_test(f3 v0, f3 v1, f3 v2, const TRay &r,
                                         float tmax, float *t_out, float *u_out,
-                                        float *v_out) {
+                                        float *v_out, bool incl = false) {
   const f3 e0 = sub(v1, v0);
   const f3 e1 = sub(v2, v0);
   const f3 h = cross(r.d, e1);
@@ -91,7 +92,7 @@ __device__ __forceinline__ bool mt_test(f3 v0, f3 v1, f3 v2, const TRay &r,
   const float v = f * dot(r.d, q);
   if (v < 0.f || u + v > 1.f) return false;
   const float t = f * dot(e1, q);
-  if (t > r.tmin && t < tmax) {
+  if (t > r.tmin && (t < tmax || (incl && t == tmax))) {
     *t_out = t;
     *u_out = u;
     *v_out = v;
@@ -117,10 +118,12 @@ __device__ __forceinline__ void load_leaf(const float4 *lt, int slot, f3 *a, f3 
 
 // The exact leaf box test (IntersectP on the triangle's AABB,
 // ploc_aggregate.cpp:154-162, 168-182) with the given t_max.
-__device__ __forceinline__ bool leaf_box_test(const TRay &r, f3 a, f3 b, f3 c, float tmax) {
+__device__ __forceinline__ bool leaf_box_test(const TRay &r, f3 a, f3 b, f3 c, float tmax,
+                                              bool incl = false) {
   const box3 bx = tri_box(a, b, c);
   float e;
-  return slab_static(r, bx.mn.x, bx.mn.y, bx.mn.z, bx.mx.x, bx.mx.y, bx.mx.z, &e) && e < tmax;
+  return slab_static(r, bx.mn.x, bx.mn.y, bx.mn.z, bx.mx.x, bx.mx.y, bx.mx.z, &e) &&
+         (e < tmax || (incl && e == tmax));
 }
 
 // The leaf box test at visit time + MT, updating the closest hit.
@@ -220,7 +223,8 @@ __device__ __forceinline__ QBoxes qload(const WideNode *w) {
 // is the next node (interior or ~leaf).
 template <int LDS>
 __device__ __forceinline__ bool closest_step(const WideNode *wide, int &node, const TRay &r,
-                                             float tmax, const StackRef &st, int &d) {
+                                             float tmax, const StackRef &st, int &d,
+                                             bool incl = false) {
   const WideNode *w = wide + node;
   const QBoxes q = qload(w);
   const int4 ref = w->r;
@@ -228,11 +232,56 @@ __device__ __forceinline__ bool closest_step(const WideNode *wide, int &node, co
   if (ref.w != kNoRef && qslab(r, q, 3, &e)) stack_push<LDS>(st, d, ref.w, e);
   if (ref.z != kNoRef && qslab(r, q, 2, &e)) stack_push<LDS>(st, d, ref.z, e);
   if (qslab(r, q, 1, &e)) stack_push<LDS>(st, d, ref.y, e);
-  if (qslab(r, q, 0, &e) && e < tmax) {
+  if (qslab(r, q, 0, &e) && (e < tmax || (incl && e == tmax))) {
     node = ref.x;
     return true;
   }
   return false;
+}
+
+// Top-level visit of the two-level closest-hit traversal: the slots that
+// pass their static slab part with entry <= t_max are visited nearest entry
+// first (equal entries in slot order); the others are pushed farthest first.
+// Entries equal to t_max still count: a tie with the current hit may go to a
+// lower instance number (the two-level tie rule, below).
+__device__ __forceinline__ void near_cs(float &ka, int &na, int &sa, float &kb, int &nb,
+                                        int &sb) {
+  if (kb < ka || (kb == ka && sb < sa)) {
+    const float k = ka;
+    const int n = na, s = sa;
+    ka = kb, na = nb, sa = sb;
+    kb = k, nb = n, sb = s;
+  }
+}
+template <int LDS>
+__device__ __forceinline__ bool closest_step_near(const WideNode *wide, int &node,
+                                                  const TRay &r, float tmax,
+                                                  const StackRef &st, int &d) {
+  const WideNode *w = wide + node;
+  const QBoxes q = qload(w);
+  const int4 ref = w->r;
+  const float inf = __int_as_float(0x7f800000);
+  float k0, k1, k2, k3;
+  int n0 = ref.x, n1 = ref.y, n2 = ref.z, n3 = ref.w;
+  int s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+  if (!(qslab(r, q, 0, &k0) && k0 <= tmax)) k0 = inf, n0 = kNoRef;
+  if (!(qslab(r, q, 1, &k1) && k1 <= tmax)) k1 = inf, n1 = kNoRef;
+  if (!(n2 != kNoRef && qslab(r, q, 2, &k2) && k2 <= tmax)) k2 = inf, n2 = kNoRef;
+  if (!(n3 != kNoRef && qslab(r, q, 3, &k3) && k3 <= tmax)) k3 = inf, n3 = kNoRef;
+  // a sorting network on (entry, slot); rejected slots carry no reference
+  near_cs(k0, n0, s0, k1, n1, s1);
+  near_cs(k2, n2, s2, k3, n3, s3);
+  near_cs(k0, n0, s0, k2, n2, s2);
+  near_cs(k1, n1, s1, k3, n3, s3);
+  near_cs(k1, n1, s1, k2, n2, s2);
+  if (n3 != kNoRef) stack_push<LDS>(st, d, n3, k3);
+  if (n2 != kNoRef) stack_push<LDS>(st, d, n2, k2);
+  if (n1 != kNoRef) stack_push<LDS>(st, d, n1, k1);
+  if (n0 != kNoRef) {
+    node = n0;
+    return true;
+  }
+  return false;  // (whatever was pushed is popped next)
 }
 
 // Any-hit visit (t_max fixed, so culling happens at push time and the visit
@@ -579,10 +628,15 @@ __device__ __forceinline__ uint32_t tlas_leaf_instance(const DevBVH &tlas, int s
   return __float_as_uint(tlas.leaf_tris[3 * slot].w);
 }
 
-// Closest hit over a two-level scene; store(i, hit, instance). The world
-// ray is not kept while a mesh's tree is traversed: leaving the tree loads
-// it again (load(i), the same operations as at the start), which keeps six
-// registers per lane out of the live state.
+// Closest hit over a two-level scene; store(i, hit, instance). The result
+// is the hit with the least (t, instance number), and within an instance
+// the one-level traversal's (the first of equal t in its visit order): an
+// instance numbered below the current hit's also accepts t equal to t_max
+// (boxes and triangles), so the top level may be visited in any order. It
+// is visited nearest entry first. The world ray is not kept while a mesh's
+// tree is traversed: leaving the tree loads it again (load(i), the same
+// operations as at the start), which keeps six registers per lane out of
+// the live state.
 template <int SPILL, int LDS, uint32_t STRIPES = 1, class Load, class Store>
 __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const DevInst *insts,
                                                       const DevBlas *blases, uint32_t n,
@@ -635,6 +689,8 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
     }
     if (ray == kNone) continue;
     bool next = false;
+    // this instance may take a tie at t_max from the current hit
+    const bool tie = base >= 0 && h_inst != kNone && cur_inst < h_inst;
     if (node < 0) {
       if (base < 0) {  // an instance: enter its mesh's tree in object space
         f3 ia, ib, ic;  // the instance box as its degenerate leaf triangle
@@ -643,11 +699,12 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         const DevInst *I = insts + cur_inst;
         const DevBlas *B = blases + I->info.x;
         const TRay ro = make_tray(xform_point(I->w2o, r.o), xform_vec(I->w2o, r.d), r.tmin, tmax);
+        const bool tie_in = h_inst != kNone && cur_inst < h_inst;
         float e;
-        if (leaf_box_test(r, ia, ib, ic, tmax) &&
+        if (leaf_box_test(r, ia, ib, ic, tmax, true) &&
             slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
                         B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
-            e < tmax) {
+            (e < tmax || (tie_in && e == tmax))) {
           r = ro;
           base = d;
           wn = B->bvh.wide;
@@ -659,7 +716,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         f3 a, b, c;
         load_leaf(leaf, ~node, &a, &b, &c);
         float t, u, v;
-        if (leaf_box_test(r, a, b, c, tmax) && mt_test(a, b, c, r, tmax, &t, &u, &v)) {
+        if (leaf_box_test(r, a, b, c, tmax, tie) && mt_test(a, b, c, r, tmax, &t, &u, &v, tie)) {
           tmax = t;
           h.t = t;
           h.u = u;
@@ -668,8 +725,10 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
           h_inst = cur_inst;
         }
       }
+    } else if (base < 0) {
+      next = closest_step_near<LDS>(wn, node, r, tmax, st, d);
     } else {
-      next = closest_step<LDS>(wn, node, r, tmax, st, d);
+      next = closest_step<LDS>(wn, node, r, tmax, st, d, tie);
     }
     if (!next) {
       for (;;) {
@@ -684,7 +743,9 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         int pn;
         float en;
         stack_pop<LDS>(st, d, &pn, &en);
-        if (en < tmax) {
+        // (top-level entries and a tying instance's keep an entry equal to t_max)
+        if (en < tmax ||
+            (en == tmax && (base < 0 || (h_inst != kNone && cur_inst < h_inst)))) {
           node = pn;
           next = true;
           break;

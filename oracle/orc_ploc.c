@@ -438,10 +438,16 @@ void orc_brute_closest(const float *tri, uint32_t n_tri, const orc_ray *rays,
  * instance the ray is mapped into the mesh's space by the instance's
  * world->object map with per-row order ((m0 x + m1 y) + m2 z) + m3 and the
  * direction left unnormalised, then the mesh's tree is traversed with the
- * same t_max. Pinned by: an identity instance equals the one-level traversal
- * bit for bit; against a brute-force MT over the world-space copy of every
+ * same t_max. The closest hit is the one with the least (t, instance
+ * number), and within an instance the one-level traversal's: an instance
+ * numbered below the current hit's also accepts boxes and triangles at
+ * t == t_max until it records a hit, and top-level boxes are kept at
+ * entry == t_max, so the result does not depend on the top-level visit
+ * order (the device visits it nearest first, this restatement left first).
+ * Pinned by: an identity instance equals the one-level traversal bit for
+ * bit; against a brute-force MT over the world-space copy of every
  * instance, hit ids agree except at near-ties and t within a relative
- * tolerance. */
+ * tolerance; coincident instances resolve ties to the lower number. */
 typedef struct ctx2 {
   const orc_node *tlas;
   const uint32_t *tlas_perm;
@@ -463,15 +469,67 @@ static tray map_ray(const ctx2 *c, uint32_t inst, const tray *rw) {
   return make_tray(&rr);
 }
 
+/* slab and mt that also accept an entry / t equal to t_max when `incl` */
+static inline int slab_incl(const orc_node *nd, const tray *r, int incl) {
+  float lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    float a = nd->mn[k], b = nd->mx[k];
+    float near_plane = r->neg[k] ? b : a;
+    float far_plane = r->neg[k] ? a : b;
+    lo[k] = (near_plane - vget(r->o, k)) * vget(r->inv, k);
+    hi[k] = (far_plane - vget(r->o, k)) * vget(r->inv, k);
+  }
+  float tmin_max = fmax_r(fmax_r(lo[0], lo[1]), lo[2]);
+  float tmax_min = fmin_r(fmin_r(hi[0], hi[1]), hi[2]);
+  if (tmin_max > tmax_min) return 0;
+  return (tmin_max < r->tmax || (incl && tmin_max == r->tmax)) && (tmax_min > r->tmin);
+}
+static inline int mt_incl(const float *p, const tray *r, float best, float *t_out,
+                          float *u_out, float *v_out, int incl) {
+  float t, u, v;
+  tray rr = *r;
+  rr.tmax = INFINITY;
+  if (!mt(p, &rr, INFINITY, &t, &u, &v)) return 0;
+  if (!((t < r->tmax && t < best) || (incl && t == r->tmax && t == best))) return 0;
+  *t_out = t;
+  *u_out = u;
+  *v_out = v;
+  return 1;
+}
+/* IntersectAux of one instance's mesh; *tie: the instance may take a tie at
+ * t_max (cleared when it records a hit). */
+static int intersect_tie_aux(trav_ctx *c, tray *r, orc_hit *h, uint32_t ni, int *tie) {
+  const orc_node *nd = &c->nodes[ni];
+  if (!slab_incl(nd, r, *tie)) return 0;
+  if (nd->count != 0) {
+    uint32_t id = c->perm[nd->first];
+    float t, u, v;
+    if (mt_incl(c->tri + 9 * (size_t)id, r, h->t, &t, &u, &v, *tie)) {
+      h->t = t;
+      h->u = u;
+      h->v = v;
+      h->id = id;
+      r->tmax = t;
+      *tie = 0;
+      return 1;
+    }
+    return 0;
+  }
+  int left = intersect_tie_aux(c, r, h, nd->first + 0, tie);
+  int right = intersect_tie_aux(c, r, h, nd->first + 1, tie);
+  return left || right;
+}
+
 static int intersect2_aux(const ctx2 *c, tray *rw, orc_hit *h, uint32_t *hi, uint32_t ni) {
   const orc_node *nd = &c->tlas[ni];
-  if (!slab(nd, rw)) return 0;
+  if (!slab_incl(nd, rw, 1)) return 0;
   if (nd->count != 0) {
     const uint32_t inst = c->tlas_perm[nd->first];
     tray ro = map_ray(c, inst, rw);
     const orc_blas *b = &c->blas[c->inst_mesh[inst]];
     trav_ctx bc = {b->nodes, b->perm, b->tri, 0, 0};
-    if (intersect_aux(&bc, &ro, h, 0)) {
+    int tie = *hi != 0xFFFFFFFFu && inst < *hi;
+    if (intersect_tie_aux(&bc, &ro, h, 0, &tie)) {
       rw->tmax = ro.tmax;
       *hi = inst;
       return 1;

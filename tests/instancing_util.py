@@ -92,3 +92,45 @@ def world_soup(scene):
         owner.append(np.stack([np.full(t.shape[0], k), np.arange(t.shape[0])], axis=1))
     return (np.ascontiguousarray(np.concatenate(tris).astype(np.float32)),
             np.concatenate(owner))
+
+
+def _mesh_plus_far_triangle(prim, offset=(40.0, 25.0, -30.0)):
+    """The mesh with one more triangle far away, appended last: the same
+    triangles under the same numbers, a much larger box."""
+    pos, nrm, f = prim.positions, prim.normal_and_vs, prim.index_words
+    n = len(pos)
+    extra = pos[:3].copy()
+    extra[:, :3] += np.asarray(offset, np.float32)
+    return scenes.mesh(np.concatenate([pos, extra]), np.concatenate([nrm, nrm[:3]]),
+                       np.concatenate([f, np.array([[n, n + 1, n + 2]], f.dtype)]),
+                       prim.material, two_sided=prim.two_sided)
+
+
+def coincident_scene(seed=7, n_inst=30, dup_of=(3, 7, 7, 12, 20), front=False):
+    """small_instanced plus copies of instances `dup_of` under the same maps:
+    a copy of instance k uses either k's mesh or that mesh with one far
+    triangle appended (alternately), so copies tie at every hit on k while
+    their top-level boxes, and with them the visit order, differ. front:
+    the copies take the lowest numbers. Returns the scene and, per instance,
+    the number of the original it copies."""
+    import dataclasses
+    sc = small_instanced(seed=seed, n_inst=n_inst)
+    prims = list(sc.primitives)
+    ext = {}
+    extra = []
+    for j, k in enumerate(dup_of):
+        ins = sc.instances[k]
+        m = ins.mesh
+        if j % 2 == 1 and prims[m].kind == scenes.GEOM_MESH:
+            if m not in ext:
+                ext[m] = len(prims)
+                prims.append(_mesh_plus_far_triangle(prims[m]))
+            m = ext[m]
+        extra.append(dataclasses.replace(ins, mesh=m))
+    if front:
+        insts = extra + list(sc.instances)
+        orig = list(dup_of) + list(range(len(sc.instances)))
+    else:
+        insts = list(sc.instances) + extra
+        orig = list(range(len(sc.instances))) + list(dup_of)
+    return dataclasses.replace(sc, primitives=prims, instances=insts), np.array(orig)

@@ -10,7 +10,8 @@ import orc
 from ajisai_render_amd import scenes
 from ajisai_render_amd._capi import HIT_DTYPE
 from ajisai_render_amd.scenes import LIGHT_DIRECTIONAL, LIGHT_ENV, Light
-from instancing_util import mesh_tree, oracle_trace, small_instanced, tlas_tree, world_soup
+from instancing_util import (coincident_scene, mesh_tree, oracle_trace, small_instanced,
+                             tlas_tree, world_soup)
 from raygen import random_rays
 
 pytestmark = pytest.mark.gpu
@@ -63,6 +64,31 @@ def test_instanced_trace_bitexact(gpu_ctx, seed, n_inst):
         o = owner[bf["id"][hit]]
         same = (inst[:5000][hit] == o[:, 0]) & (h["tri_id"][:5000][hit] == o[:, 1])
         assert same.mean() > 0.99
+    g.close()
+
+
+@pytest.mark.parametrize("front", [False, True])
+def test_instanced_ties_bitexact(gpu_ctx, front):
+    """Coincident instances tie at every hit on them: the device visits the
+    top level nearest entry first and the oracle left first, and both return
+    the least (t, instance number) bit for bit."""
+    from ajisai_render_amd.core import GpuScene
+    print(f"case: coincident instances front={front}", flush=True)
+    sc, orig = coincident_scene(front=front)
+    g = GpuScene(gpu_ctx, sc)
+    w2o, boxes = g.instances()
+    wbox = np.concatenate([boxes[:, :3].min(0), boxes[:, 3:].max(0)])
+    rays = np.concatenate([random_rays(wbox, 20000, seed=21),
+                           random_rays(wbox * 0.3, 10000, seed=22)])
+    hd, idev = g.Intersect(_dev(rays))
+    h = _hits(hd)
+    inst = idev.cpu().numpy().view(np.uint32)
+    ref, rinst = oracle_trace(sc, w2o, boxes, rays)
+    hit = ref["id"] != 0xFFFFFFFF
+    tied = hit & (np.bincount(orig, minlength=len(orig))[orig[np.minimum(rinst, len(orig) - 1)]] > 1)
+    assert tied.sum() > 200, tied.sum()
+    assert h.tobytes() == ref.tobytes()
+    assert np.array_equal(inst, rinst)
     g.close()
 
 

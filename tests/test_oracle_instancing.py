@@ -5,8 +5,8 @@ the instances."""
 import numpy as np
 
 import orc
-from instancing_util import (host_instances, mesh_tree, oracle_trace, small_instanced,
-                             tlas_tree, world_soup)
+from instancing_util import (coincident_scene, host_instances, mesh_tree, oracle_trace,
+                             small_instanced, tlas_tree, world_soup)
 from raygen import random_rays
 
 
@@ -77,3 +77,31 @@ def test_mistakes_are_caught():
         hits, _ = oracle_trace(sc, bad.reshape(-1, 12), boxes, rays)
         agree = np.mean((hits["id"] != 0xFFFFFFFF) == (ref["id"] != 0xFFFFFFFF))
         assert agree < 0.99, agree
+
+
+def test_coincident_instances_resolve_to_the_lowest_number():
+    """Copies of an instance (same map; the same mesh or the mesh with a far
+    triangle appended) tie at every hit on it: the result is the
+    lowest-numbered copy, with t, u, v and triangle of the scene without
+    copies, whether the copies come first or last. The copies' larger boxes
+    put some of them ahead of their original in the top-level visit order,
+    so the first-found rule would pick them."""
+    sc = small_instanced(seed=7, n_inst=30)
+    w2o, boxes = host_instances(sc)
+    wbox = np.concatenate([boxes[:, :3].min(0), boxes[:, 3:].max(0)])
+    rays = random_rays(wbox, 4000, seed=5)
+    base, binst = oracle_trace(sc, w2o, boxes, rays)
+    dup = (3, 7, 7, 12, 20)
+    hit = base["id"] != 0xFFFFFFFF
+    assert (hit & np.isin(binst, dup)).sum() > 50
+    for front in (False, True):
+        sd, orig = coincident_scene(dup_of=dup, front=front)
+        w2, bx = host_instances(sd)
+        h, inst = oracle_trace(sd, w2, bx, rays)
+        # (a copy with the far triangle has its own tree: the same hit except
+        # where a ray meets a shared edge exactly)
+        same = h.tobytes() == base.tobytes() or np.mean(h == base) > 0.999
+        assert same
+        assert np.array_equal(orig[inst[hit]], binst[hit])
+        low = np.array([np.flatnonzero(orig == o).min() for o in binst[hit]])
+        assert np.array_equal(inst[hit], low)
