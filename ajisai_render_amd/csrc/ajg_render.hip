@@ -596,24 +596,19 @@ k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const De
 template <int SPILL>
 __global__ void __launch_bounds__(kTravThreads, 8)
 k_pq_any2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
-          uint8_t *occ, QIn qin) {
+          uint8_t *occ, QIn qin, uint32_t *cursor) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
   __shared__ uint32_t s_pre[kQStripes + 1];
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
-  const int lane = threadIdx.x & 63;
-  const uint32_t waves = gridDim.x * (kTravThreads / 64);
-  for (uint32_t c = (blockIdx.x * kTravThreads + threadIdx.x) >> 6; c * 64 < n; c += waves) {
-    const uint32_t v = c * 64 + lane;
-    if (v < n) {
-      const uint32_t i = q_entry(s_pre, qin.R, v);
-      const DevRay ray = load_ray(rays, i);
-      const TRay r = make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
-      occ[i] = trace_any_2l<SPILL, kLdsStack>(tlas, insts, blases, r, s_node + threadIdx.x,
-                                             s_entry + threadIdx.x) ? 1 : 0;
-    }
-  }
+  persistent_any_2l<SPILL, kLdsStack, kCursorStripes>(
+      tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
+      [&](uint32_t v) {
+        const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
+        return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
+      },
+      [&](uint32_t v, bool hit) { occ[q_entry(s_pre, qin.R, v)] = hit ? 1 : 0; });
 }
 
 // C-ABI scene traces of instanced scenes (rays in the ajg_ray layout).
@@ -678,10 +673,10 @@ static void launch_pq2(int stack, const DevScene &ds, bool closest, const DevRay
   } else {
     if (stack == kSpillS)
       hipLaunchKernelGGL((k_pq_any2<kSpillS>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
-                         (uint8_t *)out, count);
+                         (uint8_t *)out, count, cursor);
     else
       hipLaunchKernelGGL((k_pq_any2<kSpillL>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
-                         (uint8_t *)out, count);
+                         (uint8_t *)out, count, cursor);
   }
 }
 
@@ -2619,7 +2614,7 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
         hipStream_t sa = rb->aux;
         timer.begin(1, sa);
         if (inst)
-          launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, aq_in, nullptr,
+          launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, aq_in, a_cursor,
                      trav_grid, sa);
         else if (counting)
           launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,

@@ -391,8 +391,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
 // cursor, so lanes stay busy whatever the spread of path lengths within a
 // wave. Each ray's traversal (order, culling, arithmetic) is the same as
 // trace_closest, so results are unchanged. The queue kernels' any-hit
-// traversal (persistent_any below) refills the same way; the instanced
-// any-hit kernel keeps a plain 64-ray chunk per wave.
+// traversals (persistent_any, persistent_any_2l below) refill the same way.
 static constexpr int kRefillMin = 48;
 
 // Wave-uniform refill: idle lanes take consecutive batch indices. Returns the
@@ -755,6 +754,103 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
     if (!next) {
       store(ray, h, h_inst);
       ray = kNone;
+    }
+  }
+}
+
+// Any hit over rays [0, n) of a two-level scene with lane refill (as
+// persistent_any): per step a lane enters an instance, tests a leaf or
+// visits a node, in trace_any_2l's order; leaving a mesh's tree reloads the
+// world ray (load(i)). store(i, occluded).
+template <int SPILL, int LDS, uint32_t STRIPES = 1, class Load, class Store>
+__device__ __forceinline__ void persistent_any_2l(const DevBVH &tlas, const DevInst *insts,
+                                                  const DevBlas *blases, uint32_t n,
+                                                  uint32_t *cursor, int *lds_node,
+                                                  float *lds_entry, Load load, Store store) {
+  int sn[SPILL];
+  float se[SPILL];
+  const StackRef st = {lds_node, lds_entry, sn, se};
+  int d = 0, node = 0, base = -1;
+  TRay r;
+  const WideNode *wn = tlas.wide;
+  const float4 *leaf = tlas.leaf_tris;
+  uint32_t ray = kNone;
+  bool drained = false;
+  for (;;) {
+    const bool idle = ray == kNone;
+    if (!drained && __popcll(__ballot(idle)) >= kRefillMin) {
+      const uint32_t idx = refill<STRIPES>(idle, n, cursor, &drained);
+      if (idx != kNone) {
+        r = load(idx);
+        d = 0;
+        base = -1;
+        wn = tlas.wide;
+        leaf = tlas.leaf_tris;
+        float e;
+        if (slab_static(r, tlas.root_mn[0], tlas.root_mn[1], tlas.root_mn[2], tlas.root_mx[0],
+                        tlas.root_mx[1], tlas.root_mx[2], &e) &&
+            e < r.tmax) {
+          node = tlas.root_ref;
+          ray = idx;
+        } else {
+          store(idx, false);
+        }
+      }
+      continue;
+    }
+    if (__ballot(ray != kNone) == 0ull) {
+      if (drained) break;
+      continue;
+    }
+    if (ray == kNone) continue;
+    bool next = false;
+    if (node < 0) {
+      if (base < 0) {
+        f3 ia, ib, ic;  // the instance box as its degenerate leaf triangle
+        load_leaf(tlas.leaf_tris, ~node, &ia, &ib, &ic);
+        const DevInst *I = insts + tlas_leaf_instance(tlas, ~node);
+        const DevBlas *B = blases + I->info.x;
+        const TRay ro = make_tray(xform_point(I->w2o, r.o), xform_vec(I->w2o, r.d), r.tmin,
+                                  r.tmax);
+        float e;
+        if (leaf_box_test(r, ia, ib, ic, r.tmax) &&
+            slab_static(ro, B->bvh.root_mn[0], B->bvh.root_mn[1], B->bvh.root_mn[2],
+                        B->bvh.root_mx[0], B->bvh.root_mx[1], B->bvh.root_mx[2], &e) &&
+            e < r.tmax) {
+          r = ro;
+          base = d;
+          wn = B->bvh.wide;
+          leaf = B->bvh.leaf_tris;
+          node = B->bvh.root_ref;
+          next = true;
+        }
+      } else {
+        f3 a, b, c;
+        load_leaf(leaf, ~node, &a, &b, &c);
+        float t, u, v;
+        if (leaf_box_test(r, a, b, c, r.tmax) && mt_test(a, b, c, r, r.tmax, &t, &u, &v)) {
+          store(ray, true);
+          ray = kNone;
+          continue;
+        }
+      }
+    } else {
+      next = any_step<LDS>(wn, node, r, r.tmax, st, d);
+    }
+    if (!next) {
+      if (d == base) {
+        base = -1;
+        r = load(ray);
+        wn = tlas.wide;
+        leaf = tlas.leaf_tris;
+      }
+      if (d == 0) {
+        store(ray, false);
+        ray = kNone;
+        continue;
+      }
+      float en;
+      stack_pop<LDS>(st, d, &node, &en);
     }
   }
 }
