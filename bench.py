@@ -453,7 +453,9 @@ def batched_bench(ctx, a):
 
 
 def instanced_bench(ctx, a):
-    """One full render of the large instanced scene."""
+    """One full render of the large instanced scene with per-sample streams
+    (seed_mode 1, `--slots` paths in flight), and one with the reference's
+    per-pixel sequential streams (seed_mode 0, one slot per traced pixel)."""
     import torch
     from ajisai_render_amd import _capi as capi
     from ajisai_render_amd import scenes
@@ -467,18 +469,22 @@ def instanced_bench(ctx, a):
     g = GpuScene(ctx, sc)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t
-    pt = PathTracer(g, seed_mode=0, flags=capi.RENDER_TIMING)
     w, h = sc.camera.res
-    wp, hp = pt.traced_dims()
-    warm = PathTracer(g, spp=1, seed_mode=0)
-    film, _ = warm.render_device()
-    film, _ = pt.render_device()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    _, st = pt.render_device(film=film)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t
     paths = w * h * sc.pt.spp
+    warm = PathTracer(g, spp=1, seed_mode=0)
+    warm.render_device()
+
+    def timed(pt):
+        film, _ = pt.render_device()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, st = pt.render_device(film=film)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, st
+
+    wall, st = timed(PathTracer(g, seed_mode=1, max_slots=a.slots, flags=capi.RENDER_TIMING))
+    wall0, st0 = timed(PathTracer(g, seed_mode=0, flags=capi.RENDER_TIMING))
+    wp, hp = PathTracer(g, seed_mode=0).traced_dims()
     q = st.closest_queries + st.any_queries
     out = {
         "instances": len(sc.instances),
@@ -489,6 +495,8 @@ def instanced_bench(ctx, a):
         "traced_pixels": wp * hp,
         "spp": sc.pt.spp,
         "max_bounces": sc.pt.max_bounces,
+        "seed_mode": 1,
+        "slots": a.slots,
         "seconds": round(wall, 3),
         "paths_per_s": round(paths / wall, 1),
         "mrays_per_s": round(q / wall / 1e6, 2),
@@ -496,6 +504,11 @@ def instanced_bench(ctx, a):
         "stage_ms": {"trace_closest": round(st.ms_trace_closest, 2),
                      "trace_any": round(st.ms_trace_any, 2), "shade": round(st.ms_shade, 2)},
         "iterations": st.iterations,
+        "seed0": {"seconds": round(wall0, 3), "paths_per_s": round(paths / wall0, 1),
+                  "stage_ms": {"trace_closest": round(st0.ms_trace_closest, 2),
+                               "trace_any": round(st0.ms_trace_any, 2),
+                               "shade": round(st0.ms_shade, 2)},
+                  "iterations": st0.iterations},
         "scene_generation_s": round(t_gen, 2),
         "scene_build_s": round(t_build, 3),
     }
