@@ -569,16 +569,18 @@ k_pq_any_b(const BatchJob *jobs, const uint16_t *job_of, const DevRay *rays, uin
 
 // Instanced scenes: two-level traversal (top-level tree over instances).
 // (6 waves per SIMD: the register budget that measured fastest; 8 spills)
-template <int SPILL>
+template <int SPILL, bool COUNT>
 __global__ void __launch_bounds__(kTravThreads, 6)
 k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
-              float4 *hits, uint32_t *hit_inst, QIn qin, uint32_t *cursor) {
+              float4 *hits, uint32_t *hit_inst, QIn qin, uint32_t *cursor,
+              unsigned long long *counters) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
   __shared__ uint32_t s_pre[kQStripes + 1];
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
-  persistent_closest_2l<SPILL, kLdsStack, kCursorStripes>(
+  uint32_t nn = 0, nt = 0;
+  persistent_closest_2l<SPILL, kLdsStack, kCursorStripes, COUNT>(
       tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
       [&](uint32_t v) {
         const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
@@ -588,27 +590,37 @@ k_pq_closest2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const De
         const uint32_t i = q_entry(s_pre, qin.R, v);
         hits[i] = make_float4(h.t, __int_as_float(h.slot), h.u, h.v);
         hit_inst[i] = inst;
-      });
+      },
+      &nn, &nt);
+  if (COUNT) {
+    wave_count_add(nn, counters);
+    wave_count_add(nt, counters + 1);
+  }
 }
 
 // (8 waves per SIMD measured faster for this kernel than the compiler's
 // default register budget)
-template <int SPILL>
+template <int SPILL, bool COUNT>
 __global__ void __launch_bounds__(kTravThreads, 8)
 k_pq_any2(DevBVH tlas, const DevInst *insts, const DevBlas *blases, const DevRay *rays,
-          uint8_t *occ, QIn qin, uint32_t *cursor) {
+          uint8_t *occ, QIn qin, uint32_t *cursor, unsigned long long *counters) {
   __shared__ int s_node[kLdsStack * kTravThreads];
   __shared__ float s_entry[kLdsStack * kTravThreads];
   __shared__ uint32_t s_pre[kQStripes + 1];
   const uint32_t n = q_prefix(qin, s_pre);
   if (n == 0) return;
-  persistent_any_2l<SPILL, kLdsStack, kCursorStripes>(
+  uint32_t nn = 0, nt = 0;
+  persistent_any_2l<SPILL, kLdsStack, kCursorStripes, COUNT>(
       tlas, insts, blases, n, cursor, s_node + threadIdx.x, s_entry + threadIdx.x,
       [&](uint32_t v) {
         const DevRay ray = load_ray(rays, q_entry(s_pre, qin.R, v));
         return make_tray(ray.o, ray.d, ray.tmin, ray.tmax);
       },
-      [&](uint32_t v, bool hit) { occ[q_entry(s_pre, qin.R, v)] = hit ? 1 : 0; });
+      [&](uint32_t v, bool hit) { occ[q_entry(s_pre, qin.R, v)] = hit ? 1 : 0; }, &nn, &nt);
+  if (COUNT) {
+    wave_count_add(nn, counters);
+    wave_count_add(nt, counters + 1);
+  }
 }
 
 // C-ABI scene traces of instanced scenes (rays in the ajg_ray layout).
@@ -659,24 +671,25 @@ k_one_level_instance(const ajg_hit *hits, uint32_t n, uint32_t *inst) {
   if (i < n) inst[i] = hits[i].tri_id == kNone ? kNone : 0u;
 }
 
+template <bool COUNT>
 static void launch_pq2(int stack, const DevScene &ds, bool closest, const DevRay *rays, void *out,
-                       uint32_t *hit_inst, QIn count, uint32_t *cursor, uint32_t grid,
-                       hipStream_t s) {
+                       uint32_t *hit_inst, QIn count, uint32_t *cursor,
+                       unsigned long long *ctr, uint32_t grid, hipStream_t s) {
   const dim3 g(grid), b(kTravThreads);
   if (closest) {
     if (stack == kSpillS)
-      hipLaunchKernelGGL((k_pq_closest2<kSpillS>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
-                         (float4 *)out, hit_inst, count, cursor);
+      hipLaunchKernelGGL((k_pq_closest2<kSpillS, COUNT>), g, b, 0, s, ds.bvh, ds.inst, ds.blas,
+                         rays, (float4 *)out, hit_inst, count, cursor, ctr);
     else
-      hipLaunchKernelGGL((k_pq_closest2<kSpillL>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
-                         (float4 *)out, hit_inst, count, cursor);
+      hipLaunchKernelGGL((k_pq_closest2<kSpillL, COUNT>), g, b, 0, s, ds.bvh, ds.inst, ds.blas,
+                         rays, (float4 *)out, hit_inst, count, cursor, ctr);
   } else {
     if (stack == kSpillS)
-      hipLaunchKernelGGL((k_pq_any2<kSpillS>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
-                         (uint8_t *)out, count, cursor);
+      hipLaunchKernelGGL((k_pq_any2<kSpillS, COUNT>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+                         (uint8_t *)out, count, cursor, ctr);
     else
-      hipLaunchKernelGGL((k_pq_any2<kSpillL>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
-                         (uint8_t *)out, count, cursor);
+      hipLaunchKernelGGL((k_pq_any2<kSpillL, COUNT>), g, b, 0, s, ds.bvh, ds.inst, ds.blas, rays,
+                         (uint8_t *)out, count, cursor, ctr);
   }
 }
 
@@ -2547,8 +2560,6 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree too deep for the traversal stack"));
 
   const bool counting = (args->flags & AJG_RENDER_COUNT) != 0;
-  if (inst && counting)
-    return done(fail(ctx, AJG_ERR_INVALID, "render: AJG_RENDER_COUNT is not supported for instanced scenes"));
   StageTimer timer;
   timer.on = (args->flags & AJG_RENDER_TIMING) != 0;
   // traversal counters: closest (nodes, tris) at +8, any (nodes, tris) at +12
@@ -2600,9 +2611,12 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       AJG_RCHK(hipEventRecord(rb->ev_fork, s));
       AJG_RCHK(hipStreamWaitEvent(rb->aux, rb->ev_fork, 0));
       timer.begin(0, s);
-      if (inst)
-        launch_pq2(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, c_cursor,
-                   trav_grid, s);
+      if (inst && counting)
+        launch_pq2<true>(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, c_cursor,
+                         trav_ctr, trav_grid, s);
+      else if (inst)
+        launch_pq2<false>(stack, ds, true, c_rays[nxt], c_hits, rb->c_inst, cq_in, c_cursor,
+                          trav_ctr, trav_grid, s);
       else if (counting)
         launch_pq<true>(stack, ds.bvh, true, c_rays[nxt], c_hits, cq_in, c_cursor,
                         trav_ctr, trav_grid, s);
@@ -2613,9 +2627,12 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       {
         hipStream_t sa = rb->aux;
         timer.begin(1, sa);
-        if (inst)
-          launch_pq2(stack, ds, false, a_rays[nxt], a_occ, nullptr, aq_in, a_cursor,
-                     trav_grid, sa);
+        if (inst && counting)
+          launch_pq2<true>(stack, ds, false, a_rays[nxt], a_occ, nullptr, aq_in, a_cursor,
+                           trav_ctr + 2, trav_grid, sa);
+        else if (inst)
+          launch_pq2<false>(stack, ds, false, a_rays[nxt], a_occ, nullptr, aq_in, a_cursor,
+                            trav_ctr + 2, trav_grid, sa);
         else if (counting)
           launch_pq<true>(stack, ds.bvh, false, a_rays[nxt], a_occ, aq_in,
                           a_cursor, trav_ctr + 2, trav_grid, sa);

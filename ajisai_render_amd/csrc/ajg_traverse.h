@@ -636,11 +636,16 @@ __device__ __forceinline__ uint32_t tlas_leaf_instance(const DevBVH &tlas, int s
 // tree is traversed: leaving the tree loads it again (load(i), the same
 // operations as at the start), which keeps six registers per lane out of
 // the live state.
-template <int SPILL, int LDS, uint32_t STRIPES = 1, class Load, class Store>
+// COUNT: *nn += interior node fetches (both levels) and instance visits
+// (an instance's map and mesh record), *nt += leaf triangles tested and
+// instance boxes.
+template <int SPILL, int LDS, uint32_t STRIPES = 1, bool COUNT = false, class Load, class Store>
 __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const DevInst *insts,
                                                       const DevBlas *blases, uint32_t n,
                                                       uint32_t *cursor, int *lds_node,
-                                                      float *lds_entry, Load load, Store store) {
+                                                      float *lds_entry, Load load, Store store,
+                                                      uint32_t *nn = nullptr,
+                                                      uint32_t *nt = nullptr) {
   int sn[SPILL];
   float se[SPILL];
   const StackRef st = {lds_node, lds_entry, sn, se};
@@ -692,6 +697,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
     const bool tie = base >= 0 && h_inst != kNone && cur_inst < h_inst;
     if (node < 0) {
       if (base < 0) {  // an instance: enter its mesh's tree in object space
+        if (COUNT) ++*nn, ++*nt;
         f3 ia, ib, ic;  // the instance box as its degenerate leaf triangle
         load_leaf(tlas.leaf_tris, ~node, &ia, &ib, &ic);
         cur_inst = tlas_leaf_instance(tlas, ~node);
@@ -712,6 +718,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
           next = true;
         }
       } else {
+        if (COUNT) ++*nt;
         f3 a, b, c;
         load_leaf(leaf, ~node, &a, &b, &c);
         float t, u, v;
@@ -725,8 +732,10 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
         }
       }
     } else if (base < 0) {
+      if (COUNT) ++*nn;
       next = closest_step_near<LDS>(wn, node, r, tmax, st, d);
     } else {
+      if (COUNT) ++*nn;
       next = closest_step<LDS>(wn, node, r, tmax, st, d, tie);
     }
     if (!next) {
@@ -762,11 +771,12 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
 // persistent_any): per step a lane enters an instance, tests a leaf or
 // visits a node, in trace_any_2l's order; leaving a mesh's tree reloads the
 // world ray (load(i)). store(i, occluded).
-template <int SPILL, int LDS, uint32_t STRIPES = 1, class Load, class Store>
+template <int SPILL, int LDS, uint32_t STRIPES = 1, bool COUNT = false, class Load, class Store>
 __device__ __forceinline__ void persistent_any_2l(const DevBVH &tlas, const DevInst *insts,
                                                   const DevBlas *blases, uint32_t n,
                                                   uint32_t *cursor, int *lds_node,
-                                                  float *lds_entry, Load load, Store store) {
+                                                  float *lds_entry, Load load, Store store,
+                                                  uint32_t *nn = nullptr, uint32_t *nt = nullptr) {
   int sn[SPILL];
   float se[SPILL];
   const StackRef st = {lds_node, lds_entry, sn, se};
@@ -806,6 +816,7 @@ __device__ __forceinline__ void persistent_any_2l(const DevBVH &tlas, const DevI
     bool next = false;
     if (node < 0) {
       if (base < 0) {
+        if (COUNT) ++*nn, ++*nt;
         f3 ia, ib, ic;  // the instance box as its degenerate leaf triangle
         load_leaf(tlas.leaf_tris, ~node, &ia, &ib, &ic);
         const DevInst *I = insts + tlas_leaf_instance(tlas, ~node);
@@ -825,6 +836,7 @@ __device__ __forceinline__ void persistent_any_2l(const DevBVH &tlas, const DevI
           next = true;
         }
       } else {
+        if (COUNT) ++*nt;
         f3 a, b, c;
         load_leaf(leaf, ~node, &a, &b, &c);
         float t, u, v;
@@ -835,6 +847,7 @@ __device__ __forceinline__ void persistent_any_2l(const DevBVH &tlas, const DevI
         }
       }
     } else {
+      if (COUNT) ++*nn;
       next = any_step<LDS>(wn, node, r, r.tmax, st, d);
     }
     if (!next) {

@@ -484,6 +484,30 @@ def instanced_bench(ctx, a):
 
     wall, st = timed(PathTracer(g, seed_mode=1, max_slots=a.slots, flags=capi.RENDER_TIMING))
     wall0, st0 = timed(PathTracer(g, seed_mode=0, flags=capi.RENDER_TIMING))
+    stages = None
+    if not a.no_count:
+        # algorithmic bytes from an untimed counting pass of the same render
+        _, cs = PathTracer(g, seed_mode=1, max_slots=a.slots,
+                           flags=capi.RENDER_COUNT).render_device()
+        cq, aq = st.closest_queries, st.any_queries
+        b_closest = (cq * (RAY_BYTES + HIT_BYTES + 4) + cs.nodes_fetched * NODE_BYTES +
+                     cs.tris_tested * LEAF_TRI_BYTES)
+        b_any = (aq * (RAY_BYTES + OCC_BYTES) + cs.nodes_fetched_any * NODE_BYTES +
+                 cs.tris_tested_any * LEAF_TRI_BYTES)
+        stages = {
+            "trace_closest": stage_roofline(
+                "k_pq_closest2", b_closest, st.ms_trace_closest, st.launches_closest,
+                {"nodes_per_query": round(cs.nodes_fetched / max(1, cq), 3),
+                 "tris_per_query": round(cs.tris_tested / max(1, cq), 3)}),
+            "trace_any": stage_roofline(
+                "k_pq_any2", b_any, st.ms_trace_any, st.launches_any,
+                {"nodes_per_query": round(cs.nodes_fetched_any / max(1, aq), 3),
+                 "tris_per_query": round(cs.tris_tested_any / max(1, aq), 3)}),
+            "byte_model": ("per query: 32 B ray + 16 B hit + 4 B instance (any-hit: 1 B) + 64 B "
+                           "per node fetch + 48 B per leaf triangle; an instance visit counts "
+                           "as one node (its 64 B map and mesh record) and one triangle (its "
+                           "48 B box); counts from an untimed AJG_RENDER_COUNT pass"),
+        }
     wp, hp = PathTracer(g, seed_mode=0).traced_dims()
     q = st.closest_queries + st.any_queries
     out = {
@@ -504,6 +528,7 @@ def instanced_bench(ctx, a):
         "stage_ms": {"trace_closest": round(st.ms_trace_closest, 2),
                      "trace_any": round(st.ms_trace_any, 2), "shade": round(st.ms_shade, 2)},
         "iterations": st.iterations,
+        "stage_roofline": stages,
         "seed0": {"seconds": round(wall0, 3), "paths_per_s": round(paths / wall0, 1),
                   "stage_ms": {"trace_closest": round(st0.ms_trace_closest, 2),
                                "trace_any": round(st0.ms_trace_any, 2),

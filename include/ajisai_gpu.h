@@ -225,7 +225,10 @@ typedef struct ajg_pt_args {
 /* ajg_pt_args.flags */
 #define AJG_RENDER_TIMING 1u   /* per-stage device times via HIP events */
 #define AJG_RENDER_COUNT 2u    /* count node fetches / triangles per query (slower;
-                                  for the byte model, not for timing) */
+                                  for the byte model, not for timing). In
+                                  instanced scenes an instance visit counts
+                                  as one node fetch (its map and mesh record)
+                                  and one triangle (its box) */
 #define AJG_RENDER_FINITE_PDF 4u /* end a path whose continuation sample has
                                   pdf 0 (the reference continues it with an
                                   infinite throughput, pt.cpp:216-221; guided

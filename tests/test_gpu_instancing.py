@@ -145,6 +145,24 @@ def test_instanced_render_matches_world_copy(gpu_ctx):
     gf.close()
 
 
+def test_instanced_render_counts_without_changing_the_image(gpu_ctx):
+    """AJG_RENDER_COUNT on an instanced scene: the same film bit for bit,
+    and counts of node fetches and triangles (instance visits included) for
+    both ray kinds."""
+    from ajisai_render_amd import _capi as capi
+    from ajisai_render_amd.core import GpuScene, PathTracer
+    print("case: instanced render with counting", flush=True)
+    sc = _lit(small_instanced(seed=5, n_inst=40))
+    g = GpuScene(gpu_ctx, sc)
+    plain = PathTracer(g, spp=4, seed_mode=1, seed_base=3).Render()
+    cnt = PathTracer(g, spp=4, seed_mode=1, seed_base=3, flags=capi.RENDER_COUNT).Render()
+    assert plain["sum"].tobytes() == cnt["sum"].tobytes()
+    st = cnt["stats"]
+    assert st.nodes_fetched > st.closest_queries > 0 and st.tris_tested > 0
+    assert st.nodes_fetched_any > 0 and st.tris_tested_any > 0
+    g.close()
+
+
 def test_instanced_guided_render_runs_unbiased(gpu_ctx):
     """Guided and plain renders of an instanced scene under the full
     lighting of the large instanced configuration (HDR env with a 1e4 sun
