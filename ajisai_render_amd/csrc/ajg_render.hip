@@ -176,8 +176,10 @@ struct Queues {
   uint32_t *a_count_out;
   uint32_t a_len;
   uint32_t *live;
-  uint32_t *work_counter;
+  uint32_t *work_counter;  // work_stripes counter words, kQStride apart
+  uint32_t work_stripes;   // 1 or kWStripes
   uint32_t first_static;  // a slot's first work item is its own index (next_sample)
+  uint32_t work_static;   // first_static: items [0, work_static) are the slots' first
   float *partial;      // total_work x 6 (sum3, sq3) or the film when n_chunks == 1
   uint32_t *partial_n;
   float *per_sample;   // optional
@@ -341,7 +343,17 @@ static constexpr uint32_t kQWords = kQStripes * kQStride;
 // Fetch-cursor stripes of the closest-hit queue kernels (lane refill); the
 // cursor words follow the four queue counter blocks, kCursorStride apart.
 static constexpr uint32_t kCursorStripes = 16;
-static constexpr uint32_t kQctrWords = 4 * kQWords + 2 * kCursorStripes * kCursorStride;
+// Work-item counter stripes follow the cursors, kQStride apart (see
+// next_sample); they are cleared once per render, not per iteration.
+static constexpr uint32_t kWStripes = 32;
+static constexpr uint32_t kWorkOff = 4 * kQWords + 2 * kCursorStripes * kCursorStride;
+static constexpr uint32_t kQctrWords = kWorkOff + kWStripes * kQStride;
+// A stripe's items are taken only by its own waves (wave w takes from
+// stripe w % stripes), so every stripe needs a wave: at most one stripe
+// per wave of slots.
+static inline uint32_t work_stripes_for(uint32_t slots) {
+  return std::max(1u, std::min(kWStripes, (slots + 63) / 64));
+}
 
 // Entries one stripe holds: every wave of the stripe appending per_slot
 // entries for each of its 64 lanes.
@@ -469,6 +481,9 @@ static constexpr uint32_t kMinTravGrid = (kCursorStripes * 64 + kTravThreads - 1
 static_assert(kMinTravGrid * (kTravThreads / 64) >= kCursorStripes, "traversal grid covers every stripe");
 static inline uint32_t trav_grid_size(uint32_t blocks) { return std::max(blocks, kMinTravGrid); }
 static constexpr int kItersPerCheck = 4;
+// Work items per path slot aimed at by a plain seed_mode-1 render's
+// automatic sample chunking (setup_params).
+static constexpr uint64_t kItemsPerSlot = 8;
 
 template <int SPILL, bool COUNT>
 __global__ void __launch_bounds__(kTravThreads)
@@ -931,23 +946,38 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   const bool fetch = need && work == kNone;
   // A slot's first work item is its own index: every slot fetches in the
   // render's first shade (its sample_end is still 0), and one atomic per
-  // wave on the single work counter would hold S/64 waves in line; the
-  // counter starts at min(S, total) instead. Later fetches take the next
-  // items from the counter. Once it has passed the last item no wave takes
-  // another, and every slot that finishes near the end of a render would
-  // otherwise add one more atomic on this word. (A plain read: the counter
-  // only grows, so a stale value only costs the atomic.)
+  // wave would hold S/64 waves in line; items [0, work_static) are taken
+  // that way. Later fetches take the next items from a counter. With
+  // work_stripes > 1 the counter is striped like the fetch cursors of the
+  // traversal kernels: wave w takes from stripe s = w % work_stripes,
+  // whose value j stands for item work_static + (s + stripes (j / 64)) 64
+  // + j % 64, so each stripe hands out every stripes-th 64-item block in
+  // order (atomics on one word are served one at a time, and with several
+  // work items per slot every wave fetches many times). A stripe whose
+  // items are used up stops taking atomics (a plain read first: the
+  // counter only grows, so a stale value only costs the atomic). A lane
+  // whose fetch finds its stripe used up stays idle for the rest of the
+  // render; the stripe's other waves finish its items.
   const bool first = q.first_static && fetch && ps.sample_end[slot] == 0u;
   const uint32_t limit = BATCH ? B.total_work : P0.total_work;
-  const bool open = __ballot(fetch && !first) != 0ull && *q.work_counter < limit;
-  uint32_t w = wave_append(fetch && !first && open, q.work_counter);
+  const uint32_t ws = q.work_stripes;
+  const uint32_t wst = ws > 1 ? (slot >> 6) % ws : 0u;
+  uint32_t *const wctr = q.work_counter + wst * kQStride;
+  auto item_at = [&](uint32_t j) -> uint64_t {
+    const uint64_t base = q.first_static ? q.work_static : 0u;
+    return ws > 1 ? base + (((uint64_t)wst + (uint64_t)ws * (j >> 6)) << 6) + (j & 63u)
+                  : base + j;
+  };
+  const bool open = __ballot(fetch && !first) != 0ull && item_at(*wctr) < limit;
+  const uint32_t jw = wave_append(fetch && !first && open, wctr);
+  uint64_t w = jw == kNone ? (uint64_t)limit : item_at(jw);
   if (first) w = slot;
   if (fetch) {
-    if (w < (BATCH ? B.total_work : P0.total_work)) {
-      uint32_t local = w;
+    if (w < (uint64_t)limit) {
+      uint32_t local = (uint32_t)w;
       if (BATCH) {
-        job = find_job(B, w);
-        local = w - B.job_begin[job];
+        job = find_job(B, (uint32_t)w);
+        local = (uint32_t)w - B.job_begin[job];
       }
       const RenderParams &P = BATCH ? B.jobs[job].P : P0;
       work = local;
@@ -2429,7 +2459,14 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   const ajg_accel *a = sc->accel;
   const uint64_t target = args->max_slots > 0 ? (uint64_t)args->max_slots : (1ull << 20);
   RenderParams P;
-  if (int st = setup_params(ctx, camera, args, target, 0, &P)) return st;
+  // seed_mode 1: about kItemsPerSlot work items per slot, so a slot that
+  // finishes early takes more work (with one item per slot most slots sat
+  // idle through the render's last iterations while the slots with the
+  // longest sample sequences finished: heavy mesh, 8M slots, 76 -> 36
+  // iterations, 433 -> 485 M paths/s); guided renders, whose items come
+  // from one counter, keep one per slot
+  const uint64_t items = guided ? target : kItemsPerSlot * target;
+  if (int st = setup_params(ctx, camera, args, items, 0, &P)) return st;
   const uint64_t total = P.total_work;
   P.slots = (uint32_t)std::min<uint64_t>(target, total);
   const uint32_t S = P.slots;
@@ -2534,8 +2571,6 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   // (their training slots are every train_stride-th slot, and a slot's
   // first item equal to its index gave the guide a markedly worse fit)
   const bool first_static = !guided;
-  if (first_static) fl.words(counters + 5, 1, (uint32_t)std::min<uint64_t>(S, total));
-  AJG_RCHK(fl.launch(s));
   unsigned long long rec_before = 0;
   if (guided) {
     uint32_t rc[4] = {0, 0, 0, 0};
@@ -2550,8 +2585,11 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   q.c_inst_in = rb->c_inst;
   q.a_occ_in = a_occ;
   q.live = counters + 4;
-  q.work_counter = counters + 5;
+  // guided renders take every item from one counter, in order
+  q.work_counter = rb->qctr + kWorkOff;
+  q.work_stripes = guided ? 1u : work_stripes_for(S);
   q.first_static = first_static ? 1u : 0u;
+  q.work_static = first_static ? (uint32_t)std::min<uint64_t>(S, total) : 0u;
   q.partial = partial;
   q.partial_n = partial_n;
   q.per_sample = film->per_sample;
@@ -2579,9 +2617,9 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   uint32_t *const c_cursor = rb->qctr + 4 * kQWords;
   uint32_t *const a_cursor = c_cursor + kCursorStripes * kCursorStride;
   const uint32_t trav_grid = trav_grid_size((uint32_t)std::max(1, sc->cu_count) * kTravBlocksPerCU);
-  // counter words: 4 live flag, 5 work items handed out, 6 closest-queue
-  // total for the guide, 8-15 u64 traversal counts, 16-19 u64 query totals
-  // (closest, shadow); the queue counters and fetch cursors are in rb->qctr.
+  // counter words: 4 live flag, 6 closest-queue total for the guide, 8-15
+  // u64 traversal counts, 16-19 u64 query totals (closest, shadow); the
+  // queue counters, fetch cursors and work-item counters are in rb->qctr.
   unsigned long long *totals = (unsigned long long *)(counters + 16);
   for (;;) {
     for (int k = 0; k < kItersPerCheck; ++k) {
@@ -2852,7 +2890,9 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
     if (J.scene->insts)
       return fail(ctx, AJG_ERR_INVALID, "batch: instanced scenes render with ajg_render_pt");
     RenderParams P;
-    if (int st = setup_params(ctx, &J.camera, &J.args, target, (uint32_t)ba.samples_per_item, &P))
+    // automatic chunking as a lone render with the same slot target picks
+    if (int st = setup_params(ctx, &J.camera, &J.args, kItemsPerSlot * target,
+                              (uint32_t)ba.samples_per_item, &P))
       return st;
     const uint32_t T = (uint32_t)ba.pixel_tile;
     const uint32_t Hp = P.n_pix / P.Wp;
@@ -2917,8 +2957,7 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   AJG_BCHK(hipMemcpyAsync(bb->jobs, hj.data(), sizeof(BatchJob) * n_jobs, hipMemcpyHostToDevice, s));
   AJG_BCHK(hipMemcpyAsync(bb->job_begin, begin.data(), 4 * ((size_t)n_jobs + 1), hipMemcpyHostToDevice, s));
   AJG_BCHK(hipMemcpyAsync(bb->remaining, work_of.data(), 4 * (size_t)n_jobs, hipMemcpyHostToDevice, s));
-  // clears as in a single render (one launch per table), the work counter
-  // starting at min(S, total)
+  // clears as in a single render (one launch per table)
   {
     Fills fl;
     fl.words(bb->clocks, 2ull * ((size_t)n_jobs + 1));
@@ -2949,8 +2988,6 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       if (f.per_sample) fl.words(f.per_sample, 3ull * np * hj[j].P.spp);
     }
     AJG_BCHK(fl.launch(s));
-    fl.words(rb->counters + 5, 1, (uint32_t)std::min<uint64_t>(S, total));
-    AJG_BCHK(fl.launch(s));
   }
 
   uint32_t *counters = rb->counters;  // word layout as in a single render
@@ -2976,8 +3013,10 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   q.c_inst_in = rb->c_inst;
   q.a_occ_in = rb->a_occ;
   q.live = counters + 4;
-  q.work_counter = counters + 5;
+  q.work_counter = rb->qctr + kWorkOff;
+  q.work_stripes = work_stripes_for(S);
   q.first_static = 1u;
+  q.work_static = S;
   const uint32_t shade_grid = cdiv(S, 256);
   // queue stripes sized for the most any job can append
   const uint32_t c_len = stripe_len(S, kCqPerSlot), a_len = stripe_len(S, kMaxLights + 1);

@@ -144,8 +144,8 @@ int ajg_accel_trace_closest(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
 int ajg_accel_trace_any(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
                         uint8_t *occluded, void *stream);
 /* Counting pass of trace_closest for the byte model: per ray, the number of
- * traversal nodes fetched (128-byte wide nodes, each covering up to two
- * tree levels) and triangles tested. Device pointers; not timed. */
+ * traversal nodes fetched (64-byte quantised nodes, each covering up to
+ * two tree levels) and triangles tested. Device pointers; not timed. */
 int ajg_accel_trace_counts(ajg_accel *accel, const ajg_ray *rays, uint32_t n,
                            uint32_t *nodes_fetched, uint32_t *tris_tested,
                            void *stream);
@@ -218,7 +218,9 @@ typedef struct ajg_pt_args {
   uint32_t seed_base;   /* seed_mode 1 only */
   int32_t env_cdf_fp64; /* env CDF prefix sums: 0 float (distribution.h:44-48),
                            1 double rounded to float */
-  int32_t max_slots;    /* path slots in flight; 0 = automatic */
+  int32_t max_slots;    /* path slots in flight; 0 = automatic (1 << 20). With
+                           seed_mode 1 each pixel's samples are split into work
+                           items of equal sample chunks, about 8 items per slot */
   uint32_t flags;       /* AJG_RENDER_* bits */
 } ajg_pt_args;
 
