@@ -536,12 +536,14 @@ k_build_wide(const FatNode *fat, uint32_t n_int, WideNode *wide, uint32_t *bad) 
     st[a] = ldexpf(1.f, e);
     ebits |= (uint32_t)(e + 127) << (8 * a);
   }
-  uint8_t q[32] = {0};
-  for (int k = 0; k < ns; ++k) q_box(mn[k], mx[k], o, st, q + 6 * k, bad);
-  uint32_t wd[8];
-  for (int k = 0; k < 8; ++k)
-    wd[k] = (uint32_t)q[4 * k] | (uint32_t)q[4 * k + 1] << 8 | (uint32_t)q[4 * k + 2] << 16 |
-            (uint32_t)q[4 * k + 3] << 24;
+  // plane-major words: word j (lo x, lo y, lo z, hi x, hi y, hi z) holds
+  // slot k's plane in byte k; empty slots stay 0
+  uint8_t q[4][6] = {};
+  for (int k = 0; k < ns; ++k) q_box(mn[k], mx[k], o, st, q[k], bad);
+  uint32_t wd[6];
+  for (int j = 0; j < 6; ++j)
+    wd[j] = (uint32_t)q[0][j] | (uint32_t)q[1][j] << 8 | (uint32_t)q[2][j] << 16 |
+            (uint32_t)q[3][j] << 24;
   WideNode w;
   w.o = make_float4(o[0], o[1], o[2], __uint_as_float(ebits));
   w.qa = make_uint4(wd[0], wd[1], wd[2], wd[3]);

@@ -156,8 +156,8 @@ struct FatNode {
 static constexpr int kNoRef = (int)0x80000000;  // empty slot
 struct WideNode {
   float4 o;  // grid origin xyz; w: the bits (e_x + 127) | (e_y + 127) << 8 | (e_z + 127) << 16
-  uint4 qa;  // plane bytes 0..15 (slot k: bytes 6k .. 6k+5 = lo xyz, hi xyz)
-  uint4 qb;  // plane bytes 16..23 in .x, .y
+  uint4 qa;  // plane words lo x, lo y, lo z, hi x (byte k of a word: slot k's plane)
+  uint4 qb;  // plane words hi y, hi z in .x, .y
   int4 r;    // slot refs (fat index, ~leaf slot, or kNoRef), left-first order
 };
 static_assert(sizeof(WideNode) == 64, "64-byte traversal node");

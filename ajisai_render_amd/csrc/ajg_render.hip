@@ -2463,10 +2463,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   // finishes early takes more work (with one item per slot most slots sat
   // idle through the render's last iterations while the slots with the
   // longest sample sequences finished: heavy mesh, 8M slots, 76 -> 36
-  // iterations, 433 -> 485 M paths/s); guided renders, whose items come
-  // from one counter, keep one per slot
-  const uint64_t items = guided ? target : kItemsPerSlot * target;
-  if (int st = setup_params(ctx, camera, args, items, 0, &P)) return st;
+  // iterations, 433 -> 485 M paths/s)
+  if (int st = setup_params(ctx, camera, args, kItemsPerSlot * target, 0, &P)) return st;
   const uint64_t total = P.total_work;
   P.slots = (uint32_t)std::min<uint64_t>(target, total);
   const uint32_t S = P.slots;
@@ -2585,9 +2583,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   q.c_inst_in = rb->c_inst;
   q.a_occ_in = a_occ;
   q.live = counters + 4;
-  // guided renders take every item from one counter, in order
   q.work_counter = rb->qctr + kWorkOff;
-  q.work_stripes = guided ? 1u : work_stripes_for(S);
+  q.work_stripes = work_stripes_for(S);
   q.first_static = first_static ? 1u : 0u;
   q.work_static = first_static ? (uint32_t)std::min<uint64_t>(S, total) : 0u;
   q.partial = partial;

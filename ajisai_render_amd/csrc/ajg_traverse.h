@@ -38,6 +38,7 @@ struct DevBVH {
 struct TRay {
   f3 o, d, inv;
   bool neg[3];
+  bool fin;  // every component of inv is finite (no zero direction component)
   float tmin, tmax;
 };
 
@@ -50,6 +51,7 @@ __device__ __forceinline__ TRay make_tray(f3 o, f3 d, float tmin, float tmax) {
   r.neg[0] = r.inv.x < 0.f;
   r.neg[1] = r.inv.y < 0.f;
   r.neg[2] = r.inv.z < 0.f;
+  r.fin = __builtin_isfinite(r.inv.x) && __builtin_isfinite(r.inv.y) && __builtin_isfinite(r.inv.z);
   r.tmin = tmin;
   r.tmax = tmax;
   return r;
@@ -118,8 +120,27 @@ __device__ __forceinline__ void load_leaf(const float4 *lt, int slot, f3 *a, f3 
 
 // The exact leaf box test (IntersectP on the triangle's AABB,
 // ploc_aggregate.cpp:154-162, 168-182) with the given t_max.
+// Where every component of 1/d is finite no NaN can arise (the builder
+// rejects non-finite vertices): the reference's compare-select Min/Max
+// then equal fminf/fmaxf up to the sign of a zero, which no comparison
+// below can see, and the plane its sign of 1/d selects is the nearer
+// (farther) of the two, since (lo - o) * inv and (hi - o) * inv keep the
+// order of lo <= hi under rounding; so min/max give the reference's
+// distances bit for bit with fewer instructions. A ray with a zero
+// direction component takes the reference's own sequence.
 __device__ __forceinline__ bool leaf_box_test(const TRay &r, f3 a, f3 b, f3 c, float tmax,
                                               bool incl = false) {
+  if (r.fin) {
+    const float t0x = (fminf(fminf(a.x, b.x), c.x) - r.o.x) * r.inv.x;
+    const float t0y = (fminf(fminf(a.y, b.y), c.y) - r.o.y) * r.inv.y;
+    const float t0z = (fminf(fminf(a.z, b.z), c.z) - r.o.z) * r.inv.z;
+    const float t1x = (fmaxf(fmaxf(a.x, b.x), c.x) - r.o.x) * r.inv.x;
+    const float t1y = (fmaxf(fmaxf(a.y, b.y), c.y) - r.o.y) * r.inv.y;
+    const float t1z = (fmaxf(fmaxf(a.z, b.z), c.z) - r.o.z) * r.inv.z;
+    const float e = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    const float x = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    return !(e > x) && x > r.tmin && (e < tmax || (incl && e == tmax));
+  }
   const box3 bx = tri_box(a, b, c);
   float e;
   return slab_static(r, bx.mn.x, bx.mn.y, bx.mn.z, bx.mx.x, bx.mx.y, bx.mx.z, &e) &&
@@ -181,37 +202,54 @@ __device__ __forceinline__ void stack_pop(const StackRef &s, int &d, int *n, flo
   }
 }
 
-// Static slab part of quantised slot k (0..3) of a node: plane bytes
-// 6k .. 6k+5 (lo xyz, hi xyz) of qa, qb.
+// Static slab part of quantised slot k (0..3) of a node, from the node's
+// plane words chosen for the ray: near[a] holds, per slot, the plane the
+// ray enters axis a's slab through (the upper plane where the direction is
+// negative), far[a] the plane it leaves through; byte k is slot k's. The
+// planes and (plane - origin) * inv_dir are computed exactly as the
+// reference computes its slab test, and a quantised box contains the exact
+// one, so per axis the near distance is at most the exact box's and the far
+// distance at least its (rounding is monotone). The three axes combine with
+// fmaxf / fminf, which ignore a NaN operand (a zero direction component
+// with the ray origin on a plane), where the reference's compare-select
+// chain would carry a NaN from the first axis and cull the box: the test
+// passes wherever the reference's test of the exact box passes, with an
+// entry distance no larger. It only ever visits more nodes; the exact leaf
+// box test at visit time decides what is hit.
 struct QBoxes {
-  float4 o;
+  float ox, oy, oz;
   float sx, sy, sz;
-  uint32_t w[6];
+  uint32_t near[3], far[3];
 };
 __device__ __forceinline__ bool qslab(const TRay &r, const QBoxes &q, int k, float *entry) {
-  const int p = 6 * k;
-  const float mnx = qplane(q.w[(p + 0) >> 2], (p + 0) & 3, q.sx, q.o.x);
-  const float mny = qplane(q.w[(p + 1) >> 2], (p + 1) & 3, q.sy, q.o.y);
-  const float mnz = qplane(q.w[(p + 2) >> 2], (p + 2) & 3, q.sz, q.o.z);
-  const float mxx = qplane(q.w[(p + 3) >> 2], (p + 3) & 3, q.sx, q.o.x);
-  const float mxy = qplane(q.w[(p + 4) >> 2], (p + 4) & 3, q.sy, q.o.y);
-  const float mxz = qplane(q.w[(p + 5) >> 2], (p + 5) & 3, q.sz, q.o.z);
-  return slab_static(r, mnx, mny, mnz, mxx, mxy, mxz, entry);
+  const float nx = (qplane(q.near[0], k, q.sx, q.ox) - r.o.x) * r.inv.x;
+  const float ny = (qplane(q.near[1], k, q.sy, q.oy) - r.o.y) * r.inv.y;
+  const float nz = (qplane(q.near[2], k, q.sz, q.oz) - r.o.z) * r.inv.z;
+  const float fx = (qplane(q.far[0], k, q.sx, q.ox) - r.o.x) * r.inv.x;
+  const float fy = (qplane(q.far[1], k, q.sy, q.oy) - r.o.y) * r.inv.y;
+  const float fz = (qplane(q.far[2], k, q.sz, q.oz) - r.o.z) * r.inv.z;
+  const float tmin_max = fmaxf(fmaxf(nx, ny), nz);
+  const float tmax_min = fminf(fminf(fx, fy), fz);
+  *entry = tmin_max;
+  return !(tmin_max > tmax_min) && tmax_min > r.tmin;
 }
-__device__ __forceinline__ QBoxes qload(const WideNode *w) {
+__device__ __forceinline__ QBoxes qload(const WideNode *w, const TRay &r) {
   QBoxes q;
-  q.o = w->o;
-  q.sx = qstep(q.o, 0);
-  q.sy = qstep(q.o, 1);
-  q.sz = qstep(q.o, 2);
+  const float4 o = w->o;
+  q.ox = o.x;
+  q.oy = o.y;
+  q.oz = o.z;
+  q.sx = qstep(o, 0);
+  q.sy = qstep(o, 1);
+  q.sz = qstep(o, 2);
   const uint4 a = w->qa;
   const uint4 b = w->qb;
-  q.w[0] = a.x;
-  q.w[1] = a.y;
-  q.w[2] = a.z;
-  q.w[3] = a.w;
-  q.w[4] = b.x;
-  q.w[5] = b.y;
+  q.near[0] = r.neg[0] ? a.w : a.x;
+  q.far[0] = r.neg[0] ? a.x : a.w;
+  q.near[1] = r.neg[1] ? b.x : a.y;
+  q.far[1] = r.neg[1] ? a.y : b.x;
+  q.near[2] = r.neg[2] ? b.y : a.z;
+  q.far[2] = r.neg[2] ? a.z : b.y;
   return q;
 }
 
@@ -226,7 +264,7 @@ __device__ __forceinline__ bool closest_step(const WideNode *wide, int &node, co
                                              float tmax, const StackRef &st, int &d,
                                              bool incl = false) {
   const WideNode *w = wide + node;
-  const QBoxes q = qload(w);
+  const QBoxes q = qload(w, r);
   const int4 ref = w->r;
   float e;
   if (ref.w != kNoRef && qslab(r, q, 3, &e)) stack_push<LDS>(st, d, ref.w, e);
@@ -258,7 +296,7 @@ __device__ __forceinline__ bool closest_step_near(const WideNode *wide, int &nod
                                                   const TRay &r, float tmax,
                                                   const StackRef &st, int &d) {
   const WideNode *w = wide + node;
-  const QBoxes q = qload(w);
+  const QBoxes q = qload(w, r);
   const int4 ref = w->r;
   const float inf = __int_as_float(0x7f800000);
   float k0, k1, k2, k3;
@@ -290,7 +328,7 @@ template <int LDS>
 __device__ __forceinline__ bool any_step(const WideNode *wide, int &node, const TRay &r,
                                          float tmax, const StackRef &st, int &d) {
   const WideNode *w = wide + node;
-  const QBoxes q = qload(w);
+  const QBoxes q = qload(w, r);
   const int4 ref = w->r;
   float e;
   if (ref.w != kNoRef && qslab(r, q, 3, &e) && e < tmax) stack_push<LDS>(st, d, ref.w, e);
@@ -392,7 +430,7 @@ __device__ __forceinline__ bool trace_any(const DevBVH &bvh, const TRay &r, int 
 // wave. Each ray's traversal (order, culling, arithmetic) is the same as
 // trace_closest, so results are unchanged. The queue kernels' any-hit
 // traversals (persistent_any, persistent_any_2l below) refill the same way.
-static constexpr int kRefillMin = 48;
+static constexpr int kRefillMin = 16;
 
 // Wave-uniform refill: idle lanes take consecutive batch indices. Returns the
 // lane's new index (kNone if none) and sets *drained when the batch is used up.
@@ -465,7 +503,7 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
           store(idx, h);
         }
       }
-      continue;
+      // (no continue: the lanes that kept their rays take this step too)
     }
     if (__ballot(ray != kNone) == 0ull) {
       if (drained) break;
@@ -550,7 +588,7 @@ __device__ __forceinline__ void persistent_any(const DevBVH &bvh_uniform, uint32
           store(idx, false);
         }
       }
-      continue;
+      // (no continue: the lanes that kept their rays take this step too)
     }
     if (__ballot(ray != kNone) == 0ull) {
       if (drained) break;
@@ -685,7 +723,7 @@ __device__ __forceinline__ void persistent_closest_2l(const DevBVH &tlas, const 
           store(idx, h, h_inst);
         }
       }
-      continue;
+      // (no continue: the lanes that kept their rays take this step too)
     }
     if (__ballot(ray != kNone) == 0ull) {
       if (drained) break;
@@ -806,7 +844,7 @@ __device__ __forceinline__ void persistent_any_2l(const DevBVH &tlas, const DevI
           store(idx, false);
         }
       }
-      continue;
+      // (no continue: the lanes that kept their rays take this step too)
     }
     if (__ballot(ray != kNone) == 0ull) {
       if (drained) break;
