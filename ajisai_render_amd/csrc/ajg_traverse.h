@@ -551,9 +551,13 @@ __device__ __forceinline__ void persistent_closest(const DevBVH &bvh_uniform, ui
   }
 }
 
-// Any-hit traversal over rays [0, n) with the same lane refill: per step a
-// lane tests its leaf and, when that finds nothing and the next node is
-// interior, visits it, in trace_any's order. load(i, bvh) as for
+// Any-hit traversal over rays [0, n) with the same lane refill. Its result
+// does not depend on the order in which a ray's triangles are tested (t_max
+// is fixed), so a lane that reaches a leaf parks it and goes on with the
+// next stacked node; the parked leaves of the wave are tested together once
+// at least half of the traversing lanes hold one (or none can go on), which
+// keeps more lanes busy in both the node and the triangle code than a leaf
+// test whenever one lane stands on a leaf. load(i, bvh) as for
 // persistent_closest; store(i, occluded).
 template <int SPILL, bool COUNT, bool PER_RAY_BVH = false, int LDS = kLdsStack,
           uint32_t STRIPES = 1, class Load, class Store>
@@ -568,7 +572,8 @@ __device__ __forceinline__ void persistent_any(const DevBVH &bvh_uniform, uint32
   const StackRef st = {lds_node, lds_entry, sn, se};
   int d = 0;
   TRay r;
-  int node = 0;
+  int node = kNoRef;    // next node: >= 0 interior, < 0 ~leaf slot, kNoRef none left
+  int parked = kNoRef;  // a leaf waiting for the wave's next leaf tests
   uint32_t ray = kNone;
   bool drained = false;
   for (;;) {
@@ -578,6 +583,7 @@ __device__ __forceinline__ void persistent_any(const DevBVH &bvh_uniform, uint32
       if (idx != kNone) {
         r = load(idx, bvh_lane);
         d = 0;
+        parked = kNoRef;
         float e;
         if (slab_static(r, bvh.root_mn[0], bvh.root_mn[1], bvh.root_mn[2], bvh.root_mx[0],
                         bvh.root_mx[1], bvh.root_mx[2], &e) &&
@@ -590,36 +596,44 @@ __device__ __forceinline__ void persistent_any(const DevBVH &bvh_uniform, uint32
       }
       // (no continue: the lanes that kept their rays take this step too)
     }
-    if (__ballot(ray != kNone) == 0ull) {
+    const uint64_t act = __ballot(ray != kNone);
+    if (act == 0ull) {
       if (drained) break;
       continue;
     }
     if (ray == kNone) continue;
-    bool next = true, hit = false;
-    if (node < 0) {
-      if (COUNT) ++*nt;
-      f3 a, b, c;
-      load_leaf(bvh.leaf_tris, ~node, &a, &b, &c);
-      float t, u, v;
-      hit = leaf_box_test(r, a, b, c, r.tmax) && mt_test(a, b, c, r, r.tmax, &t, &u, &v);
-      if (!hit) {
-        next = d > 0;
-        if (next) {
+    if (node >= 0) {
+      if (COUNT) ++*nn;
+      if (!any_step<LDS>(bvh.wide, node, r, r.tmax, st, d)) {
+        node = kNoRef;
+        if (d > 0) {
           float en;
           stack_pop<LDS>(st, d, &node, &en);
         }
       }
     }
-    if (!hit && next && node >= 0) {
-      if (COUNT) ++*nn;
-      next = any_step<LDS>(bvh.wide, node, r, r.tmax, st, d);
-      if (!next && d > 0) {
+    if (node < 0 && node != kNoRef && parked == kNoRef) {
+      parked = node;
+      node = kNoRef;
+      if (d > 0) {
         float en;
         stack_pop<LDS>(st, d, &node, &en);
-        next = true;
       }
     }
-    if (hit || !next) {
+    const uint64_t holding = __ballot(parked != kNoRef);
+    const uint64_t going = __ballot(node >= 0);
+    bool hit = false;
+    if (holding != 0ull && (going == 0ull || 2 * __popcll(holding) >= __popcll(act))) {
+      if (parked != kNoRef) {
+        if (COUNT) ++*nt;
+        f3 a, b, c;
+        load_leaf(bvh.leaf_tris, ~parked, &a, &b, &c);
+        float t, u, v;
+        hit = leaf_box_test(r, a, b, c, r.tmax) && mt_test(a, b, c, r, r.tmax, &t, &u, &v);
+        parked = kNoRef;
+      }
+    }
+    if (hit || (node == kNoRef && parked == kNoRef)) {
       store(ray, hit);
       ray = kNone;
     }
