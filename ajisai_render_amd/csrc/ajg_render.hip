@@ -180,6 +180,16 @@ struct Queues {
   uint32_t work_stripes;   // 1 or kWStripes
   uint32_t first_static;  // a slot's first work item is its own index (next_sample)
   uint32_t work_static;   // first_static: items [0, work_static) are the slots' first
+  // Live-slot lists: the shade stage runs over the slots the previous
+  // shade left live (dense index -> stripe entry -> slot), and appends the
+  // slots it leaves live to the next list, so slots with nothing left to
+  // do take no lanes (see k_shade).
+  const uint32_t *lv_in;
+  QIn lv_q;                  // the input list's stripes
+  const uint32_t *lv_total;  // its length (written by k_iter_begin)
+  uint32_t *lv_out;
+  uint32_t *lv_count_out;    // kQStripes counter words of the output list
+  uint32_t lv_len;           // entries per stripe
   float *partial;      // total_work x 6 (sum3, sq3) or the film when n_chunks == 1
   uint32_t *partial_n;
   float *per_sample;   // optional
@@ -347,7 +357,10 @@ static constexpr uint32_t kCursorStripes = 16;
 // next_sample); they are cleared once per render, not per iteration.
 static constexpr uint32_t kWStripes = 32;
 static constexpr uint32_t kWorkOff = 4 * kQWords + 2 * kCursorStripes * kCursorStride;
-static constexpr uint32_t kQctrWords = kWorkOff + kWStripes * kQStride;
+// The live-slot lists (two generations, striped like the ray queues) take
+// the last two counter blocks.
+static constexpr uint32_t kListOff = kWorkOff + kWStripes * kQStride;
+static constexpr uint32_t kQctrWords = kListOff + 2 * kQWords;
 // A stripe's items are taken only by its own waves (wave w takes from
 // stripe w % stripes), so every stripe needs a wave: at most one stripe
 // per wave of slots.
@@ -790,28 +803,51 @@ struct StageTimer {
 
 // Start of a wavefront iteration: add the sizes of the queues traced in the
 // previous iteration to the totals, clear the queues the shade stage is
-// about to fill, the live flag and the traversal fetch cursors. qctr holds
-// the striped counters of closest queues 0 and 1, then shadow queues 0 and
-// 1, kQWords apart, then the closest-hit fetch-cursor stripes. One wave.
+// about to fill, the live flag and the traversal fetch cursors, and write
+// the length of the live-slot list the shade stage reads to ctr[5]. qctr
+// holds the striped counters of closest queues 0 and 1, then shadow queues
+// 0 and 1, kQWords apart, then the fetch-cursor stripes, the work-item
+// counters and the live-slot lists 0 and 1. One wave.
 __global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
                              unsigned long long *totals) {
   const uint32_t lane = threadIdx.x;
-  uint32_t c = 0, a = 0;
+  uint32_t c = 0, a = 0, l = 0;
   if (lane < kQStripes) {
     c = qctr[cur * kQWords + lane * kQStride];
     a = qctr[(2 + cur) * kQWords + lane * kQStride];
+    l = qctr[kListOff + cur * kQWords + lane * kQStride];
     qctr[nxt * kQWords + lane * kQStride] = 0;
     qctr[(2 + nxt) * kQWords + lane * kQStride] = 0;
+    qctr[kListOff + nxt * kQWords + lane * kQStride] = 0;
   }
   if (lane < 2 * kCursorStripes) qctr[4 * kQWords + lane * kCursorStride] = 0;
   for (uint32_t o = 32; o > 0; o >>= 1) {
     c += __shfl_down(c, o, 64);
     a += __shfl_down(a, o, 64);
+    l += __shfl_down(l, o, 64);
   }
   if (lane == 0) {
     totals[0] += c;
     totals[1] += a;
     ctr[4] = 0;
+    ctr[5] = l;
+  }
+}
+
+// The first live-slot list: every slot, in the order a shade launch over
+// the slots would append them (wave w to stripe w % kQStripes), with the
+// stripe counts. qctr_list: the list's kQStripes counter words.
+__global__ void __launch_bounds__(256)
+k_lv_init(uint32_t *lv, uint32_t *qctr_list, uint32_t S, uint32_t R) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < S) {
+    const uint32_t w = i >> 6;
+    lv[(w % kQStripes) * R + (w / kQStripes) * 64 + (i & 63)] = i;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kQStripes) {
+    const uint32_t st = threadIdx.x, full = S >> 6, rem = S & 63;
+    const uint32_t n = full > st ? (full - 1 - st) / kQStripes + 1 : 0u;
+    qctr_list[st * kQStride] = 64 * n + ((rem && full % kQStripes == st) ? rem : 0u);
   }
 }
 
@@ -934,7 +970,7 @@ __device__ __forceinline__ AccTarget acc_job(const BatchParams &B, uint32_t j) {
 template <bool BATCH>
 __device__ void next_sample(const PathState &ps, const Queues &q, const RenderParams &P0,
                             const BatchParams &B, uint32_t slot, bool need, Pcg32 &rng,
-                            bool &live) {
+                            bool &live, uint32_t stripe) {
   uint32_t work = need ? ps.work[slot] : kNone;
   uint32_t job = BATCH && need ? ps.job[slot] : 0u;
   uint32_t sample = 0;
@@ -957,7 +993,10 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   // items are used up stops taking atomics (a plain read first: the
   // counter only grows, so a stale value only costs the atomic). A lane
   // whose fetch finds its stripe used up stays idle for the rest of the
-  // render; the stripe's other waves finish its items.
+  // render; the stripe's other waves finish its items. The stripe belongs
+  // to the slot (its wave of slots), and a shade wave runs the slots of a
+  // live-slot list, whose lanes may come from several waves of slots: one
+  // append per distinct stripe among the fetching lanes.
   const bool first = q.first_static && fetch && ps.sample_end[slot] == 0u;
   const uint32_t limit = BATCH ? B.total_work : P0.total_work;
   const uint32_t ws = q.work_stripes;
@@ -968,8 +1007,15 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     return ws > 1 ? base + (((uint64_t)wst + (uint64_t)ws * (j >> 6)) << 6) + (j & 63u)
                   : base + j;
   };
-  const bool open = __ballot(fetch && !first) != 0ull && item_at(*wctr) < limit;
-  const uint32_t jw = wave_append(fetch && !first && open, wctr);
+  const bool want = fetch && !first && __ballot(fetch && !first) != 0ull && item_at(*wctr) < limit;
+  uint32_t jw = kNone;
+  for (uint64_t left = __ballot(want); left != 0ull;) {
+    const uint32_t ls = __shfl(wst, __ffsll((unsigned long long)left) - 1, 64);
+    const bool mine = want && wst == ls;
+    const uint32_t j = wave_append(mine, q.work_counter + ls * kQStride);
+    if (mine) jw = j;
+    left &= ~__ballot(mine);
+  }
   uint64_t w = jw == kNone ? (uint64_t)limit : item_at(jw);
   if (first) w = slot;
   if (fetch) {
@@ -1027,7 +1073,7 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   }
   if (P.max_bounces <= 0) alive = false;
   // A path that dies before its first intersection contributes zero.
-  const uint32_t idx = q_append(alive, q.c_count_out, q.c_len, (slot >> 6) % kQStripes);
+  const uint32_t idx = q_append(alive, q.c_count_out, q.c_len, stripe);
   if (alive) {
     store_ray(q.c_rays_out, idx, r);
     if (BATCH) B.c_job_out[idx] = (uint16_t)job;
@@ -1053,12 +1099,22 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
 template <bool GUIDED, bool BATCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GUIDED ? 1 : 3)))
 k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, BatchParams B) {
+  // The launch covers the largest possible list (every slot); blocks past
+  // the list's end leave at once. Thread v of the launch shades the slot at
+  // dense index v of the live-slot list. Slots without work, pending
+  // light or rays are not in the list: late in a render most slots are
+  // idle, and shading by slot would keep their lanes in every wave.
+  const uint32_t total = *q.lv_total;
+  if (blockIdx.x * 256 >= total) return;
   // guided: the guide mixtures are stored by dense index of the closest queue
   __shared__ uint32_t s_pre[GUIDED ? kQStripes + 1 : 1];
+  __shared__ uint32_t s_lv[kQStripes + 1];
   if (GUIDED) q_prefix(q.c_in, s_pre);
-  const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t stripe = (slot >> 6) % kQStripes;  // queue stripe of this wave
-  const bool valid = slot < (BATCH ? B.slots : P0.slots);
+  q_prefix(q.lv_q, s_lv);
+  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = v < total;
+  const uint32_t slot = valid ? q.lv_in[q_entry(s_lv, q.lv_q.R, v)] : 0u;
+  const uint32_t stripe = (v >> 6) % kQStripes;  // queue stripe of this wave
   const bool train = GUIDED && valid && G.train_stride && slot % G.train_stride == 0;
   const uint32_t hslot = train ? slot / G.train_stride : 0u;
   bool live = false;
@@ -1450,8 +1506,12 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
   }
 
   // ---- 3. next sample / ray generation (wave-uniform control flow)
-  next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live);
+  next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live, stripe);
   if (valid) ps.rng[slot] = rng.state;
+  // the slots left live (a path, pending light or a sample to start) are
+  // the next iteration's list
+  const uint32_t li = q_append(valid && live, q.lv_count_out, q.lv_len, stripe);
+  if (valid && live) q.lv_out[li] = slot;
   const bool any_live = __ballot(valid && live) != 0ull;
   if ((threadIdx.x & 63) == 0 && any_live) *q.live = 1u;
 }
@@ -1634,6 +1694,7 @@ struct RenderBuffers {
   uint8_t *a_occ = nullptr;
   uint32_t *counters = nullptr;
   uint32_t *qctr = nullptr;  // striped queue counters: closest 0, 1, shadow 0, 1
+  uint32_t *lv[2] = {nullptr, nullptr};  // live-slot lists (kQStripes * lv_stripe_len(S))
   float *partial = nullptr;
   uint32_t *partial_n = nullptr;
   // shadow rays are traced on a second stream, concurrently with the
@@ -1705,6 +1766,8 @@ static int ensure_guide_buffers(ajg_ctx *ctx, GuideBuffers *gb, uint32_t S, uint
 }
 
 static constexpr size_t kCounterWords = 64;  // 256 bytes
+// Entries per stripe of a live-slot list (one per slot).
+static inline uint32_t lv_stripe_len(uint32_t S) { return stripe_len(S, 1); }
 
 static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t total) {
   if (rb->slots >= S && rb->total >= total && !rb->bufs.empty()) return AJG_OK;
@@ -1741,6 +1804,8 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&rb->a_rays[0], sizeof(DevRay) * aq_cap, "shadow queue")) ||
       (st = al((void **)&rb->a_rays[1], sizeof(DevRay) * aq_cap, "shadow queue")) ||
       (st = al((void **)&rb->a_occ, aq_cap, "shadow results")) ||
+      (st = al((void **)&rb->lv[0], 4 * (size_t)kQStripes * lv_stripe_len(S), "live-slot list")) ||
+      (st = al((void **)&rb->lv[1], 4 * (size_t)kQStripes * lv_stripe_len(S), "live-slot list")) ||
       (st = al((void **)&rb->counters, 4 * kCounterWords, "counters")) ||
       (st = al((void **)&rb->qctr, 4 * kQctrWords, "queue counters")) ||
       (st = al((void **)&rb->partial, sizeof(float) * 6 * total, "partial sums")) ||
@@ -2565,6 +2630,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   if (film->per_sample) fl.words(film->per_sample, 3ull * n_pix_sz * P.spp);
   if (guided) fl.words(gb->hist_n, gb->train_slots);
   AJG_RCHK(fl.launch(s));
+  // the first shade runs over every slot (list 0 holds them all)
+  const uint32_t lv_len = lv_stripe_len(S);
+  hipLaunchKernelGGL(k_lv_init, dim3(cdiv(S, 256)), dim3(256), 0, s, rb->lv[0], rb->qctr + kListOff, S,
+                     lv_len);
   // guided renders keep the fetch order through the counter for every item
   // (their training slots are every train_stride-th slot, and a slot's
   // first item equal to its index gave the guide a markedly worse fit)
@@ -2590,6 +2659,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   q.partial = partial;
   q.partial_n = partial_n;
   q.per_sample = film->per_sample;
+  q.lv_total = counters + 5;
+  q.lv_len = lv_len;
   const bool inst = ds.inst != nullptr;
   const int stack = spill_for_depth(inst ? sc->depth2 : a->depth);
   if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree too deep for the traversal stack"));
@@ -2631,6 +2702,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       q.a_rays_out = a_rays[nxt];
       q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
       q.a_len = a_len;
+      q.lv_in = rb->lv[cur];
+      q.lv_q = {rb->qctr + kListOff + cur * kQWords, lv_len};
+      q.lv_out = rb->lv[nxt];
+      q.lv_count_out = rb->qctr + kListOff + nxt * kQWords;
       const QIn cq_in = {rb->qctr + nxt * kQWords, c_len};
       const QIn aq_in = {rb->qctr + (2 + nxt) * kQWords, a_len};
       hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, rb->qctr, cur, nxt,
@@ -2985,6 +3060,9 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       if (f.per_sample) fl.words(f.per_sample, 3ull * np * hj[j].P.spp);
     }
     AJG_BCHK(fl.launch(s));
+    // the first shade runs over every slot (list 0 holds them all)
+    hipLaunchKernelGGL(k_lv_init, dim3(cdiv(S, 256)), dim3(256), 0, s, rb->lv[0], rb->qctr + kListOff,
+                       S, lv_stripe_len(S));
   }
 
   uint32_t *counters = rb->counters;  // word layout as in a single render
@@ -3014,6 +3092,9 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
   q.work_stripes = work_stripes_for(S);
   q.first_static = 1u;
   q.work_static = S;
+  const uint32_t lv_len = lv_stripe_len(S);
+  q.lv_total = counters + 5;
+  q.lv_len = lv_len;
   const uint32_t shade_grid = cdiv(S, 256);
   // queue stripes sized for the most any job can append
   const uint32_t c_len = stripe_len(S, kCqPerSlot), a_len = stripe_len(S, kMaxLights + 1);
@@ -3040,6 +3121,10 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       q.a_rays_in = rb->a_rays[cur];
       q.a_rays_out = rb->a_rays[nxt];
       q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
+      q.lv_in = rb->lv[cur];
+      q.lv_q = {rb->qctr + kListOff + cur * kQWords, lv_len};
+      q.lv_out = rb->lv[nxt];
+      q.lv_count_out = rb->qctr + kListOff + nxt * kQWords;
       const QIn cq_in = {rb->qctr + nxt * kQWords, c_len};
       const QIn aq_in = {rb->qctr + (2 + nxt) * kQWords, a_len};
       B.c_job_out = bb->c_job[nxt];
