@@ -286,8 +286,14 @@ __device__ __forceinline__ void mask_grad(const f32x4 (&acc)[MT][1], const s16x4
 // MODE 0: inference (decoded mixture, optional raw outputs), 4 tiles of 16
 // samples per wave iteration. MODE 1: training (forward, vMF loss gradient,
 // backward data), one tile per iteration, feature-major activations out.
+// Inference runs two waves per SIMD (256 registers, a few spilled): 1M
+// points in 0.125 ms against 0.158 ms at the compiler's own budget (one
+// wave, 234 + 72 registers); fewer tiles per wave or three waves spill far
+// more and are 2x slower. The training pass keeps one wave (two: 13%
+// slower).
 template <int MODE>
 __global__ void __launch_bounds__(64 * NetCfg<MODE>::kWaves)
+__attribute__((amdgpu_waves_per_eu(MODE ? 1 : 2)))
 k_guide_fwd(FwdArgs a) {
   constexpr int W = NetCfg<MODE>::kWaves, NT = NetCfg<MODE>::kNT;
   __shared__ NetLds<MODE> s;

@@ -10,24 +10,27 @@ closest-hit MIS-BSDF ray and one shadow ray per bounce. Inputs (scene, BVH,
 env tables) are resident in HBM before the timed region.
 
 Random numbers: by default one counter-based PCG32 stream per (pixel,
-sample) (seed_mode 1), so the 64 samples of a pixel run in parallel and 8M
+sample) (seed_mode 1), so the 64 samples of a pixel run in parallel and 16M
 paths are in flight; the reference's per-pixel sequential stream (seed_mode
 0: a pixel's samples one after another, 1M paths in flight at 1024x1024) is
 timed as well (paths_per_s_seed_mode0). Both trace the same paths per pixel
 with bit-exact hits; only which random numbers a sample draws differs.
 
 Prints one JSON line: paths/s over the timed steps (whole job), Mrays/s, PLOC
-build Mtris/s, the roofline of the dominant kernel (closest-hit traversal)
-and the CPU baseline (the oracle's all-core CPU restatement of the reference
-path tracer on a bounded sample of the same workload).
+build Mtris/s, the roofline of the dominant kernel (the stage with the most
+device time: shading, closest-hit or shadow-ray traversal) with every
+stage's roofline beside it, and the CPU baseline (the oracle's all-core CPU
+restatement of the reference path tracer on a bounded sample of the same
+workload).
 
 The "guided" object is the equal-time comparison of learned path guiding
 against plain path tracing on the occluded room (4 x 3 x 4 room lit only
 through a 0.2 x 1.0 slit by the same HDR map, 204,866 triangles, 512 x 512,
 max 16 bounces): each method renders for the same wall-clock budget (the
-guided one's warm-up training included), and relMSE = mean over pixels of
-mean_c (I - R)^2 / (R_lum^2 + 0.01 mean(R)^2) against a high-spp reference R
-(guided by default, unbiased; --guided-ref plain for a plain-PT reference).
+guided one's warm-up training included), and relMSE = mean over pixels and
+channels of (I - R)^2 / (R^2 + 1e-2) against the committed 65,536-spp
+plain path-tracing reference R (tests/golden; --guided-ref guided or plain
+renders one instead).
 "mlp" times the guide network's kernels alone (MFMA utilisation).
 
 The "batched" object is the batched uneven workload: 64 independent scenes
@@ -105,7 +108,7 @@ def parse():
     ap.add_argument("--seed-mode", type=int, default=1,
                     help="1: one PCG32 stream per (pixel, sample), samples of a pixel in parallel; "
                          "0: the reference's per-pixel sequential stream (one path per pixel in flight)")
-    ap.add_argument("--slots", type=int, default=1 << 23, help="paths in flight (seed_mode 1)")
+    ap.add_argument("--slots", type=int, default=1 << 24, help="paths in flight (seed_mode 1)")
     ap.add_argument("--no-seed0", action="store_true",
                     help="skip the per-pixel-stream (seed_mode 0) comparison render")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
