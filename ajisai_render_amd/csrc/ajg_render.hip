@@ -145,12 +145,8 @@ struct PathState {
   uint32_t *bank;       // accumulator bank of the current work item
   uint64_t *rng;
   float *L, *beta;      // 3 per slot
-  // pending direct lighting of the last shaded bounce; the per-light
-  // records (shadow-ray index, contribution) are light-major, light k of
-  // slot i at k * stride + i, so a wave's accesses for one light are
-  // contiguous whatever the number of lights
+  // pending direct lighting of the last shaded bounce
   uint32_t *pend_flags, *pend_mis, *pend_shadow;  // shadow: kMaxLights per slot
-  uint32_t stride;      // slots the per-light arrays are sized for
   uint32_t *pend_work, *pend_sample, *pend_bank;
   float *pend_L, *pend_beta, *pend_c, *pend_f, *pend_pdf, *pend_ns;
   float *pend_pb;       // the BSDF's own pdf of the MIS-BSDF direction
@@ -220,7 +216,6 @@ struct GuideParams {
   uint32_t rec_cap;
   float *pend_pos;      // 3 per slot
   float *pend_tw;       // kMaxLights per slot: NEE training weight before visibility
-  uint32_t tw_stride;   // light k of slot i at k * tw_stride + i
 };
 
 static constexpr int kGRec = 10;
@@ -1142,12 +1137,12 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
       for (uint32_t k = 0; k < sc.n_lights; ++k) {
         f3 c = mk3(0.f, 0.f, 0.f);
         if (pf & (kPendLight0 << k)) {
-          const uint32_t si = ps.pend_shadow[(size_t)k * ps.stride + slot];
+          const uint32_t si = ps.pend_shadow[(size_t)slot * kMaxLights + k];
           if (!q.a_occ_in[si]) {
-            c = ld_f3(ps.pend_c, k * ps.stride + slot);
+            c = ld_f3(ps.pend_c, slot * kMaxLights + k);
             if (train)
               emit_record(G, tpos, tnrm, load_ray(q.a_rays_in, si).d,
-                          G.pend_tw[(size_t)k * G.tw_stride + slot]);
+                          G.pend_tw[(size_t)slot * kMaxLights + k]);
           }
         }
         direct = add(direct, c);
@@ -1357,9 +1352,9 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
                   bpdf = alpha * gmix_pdf(gm, to_light) + (1.f - alpha) * bpdf;
                 const float wl = power_heuristic(lsmp.pdf, bpdf);
                 const f3 c = mul(beta, scale(divs(f, lsmp.pdf), wl));
-                st_f3(ps.pend_c, k * ps.stride + slot, c);
+                st_f3(ps.pend_c, slot * kMaxLights + k, c);
                 if (train)
-                  G.pend_tw[(size_t)k * G.tw_stride + slot] =
+                  G.pend_tw[(size_t)slot * kMaxLights + k] =
                       l.type != AJG_LIGHT_DIRECTIONAL
                           ? lum_f(lsmp.radiance) * fabsf(dot(to_light, nrm)) / lsmp.pdf * wl
                           : 0.f;
@@ -1372,7 +1367,7 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
           if (emit) {
             store_ray(q.a_rays_out, si, sr);
             if (BATCH) B.a_job_out[si] = (uint16_t)j;
-            ps.pend_shadow[(size_t)k * ps.stride + slot] = si;
+            ps.pend_shadow[(size_t)slot * kMaxLights + k] = si;
             pf2 |= kPendLight0 << k;
           }
         }
@@ -1819,7 +1814,6 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
     return st;
   }
   rb->slots = S;
-  ps.stride = S;
   rb->total = total;
   return AJG_OK;
 }
@@ -2587,7 +2581,6 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
     G.hist_n = gb->hist_n;
     G.pend_pos = gb->pend_pos;
     G.pend_tw = gb->pend_tw;
-    G.tw_stride = gb->slots;
   }
   Queues q;
   memset(&q, 0, sizeof(q));
