@@ -1092,6 +1092,402 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   }
 }
 
+// Steps 1 and 2 of k_shade for one live slot: finish the previous bounce's
+// direct lighting, then shade the extension hit of the current sample
+// (NEE, MIS-BSDF and continuation rays). (As a function of its own the
+// plain kernel compiles to fewer spills: heavy mesh shading 50.9 -> 48.7 ms
+// per render. A second, wave-uniform-job copy for batches, taken when a
+// ballot shows that all lanes shade slots of one job, made the batched
+// kernel spill 34 registers and the batch 3% slower, and was dropped.)
+template <bool GUIDED, bool BATCH>
+__device__ __forceinline__ void shade_slot(const DevScene &sc0, const RenderParams &P0,
+                                           const PathState &ps, const Queues &q,
+                                           const GuideParams &G, const BatchParams &B,
+                                           uint32_t slot, uint32_t stripe, bool train,
+                                           uint32_t hslot, const uint32_t *s_pre, Pcg32 &rng,
+                                           bool &live, bool &need_new) {
+  // ---- 1. finish the previous bounce's direct lighting
+  const uint32_t pf = ps.pend_flags[slot];
+  if (pf & kPendActive) {
+    const uint32_t pj = BATCH ? ps.pend_job[slot] : 0u;
+    const DevScene &sc = BATCH ? B.jobs[pj].sc : sc0;
+    f3 tpos, tnrm;
+    if (train) {
+      tpos = ld_f3(G.pend_pos, slot);
+      tnrm = normalize(ld_f3(ps.pend_ns, slot));
+    }
+    f3 direct = mk3(0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < sc.n_lights; ++k) {
+      f3 c = mk3(0.f, 0.f, 0.f);
+      if (pf & (kPendLight0 << k)) {
+        const uint32_t si = ps.pend_shadow[(size_t)slot * kMaxLights + k];
+        if (!q.a_occ_in[si]) {
+          c = ld_f3(ps.pend_c, slot * kMaxLights + k);
+          if (train)
+            emit_record(G, tpos, tnrm, load_ray(q.a_rays_in, si).d,
+                        G.pend_tw[(size_t)slot * kMaxLights + k]);
+        }
+      }
+      direct = add(direct, c);
+    }
+    f3 mis = mk3(0.f, 0.f, 0.f);
+    if (pf & kPendMis) {
+      // MISSampleBsdf (helper.h:208-262). Without area lights the ray
+      // contributes only when it escapes to the environment, so it is
+      // traced as an any-hit query (it escapes iff the reference's
+      // closest-hit query misses); with area lights it is a closest-hit
+      // query, and a hit on the emitting side of an emissive primitive
+      // adds that light.
+      const uint32_t mi = ps.pend_mis[slot];
+      const bool closest = (pf & kPendMisClosest) != 0;
+      const DevRay mr = load_ray(closest ? q.c_rays_in : q.a_rays_in, mi);
+      float4 mh = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (closest) mh = q.c_hits_in[mi];
+      const bool escaped = closest ? __float_as_int(mh.y) < 0 : !q.a_occ_in[mi];
+      if (!escaped && closest) {
+        const Inct hi = post_intersect(sc, mr, mh.x, __float_as_int(mh.y), mh.z, mh.w,
+                                       sc.inst ? q.c_inst_in[mi] : 0u);
+        if (hi.light && dot(hi.ng, hi.wr) > 0) {  // AreaLight::Radiance
+          const DevLight &l = sc.lights[hi.light - 1];
+          if (!is_black(l.radiance)) {
+            const f3 f = scale(mul(l.radiance, ld_f3(ps.pend_f, slot)),
+                               fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
+            const float pdf = ps.pend_pdf[slot];
+            const float lpdf = area_pdf(l, mr.o, hi.pos, hi.ng);
+            // guided: the weight keeps the BSDF's own pdf (the area
+            // light's pdf is not a density of the same measure, so the
+            // two weights do not sum to one; keeping the plain path's
+            // weight functions keeps the value the estimator converges to)
+            const float wb = power_heuristic(GUIDED ? ps.pend_pb[slot] : pdf, lpdf);
+            mis = scale(divs(f, pdf), wb);
+            if (train)
+              emit_record(G, tpos, tnrm, mr.d, lum_f(l.radiance) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
+          }
+        }
+      }
+      if (escaped && sc.env_index >= 0) {
+        const DevEnv &e = sc.lights[sc.env_index].env;
+        const f3 le = env_radiance(e, mr.d);
+        if (!is_black(le)) {
+          const f3 f = scale(mul(le, ld_f3(ps.pend_f, slot)),
+                             fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
+          const float pdf = ps.pend_pdf[slot];
+          const float lpdf = env_pdf(e, mr.d);
+          // guided: the mixture density is the MIS partner only where the
+          // light-side and BSDF-side integrands are the same function
+          // (flag set in the shade that queued this ray); elsewhere the
+          // plain path's weight functions, still dividing by the density
+          // the direction was drawn from
+          const float wb = power_heuristic(
+              (!GUIDED || (pf & kPendMisMix)) ? pdf : ps.pend_pb[slot], lpdf);
+          mis = scale(divs(f, pdf), wb);
+          if (train) emit_record(G, tpos, tnrm, mr.d, lum_f(le) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
+        }
+      }
+    }
+    direct = add(direct, mul(ld_f3(ps.pend_beta, slot), mis));
+    const f3 Lf = add(ld_f3(ps.pend_L, slot), scale(direct, 1.f));
+    if (train) {
+      if (pf & kPendEnded) {
+        emit_history(G, hslot, Lf);
+      } else if (pf & kPendHist) {
+        const uint32_t hn = G.hist_n[hslot];
+        st3(G.hist + ((size_t)hslot * G.hist_depth + hn - 1) * kGHist + 10, Lf);
+      }
+    }
+    if (pf & kPendEnded) {
+      accumulate(ps, BATCH ? acc_job(B, pj) : acc_single(q, P0), slot, ps.pend_bank[slot],
+                 ps.pend_work[slot], ps.pend_sample[slot], Lf, (pf & kPendLast) != 0);
+    } else {
+      st_f3(ps.L, slot, Lf);
+    }
+    ps.pend_flags[slot] = 0;
+  }
+
+  // ---- 2. shade the extension hit of the current sample
+  const uint32_t ei = ps.ext_idx[slot];
+  if (ei == kNone) {
+    if (ps.work[slot] != kNone) {
+      // a sample ended before its first intersection: start the next one
+      need_new = true;
+      live = true;
+    } else if (ps.sample_end[slot] == 0) {
+      need_new = true;  // never had work: try to fetch some
+    }
+  }
+  if (ei != kNone) {
+    live = true;
+    const uint32_t j = BATCH ? ps.job[slot] : 0u;
+    const DevScene &sc = BATCH ? B.jobs[j].sc : sc0;
+    const RenderParams &P = BATCH ? B.jobs[j].P : P0;
+    const DevRay ray = load_ray(q.c_rays_in, ei);
+    const float4 h = q.c_hits_in[ei];
+    const int ls = __float_as_int(h.y);
+    const uint32_t b = ps.bounce[slot];
+    f3 L = ld_f3(ps.L, slot);
+    f3 beta = ld_f3(ps.beta, slot);
+    const uint32_t work = ps.work[slot], sample = ps.sample[slot];
+    const bool last = sample + 1 >= ps.sample_end[slot];
+    if (ls < 0) {
+      // TraceMIS adds the environment at the first bounce only (pt.cpp:
+      // 114-121; later escapes are its MIS-BSDF term); Trace adds it at
+      // every bounce, unweighted by the throughput (pt.cpp:245-249)
+      if ((b == 0 || !P.use_mis) && sc.env_index >= 0)
+        L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
+      if (train) emit_history(G, hslot, L);
+      accumulate(ps, BATCH ? acc_job(B, j) : acc_single(q, P), slot, ps.bank[slot], work, sample,
+                 L, last);
+      need_new = true;
+    } else {
+      const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w, sc.inst ? q.c_inst_in[ei] : 0u);
+      const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
+      const _Float16 *gm = nullptr;
+      if (GUIDED) {
+        const uint32_t k = ei / q.c_in.R;
+        gm = G.mix + (size_t)(s_pre[k] + (ei - k * q.c_in.R)) * kGMix;
+      }
+      // The guide is mixed in only where the BSDF's pdf is the density of
+      // its own sampler (Lambert). The Disney sampler's selection between
+      // its specular and diffuse branches reuses the half-vector's random
+      // numbers (material/disney.cpp:113-130), so its Pdf (:186-220) is not
+      // that density
+      // and the plain estimator converges to E[f cos / Pdf] rather than to
+      // the integral of f cos; mixing another strategy in would change the
+      // value the render converges to. Disney vertices therefore sample
+      // the BSDF alone (alpha 0), exactly as the plain path does.
+      const float alpha = (GUIDED && bs.m.type != AJG_MAT_DISNEY) ? G.alpha : 0.f;
+      // The env light's two estimates measure different integrands where
+      // the shading normal differs from the geometric one: the light side
+      // takes Eval and |wi . normalize(ns)| (helper.h:120-131), the BSDF
+      // side the sample's f, which carries |ns . wi| / |ng . wi|
+      // (aggregate_bsdf.cpp:82-89), times |ns . wi| (helper.h:228-231).
+      // The plain render converges to the sum of the two weighted by the
+      // BSDF-pdf power heuristic, so guided weights may differ from plain
+      // only where the two integrands coincide: ns equal to ng. Their
+      // ratio is |ns|^2 |ns^ . wi| / |ng . wi|, within 1 + 2e-5 + theta tan(beta)
+      // of 1 for ns at angle theta from ng; the test below admits
+      // theta <= 1e-5 and |ns|^2 within 2e-5 of 1 (quads and cubes, whose
+      // ns is ng, and flat-shaded meshes up to rounding): there the two
+      // integrands differ by at most 2e-5 + 1e-5 tan(beta) of their value
+      // at angle beta from ng, so the weights' choice moves the converged
+      // value by no more than that.
+      bool mix_w = false;
+      if (GUIDED && alpha > 0.f) {
+        const float n2 = dot(it.ns, it.ns);
+        const f3 cx = cross(it.ns, it.ng);
+        mix_w = dot(cx, cx) <= 1e-10f * n2 && fabsf(n2 - 1.f) <= 2e-5f;
+      }
+      // emitted radiance at the first hit (pt.cpp:188-194); Trace adds
+      // it at every hit (pt.cpp:289-293)
+      if ((b == 0 || !P.use_mis) && it.light && dot(it.ng, it.wr) > 0)
+        L = add(L, mul(beta, sc.lights[it.light - 1].radiance));
+      uint32_t pf2 = kPendActive;
+      st_f3(ps.pend_L, slot, L);
+      st_f3(ps.pend_beta, slot, beta);
+      ps.pend_work[slot] = work;
+      ps.pend_sample[slot] = sample;
+      ps.pend_bank[slot] = ps.bank[slot];
+      if (BATCH) ps.pend_job[slot] = j;
+      if (train) {
+        st_f3(G.pend_pos, slot, it.pos);
+        st_f3(ps.pend_ns, slot, it.ns);
+      }
+      // NEE for each light (MISSampleLight, helper.h:100-133); Trace
+      // samples no light (its direct-light loop is empty, pt.cpp:296-301)
+      const uint32_t n_nee = P.use_mis ? sc.n_lights : 0u;
+      for (uint32_t k = 0; k < n_nee; ++k) {
+        const DevLight &l = sc.lights[k];
+        LightSample lsmp;
+        if (l.type == AJG_LIGHT_ENV) {
+          const float first = pcg_next_float(rng);
+          const float second = pcg_next_float(rng);
+          lsmp = env_sample(l.env, it.pos, second, first, sc.radius);
+        } else if (l.type == kLightArea) {
+          f3 sam;
+          sam.x = pcg_next_float(rng);
+          sam.y = pcg_next_float(rng);
+          sam.z = pcg_next_float(rng);
+          lsmp = area_sample(l, it.pos, sam);
+        } else {
+          const float u0 = pcg_next_float(rng);
+          const float u1 = pcg_next_float(rng);
+          lsmp = dir_sample(l, it.pos, u0, u1, sc.radius);
+        }
+        bool emit = false;
+        DevRay sr;
+        if (!(is_black(lsmp.radiance) || lsmp.pdf <= 0.f)) {
+          const float len = length(sub(lsmp.pos, it.pos)) - AJG_EPS;
+          if (!(len < AJG_EPS)) {
+            const f3 to_light = normalize(sub(lsmp.pos, it.pos));
+            const f3 bf = bsdf_eval_all(bs, to_light, it.wr);
+            if (!is_black(bf)) {
+              // MISSampleAreaLight takes the shading normal as it is,
+              // the other lights normalise it (helper.h:64, 92, 125)
+              const f3 nrm = normalize(it.ns);
+              const f3 f = scale(mul(lsmp.radiance, bf),
+                                 fabsf(dot(to_light, l.type == kLightArea ? it.ns : nrm)));
+              float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
+              // guided: the env light's MIS partner is the mixture density
+              // where ns equals ng (see mix_w); a delta light is never
+              // reached by directional sampling, so its weight keeps the
+              // BSDF pdf, as in the plain path
+              if (mix_w && l.type == AJG_LIGHT_ENV)
+                bpdf = alpha * gmix_pdf(gm, to_light) + (1.f - alpha) * bpdf;
+              const float wl = power_heuristic(lsmp.pdf, bpdf);
+              const f3 c = mul(beta, scale(divs(f, lsmp.pdf), wl));
+              st_f3(ps.pend_c, slot * kMaxLights + k, c);
+              if (train)
+                G.pend_tw[(size_t)slot * kMaxLights + k] =
+                    l.type != AJG_LIGHT_DIRECTIONAL
+                        ? lum_f(lsmp.radiance) * fabsf(dot(to_light, nrm)) / lsmp.pdf * wl
+                        : 0.f;
+              sr = make_ray(it.pos, to_light, AJG_EPS, 0.99f * len);
+              emit = true;
+            }
+          }
+        }
+        const uint32_t si = q_append(emit, q.a_count_out, q.a_len, stripe);
+        if (emit) {
+          store_ray(q.a_rays_out, si, sr);
+          if (BATCH) B.a_job_out[si] = (uint16_t)j;
+          ps.pend_shadow[(size_t)slot * kMaxLights + k] = si;
+          pf2 |= kPendLight0 << k;
+        }
+      }
+      // Two independent draws of the BSDF sampler (of the guide mixture
+      // when guided): the MIS-BSDF sample (MISSampleBsdf, helper.h:209-246),
+      // then the continuation (pt.cpp:215-223). One loop over the two keeps
+      // a single inlined copy of the sampler in the kernel.
+      // Where ns differs from ng the MIS-BSDF estimate carries the factor
+      // |ns . wi| / |ng . wi| (aggregate_bsdf.cpp:82-89), unbounded for
+      // directions grazing ng; the plain sampler draws those rarely, a
+      // guide may not, so there the MIS-BSDF direction comes from the BSDF
+      // alone (alpha 0, with the plain weights) and only the continuation,
+      // whose weight stays bounded, is guided.
+      GSample g_mis, g_cont;
+      g_mis.ok = false;
+#pragma unroll 1
+      for (int k = P.use_mis ? 0 : 1; k < 2; ++k) {
+        GSample g;
+        if (GUIDED) {
+          g = guided_sample(bs, it.wr, gm, (k == 0 && !mix_w) ? 0.f : alpha, rng);
+        } else {
+          f3 sam;
+          sam.x = pcg_next_float(rng);
+          sam.y = pcg_next_float(rng);
+          sam.z = pcg_next_float(rng);
+          const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
+          g.dir = s.dir;
+          g.f = s.f;
+          g.pdf = s.pdf;
+          g.pb = s.pdf;
+          g.ok = s.valid;
+        }
+        if (k == 0)
+          g_mis = g;
+        else
+          g_cont = g;
+      }
+      // MIS-BSDF ray
+      {
+        bool emit = false;
+        DevRay mr;
+        const bool lit = sc.env_index >= 0 || sc.n_area > 0;
+        if (GUIDED) {
+          if (g_mis.ok && lit) {
+            mr = make_ray(eps_offset(it, g_mis.dir), g_mis.dir, AJG_EPS, __int_as_float(0x7f800000));
+            st_f3(ps.pend_f, slot, g_mis.f);
+            ps.pend_pdf[slot] = g_mis.pdf;
+            ps.pend_pb[slot] = g_mis.pb;
+            st_f3(ps.pend_ns, slot, it.ns);
+            emit = true;
+          }
+        } else {
+          if (g_mis.ok && !(is_black(g_mis.f) || g_mis.pdf == 0.f) && lit) {
+            const f3 dn = normalize(g_mis.dir);
+            mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
+            st_f3(ps.pend_f, slot, g_mis.f);
+            ps.pend_pdf[slot] = g_mis.pdf;
+            st_f3(ps.pend_ns, slot, it.ns);
+            emit = true;
+          }
+        }
+        // with area lights: a closest-hit query (the emitter it hits)
+        const bool to_c = emit && sc.n_area > 0;
+        const uint32_t mc = q_append(to_c, q.c_count_out, q.c_len, stripe);
+        const uint32_t ma = q_append(emit && !to_c, q.a_count_out, q.a_len, stripe);
+        if (to_c) {
+          store_ray(q.c_rays_out, mc, mr);
+          if (BATCH) B.c_job_out[mc] = (uint16_t)j;
+          ps.pend_mis[slot] = mc;
+          pf2 |= kPendMis | kPendMisClosest;
+        } else if (emit) {
+          store_ray(q.a_rays_out, ma, mr);
+          if (BATCH) B.a_job_out[ma] = (uint16_t)j;
+          ps.pend_mis[slot] = ma;
+          pf2 |= kPendMis;
+        }
+        if (mix_w) pf2 |= kPendMisMix;
+      }
+      // continuation (pt.cpp:215-223) and Russian roulette of the next
+      // bounce (pt.cpp:107-110)
+      bool cont;
+      DevRay nr;
+      float cw = 0.f;
+      {
+        const f3 dir = g_cont.dir, fv = g_cont.f;
+        const float pdf = g_cont.pdf;
+        cont = g_cont.ok;
+        if (!GUIDED) {
+          cont = cont && !(is_black(fv) || pdf < 0.f);
+          if ((P.flags & AJG_RENDER_FINITE_PDF) && pdf == 0.f) cont = false;
+        }
+        if (cont) {
+          beta = mul(beta, divs(scale(fv, fabsf(dot(it.ng, dir))), pdf));
+          nr = make_ray(eps_offset(it, dir), dir, AJG_EPS, __int_as_float(0x7f800000));
+          if (train) cw = fabsf(dot(normalize(it.ns), dir)) / pdf;
+          const int nb = (int)b + 1;
+          if (nb >= P.max_bounces) {
+            cont = false;
+          } else if (nb > P.min_bounces) {
+            if (pcg_next_float(rng) > P.cont_prob)
+              cont = false;
+            else
+              beta = divs(beta, P.cont_prob);
+          }
+        }
+      }
+      const uint32_t ni = q_append(cont, q.c_count_out, q.c_len, stripe);
+      if (cont) {
+        store_ray(q.c_rays_out, ni, nr);
+        if (BATCH) B.c_job_out[ni] = (uint16_t)j;
+        ps.ext_idx[slot] = ni;
+        ps.bounce[slot] = b + 1;
+        st_f3(ps.beta, slot, beta);
+        if (train) {
+          const uint32_t hn = G.hist_n[hslot];
+          if (hn < G.hist_depth) {
+            float *hr = G.hist + ((size_t)hslot * G.hist_depth + hn) * kGHist;
+            st3(hr, it.pos);
+            st3(hr + 3, normalize(it.ns));
+            st3(hr + 6, nr.d);
+            hr[9] = cw;
+            st3(hr + 13, beta);
+            G.hist_n[hslot] = hn + 1;
+            pf2 |= kPendHist;
+          }
+        }
+      } else {
+        pf2 |= kPendEnded;
+        if (last) pf2 |= kPendLast;
+        need_new = true;
+      }
+      ps.pend_flags[slot] = pf2;
+    }
+  }
+  if (ps.pend_flags[slot] & kPendActive) live = true;
+}
+
 // Plain and batched shading run three waves per SIMD (at most 168
 // registers): measured faster than the compiler's own budget (two waves)
 // and than four (128 registers, with spills). The guided variant keeps the
@@ -1122,388 +1518,8 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
   rng.state = valid ? ps.rng[slot] : 0ull;
   bool need_new = false;
 
-  if (valid) {
-    // ---- 1. finish the previous bounce's direct lighting
-    const uint32_t pf = ps.pend_flags[slot];
-    if (pf & kPendActive) {
-      const uint32_t pj = BATCH ? ps.pend_job[slot] : 0u;
-      const DevScene &sc = BATCH ? B.jobs[pj].sc : sc0;
-      f3 tpos, tnrm;
-      if (train) {
-        tpos = ld_f3(G.pend_pos, slot);
-        tnrm = normalize(ld_f3(ps.pend_ns, slot));
-      }
-      f3 direct = mk3(0.f, 0.f, 0.f);
-      for (uint32_t k = 0; k < sc.n_lights; ++k) {
-        f3 c = mk3(0.f, 0.f, 0.f);
-        if (pf & (kPendLight0 << k)) {
-          const uint32_t si = ps.pend_shadow[(size_t)slot * kMaxLights + k];
-          if (!q.a_occ_in[si]) {
-            c = ld_f3(ps.pend_c, slot * kMaxLights + k);
-            if (train)
-              emit_record(G, tpos, tnrm, load_ray(q.a_rays_in, si).d,
-                          G.pend_tw[(size_t)slot * kMaxLights + k]);
-          }
-        }
-        direct = add(direct, c);
-      }
-      f3 mis = mk3(0.f, 0.f, 0.f);
-      if (pf & kPendMis) {
-        // MISSampleBsdf (helper.h:208-262). Without area lights the ray
-        // contributes only when it escapes to the environment, so it is
-        // traced as an any-hit query (it escapes iff the reference's
-        // closest-hit query misses); with area lights it is a closest-hit
-        // query, and a hit on the emitting side of an emissive primitive
-        // adds that light.
-        const uint32_t mi = ps.pend_mis[slot];
-        const bool closest = (pf & kPendMisClosest) != 0;
-        const DevRay mr = load_ray(closest ? q.c_rays_in : q.a_rays_in, mi);
-        float4 mh = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (closest) mh = q.c_hits_in[mi];
-        const bool escaped = closest ? __float_as_int(mh.y) < 0 : !q.a_occ_in[mi];
-        if (!escaped && closest) {
-          const Inct hi = post_intersect(sc, mr, mh.x, __float_as_int(mh.y), mh.z, mh.w,
-                                         sc.inst ? q.c_inst_in[mi] : 0u);
-          if (hi.light && dot(hi.ng, hi.wr) > 0) {  // AreaLight::Radiance
-            const DevLight &l = sc.lights[hi.light - 1];
-            if (!is_black(l.radiance)) {
-              const f3 f = scale(mul(l.radiance, ld_f3(ps.pend_f, slot)),
-                                 fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
-              const float pdf = ps.pend_pdf[slot];
-              const float lpdf = area_pdf(l, mr.o, hi.pos, hi.ng);
-              // guided: the weight keeps the BSDF's own pdf (the area
-              // light's pdf is not a density of the same measure, so the
-              // two weights do not sum to one; keeping the plain path's
-              // weight functions keeps the value the estimator converges to)
-              const float wb = power_heuristic(GUIDED ? ps.pend_pb[slot] : pdf, lpdf);
-              mis = scale(divs(f, pdf), wb);
-              if (train)
-                emit_record(G, tpos, tnrm, mr.d, lum_f(l.radiance) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
-            }
-          }
-        }
-        if (escaped && sc.env_index >= 0) {
-          const DevEnv &e = sc.lights[sc.env_index].env;
-          const f3 le = env_radiance(e, mr.d);
-          if (!is_black(le)) {
-            const f3 f = scale(mul(le, ld_f3(ps.pend_f, slot)),
-                               fabsf(dot(ld_f3(ps.pend_ns, slot), mr.d)));
-            const float pdf = ps.pend_pdf[slot];
-            const float lpdf = env_pdf(e, mr.d);
-            // guided: the mixture density is the MIS partner only where the
-            // light-side and BSDF-side integrands are the same function
-            // (flag set in the shade that queued this ray); elsewhere the
-            // plain path's weight functions, still dividing by the density
-            // the direction was drawn from
-            const float wb = power_heuristic(
-                (!GUIDED || (pf & kPendMisMix)) ? pdf : ps.pend_pb[slot], lpdf);
-            mis = scale(divs(f, pdf), wb);
-            if (train) emit_record(G, tpos, tnrm, mr.d, lum_f(le) * fabsf(dot(tnrm, mr.d)) / pdf * wb);
-          }
-        }
-      }
-      direct = add(direct, mul(ld_f3(ps.pend_beta, slot), mis));
-      const f3 Lf = add(ld_f3(ps.pend_L, slot), scale(direct, 1.f));
-      if (train) {
-        if (pf & kPendEnded) {
-          emit_history(G, hslot, Lf);
-        } else if (pf & kPendHist) {
-          const uint32_t hn = G.hist_n[hslot];
-          st3(G.hist + ((size_t)hslot * G.hist_depth + hn - 1) * kGHist + 10, Lf);
-        }
-      }
-      if (pf & kPendEnded) {
-        accumulate(ps, BATCH ? acc_job(B, pj) : acc_single(q, P0), slot, ps.pend_bank[slot],
-                   ps.pend_work[slot], ps.pend_sample[slot], Lf, (pf & kPendLast) != 0);
-      } else {
-        st_f3(ps.L, slot, Lf);
-      }
-      ps.pend_flags[slot] = 0;
-    }
-
-    // ---- 2. shade the extension hit of the current sample
-    const uint32_t ei = ps.ext_idx[slot];
-    if (ei == kNone) {
-      if (ps.work[slot] != kNone) {
-        // a sample ended before its first intersection: start the next one
-        need_new = true;
-        live = true;
-      } else if (ps.sample_end[slot] == 0) {
-        need_new = true;  // never had work: try to fetch some
-      }
-    }
-    if (ei != kNone) {
-      live = true;
-      const uint32_t j = BATCH ? ps.job[slot] : 0u;
-      const DevScene &sc = BATCH ? B.jobs[j].sc : sc0;
-      const RenderParams &P = BATCH ? B.jobs[j].P : P0;
-      const DevRay ray = load_ray(q.c_rays_in, ei);
-      const float4 h = q.c_hits_in[ei];
-      const int ls = __float_as_int(h.y);
-      const uint32_t b = ps.bounce[slot];
-      f3 L = ld_f3(ps.L, slot);
-      f3 beta = ld_f3(ps.beta, slot);
-      const uint32_t work = ps.work[slot], sample = ps.sample[slot];
-      const bool last = sample + 1 >= ps.sample_end[slot];
-      if (ls < 0) {
-        // TraceMIS adds the environment at the first bounce only (pt.cpp:
-        // 114-121; later escapes are its MIS-BSDF term); Trace adds it at
-        // every bounce, unweighted by the throughput (pt.cpp:245-249)
-        if ((b == 0 || !P.use_mis) && sc.env_index >= 0)
-          L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
-        if (train) emit_history(G, hslot, L);
-        accumulate(ps, BATCH ? acc_job(B, j) : acc_single(q, P), slot, ps.bank[slot], work, sample,
-                   L, last);
-        need_new = true;
-      } else {
-        const Inct it = post_intersect(sc, ray, h.x, ls, h.z, h.w, sc.inst ? q.c_inst_in[ei] : 0u);
-        const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
-        const _Float16 *gm = nullptr;
-        if (GUIDED) {
-          const uint32_t k = ei / q.c_in.R;
-          gm = G.mix + (size_t)(s_pre[k] + (ei - k * q.c_in.R)) * kGMix;
-        }
-        // The guide is mixed in only where the BSDF's pdf is the density of
-        // its own sampler (Lambert). The Disney sampler's selection between
-        // its specular and diffuse branches reuses the half-vector's random
-        // numbers (material/disney.cpp:113-130), so its Pdf (:186-220) is not
-        // that density
-        // and the plain estimator converges to E[f cos / Pdf] rather than to
-        // the integral of f cos; mixing another strategy in would change the
-        // value the render converges to. Disney vertices therefore sample
-        // the BSDF alone (alpha 0), exactly as the plain path does.
-        const float alpha = (GUIDED && bs.m.type != AJG_MAT_DISNEY) ? G.alpha : 0.f;
-        // The env light's two estimates measure different integrands where
-        // the shading normal differs from the geometric one: the light side
-        // takes Eval and |wi . normalize(ns)| (helper.h:120-131), the BSDF
-        // side the sample's f, which carries |ns . wi| / |ng . wi|
-        // (aggregate_bsdf.cpp:82-89), times |ns . wi| (helper.h:228-231).
-        // The plain render converges to the sum of the two weighted by the
-        // BSDF-pdf power heuristic, so guided weights may differ from plain
-        // only where the two integrands coincide: ns equal to ng. Their
-        // ratio is |ns|^2 |ns^ . wi| / |ng . wi|, within 1 + 2e-5 + theta tan(beta)
-        // of 1 for ns at angle theta from ng; the test below admits
-        // theta <= 1e-5 and |ns|^2 within 2e-5 of 1 (quads and cubes, whose
-        // ns is ng, and flat-shaded meshes up to rounding): there the two
-        // integrands differ by at most 2e-5 + 1e-5 tan(beta) of their value
-        // at angle beta from ng, so the weights' choice moves the converged
-        // value by no more than that.
-        bool mix_w = false;
-        if (GUIDED && alpha > 0.f) {
-          const float n2 = dot(it.ns, it.ns);
-          const f3 cx = cross(it.ns, it.ng);
-          mix_w = dot(cx, cx) <= 1e-10f * n2 && fabsf(n2 - 1.f) <= 2e-5f;
-        }
-        // emitted radiance at the first hit (pt.cpp:188-194); Trace adds
-        // it at every hit (pt.cpp:289-293)
-        if ((b == 0 || !P.use_mis) && it.light && dot(it.ng, it.wr) > 0)
-          L = add(L, mul(beta, sc.lights[it.light - 1].radiance));
-        uint32_t pf2 = kPendActive;
-        st_f3(ps.pend_L, slot, L);
-        st_f3(ps.pend_beta, slot, beta);
-        ps.pend_work[slot] = work;
-        ps.pend_sample[slot] = sample;
-        ps.pend_bank[slot] = ps.bank[slot];
-        if (BATCH) ps.pend_job[slot] = j;
-        if (train) {
-          st_f3(G.pend_pos, slot, it.pos);
-          st_f3(ps.pend_ns, slot, it.ns);
-        }
-        // NEE for each light (MISSampleLight, helper.h:100-133); Trace
-        // samples no light (its direct-light loop is empty, pt.cpp:296-301)
-        const uint32_t n_nee = P.use_mis ? sc.n_lights : 0u;
-        for (uint32_t k = 0; k < n_nee; ++k) {
-          const DevLight &l = sc.lights[k];
-          LightSample lsmp;
-          if (l.type == AJG_LIGHT_ENV) {
-            const float first = pcg_next_float(rng);
-            const float second = pcg_next_float(rng);
-            lsmp = env_sample(l.env, it.pos, second, first, sc.radius);
-          } else if (l.type == kLightArea) {
-            f3 sam;
-            sam.x = pcg_next_float(rng);
-            sam.y = pcg_next_float(rng);
-            sam.z = pcg_next_float(rng);
-            lsmp = area_sample(l, it.pos, sam);
-          } else {
-            const float u0 = pcg_next_float(rng);
-            const float u1 = pcg_next_float(rng);
-            lsmp = dir_sample(l, it.pos, u0, u1, sc.radius);
-          }
-          bool emit = false;
-          DevRay sr;
-          if (!(is_black(lsmp.radiance) || lsmp.pdf <= 0.f)) {
-            const float len = length(sub(lsmp.pos, it.pos)) - AJG_EPS;
-            if (!(len < AJG_EPS)) {
-              const f3 to_light = normalize(sub(lsmp.pos, it.pos));
-              const f3 bf = bsdf_eval_all(bs, to_light, it.wr);
-              if (!is_black(bf)) {
-                // MISSampleAreaLight takes the shading normal as it is,
-                // the other lights normalise it (helper.h:64, 92, 125)
-                const f3 nrm = normalize(it.ns);
-                const f3 f = scale(mul(lsmp.radiance, bf),
-                                   fabsf(dot(to_light, l.type == kLightArea ? it.ns : nrm)));
-                float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
-                // guided: the env light's MIS partner is the mixture density
-                // where ns equals ng (see mix_w); a delta light is never
-                // reached by directional sampling, so its weight keeps the
-                // BSDF pdf, as in the plain path
-                if (mix_w && l.type == AJG_LIGHT_ENV)
-                  bpdf = alpha * gmix_pdf(gm, to_light) + (1.f - alpha) * bpdf;
-                const float wl = power_heuristic(lsmp.pdf, bpdf);
-                const f3 c = mul(beta, scale(divs(f, lsmp.pdf), wl));
-                st_f3(ps.pend_c, slot * kMaxLights + k, c);
-                if (train)
-                  G.pend_tw[(size_t)slot * kMaxLights + k] =
-                      l.type != AJG_LIGHT_DIRECTIONAL
-                          ? lum_f(lsmp.radiance) * fabsf(dot(to_light, nrm)) / lsmp.pdf * wl
-                          : 0.f;
-                sr = make_ray(it.pos, to_light, AJG_EPS, 0.99f * len);
-                emit = true;
-              }
-            }
-          }
-          const uint32_t si = q_append(emit, q.a_count_out, q.a_len, stripe);
-          if (emit) {
-            store_ray(q.a_rays_out, si, sr);
-            if (BATCH) B.a_job_out[si] = (uint16_t)j;
-            ps.pend_shadow[(size_t)slot * kMaxLights + k] = si;
-            pf2 |= kPendLight0 << k;
-          }
-        }
-        // Two independent draws of the BSDF sampler (of the guide mixture
-        // when guided): the MIS-BSDF sample (MISSampleBsdf, helper.h:209-246),
-        // then the continuation (pt.cpp:215-223). One loop over the two keeps
-        // a single inlined copy of the sampler in the kernel.
-        // Where ns differs from ng the MIS-BSDF estimate carries the factor
-        // |ns . wi| / |ng . wi| (aggregate_bsdf.cpp:82-89), unbounded for
-        // directions grazing ng; the plain sampler draws those rarely, a
-        // guide may not, so there the MIS-BSDF direction comes from the BSDF
-        // alone (alpha 0, with the plain weights) and only the continuation,
-        // whose weight stays bounded, is guided.
-        GSample g_mis, g_cont;
-        g_mis.ok = false;
-#pragma unroll 1
-        for (int k = P.use_mis ? 0 : 1; k < 2; ++k) {
-          GSample g;
-          if (GUIDED) {
-            g = guided_sample(bs, it.wr, gm, (k == 0 && !mix_w) ? 0.f : alpha, rng);
-          } else {
-            f3 sam;
-            sam.x = pcg_next_float(rng);
-            sam.y = pcg_next_float(rng);
-            sam.z = pcg_next_float(rng);
-            const BsdfSample s = bsdf_sample_all(bs, it.wr, sam);
-            g.dir = s.dir;
-            g.f = s.f;
-            g.pdf = s.pdf;
-            g.pb = s.pdf;
-            g.ok = s.valid;
-          }
-          if (k == 0)
-            g_mis = g;
-          else
-            g_cont = g;
-        }
-        // MIS-BSDF ray
-        {
-          bool emit = false;
-          DevRay mr;
-          const bool lit = sc.env_index >= 0 || sc.n_area > 0;
-          if (GUIDED) {
-            if (g_mis.ok && lit) {
-              mr = make_ray(eps_offset(it, g_mis.dir), g_mis.dir, AJG_EPS, __int_as_float(0x7f800000));
-              st_f3(ps.pend_f, slot, g_mis.f);
-              ps.pend_pdf[slot] = g_mis.pdf;
-              ps.pend_pb[slot] = g_mis.pb;
-              st_f3(ps.pend_ns, slot, it.ns);
-              emit = true;
-            }
-          } else {
-            if (g_mis.ok && !(is_black(g_mis.f) || g_mis.pdf == 0.f) && lit) {
-              const f3 dn = normalize(g_mis.dir);
-              mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, __int_as_float(0x7f800000));
-              st_f3(ps.pend_f, slot, g_mis.f);
-              ps.pend_pdf[slot] = g_mis.pdf;
-              st_f3(ps.pend_ns, slot, it.ns);
-              emit = true;
-            }
-          }
-          // with area lights: a closest-hit query (the emitter it hits)
-          const bool to_c = emit && sc.n_area > 0;
-          const uint32_t mc = q_append(to_c, q.c_count_out, q.c_len, stripe);
-          const uint32_t ma = q_append(emit && !to_c, q.a_count_out, q.a_len, stripe);
-          if (to_c) {
-            store_ray(q.c_rays_out, mc, mr);
-            if (BATCH) B.c_job_out[mc] = (uint16_t)j;
-            ps.pend_mis[slot] = mc;
-            pf2 |= kPendMis | kPendMisClosest;
-          } else if (emit) {
-            store_ray(q.a_rays_out, ma, mr);
-            if (BATCH) B.a_job_out[ma] = (uint16_t)j;
-            ps.pend_mis[slot] = ma;
-            pf2 |= kPendMis;
-          }
-          if (mix_w) pf2 |= kPendMisMix;
-        }
-        // continuation (pt.cpp:215-223) and Russian roulette of the next
-        // bounce (pt.cpp:107-110)
-        bool cont;
-        DevRay nr;
-        float cw = 0.f;
-        {
-          const f3 dir = g_cont.dir, fv = g_cont.f;
-          const float pdf = g_cont.pdf;
-          cont = g_cont.ok;
-          if (!GUIDED) {
-            cont = cont && !(is_black(fv) || pdf < 0.f);
-            if ((P.flags & AJG_RENDER_FINITE_PDF) && pdf == 0.f) cont = false;
-          }
-          if (cont) {
-            beta = mul(beta, divs(scale(fv, fabsf(dot(it.ng, dir))), pdf));
-            nr = make_ray(eps_offset(it, dir), dir, AJG_EPS, __int_as_float(0x7f800000));
-            if (train) cw = fabsf(dot(normalize(it.ns), dir)) / pdf;
-            const int nb = (int)b + 1;
-            if (nb >= P.max_bounces) {
-              cont = false;
-            } else if (nb > P.min_bounces) {
-              if (pcg_next_float(rng) > P.cont_prob)
-                cont = false;
-              else
-                beta = divs(beta, P.cont_prob);
-            }
-          }
-        }
-        const uint32_t ni = q_append(cont, q.c_count_out, q.c_len, stripe);
-        if (cont) {
-          store_ray(q.c_rays_out, ni, nr);
-          if (BATCH) B.c_job_out[ni] = (uint16_t)j;
-          ps.ext_idx[slot] = ni;
-          ps.bounce[slot] = b + 1;
-          st_f3(ps.beta, slot, beta);
-          if (train) {
-            const uint32_t hn = G.hist_n[hslot];
-            if (hn < G.hist_depth) {
-              float *hr = G.hist + ((size_t)hslot * G.hist_depth + hn) * kGHist;
-              st3(hr, it.pos);
-              st3(hr + 3, normalize(it.ns));
-              st3(hr + 6, nr.d);
-              hr[9] = cw;
-              st3(hr + 13, beta);
-              G.hist_n[hslot] = hn + 1;
-              pf2 |= kPendHist;
-            }
-          }
-        } else {
-          pf2 |= kPendEnded;
-          if (last) pf2 |= kPendLast;
-          need_new = true;
-        }
-        ps.pend_flags[slot] = pf2;
-      }
-    }
-    if (ps.pend_flags[slot] & kPendActive) live = true;
-  }
+  if (valid) shade_slot<GUIDED, BATCH>(sc0, P0, ps, q, G, B, slot, stripe, train, hslot, s_pre, rng, live,
+                                       need_new);
 
   // ---- 3. next sample / ray generation (wave-uniform control flow)
   next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live, stripe);
