@@ -646,8 +646,15 @@ k_guide_update(const float *partial, int nb, const float *scal, int nfb, float *
   }
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= kGParams) return;
-  float gs = 0.f;
-  for (int b = 0; b < nb; ++b) gs += partial[(size_t)b * kGParams + i];
+  // eight independent sums (partials b = 8 k + j go to sum j), combined in
+  // a fixed order: eight loads in flight per thread instead of one chain
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int pb = 0;
+  for (; pb + 8 <= nb; pb += 8)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] += partial[(size_t)(pb + j) * kGParams + i];
+  for (; pb < nb; ++pb) acc[pb & 7] += partial[(size_t)pb * kGParams + i];
+  const float gs = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   const float gi = S > 0.f ? gs / S : 0.f;
   grad[i] = gi;
   if (!(lr > 0.f) || !(S > 0.f)) return;
