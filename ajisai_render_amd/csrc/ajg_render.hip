@@ -3320,7 +3320,7 @@ int ajg_batch_autotune(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs, uint3
     std::vector<int32_t> values;
   };
   const Knob knobs[] = {
-      {&ajg_batch_args::max_slots, {1 << 19, 1 << 20, 1 << 21, 1 << 22}},
+      {&ajg_batch_args::max_slots, {1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23}},
       {&ajg_batch_args::samples_per_item, {0, 1, 2, 4, 8}},
       {&ajg_batch_args::pixel_tile, {0, 4, 8, 16}},
       {&ajg_batch_args::lds_stack, {4, 8, 16}},
