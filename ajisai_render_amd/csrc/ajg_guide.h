@@ -94,15 +94,23 @@ __device__ __forceinline__ f3 vmf_sample_dir(const VmfLobe &l, float u1, float u
   return normalize(add(add(scale(t1, s * cp), scale(t2, s * sp)), scale(n, wz)));
 }
 
-// Sample the mixture: u0 picks the lobe, (u1, u2) the direction.
+// Sample the mixture: u0 picks the lobe (the first whose running weight sum
+// exceeds u0, else the last), (u1, u2) the direction. The chosen lobe is
+// copied out by a fully unrolled scan, so the lobes stay in registers (an
+// indexed read would place them in scratch memory).
 __device__ __forceinline__ f3 mixture_sample(const VmfLobe *lobes, float u0, float u1, float u2) {
-  int j = 0;
+  VmfLobe sel = lobes[kGLobes - 1];
   float c = 0.f;
-  for (; j < kGLobes - 1; ++j) {
+  bool found = false;
+#pragma unroll
+  for (int j = 0; j < kGLobes - 1; ++j) {
     c += lobes[j].w;
-    if (u0 < c) break;
+    if (!found && u0 < c) {
+      sel = lobes[j];
+      found = true;
+    }
   }
-  return vmf_sample_dir(lobes[j], u1, u2);
+  return vmf_sample_dir(sel, u1, u2);
 }
 
 // The mixture as the renderer stores it per closest-hit queue entry: 40

@@ -1092,20 +1092,14 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   }
 }
 
-// Steps 1 and 2 of k_shade for one live slot: finish the previous bounce's
-// direct lighting, then shade the extension hit of the current sample
-// (NEE, MIS-BSDF and continuation rays). (As a function of its own the
-// plain kernel compiles to fewer spills: heavy mesh shading 50.9 -> 48.7 ms
-// per render. A second, wave-uniform-job copy for batches, taken when a
-// ballot shows that all lanes shade slots of one job, made the batched
-// kernel spill 34 registers and the batch 3% slower, and was dropped.)
+// Shading step 1 for one live slot (k_shade_pend): finish the previous
+// bounce's direct lighting (shadow-ray results, the MIS-BSDF ray's light)
+// and accumulate a finished sample.
 template <bool GUIDED, bool BATCH>
-__device__ __forceinline__ void shade_slot(const DevScene &sc0, const RenderParams &P0,
+__device__ __forceinline__ void shade_pend(const DevScene &sc0, const RenderParams &P0,
                                            const PathState &ps, const Queues &q,
                                            const GuideParams &G, const BatchParams &B,
-                                           uint32_t slot, uint32_t stripe, bool train,
-                                           uint32_t hslot, const uint32_t *s_pre, Pcg32 &rng,
-                                           bool &live, bool &need_new) {
+                                           uint32_t slot, bool train, uint32_t hslot) {
   // ---- 1. finish the previous bounce's direct lighting
   const uint32_t pf = ps.pend_flags[slot];
   if (pf & kPendActive) {
@@ -1204,6 +1198,21 @@ __device__ __forceinline__ void shade_slot(const DevScene &sc0, const RenderPara
     ps.pend_flags[slot] = 0;
   }
 
+}
+
+// Shading step 2 for one live slot (k_shade): shade the extension hit of
+// the current sample (NEE, MIS-BSDF and continuation rays). (As a function
+// of its own the plain kernel compiles to fewer spills. A second,
+// wave-uniform-job copy for batches, taken when a ballot shows that all
+// lanes shade slots of one job, made the batched kernel spill 34 registers
+// and the batch 3% slower, and was dropped.)
+template <bool GUIDED, bool BATCH>
+__device__ __forceinline__ void shade_slot(const DevScene &sc0, const RenderParams &P0,
+                                           const PathState &ps, const Queues &q,
+                                           const GuideParams &G, const BatchParams &B,
+                                           uint32_t slot, uint32_t stripe, bool train,
+                                           uint32_t hslot, const uint32_t *s_pre, Pcg32 &rng,
+                                           bool &live, bool &need_new) {
   // ---- 2. shade the extension hit of the current sample
   const uint32_t ei = ps.ext_idx[slot];
   if (ei == kNone) {
@@ -1530,6 +1539,27 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
   if (valid && live) q.lv_out[li] = slot;
   const bool any_live = __ballot(valid && live) != 0ull;
   if ((threadIdx.x & 63) == 0 && any_live) *q.live = 1u;
+}
+
+// Step 1 of shading, finishing the previous bounce's direct lighting for
+// every slot of the live-slot list, as a kernel of its own: it needs about
+// 50 registers and runs eight waves per SIMD, where inside k_shade it ran
+// at k_shade's three (heavy mesh shading 48.7 -> 45.7 ms per render, both
+// kernels). k_shade runs steps 2 and 3 after it on the same stream.
+template <bool GUIDED, bool BATCH>
+__global__ void __launch_bounds__(256)
+k_shade_pend(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, BatchParams B) {
+  const uint32_t total = *q.lv_total;
+  if (blockIdx.x * 256 >= total) return;
+  __shared__ uint32_t s_lv[kQStripes + 1];
+  q_prefix(q.lv_q, s_lv);
+  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+  if (v < total) {
+    const uint32_t slot = q.lv_in[q_entry(s_lv, q.lv_q.R, v)];
+    const bool train = GUIDED && G.train_stride && slot % G.train_stride == 0;
+    const uint32_t hslot = train ? slot / G.train_stride : 0u;
+    shade_pend<GUIDED, BATCH>(sc0, P0, ps, q, G, B, slot, train, hslot);
+  }
 }
 
 // Guided rendering: position and unit shading normal of each closest-queue
@@ -2728,6 +2758,12 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
                          totals);
       timer.begin(2, s);
       if (guided)
+        hipLaunchKernelGGL((k_shade_pend<true, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q,
+                           G, nob);
+      else
+        hipLaunchKernelGGL((k_shade_pend<false, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q,
+                           G, nob);
+      if (guided)
         hipLaunchKernelGGL((k_shade<true, false>), dim3(shade_grid), dim3(256), 0, s, ds, P, ps, q, G,
                            nob);
       else
@@ -3149,6 +3185,8 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       B.a_job_in = bb->a_job[cur];
       hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(64), 0, s, counters, rb->qctr, cur, nxt,
                          totals);
+      hipLaunchKernelGGL((k_shade_pend<false, true>), dim3(shade_grid), dim3(256), 0, s, sc0, P0, ps, q,
+                         G, B);
       hipLaunchKernelGGL((k_shade<false, true>), dim3(shade_grid), dim3(256), 0, s, sc0, P0, ps, q, G,
                          B);
       q_ok(hipEventRecord(rb->ev_fork, s));

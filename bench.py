@@ -68,8 +68,8 @@ OCC_BYTES = 1
 def shade_vertex_bytes(n_lights):
     """Algorithmic bytes of the shade stage per shaded vertex (a closest-hit
     result turned into the next rays), one environment or directional light
-    per entry of n_lights, no area lights: the records k_shade reads and
-    writes for a live slot.
+    per entry of n_lights, no area lights: the records the shade stage
+    (k_shade_pend, then k_shade) reads and writes for a live slot.
     Reads: the previous bounce's pending record (flags 4, per light shadow
     index 4 + occlusion byte 1 + contribution 12, MIS index 4, MIS ray 32,
     f 12, pdf 4, shading normal 12, throughput 12, radiance 12, work / sample
@@ -844,6 +844,13 @@ def profiled_traffic(kernel):
     return None
 
 
+def sum_traffic(*kernels):
+    """Summed profiled HBM bytes per launch of kernels that run back to back
+    as one stage; None unless every one of them is in the summary."""
+    t = [profiled_traffic(k) for k in kernels]
+    return None if any(x is None for x in t) else sum(t)
+
+
 def host_cpu():
     """The host's CPU model, logical CPU count and the CPUs this process may
     run on."""
@@ -1013,9 +1020,12 @@ def main():
                 {"nodes_per_query": round(cs.nodes_fetched_any / max(1, any_q), 3),
                  "tris_per_query": round(cs.tris_tested_any / max(1, any_q), 3),
                  "traffic": profiled_traffic("k_pq_any")}),
-            "shade": stage_roofline("k_shade", b_shade, ms_shade, stats[0].launches_shade,
+            # two kernels per launch of the stage: k_shade_pend (the previous
+            # bounce's direct light) then k_shade (the new hit), one stream
+            "shade": stage_roofline("k_shade_pend + k_shade", b_shade, ms_shade,
+                                    stats[0].launches_shade,
                                     {"bytes_per_vertex": shade_vertex_bytes(1),
-                                     "traffic": profiled_traffic("k_shade")}),
+                                     "traffic": sum_traffic("k_shade_pend", "k_shade")}),
         }
         # the dominant kernel: the stage with the most device time per step
         dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
