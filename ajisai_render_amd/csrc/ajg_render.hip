@@ -1884,7 +1884,6 @@ static void scene_free(ajg_scene *s) {
   if (!s) return;
   destroy_accel(s->accel);
   (void)hipFree(s->tri_shade);
-  (void)hipFree(s->normals);
   (void)hipFree(s->mats);
   for (void *p : s->light_bufs) (void)hipFree(p);
   delete (RenderBuffers *)s->render_cache;
@@ -1991,6 +1990,19 @@ static void shade_records(const ajg_primitive *prims, uint32_t p0, uint32_t p1,
       }
     }
   }
+}
+
+// The leaf slot's record: the shade flags and, for a mesh triangle, its
+// vertex normals copied from the concatenated normal array.
+static TriShade tri_record(const uint4 &info, const std::vector<float4> &normals) {
+  TriShade t;
+  t.info = info;
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool mesh = (info.x >> kKindShift) == AJG_GEOM_MESH;
+  t.n[0] = mesh ? normals[info.y] : z;
+  t.n[1] = mesh ? normals[info.z] : z;
+  t.n[2] = mesh ? normals[info.w] : z;
+  return t;
 }
 
 static int download_perm(ajg_ctx *ctx, const ajg_accel *a, std::vector<uint32_t> *perm) {
@@ -2277,16 +2289,12 @@ int ajg_scene_create(ajg_ctx *ctx, const ajg_primitive *prims, uint32_t n_prims,
   by_id.reserve(n);
   normals.reserve(n_normals);
   shade_records(prims, 0, n_prims, &by_id, &normals);
-  if (normals.empty()) normals.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
-  std::vector<uint4> by_slot(n);
-  for (uint32_t i = 0; i < n; ++i) by_slot[i] = by_id[perm[i]];
+  std::vector<TriShade> by_slot(n);
+  for (uint32_t i = 0; i < n; ++i) by_slot[i] = tri_record(by_id[perm[i]], normals);
   int st = 0;
-  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(uint4) * n, "tri shade")) ||
-      (st = dev_alloc(ctx, (void **)&s->normals, sizeof(float4) * normals.size(), "normals")))
+  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(TriShade) * n, "tri shade")))
     return bail(st);
-  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(uint4) * n, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(TriShade) * n, hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(hip_fail(ctx, e, "scene upload"));
   if ((st = finish_scene(ctx, s, mats, n_mats, lights, n_lights, env_cdf_fp64, prims, n_prims,
                           tri.data())))
@@ -2321,7 +2329,7 @@ int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes, uint32
     return st;
   };
   // bottom level: one PLOC tree per mesh, in the mesh's own space
-  std::vector<uint4> by_slot;
+  std::vector<TriShade> by_slot;
   std::vector<float4> normals;
   normals.reserve(n_normals);
   std::vector<size_t> shade_off(n_meshes);
@@ -2339,10 +2347,9 @@ int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes, uint32
     by_id.reserve(acc->n_tri);
     shade_records(meshes, m, m + 1, &by_id, &normals);
     shade_off[m] = by_slot.size();
-    for (uint32_t i = 0; i < acc->n_tri; ++i) by_slot.push_back(by_id[perm[i]]);
+    for (uint32_t i = 0; i < acc->n_tri; ++i) by_slot.push_back(tri_record(by_id[perm[i]], normals));
     max_depth = std::max(max_depth, acc->depth);
   }
-  if (normals.empty()) normals.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
   // top level: PLOC over the instance boxes, each given to the builder as the
   // degenerate triangle (min, max, min), whose box is exactly the instance box
   s->n_inst = n_instances;
@@ -2377,8 +2384,7 @@ int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes, uint32
   set_world(s, wbox);
   std::vector<DevBlas> db(n_meshes);
   int st = 0;
-  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(uint4) * by_slot.size(), "tri shade")) ||
-      (st = dev_alloc(ctx, (void **)&s->normals, sizeof(float4) * normals.size(), "normals")) ||
+  if ((st = dev_alloc(ctx, (void **)&s->tri_shade, sizeof(TriShade) * by_slot.size(), "tri shade")) ||
       (st = dev_alloc(ctx, (void **)&s->insts, sizeof(DevInst) * n_instances, "instances")) ||
       (st = dev_alloc(ctx, (void **)&s->blases, sizeof(DevBlas) * n_meshes, "mesh trees")))
     return bail(st);
@@ -2394,9 +2400,7 @@ int ajg_scene_create_instanced(ajg_ctx *ctx, const ajg_primitive *meshes, uint32
     db[m].tri_shade = s->tri_shade + shade_off[m];
     db[m].perm = a->perm;
   }
-  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(uint4) * by_slot.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(s->normals, normals.data(), sizeof(float4) * normals.size(), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(s->tri_shade, by_slot.data(), sizeof(TriShade) * by_slot.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(s->insts, di.data(), sizeof(DevInst) * n_instances, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(s->blases, db.data(), sizeof(DevBlas) * n_meshes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return bail(hip_fail(ctx, e, "instanced scene upload"));
@@ -2551,7 +2555,6 @@ DevScene dev_scene(const ajg_scene *sc) {
     ds.bvh.root_mx[i] = a->root_box[3 + i];
   }
   ds.tri_shade = sc->tri_shade;
-  ds.normals = sc->normals;
   ds.mats = sc->mats;
   for (int i = 0; i < kMaxLights; ++i) ds.lights[i] = sc->lights[i];
   ds.n_lights = sc->n_lights;

@@ -11,8 +11,7 @@
 struct ajg_scene {
   ajg_ctx *ctx = nullptr;
   ajg_accel *accel = nullptr;
-  uint4 *tri_shade = nullptr;  // per leaf slot: flags|material, n_i, n_j, n_k
-  float4 *normals = nullptr;   // mesh normal_and_vs, all meshes concatenated
+  ajg::TriShade *tri_shade = nullptr;  // per leaf slot: flags|material and vertex normals
   ajg::DevMaterial *mats = nullptr;
   uint32_t n_mats = 0;
   ajg::DevLight lights[ajg::kMaxLights];
@@ -48,8 +47,7 @@ static constexpr uint32_t kMatMask = (1u << 24) - 1;
 
 struct DevScene {
   DevBVH bvh;                  // instanced: the top-level tree
-  const uint4 *tri_shade;
-  const float4 *normals;
+  const TriShade *tri_shade;
   const DevInst *inst;         // instanced scenes only (else null)
   const DevBlas *blas;
   const DevMaterial *mats;
@@ -79,7 +77,7 @@ __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay 
   Inct it;
   f3 a, b, c;
   const float4 *leaf = sc.bvh.leaf_tris;
-  const uint4 *shade = sc.tri_shade;
+  const TriShade *shade = sc.tri_shade;
   const DevInst *I = nullptr;
   if (sc.inst) {
     I = sc.inst + inst;
@@ -88,7 +86,8 @@ __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay 
     shade = B.tri_shade;
   }
   load_leaf(leaf, ls, &a, &b, &c);
-  const uint4 info = shade[ls];
+  const TriShade ts = shade[ls];
+  const uint4 info = ts.info;
   const uint32_t kind = info.x >> kKindShift;
   it.material = (I && I->info.y != kKeepMaterial) ? I->info.y : (info.x & kMatMask);
   it.light = (info.x >> kLightShift) & 7u;
@@ -101,7 +100,7 @@ __device__ __forceinline__ Inct post_intersect(const DevScene &sc, const DevRay 
     it.ng = normalize(cross(sub(b, a), sub(c, a)));
     if (kind == AJG_GEOM_MESH) {
       const float w = 1 - u - v;
-      const float4 ni = sc.normals[info.y], nj = sc.normals[info.z], nk = sc.normals[info.w];
+      const float4 ni = ts.n[0], nj = ts.n[1], nk = ts.n[2];
       it.ns = add(add(scale(mk3(ni.x, ni.y, ni.z), w), scale(mk3(nj.x, nj.y, nj.z), u)),
                   scale(mk3(nk.x, nk.y, nk.z), v));
     } else {

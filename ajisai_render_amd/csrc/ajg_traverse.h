@@ -657,9 +657,18 @@ struct DevInst {
 };
 static constexpr uint32_t kKeepMaterial = 0xFFFFFFFFu;
 
+// Per leaf slot shading record, one 64-byte line: flags | material (and
+// the normal indices it was built from), then, for mesh triangles, the
+// three vertex normals (normal_and_vs) in index order, so PostIntersect
+// reads the record and the normals with one dependent load instead of two.
+struct TriShade {
+  uint4 info;
+  float4 n[3];
+};
+
 struct DevBlas {
   DevBVH bvh;
-  const uint4 *tri_shade;  // per leaf slot (as the one-level scene)
+  const TriShade *tri_shade;  // per leaf slot (as the one-level scene)
   const uint32_t *perm;    // leaf slot -> triangle index within the mesh
 };
 

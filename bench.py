@@ -75,14 +75,14 @@ def shade_vertex_bytes(n_lights):
     f 12, pdf 4, shading normal 12, throughput 12, radiance 12, work / sample
     / bank 12), the extension ray 32 and hit 16, slot state (index 4,
     bounce 4, radiance 12, throughput 12, work / sample / end 12, rng 8),
-    the triangle's shade record 16 and vertices 48, three vertex normals 48,
-    the material 48.
+    the triangle's shade record with its three vertex normals 64 and its
+    vertices 48, the material 48.
     Writes: the new pending record (radiance 12, throughput 12, work /
     sample / bank 12, flags 4, per light contribution 12 + shadow index 4 +
     shadow ray 32, MIS ray 32 + f / pdf / normal 28 + index 4), the
     continuation ray 32, slot index 4, bounce 4, throughput 12, rng 8."""
     reads = 4 + 17 * n_lights + 4 + 32 + 12 + 4 + 12 + 12 + 12 + 12 + \
-        32 + 16 + 4 + 4 + 12 + 12 + 12 + 8 + 16 + 48 + 48 + 48
+        32 + 16 + 4 + 4 + 12 + 12 + 12 + 8 + 64 + 48 + 48
     writes = 12 + 12 + 12 + 4 + 48 * n_lights + 32 + 28 + 4 + 32 + 4 + 4 + 12 + 8
     return reads + writes
 
