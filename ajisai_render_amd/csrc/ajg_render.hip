@@ -141,7 +141,6 @@ struct PathState {
   uint32_t *sample;     // sample index within the work item's range (absolute)
   uint32_t *sample_end;
   uint32_t *bounce;
-  uint32_t *ext_idx;    // index of the extension ray in the closest queue, kNone
   uint32_t *bank;       // accumulator bank of the current work item
   uint64_t *rng;
   float *L, *beta;      // 3 per slot
@@ -186,9 +185,10 @@ struct Queues {
   // slots it leaves live to the next list, so slots with nothing left to
   // do take no lanes (see k_shade).
   const uint32_t *lv_in;
+  const uint32_t *lv_ei_in;  // per entry: the slot's extension ray (closest queue index, kNone)
   QIn lv_q;                  // the input list's stripes
   const uint32_t *lv_total;  // its length (written by k_iter_begin)
-  uint32_t *lv_out;
+  uint32_t *lv_out, *lv_ei_out;
   uint32_t *lv_count_out;    // kQStripes counter words of the output list
   uint32_t lv_len;           // entries per stripe
   float *partial;      // total_work x 6 (sum3, sq3) or the film when n_chunks == 1
@@ -839,11 +839,13 @@ __global__ void k_iter_begin(uint32_t *ctr, uint32_t *qctr, int cur, int nxt,
 // the slots would append them (wave w to stripe w % kQStripes), with the
 // stripe counts. qctr_list: the list's kQStripes counter words.
 __global__ void __launch_bounds__(256)
-k_lv_init(uint32_t *lv, uint32_t *qctr_list, uint32_t S, uint32_t R) {
+k_lv_init(uint32_t *lv, uint32_t *lv_ei, uint32_t *qctr_list, uint32_t S, uint32_t R) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i < S) {
     const uint32_t w = i >> 6;
-    lv[(w % kQStripes) * R + (w / kQStripes) * 64 + (i & 63)] = i;
+    const uint32_t e = (w % kQStripes) * R + (w / kQStripes) * 64 + (i & 63);
+    lv[e] = i;
+    lv_ei[e] = kNone;
   }
   if (blockIdx.x == 0 && threadIdx.x < kQStripes) {
     const uint32_t st = threadIdx.x, full = S >> 6, rem = S & 63;
@@ -980,7 +982,7 @@ __device__ __forceinline__ AccTarget acc_job(const BatchParams &B, uint32_t j) {
 template <bool BATCH>
 __device__ void next_sample(const PathState &ps, const Queues &q, const RenderParams &P0,
                             const BatchParams &B, uint32_t slot, bool need, Pcg32 &rng,
-                            bool &live, uint32_t stripe) {
+                            bool &live, uint32_t stripe, uint32_t &ei_out) {
   uint32_t work = need ? ps.work[slot] : kNone;
   uint32_t job = BATCH && need ? ps.job[slot] : 0u;
   uint32_t sample = 0;
@@ -1049,7 +1051,7 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   if (!need) return;
   ps.work[slot] = work;
   if (work == kNone) {
-    ps.ext_idx[slot] = kNone;
+    ei_out = kNone;
     return;
   }
   if (BATCH) ps.job[slot] = job;
@@ -1087,9 +1089,9 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
   if (alive) {
     store_ray(q.c_rays_out, idx, r);
     if (BATCH) B.c_job_out[idx] = (uint16_t)job;
-    ps.ext_idx[slot] = idx;
+    ei_out = idx;
   } else {
-    ps.ext_idx[slot] = kNone;
+    ei_out = kNone;
   }
   if (!alive) {
     // zero-radiance sample: accumulate immediately, then the next sample
@@ -1097,7 +1099,6 @@ __device__ void next_sample(const PathState &ps, const Queues &q, const RenderPa
     const bool last = sample + 1 >= ps.sample_end[slot];
     accumulate(ps, BATCH ? acc_job(B, job) : acc_single(q, P), slot, ps.bank[slot], work, sample,
                mk3(0.f, 0.f, 0.f), last);
-    ps.ext_idx[slot] = kNone;
     ps.pend_flags[slot] = 0;
   }
 }
@@ -1222,9 +1223,9 @@ __device__ __forceinline__ void shade_slot(const DevScene &sc0, const RenderPara
                                            const GuideParams &G, const BatchParams &B,
                                            uint32_t slot, uint32_t stripe, bool train,
                                            uint32_t hslot, const uint32_t *s_pre, Pcg32 &rng,
-                                           bool &live, bool &need_new) {
+                                           uint32_t ei, uint32_t &ei_next, bool &live,
+                                           bool &need_new) {
   // ---- 2. shade the extension hit of the current sample
-  const uint32_t ei = ps.ext_idx[slot];
   if (ei == kNone) {
     if (ps.work[slot] != kNone) {
       // a sample ended before its first intersection: start the next one
@@ -1480,7 +1481,7 @@ __device__ __forceinline__ void shade_slot(const DevScene &sc0, const RenderPara
       if (cont) {
         store_ray(q.c_rays_out, ni, nr);
         if (BATCH) B.c_job_out[ni] = (uint16_t)j;
-        ps.ext_idx[slot] = ni;
+        ei_next = ni;
         ps.bounce[slot] = b + 1;
         st_f3(ps.beta, slot, beta);
         if (train) {
@@ -1528,7 +1529,9 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
   q_prefix(q.lv_q, s_lv);
   const uint32_t v = blockIdx.x * 256 + threadIdx.x;
   const bool valid = v < total;
-  const uint32_t slot = valid ? q.lv_in[q_entry(s_lv, q.lv_q.R, v)] : 0u;
+  const uint32_t le = valid ? q_entry(s_lv, q.lv_q.R, v) : 0u;
+  const uint32_t slot = valid ? q.lv_in[le] : 0u;
+  const uint32_t ei = valid ? q.lv_ei_in[le] : kNone;
   const uint32_t stripe = (v >> 6) % kQStripes;  // queue stripe of this wave
   const bool train = GUIDED && valid && G.train_stride && slot % G.train_stride == 0;
   const uint32_t hslot = train ? slot / G.train_stride : 0u;
@@ -1537,16 +1540,22 @@ k_shade(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams G, Ba
   rng.state = valid ? ps.rng[slot] : 0ull;
   bool need_new = false;
 
-  if (valid) shade_slot<GUIDED, BATCH>(sc0, P0, ps, q, G, B, slot, stripe, train, hslot, s_pre, rng, live,
-                                       need_new);
+  uint32_t ei_next = kNone;
+  if (valid)
+    shade_slot<GUIDED, BATCH>(sc0, P0, ps, q, G, B, slot, stripe, train, hslot, s_pre, rng, ei, ei_next,
+                              live, need_new);
 
   // ---- 3. next sample / ray generation (wave-uniform control flow)
-  next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live, stripe);
+  next_sample<BATCH>(ps, q, P0, B, slot, valid && need_new, rng, live, stripe, ei_next);
   if (valid) ps.rng[slot] = rng.state;
   // the slots left live (a path, pending light or a sample to start) are
-  // the next iteration's list
+  // the next iteration's list, each with its extension ray (the list
+  // carries it, so shading reads it with the slot instead of after it)
   const uint32_t li = q_append(valid && live, q.lv_count_out, q.lv_len, stripe);
-  if (valid && live) q.lv_out[li] = slot;
+  if (valid && live) {
+    q.lv_out[li] = slot;
+    q.lv_ei_out[li] = ei_next;
+  }
   const bool any_live = __ballot(valid && live) != 0ull;
   if ((threadIdx.x & 63) == 0 && any_live) *q.live = 1u;
 }
@@ -1751,6 +1760,7 @@ struct RenderBuffers {
   uint32_t *counters = nullptr;
   uint32_t *qctr = nullptr;  // striped queue counters: closest 0, 1, shadow 0, 1
   uint32_t *lv[2] = {nullptr, nullptr};  // live-slot lists (kQStripes * lv_stripe_len(S))
+  uint32_t *lv_ei[2] = {nullptr, nullptr};  // their entries' extension rays
   float *partial = nullptr;
   uint32_t *partial_n = nullptr;
   // shadow rays are traced on a second stream, concurrently with the
@@ -1838,7 +1848,7 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
   int st = 0;
   if ((st = al((void **)&ps.work, 4 * (size_t)S, "state")) || (st = al((void **)&ps.sample, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.sample_end, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bounce, 4 * (size_t)S, "state")) ||
-      (st = al((void **)&ps.ext_idx, 4 * (size_t)S, "state")) || (st = al((void **)&ps.bank, 4 * (size_t)S, "state")) ||
+      (st = al((void **)&ps.bank, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.rng, 8 * (size_t)S, "state")) || (st = al((void **)&ps.L, 12 * (size_t)S, "state")) ||
       (st = al((void **)&ps.beta, 12 * (size_t)S, "state")) || (st = al((void **)&ps.pend_flags, 4 * (size_t)S, "state")) ||
       (st = al((void **)&ps.pend_mis, 4 * (size_t)S, "state")) ||
@@ -1862,6 +1872,8 @@ static int ensure_buffers(ajg_ctx *ctx, RenderBuffers *rb, uint32_t S, uint64_t 
       (st = al((void **)&rb->a_occ, aq_cap, "shadow results")) ||
       (st = al((void **)&rb->lv[0], 4 * (size_t)kQStripes * lv_stripe_len(S), "live-slot list")) ||
       (st = al((void **)&rb->lv[1], 4 * (size_t)kQStripes * lv_stripe_len(S), "live-slot list")) ||
+      (st = al((void **)&rb->lv_ei[0], 4 * (size_t)kQStripes * lv_stripe_len(S), "live-slot list")) ||
+      (st = al((void **)&rb->lv_ei[1], 4 * (size_t)kQStripes * lv_stripe_len(S), "live-slot list")) ||
       (st = al((void **)&rb->counters, 4 * kCounterWords, "counters")) ||
       (st = al((void **)&rb->qctr, 4 * kQctrWords, "queue counters")) ||
       (st = al((void **)&rb->partial, sizeof(float) * 6 * total, "partial sums")) ||
@@ -2671,7 +2683,6 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   fl.words(ps.work, S, kNone);
   fl.words(ps.sample, S);
   fl.words(ps.sample_end, S);
-  fl.words(ps.ext_idx, S, kNone);
   fl.words(ps.bank, S);
   fl.words(ps.rng, 2ull * S);
   fl.words(ps.pend_flags, S);
@@ -2692,7 +2703,8 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   AJG_RCHK(fl.launch(s));
   // the first shade runs over every slot (list 0 holds them all)
   const uint32_t lv_len = lv_stripe_len(S);
-  hipLaunchKernelGGL(k_lv_init, dim3(cdiv(S, 256)), dim3(256), 0, s, rb->lv[0], rb->qctr + kListOff, S,
+  hipLaunchKernelGGL(k_lv_init, dim3(cdiv(S, 256)), dim3(256), 0, s, rb->lv[0], rb->lv_ei[0],
+                     rb->qctr + kListOff, S,
                      lv_len);
   // guided renders keep the fetch order through the counter for every item
   // (their training slots are every train_stride-th slot, and a slot's
@@ -2763,8 +2775,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
       q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
       q.a_len = a_len;
       q.lv_in = rb->lv[cur];
+      q.lv_ei_in = rb->lv_ei[cur];
       q.lv_q = {rb->qctr + kListOff + cur * kQWords, lv_len};
       q.lv_out = rb->lv[nxt];
+      q.lv_ei_out = rb->lv_ei[nxt];
       q.lv_count_out = rb->qctr + kListOff + nxt * kQWords;
       const QIn cq_in = {rb->qctr + nxt * kQWords, c_len};
       const QIn aq_in = {rb->qctr + (2 + nxt) * kQWords, a_len};
@@ -3102,7 +3116,6 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
     fl.words(ps.work, S, kNone);
     fl.words(ps.sample, S);
     fl.words(ps.sample_end, S);
-    fl.words(ps.ext_idx, S, kNone);
     fl.words(ps.bank, S);
     fl.words(ps.rng, 2ull * S);
     fl.words(ps.pend_flags, S);
@@ -3127,7 +3140,8 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
     }
     AJG_BCHK(fl.launch(s));
     // the first shade runs over every slot (list 0 holds them all)
-    hipLaunchKernelGGL(k_lv_init, dim3(cdiv(S, 256)), dim3(256), 0, s, rb->lv[0], rb->qctr + kListOff,
+    hipLaunchKernelGGL(k_lv_init, dim3(cdiv(S, 256)), dim3(256), 0, s, rb->lv[0], rb->lv_ei[0],
+                       rb->qctr + kListOff,
                        S, lv_stripe_len(S));
   }
 
@@ -3188,8 +3202,10 @@ static int render_batch_impl(ajg_ctx *ctx, const ajg_job *jobs, uint32_t n_jobs,
       q.a_rays_out = rb->a_rays[nxt];
       q.a_count_out = rb->qctr + (2 + nxt) * kQWords;
       q.lv_in = rb->lv[cur];
+      q.lv_ei_in = rb->lv_ei[cur];
       q.lv_q = {rb->qctr + kListOff + cur * kQWords, lv_len};
       q.lv_out = rb->lv[nxt];
+      q.lv_ei_out = rb->lv_ei[nxt];
       q.lv_count_out = rb->qctr + kListOff + nxt * kQWords;
       const QIn cq_in = {rb->qctr + nxt * kQWords, c_len};
       const QIn aq_in = {rb->qctr + (2 + nxt) * kQWords, a_len};
