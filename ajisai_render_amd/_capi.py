@@ -78,6 +78,7 @@ class PTArgs(C.Structure):
 RENDER_TIMING = 1
 RENDER_COUNT = 2
 RENDER_FINITE_PDF = 4
+RENDER_INLINE = 8
 
 
 class Film(C.Structure):

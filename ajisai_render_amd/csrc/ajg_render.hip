@@ -1581,6 +1581,259 @@ k_shade_pend(DevScene sc0, RenderParams P0, PathState ps, Queues q, GuideParams 
   }
 }
 
+// AJG_RENDER_INLINE: the whole render as one kernel, one thread per work
+// item tracing its samples' paths to their ends with single-ray traversal
+// (no queues, no per-bounce launches). Each step is the wavefront's own
+// arithmetic in the wavefront's order -- the same random draws, the same
+// hits (trace_closest / trace_any visit nodes as the queue kernels do), the
+// direct light summed light by light and then the MIS-BSDF term, the same
+// Kahan accumulation -- so films and per-sample radiance are bitwise those
+// of the wavefront render. For renders with few paths in flight (per-pixel
+// streams on small images), where each wavefront iteration costs its
+// kernels' latency floors. One-level scenes, plain (unguided) sampling.
+// A tree of at most kInlineLdsTris triangles (its nodes and leaf triangles,
+// 112 bytes per triangle) is copied into LDS first and traversed there.
+static constexpr uint32_t kInlineLdsTris = 256;
+
+template <int SPILL>
+__global__ void __launch_bounds__(256)
+k_pt_inline(DevScene sc, RenderParams P, float *partial, uint32_t *partial_n, float *per_sample,
+            unsigned long long *totals, uint32_t lds_tris) {
+  __shared__ int s_node[kLdsStack * 256];
+  __shared__ float s_entry[kLdsStack * 256];
+  extern __shared__ uint4 s_tree[];
+  int *ln = s_node + threadIdx.x;
+  float *le = s_entry + threadIdx.x;
+  if (lds_tris) {
+    // nodes (lds_tris - 1) x 64 B, then leaf triangles lds_tris x 48 B
+    const uint32_t nw = (lds_tris - 1) * 4, nl = lds_tris * 3;
+    const uint4 *gw = reinterpret_cast<const uint4 *>(sc.bvh.wide);
+    const uint4 *gl = reinterpret_cast<const uint4 *>(sc.bvh.leaf_tris);
+    for (uint32_t i = threadIdx.x; i < nw + nl; i += 256) s_tree[i] = i < nw ? gw[i] : gl[i - nw];
+    __syncthreads();
+    sc.bvh.wide = reinterpret_cast<const WideNode *>(s_tree);
+    sc.bvh.leaf_tris = reinterpret_cast<const float4 *>(s_tree + nw);
+  }
+  const uint32_t w = blockIdx.x * 256 + threadIdx.x;
+  uint32_t n_closest = 0, n_any = 0;
+  uint32_t nn = 0, nt = 0;
+  if (w < P.total_work) {
+    const float kInf = __int_as_float(0x7f800000);
+    const uint32_t chunk = w / P.n_pix;
+    const uint32_t p = pixel_of(w - chunk * P.n_pix, P.tile, P.tiles_x, P.Wp);
+    const uint32_t x = p % P.Wp, y = p / P.Wp;
+    const uint32_t s0 = chunk * P.chunk, s1 = min(P.spp, s0 + P.chunk);
+    Pcg32 rng;
+    rng.state = 0ull;
+    if (P.seed_mode == 0) pcg_seed(rng, (uint64_t)(int64_t)(int)(x + y * P.W));
+    float a[12];
+    for (int k = 0; k < 12; ++k) a[k] = 0.f;
+    uint32_t na = 0;
+    const bool lit = sc.env_index >= 0 || sc.n_area > 0;
+    // Each lane moves on to its next sample as soon as its path ends (one
+    // vertex per loop pass), so a wave waits for the longest sum of path
+    // lengths among its lanes, not for the longest path of every sample.
+    // A lane's random draws stay in their sequential order.
+    auto finish = [&](uint32_t smp, f3 Ls) {
+      kahan_add(a[0], a[3], Ls.x);
+      kahan_add(a[1], a[4], Ls.y);
+      kahan_add(a[2], a[5], Ls.z);
+      kahan_add(a[6], a[9], Ls.x * Ls.x);
+      kahan_add(a[7], a[10], Ls.y * Ls.y);
+      kahan_add(a[8], a[11], Ls.z * Ls.z);
+      ++na;
+      if (per_sample) {
+        float *o = per_sample + 3 * ((size_t)p * P.spp + smp);
+        o[0] = Ls.x;
+        o[1] = Ls.y;
+        o[2] = Ls.z;
+      }
+    };
+    uint32_t sample = s0, b = 0;
+    bool start = true;
+    DevRay ray;
+    f3 L = mk3(0.f, 0.f, 0.f), beta = mk3(1.f, 1.f, 1.f);
+    for (;;) {
+      if (start) {
+        if (sample >= s1) break;
+        if (P.seed_mode != 0) pcg_seed(rng, sample_seed(P.seed_base, p, sample));
+        // TiledIntegrator::Render per-sample draws (pt.cpp:58-61)
+        const float u = (float)x + pcg_next_float(rng);
+        const float v = (float)y + pcg_next_float(rng);
+        const float l0 = pcg_next_float(rng);
+        const float l1 = pcg_next_float(rng);
+        ray = camera_ray(P.cam, u, v, l0, l1);
+        L = mk3(0.f, 0.f, 0.f);
+        beta = mk3(1.f, 1.f, 1.f);
+        b = 0;
+        bool alive = true;
+        if (0 > P.min_bounces) {
+          if (pcg_next_float(rng) > P.cont_prob)
+            alive = false;
+          else
+            beta = divs(mk3(1.f, 1.f, 1.f), P.cont_prob);
+        }
+        if (P.max_bounces <= 0) alive = false;
+        if (!alive) {  // a zero-radiance sample
+          finish(sample, mk3(0.f, 0.f, 0.f));
+          ++sample;
+          continue;
+        }
+        start = false;
+      }
+      ++n_closest;
+      const THit h = trace_closest<SPILL, false>(sc.bvh, make_tray(ray.o, ray.d, ray.tmin, ray.tmax),
+                                                 ln, le, &nn, &nt);
+      if (h.slot < 0) {
+        if ((b == 0 || !P.use_mis) && sc.env_index >= 0)
+          L = add(L, env_radiance(sc.lights[sc.env_index].env, ray.d));
+        finish(sample, L);
+        ++sample;
+        start = true;
+        continue;
+      }
+      const Inct it = post_intersect(sc, ray, h.t, h.slot, h.u, h.v, 0u);
+      const Bsdf bs = make_bsdf(sc.mats[it.material], it.ng, it.ns);
+      if ((b == 0 || !P.use_mis) && it.light && dot(it.ng, it.wr) > 0)
+        L = add(L, mul(beta, sc.lights[it.light - 1].radiance));
+      // NEE (MISSampleLight, helper.h:100-133), visibility traced at once
+      f3 direct = mk3(0.f, 0.f, 0.f);
+      const uint32_t n_nee = P.use_mis ? sc.n_lights : 0u;
+      for (uint32_t k = 0; k < sc.n_lights; ++k) {
+        f3 c = mk3(0.f, 0.f, 0.f);
+        if (k < n_nee) {
+          const DevLight &l = sc.lights[k];
+          LightSample lsmp;
+          if (l.type == AJG_LIGHT_ENV) {
+            const float first = pcg_next_float(rng);
+            const float second = pcg_next_float(rng);
+            lsmp = env_sample(l.env, it.pos, second, first, sc.radius);
+          } else if (l.type == kLightArea) {
+            f3 sam;
+            sam.x = pcg_next_float(rng);
+            sam.y = pcg_next_float(rng);
+            sam.z = pcg_next_float(rng);
+            lsmp = area_sample(l, it.pos, sam);
+          } else {
+            const float u0 = pcg_next_float(rng);
+            const float u1 = pcg_next_float(rng);
+            lsmp = dir_sample(l, it.pos, u0, u1, sc.radius);
+          }
+          if (!(is_black(lsmp.radiance) || lsmp.pdf <= 0.f)) {
+            const float len = length(sub(lsmp.pos, it.pos)) - AJG_EPS;
+            if (!(len < AJG_EPS)) {
+              const f3 to_light = normalize(sub(lsmp.pos, it.pos));
+              const f3 bf = bsdf_eval_all(bs, to_light, it.wr);
+              if (!is_black(bf)) {
+                const f3 nrm = normalize(it.ns);
+                const f3 f = scale(mul(lsmp.radiance, bf),
+                                   fabsf(dot(to_light, l.type == kLightArea ? it.ns : nrm)));
+                const float bpdf = bsdf_pdf_all(bs, to_light, it.wr);
+                const float wl = power_heuristic(lsmp.pdf, bpdf);
+                const f3 cl = mul(beta, scale(divs(f, lsmp.pdf), wl));
+                const DevRay sr = make_ray(it.pos, to_light, AJG_EPS, 0.99f * len);
+                ++n_any;
+                if (!trace_any<SPILL>(sc.bvh, make_tray(sr.o, sr.d, sr.tmin, sr.tmax), ln, le))
+                  c = cl;
+              }
+            }
+          }
+        }
+        direct = add(direct, c);
+      }
+      // the BSDF sampler's two draws: MIS-BSDF (helper.h:209-246), then
+      // the continuation (pt.cpp:215-223)
+      BsdfSample g_mis, g_cont;
+      g_mis.valid = false;
+      for (int k = P.use_mis ? 0 : 1; k < 2; ++k) {
+        f3 sam;
+        sam.x = pcg_next_float(rng);
+        sam.y = pcg_next_float(rng);
+        sam.z = pcg_next_float(rng);
+        const BsdfSample smp = bsdf_sample_all(bs, it.wr, sam);
+        if (k == 0)
+          g_mis = smp;
+        else
+          g_cont = smp;
+      }
+      f3 mis = mk3(0.f, 0.f, 0.f);
+      if (g_mis.valid && !(is_black(g_mis.f) || g_mis.pdf == 0.f) && lit) {
+        const f3 dn = normalize(g_mis.dir);
+        const DevRay mr = make_ray(eps_offset(it, dn), dn, AJG_EPS, kInf);
+        bool escaped;
+        if (sc.n_area > 0) {
+          ++n_closest;
+          const THit mh = trace_closest<SPILL, false>(sc.bvh, make_tray(mr.o, mr.d, mr.tmin, mr.tmax),
+                                                      ln, le, &nn, &nt);
+          escaped = mh.slot < 0;
+          if (!escaped) {
+            const Inct hi = post_intersect(sc, mr, mh.t, mh.slot, mh.u, mh.v, 0u);
+            if (hi.light && dot(hi.ng, hi.wr) > 0) {  // AreaLight::Radiance
+              const DevLight &l = sc.lights[hi.light - 1];
+              if (!is_black(l.radiance)) {
+                const f3 f = scale(mul(l.radiance, g_mis.f), fabsf(dot(it.ns, mr.d)));
+                const float lpdf = area_pdf(l, mr.o, hi.pos, hi.ng);
+                mis = scale(divs(f, g_mis.pdf), power_heuristic(g_mis.pdf, lpdf));
+              }
+            }
+          }
+        } else {
+          ++n_any;
+          escaped = !trace_any<SPILL>(sc.bvh, make_tray(mr.o, mr.d, mr.tmin, mr.tmax), ln, le);
+        }
+        if (escaped && sc.env_index >= 0) {
+          const DevEnv &e = sc.lights[sc.env_index].env;
+          const f3 lev = env_radiance(e, mr.d);
+          if (!is_black(lev)) {
+            const f3 f = scale(mul(lev, g_mis.f), fabsf(dot(it.ns, mr.d)));
+            mis = scale(divs(f, g_mis.pdf), power_heuristic(g_mis.pdf, env_pdf(e, mr.d)));
+          }
+        }
+      }
+      direct = add(direct, mul(beta, mis));
+      // continuation and Russian roulette of the next bounce (pt.cpp:
+      // 107-110, 215-223); beta above is the vertex's own throughput
+      bool cont = g_cont.valid && !(is_black(g_cont.f) || g_cont.pdf < 0.f);
+      if ((P.flags & AJG_RENDER_FINITE_PDF) && g_cont.pdf == 0.f) cont = false;
+      f3 nbeta = beta;
+      if (cont) {
+        const f3 dir = g_cont.dir;
+        nbeta = mul(beta, divs(scale(g_cont.f, fabsf(dot(it.ng, dir))), g_cont.pdf));
+        ray = make_ray(eps_offset(it, dir), dir, AJG_EPS, kInf);
+        const int nb = (int)b + 1;
+        if (nb >= P.max_bounces) {
+          cont = false;
+        } else if (nb > P.min_bounces) {
+          if (pcg_next_float(rng) > P.cont_prob)
+            cont = false;
+          else
+            nbeta = divs(nbeta, P.cont_prob);
+        }
+      }
+      L = add(L, scale(direct, 1.f));
+      if (!cont) {
+        finish(sample, L);
+        ++sample;
+        start = true;
+        continue;
+      }
+      beta = nbeta;
+      ++b;
+    }
+    const size_t cell = (size_t)chunk * P.n_pix + p;  // chunk-major, pixel-minor
+    float *dst = partial + cell * 6;
+    dst[0] = a[0] - a[3];
+    dst[1] = a[1] - a[4];
+    dst[2] = a[2] - a[5];
+    dst[3] = a[6] - a[9];
+    dst[4] = a[7] - a[10];
+    dst[5] = a[8] - a[11];
+    partial_n[cell] = na;
+  }
+  wave_count_add(n_closest, totals);
+  wave_count_add(n_any, totals + 1);
+}
+
 // Guided rendering: position and unit shading normal of each closest-queue
 // hit (the guide network's input), by dense queue index; misses get a zero
 // record. Writes the queue's total to *total (the guide's batch size).
@@ -2738,6 +2991,10 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   if (!stack) return done(fail(ctx, AJG_ERR_INTERNAL, "render: tree too deep for the traversal stack"));
 
   const bool counting = (args->flags & AJG_RENDER_COUNT) != 0;
+  const bool inl = (args->flags & AJG_RENDER_INLINE) != 0;
+  if (inl && (guided || inst || counting))
+    return done(fail(ctx, AJG_ERR_INVALID,
+                     "render: AJG_RENDER_INLINE takes one-level scenes, no guide, no AJG_RENDER_COUNT"));
   StageTimer timer;
   timer.on = (args->flags & AJG_RENDER_TIMING) != 0;
   // traversal counters: closest (nodes, tris) at +8, any (nodes, tris) at +12
@@ -2761,7 +3018,18 @@ static int render_impl(ajg_scene *sc, const ajg_camera *camera, const ajg_pt_arg
   // u64 traversal counts, 16-19 u64 query totals (closest, shadow); the
   // queue counters, fetch cursors and work-item counters are in rb->qctr.
   unsigned long long *totals = (unsigned long long *)(counters + 16);
-  for (;;) {
+  if (inl) {
+    const uint32_t lt = a->n_tri >= 2 && a->n_tri <= kInlineLdsTris ? a->n_tri : 0u;
+    const size_t lds = lt ? 16 * (size_t)((lt - 1) * 4 + lt * 3) : 0;
+    if (stack == kSpillS)
+      hipLaunchKernelGGL(k_pt_inline<kSpillS>, dim3(cdiv(total, 256)), dim3(256), lds, s, ds, P, partial,
+                         partial_n, film->per_sample, totals, lt);
+    else
+      hipLaunchKernelGGL(k_pt_inline<kSpillL>, dim3(cdiv(total, 256)), dim3(256), lds, s, ds, P, partial,
+                         partial_n, film->per_sample, totals, lt);
+    AJG_RCHK(hipGetLastError());
+  }
+  if (!inl) for (;;) {
     for (int k = 0; k < kItersPerCheck; ++k) {
       // shade reads queue `cur` (traced) and writes queue `nxt`
       const int nxt = cur ^ 1;

@@ -727,7 +727,9 @@ def cornell_bench(ctx, a):
     radiance 5 plus a constant 0.1 environment, pinhole camera), 256 x 256,
     16 spp, max 8 bounces, Russian roulette after bounce 3: the GPU render
     with the reference's per-pixel sequential streams (seed_mode 0) and with
-    per-sample streams (seed_mode 1), each the mean of 20 timed renders, and
+    per-sample streams (seed_mode 1), each the mean of 20 timed renders, both
+    as the wavefront and as the inline kernel (AJG_RENDER_INLINE: each pixel's
+    paths traced to their ends by one thread, bitwise the same films), and
     the oracle's full render of the same image on the host (median of 3)."""
     import numpy as np
     import torch
@@ -742,8 +744,8 @@ def cornell_bench(ctx, a):
     out = {"workload": "cornell: 34 triangles, Lambert, directional + constant env, "
                        f"{w}x{h}, {sc.pt.spp} spp, max {sc.pt.max_bounces} bounces",
            "paths": paths}
-    for mode in (0, 1):
-        pt = PathTracer(g, seed_mode=mode)
+    for mode, inline in ((0, False), (1, False), (0, True), (1, True)):
+        pt = PathTracer(g, seed_mode=mode, flags=capi.RENDER_INLINE if inline else 0)
         film, _ = pt.render_device()
         for _ in range(3):
             pt.render_device(film=film)
@@ -756,9 +758,10 @@ def cornell_bench(ctx, a):
             q += st.closest_queries + st.any_queries
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / reps
-        out[f"seed_mode{mode}"] = {"ms": round(dt * 1e3, 3), "paths_per_s": round(paths / dt, 1),
-                                   "mrays_per_s": round(q / reps / dt / 1e6, 2),
-                                   "slots": int(st.slots)}
+        out[f"seed_mode{mode}" + ("_inline" if inline else "")] = {
+            "ms": round(dt * 1e3, 3), "paths_per_s": round(paths / dt, 1),
+            "mrays_per_s": round(q / reps / dt / 1e6, 2),
+            "slots": int(st.slots) if not inline else w * h}
     g.close()
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -781,6 +784,9 @@ def cornell_bench(ctx, a):
         out["gpu_over_cpu_paths_seed_mode0"] = round(out["seed_mode0"]["paths_per_s"] /
                                                      out["cpu_baseline"]["value"], 1)
         out["gpu_over_cpu_paths_seed_mode1"] = round(out["seed_mode1"]["paths_per_s"] /
+                                                     out["cpu_baseline"]["value"], 1)
+        for key in ("seed_mode0_inline", "seed_mode1_inline"):
+            out["gpu_over_cpu_paths_" + key] = round(out[key]["paths_per_s"] /
                                                      out["cpu_baseline"]["value"], 1)
     return out
 

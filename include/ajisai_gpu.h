@@ -235,6 +235,17 @@ typedef struct ajg_pt_args {
                                   pdf 0 (the reference continues it with an
                                   infinite throughput, pt.cpp:216-221; guided
                                   sampling always ends it) */
+#define AJG_RENDER_INLINE 8u   /* no counterpart in the reference: trace each
+                                  work item's paths to their ends in one
+                                  thread, the whole render one kernel, instead
+                                  of the wavefront's per-bounce launches. For
+                                  renders with few paths in flight (per-pixel
+                                  streams on small images), where each
+                                  wavefront iteration costs its kernels'
+                                  latency floors. Films and per-sample
+                                  radiance are bitwise the wavefront's. Not
+                                  with instanced scenes, guiding or
+                                  AJG_RENDER_COUNT (AJG_ERR_INVALID) */
 
 /* Film accumulators per traced pixel (traced area = whole tiles,
  * pt.cpp:43-49): Kahan-compensated float sums of L and L^2 per channel and
